@@ -1,0 +1,194 @@
+/*
+ * rawops_hip.h -- C ABI of the MI355X (gfx950) tiled RAW render path.
+ *
+ * This is the boundary the C++ drop-in layer (include/rawengine/RawEngine.hpp,
+ * rawengine::ImageGraph / Renderer) and the Python/PyTorch module call.  It
+ * replaces, for the render hot path, the node kernels the reference runs
+ * under Renderer::render_tiles (RawEngine.cpp:1390-1412):
+ *   - RawUnpackNode::render (bilinear RawEngine.cpp:299-344, Menon
+ *     MenonDemosaic.cpp:177-218),
+ *   - WhiteBalanceNode/ExposureNode/CameraToWorkingNode/WorkingSpaceConvert/
+ *     WorkingToSrgb/ToneCurve/OutputClip/SrgbEncode (RawEngine.cpp:408-758,
+ *     1215-1277) and ExtendedCurvesNode (ToneOps.cpp:795-844) as a fused
+ *     per-pixel op program,
+ *   - the mip-1/2 footprint reduction of CameraToWorkingNode::render_level and
+ *     RasterSourceNode::render_level (RawEngine.cpp:363-406, 580-601),
+ *   - working_y_guided_filter_rgb (SpatialOps.cpp:168-287).
+ * Plain pointers and sizes only.  `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream).  Device pointers are HBM addresses; host pointers
+ * are only taken where the name says so.  No exceptions cross this boundary:
+ * every call returns a rawops_status and rawops_last_error() holds the message
+ * (per thread).  Status codes map one to one onto the reference's exception
+ * types (RawEngine.hpp:646-649 and the std:: types it throws).
+ */
+#ifndef RAWOPS_HIP_H
+#define RAWOPS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RAWOPS_OK = 0,
+    RAWOPS_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+    RAWOPS_OUT_OF_RANGE = 2,     /* std::out_of_range (viewport outside source) */
+    RAWOPS_DOMAIN_ERROR = 3,     /* std::domain_error (non-finite at an output boundary) */
+    RAWOPS_LENGTH_ERROR = 4,     /* std::length_error */
+    RAWOPS_OVERFLOW_ERROR = 5,   /* std::overflow_error (guided filter) */
+    RAWOPS_CANCELLED = 6,        /* rawengine::RenderCancelled */
+    RAWOPS_LOGIC_ERROR = 7,      /* std::logic_error */
+    RAWOPS_DEVICE_ERROR = 8      /* HIP runtime failure */
+} rawops_status;
+
+typedef struct { uint32_t x, y, width, height; } rawops_rect;
+
+/* RawMetadata, RawEngine.hpp:57-65 (same fields, same normalisation rules:
+ * stride 0 = width, all-zero active area = full sensor). */
+typedef struct {
+    uint32_t width, height, row_stride_samples;
+    uint32_t pattern;                 /* 0 RGGB, 1 BGGR, 2 GRBG, 3 GBRG */
+    uint32_t cfa_phase_x, cfa_phase_y;
+    rawops_rect active_area;
+    uint16_t black_levels[4], white_levels[4];
+} rawops_raw_metadata;
+
+enum { RAWOPS_DEMOSAIC_BILINEAR = 0, RAWOPS_DEMOSAIC_MENON = 1 };
+enum { RAWOPS_PROPHOTO_D50 = 0, RAWOPS_REC2020_D65 = 1 };
+enum { RAWOPS_QUALITY_PREVIEW = 0, RAWOPS_QUALITY_FINAL = 1 };
+
+/* Per-pixel operations of the fused op program. */
+enum {
+    RAWOPS_OP_GAINS = 1,        /* rgb *= v[0..2]                WhiteBalanceNode */
+    RAWOPS_OP_SCALE = 2,        /* rgb *= v[0]                   ExposureNode */
+    RAWOPS_OP_MATRIX = 3,       /* (v0*r + v1*g) + v2*b ...      CameraToWorking etc. */
+    RAWOPS_OP_TONE = 4,         /* v0 shoulder, v1 inverse gamma ToneCurveNode */
+    RAWOPS_OP_CLIP = 5,         /* non-finite -> domain error    OutputClipNode */
+    RAWOPS_OP_SRGB_ENCODE = 6,  /* non-finite -> domain error    SrgbEncodeNode */
+    RAWOPS_OP_CURVES = 7        /* knots below                   ExtendedCurvesNode */
+};
+
+typedef struct {
+    uint32_t kind;
+    float v[9];
+    /* RAWOPS_OP_CURVES: master, R, G, B knot lists concatenated (host memory,
+     * copied by rawops_plan_create); knot_offsets[c]..knot_offsets[c+1]. */
+    const double* knots_x;
+    const double* knots_y;
+    uint32_t knot_offsets[5];
+    uint32_t cubic;
+} rawops_op_desc;
+
+/* A compiled chain: demosaic, ops at native resolution (up to and including
+ * the reduction anchor), ops after the mip reduction.  For mip 0 both lists
+ * run back to back.  `mip_anchor` is nonzero when the chain has a reduction
+ * anchor (CameraToWorkingNode with native bounds, RawEngine.cpp:550-601);
+ * reduced levels are rejected without one. */
+typedef struct {
+    uint32_t demosaic;
+    uint32_t mip_anchor;
+    uint32_t n_native, n_reduced;
+    const rawops_op_desc* native_ops;
+    const rawops_op_desc* reduced_ops;
+} rawops_plan_desc;
+
+typedef struct rawops_source rawops_source;
+typedef struct rawops_plan rawops_plan;
+
+const char* rawops_last_error(void);
+const char* rawops_version(void);
+
+/* ---- colour math on the host (RawEngine.cpp:116-219, 502-530, 603-672) ---- */
+/* CameraToWorkingNode ctor: float32 camera->working matrix, row-major. */
+rawops_status rawops_camera_to_working_matrix(const double camera_to_xyz_d50[9],
+                                              uint32_t working_space, float out[9]);
+/* WorkingToSrgbNode ctor: working -> linear sRGB. */
+rawops_status rawops_working_to_srgb_matrix(uint32_t working_space, float out[9]);
+/* WorkingSpaceConvertNode ctor: source -> target working space. */
+rawops_status rawops_working_to_working_matrix(uint32_t source, uint32_t target, float out[9]);
+/* FP64 XYZ(D50) -> working-space matrix (RawEngine.cpp:183-196), row-major. */
+rawops_status rawops_xyz_d50_to_working_matrix(uint32_t working_space, double out[9]);
+/* ExposureNode multiplier, std::exp2(stops) in float with the reference's
+ * stop range check (RawEngine.cpp:465-468). */
+rawops_status rawops_exposure_multiplier(float stops, float* out);
+
+/* ---- sources (device-resident) ---- */
+/* RawImage ctor validation (RawEngine.cpp:223-250) + upload of host samples. */
+rawops_status rawops_source_create_raw(const rawops_raw_metadata* metadata,
+                                       const uint16_t* host_samples, size_t sample_count,
+                                       rawops_source** out);
+/* Same validation; wraps caller-owned device samples without copying. */
+rawops_status rawops_source_wrap_raw_device(const rawops_raw_metadata* metadata,
+                                            const uint16_t* device_samples, size_t sample_count,
+                                            rawops_source** out);
+/* RasterImage (RawEngine.hpp:96-119): interleaved float32 RGB, stride in pixels. */
+rawops_status rawops_source_create_raster(uint32_t width, uint32_t height, uint32_t row_stride_pixels,
+                                          uint32_t working_space, const float* host_pixels,
+                                          rawops_source** out);
+rawops_status rawops_source_wrap_raster_device(uint32_t width, uint32_t height,
+                                               uint32_t row_stride_pixels, uint32_t working_space,
+                                               const float* device_pixels, rawops_source** out);
+/* Normalised metadata of a RAW source (stride and active area filled in). */
+rawops_status rawops_source_raw_metadata(const rawops_source* source, rawops_raw_metadata* out);
+void rawops_source_destroy(rawops_source* source);
+
+/* ---- plans ---- */
+rawops_status rawops_plan_create(const rawops_plan_desc* desc, rawops_plan** out);
+void rawops_plan_destroy(rawops_plan* plan);
+
+/* Source bounds of a request level (request_bounds, RawEngine.cpp:31-43). */
+rawops_status rawops_request_bounds(rawops_rect source_bounds, uint32_t mip, uint32_t quality,
+                                    rawops_rect* out);
+
+/* Render `viewport` (coordinates at level `mip`) of a RAW source through
+ * `plan` into device memory: dst_rgb (float32 interleaved RGB, row stride
+ * dst_stride_pixels, may be NULL) and/or dst_u8 (uint8 sRGB quantisation of
+ * the final values, round(clamp(v,0,1)*255), row stride dst_stride_pixels*3
+ * bytes, may be NULL).  Asynchronous on `stream`.  device_flags (may be NULL)
+ * is a device uint32 the kernels OR error bits into (bit 0: non-finite at an
+ * output boundary); rawops_check_flags maps it to a status. */
+rawops_status rawops_render(const rawops_plan* plan, const rawops_source* source,
+                            rawops_rect viewport, uint32_t mip, uint32_t quality,
+                            float* dst_rgb, uint8_t* dst_u8, uint32_t dst_stride_pixels,
+                            uint32_t* device_flags, void* stream);
+/* Reads a device_flags word (synchronising `stream`) and resets it to zero. */
+rawops_status rawops_check_flags(uint32_t* device_flags, void* stream);
+
+/* Synchronous host render with the reference's tile loop semantics
+ * (Renderer::render_tiles RawEngine.cpp:1390-1412): the viewport is rendered on
+ * the device in one launch, then `callback(user, tile_rect, rgb)` receives each
+ * tile_size x tile_size tile in row-major order from host memory; a nonzero
+ * *cancel (polled before the render and before each tile) returns
+ * RAWOPS_CANCELLED.  A nonzero callback return stops the loop with that value
+ * passed through as RAWOPS_CANCELLED.  The callback's pointer is valid only
+ * during the call. */
+typedef int (*rawops_tile_callback)(void* user, rawops_rect tile, const float* rgb);
+rawops_status rawops_render_tiles_host(const rawops_plan* plan, const rawops_source* source,
+                                       rawops_rect viewport, uint32_t mip, uint32_t quality,
+                                       uint32_t tile_size, rawops_tile_callback callback,
+                                       void* user, const volatile int* cancel);
+
+/* ---- guided filter (SpatialOps.cpp:159-317) + fused point epilogue ---- */
+/* Working-Y guided filter of a raster source at level `mip`, output rect
+ * `viewport` (level coordinates), radius 0..32 (the reference admits 0..8,
+ * SpatialOps.cpp:159-163; larger radii are an extension), eps in
+ * [2^-24, 65536], then the plan's `reduced_ops` program per pixel (post may be
+ * NULL).  Variance/covariance box moments are accumulated in FP64.
+ * `workspace` is device scratch of rawops_guided_workspace_size() bytes for
+ * the per-centre coefficients; NULL makes the call allocate it stream-ordered. */
+rawops_status rawops_guided_workspace_size(uint32_t raster_width, uint32_t raster_height,
+                                           rawops_rect viewport, uint32_t mip, uint32_t radius,
+                                           size_t* bytes);
+rawops_status rawops_guided_filter(const rawops_source* raster, rawops_rect viewport,
+                                   uint32_t mip, uint32_t radius, double epsilon,
+                                   const rawops_plan* post, float* dst_rgb,
+                                   uint32_t dst_stride_pixels, void* workspace,
+                                   size_t workspace_bytes, uint32_t* device_flags,
+                                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAWOPS_HIP_H */

@@ -1,0 +1,521 @@
+// C ABI of the MI355X render path (declared in include/rawops_hip.h).
+// Validation mirrors the reference's constructors and renderer checks so that
+// the C++ drop-in layer can rethrow the same exception types.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rawops_hip.h"
+#include "color.hpp"
+#include "render_launch.hpp"
+
+using namespace rawops;
+
+struct rawops_source {
+    bool is_raw = true;
+    rawops_raw_metadata meta{};     // RAW: normalised metadata
+    const uint16_t* samples = nullptr;
+    uint32_t width = 0, height = 0, stride = 0, space = 0;  // raster
+    const float* pixels = nullptr;
+    void* owned = nullptr;          // device allocation owned by the source
+};
+
+struct rawops_plan {
+    uint32_t demosaic = 0, mip_anchor = 0;
+    OpList native{}, reduced{};
+    OpList* device = nullptr;       // device copies: [0] native, [1] reduced, [2] empty
+    std::vector<void*> tables;      // device curve tables
+    ~rawops_plan() {
+        for (void* t : tables) (void)hipFree(t);
+        if (device) (void)hipFree(device);
+    }
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Failure {
+    rawops_status status;
+    std::string message;
+};
+
+[[noreturn]] void fail(rawops_status s, const std::string& m) { throw Failure{s, m}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(RAWOPS_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+rawops_status guarded(F&& body) {
+    try {
+        body();
+        g_error.clear();
+        return RAWOPS_OK;
+    } catch (const Failure& f) {
+        g_error = f.message;
+        return f.status;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return RAWOPS_INVALID_ARGUMENT;
+    } catch (const std::bad_alloc&) {
+        g_error = "host allocation failed";
+        return RAWOPS_LENGTH_ERROR;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return RAWOPS_LOGIC_ERROR;
+    }
+}
+
+// RawImage ctor checks, RawEngine.cpp:223-250.
+rawops_raw_metadata normalise(const rawops_raw_metadata* in, size_t count) {
+    if (!in) fail(RAWOPS_INVALID_ARGUMENT, "RAW metadata is required");
+    rawops_raw_metadata m = *in;
+    if (!m.width || !m.height || m.pattern > 3 || m.cfa_phase_x > 1 || m.cfa_phase_y > 1)
+        fail(RAWOPS_INVALID_ARGUMENT, "invalid RAW dimensions, Bayer pattern, or CFA phase");
+    if (m.row_stride_samples == 0) m.row_stride_samples = m.width;
+    if (m.row_stride_samples < m.width) fail(RAWOPS_INVALID_ARGUMENT, "RAW row stride is smaller than width");
+    const rawops_rect a0 = m.active_area;
+    if (!a0.x && !a0.y && !a0.width && !a0.height) m.active_area = {0, 0, m.width, m.height};
+    const rawops_rect a = m.active_area;
+    if (!a.width || !a.height || uint64_t(a.x) + a.width > m.width || uint64_t(a.y) + a.height > m.height)
+        fail(RAWOPS_INVALID_ARGUMENT, "RAW active area is outside the sensor");
+    for (int s = 0; s < 4; ++s)
+        if (m.white_levels[s] <= m.black_levels[s])
+            fail(RAWOPS_INVALID_ARGUMENT, "RAW white level must exceed black level at every site");
+    if (count != size_t(m.row_stride_samples) * m.height)
+        fail(RAWOPS_INVALID_ARGUMENT, "Bayer sample count does not match stride and height");
+    return m;
+}
+
+SensorArgs sensor_args(const rawops_source& src) {
+    static const int cells[4][4] = {{0, 1, 1, 2}, {2, 1, 1, 0}, {1, 0, 2, 1}, {1, 2, 0, 1}};
+    const auto& m = src.meta;
+    SensorArgs s{};
+    s.samples = src.samples;
+    s.stride = m.row_stride_samples;
+    s.ax = int(m.active_area.x); s.ay = int(m.active_area.y);
+    s.aw = int(m.active_area.width); s.ah = int(m.active_area.height);
+    s.phase_x = m.cfa_phase_x; s.phase_y = m.cfa_phase_y;
+    for (uint32_t site = 0; site < 4; ++site) s.colors |= uint32_t(cells[m.pattern][site]) << (2 * site);
+    for (uint32_t b = 0; b < 2; ++b) {
+        const uint32_t row_site = ((b + m.cfa_phase_y) & 1u) * 2u;
+        const uint32_t site0 = row_site + ((0u + m.cfa_phase_x) & 1u);
+        const bool even_is_green = cells[m.pattern][site0] == 1;
+        s.ng_parity |= (even_is_green ? 1u : 0u) << b;
+    }
+    for (int site = 0; site < 4; ++site) {
+        s.black[site] = float(m.black_levels[site]);
+        s.denom[site] = float(int(m.white_levels[site]) - int(m.black_levels[site]));
+    }
+    return s;
+}
+
+// validate_curve, ToneOps.cpp:17-32, and the ExtendedCurvesNode ctor tables
+// (ToneOps.cpp:795-816).
+void prepare_curve(DevCurve& c, const double* xs, const double* ys, uint32_t n, bool cubic) {
+    if (n < 2 || n > uint32_t(kMaxKnots))
+        fail(RAWOPS_INVALID_ARGUMENT, "curve knot count exceeds selected operation bounds");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!std::isfinite(xs[i]) || !std::isfinite(ys[i]) || std::abs(xs[i]) > 65536 || std::abs(ys[i]) > 65536)
+            fail(RAWOPS_INVALID_ARGUMENT, "curve coordinates must be finite with magnitude <=65536");
+        if (i) {
+            if (!(xs[i] > xs[i - 1])) fail(RAWOPS_INVALID_ARGUMENT, "curve x coordinates must increase strictly");
+            const double slope = (ys[i] - ys[i - 1]) / (xs[i] - xs[i - 1]);
+            if (!std::isfinite(slope) || std::abs(slope) > 65536)
+                fail(RAWOPS_INVALID_ARGUMENT, "curve slope must be finite with magnitude <=65536");
+        }
+    }
+    std::memset(&c, 0, sizeof(c));
+    c.n = n;
+    c.identity = 1;
+    for (uint32_t i = 0; i < n; ++i) {
+        c.x[i] = xs[i];
+        c.y[i] = ys[i];
+        if (xs[i] != ys[i]) c.identity = 0;
+    }
+    for (uint32_t i = 1; i < n; ++i) c.slope[i - 1] = (ys[i] - ys[i - 1]) / (xs[i] - xs[i - 1]);
+    if (!cubic) return;
+    std::vector<double> tangent(n);
+    tangent[0] = c.slope[0];
+    for (uint32_t i = 1; i + 1 < n; ++i) {
+        const double a = c.slope[i - 1], b = c.slope[i];
+        tangent[i] = a == 0 || b == 0 || std::signbit(a) != std::signbit(b)
+                         ? 0.0 : std::copysign(std::min(std::abs(a), std::abs(b)), a);
+    }
+    tangent[n - 1] = c.slope[n - 2];
+    for (uint32_t i = 0; i + 1 < n; ++i) {
+        const double h = xs[i + 1] - xs[i];
+        c.c0[i] = ys[i] + (h * tangent[i]) / 3;
+        c.c1[i] = ys[i + 1] - (h * tangent[i + 1]) / 3;
+    }
+}
+
+void compile_ops(rawops_plan& plan, const rawops_op_desc* ops, uint32_t n, OpList& out) {
+    if (n > uint32_t(kMaxOps)) fail(RAWOPS_INVALID_ARGUMENT, "op program is too long");
+    if (n && !ops) fail(RAWOPS_INVALID_ARGUMENT, "op program is null");
+    out.n = n;
+    for (uint32_t i = 0; i < n; ++i) {
+        const rawops_op_desc& d = ops[i];
+        DevOp& o = out.op[i];
+        o.kind = d.kind;
+        std::memcpy(o.v, d.v, sizeof(o.v));
+        o.curves = nullptr;
+        switch (d.kind) {
+        case OP_GAINS:
+            for (int c = 0; c < 3; ++c)
+                if (!std::isfinite(d.v[c]) || d.v[c] <= 0.0f || d.v[c] > 65536.0f)
+                    fail(RAWOPS_INVALID_ARGUMENT, "white balance gains must be finite and in (0, 65536]");
+            break;
+        case OP_SCALE:
+        case OP_MATRIX:
+        case OP_CLIP:
+        case OP_SRGB_ENCODE:
+            break;
+        case OP_TONE:
+            if (!std::isfinite(d.v[0]) || d.v[0] <= 0.0f || !std::isfinite(d.v[1]) || d.v[1] <= 0.0f)
+                fail(RAWOPS_INVALID_ARGUMENT, "tone shoulder and gamma must be finite and positive");
+            break;
+        case OP_CURVES: {
+            if (!d.knots_x || !d.knots_y) fail(RAWOPS_INVALID_ARGUMENT, "curve knots are required");
+            auto host = std::make_unique<DevCurves>();
+            host->cubic = d.cubic ? 1u : 0u;
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t b = d.knot_offsets[c], e = d.knot_offsets[c + 1];
+                if (e < b) fail(RAWOPS_INVALID_ARGUMENT, "curve knot offsets must be nondecreasing");
+                prepare_curve(host->c[c], d.knots_x + b, d.knots_y + b, e - b, d.cubic != 0);
+            }
+            void* dev = nullptr;
+            hip_check(hipMalloc(&dev, sizeof(DevCurves)), "hipMalloc(curves)");
+            plan.tables.push_back(dev);
+            hip_check(hipMemcpy(dev, host.get(), sizeof(DevCurves), hipMemcpyHostToDevice), "hipMemcpy(curves)");
+            o.curves = static_cast<const DevCurves*>(dev);
+            break;
+        }
+        default:
+            fail(RAWOPS_INVALID_ARGUMENT, "unknown op kind");
+        }
+    }
+}
+
+rawops_rect level_bounds(rawops_rect source, uint32_t mip, uint32_t quality) {
+    if (mip == 0 && (quality == RAWOPS_QUALITY_FINAL || quality == RAWOPS_QUALITY_PREVIEW)) return source;
+    if (mip >= 1 && mip <= 2 && quality == RAWOPS_QUALITY_PREVIEW) {
+        const uint32_t s = 1u << mip;
+        return {0, 0, source.width / s + (source.width % s != 0), source.height / s + (source.height % s != 0)};
+    }
+    fail(RAWOPS_INVALID_ARGUMENT, "render level is not implemented for these source bounds");
+}
+
+void inside_or_throw(rawops_rect area, rawops_rect r) {
+    if (r.x < area.x || r.y < area.y || uint64_t(r.x) + r.width > uint64_t(area.x) + area.width ||
+        uint64_t(r.y) + r.height > uint64_t(area.y) + area.height)
+        fail(RAWOPS_OUT_OF_RANGE, "viewport is outside the active RAW area");
+}
+
+void validate_render(const rawops_plan* plan, const rawops_source* src, rawops_rect viewport, uint32_t mip,
+                     uint32_t quality) {
+    if (!plan || !src) fail(RAWOPS_INVALID_ARGUMENT, "plan and source are required");
+    if (!src->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "render needs a RAW source");
+    inside_or_throw(level_bounds(src->meta.active_area, mip, quality), viewport);
+    if (mip == 0 && quality != RAWOPS_QUALITY_FINAL)
+        fail(RAWOPS_INVALID_ARGUMENT, "node does not support this render level");
+    if (mip > 0 && !plan->mip_anchor)
+        fail(RAWOPS_INVALID_ARGUMENT, "node does not support this render level");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rawops_last_error(void) { return g_error.c_str(); }
+const char* rawops_version(void) { return "rawops-mi355x 0.1 (gfx950)"; }
+
+rawops_status rawops_camera_to_working_matrix(const double cam[9], uint32_t space, float out[9]) {
+    return guarded([&] {
+        if (!cam || !out) fail(RAWOPS_INVALID_ARGUMENT, "null matrix pointer");
+        Mat3 m;
+        for (int i = 0; i < 9; ++i) m[i] = cam[i];
+        const auto f = camera_to_working(m, space);
+        std::memcpy(out, f.data(), sizeof(float) * 9);
+    });
+}
+
+rawops_status rawops_working_to_srgb_matrix(uint32_t space, float out[9]) {
+    return guarded([&] {
+        const auto f = working_to_srgb(space);
+        std::memcpy(out, f.data(), sizeof(float) * 9);
+    });
+}
+
+rawops_status rawops_working_to_working_matrix(uint32_t source, uint32_t target, float out[9]) {
+    return guarded([&] {
+        const auto f = working_to_working(source, target);
+        std::memcpy(out, f.data(), sizeof(float) * 9);
+    });
+}
+
+rawops_status rawops_xyz_d50_to_working_matrix(uint32_t space, double out[9]) {
+    return guarded([&] {
+        if (!out) fail(RAWOPS_INVALID_ARGUMENT, "null matrix pointer");
+        const auto m = xyz_d50_to_working(space);
+        for (int i = 0; i < 9; ++i) out[i] = m[i];
+    });
+}
+
+rawops_status rawops_exposure_multiplier(float stops, float* out) {
+    return guarded([&] {
+        if (!std::isfinite(stops) || stops < -32.0f || stops > 32.0f)
+            fail(RAWOPS_INVALID_ARGUMENT, "exposure stops must be finite and in [-32, 32]");
+        *out = std::exp2(stops);
+    });
+}
+
+rawops_status rawops_source_create_raw(const rawops_raw_metadata* meta, const uint16_t* host, size_t count,
+                                       rawops_source** out) {
+    return guarded([&] {
+        if (!out) fail(RAWOPS_INVALID_ARGUMENT, "output handle is null");
+        auto src = std::make_unique<rawops_source>();
+        src->meta = normalise(meta, count);
+        if (!host && count) fail(RAWOPS_INVALID_ARGUMENT, "samples are required");
+        void* dev = nullptr;
+        hip_check(hipMalloc(&dev, count * sizeof(uint16_t)), "hipMalloc(samples)");
+        src->owned = dev;
+        hip_check(hipMemcpy(dev, host, count * sizeof(uint16_t), hipMemcpyHostToDevice), "hipMemcpy(samples)");
+        src->samples = static_cast<const uint16_t*>(dev);
+        *out = src.release();
+    });
+}
+
+rawops_status rawops_source_wrap_raw_device(const rawops_raw_metadata* meta, const uint16_t* dev, size_t count,
+                                            rawops_source** out) {
+    return guarded([&] {
+        if (!out) fail(RAWOPS_INVALID_ARGUMENT, "output handle is null");
+        auto src = std::make_unique<rawops_source>();
+        src->meta = normalise(meta, count);
+        if (!dev) fail(RAWOPS_INVALID_ARGUMENT, "samples are required");
+        src->samples = dev;
+        *out = src.release();
+    });
+}
+
+static rawops_status raster_common(uint32_t w, uint32_t h, uint32_t stride, uint32_t space, const float* px,
+                                   bool device, rawops_source** out) {
+    return guarded([&] {
+        if (!out) fail(RAWOPS_INVALID_ARGUMENT, "output handle is null");
+        if (!w || !h || space > 1) fail(RAWOPS_INVALID_ARGUMENT, "invalid scene-linear raster metadata");
+        if (!stride) stride = w;
+        if (stride < w) fail(RAWOPS_INVALID_ARGUMENT, "raster float count does not match stride and height");
+        if (!px) fail(RAWOPS_INVALID_ARGUMENT, "raster pixels are required");
+        auto src = std::make_unique<rawops_source>();
+        src->is_raw = false;
+        src->width = w; src->height = h; src->stride = stride; src->space = space;
+        if (device) {
+            src->pixels = px;
+        } else {
+            const size_t n = size_t(stride) * h * 3;
+            for (size_t i = 0; i < n; ++i)
+                if (!std::isfinite(px[i])) fail(RAWOPS_INVALID_ARGUMENT, "scene-linear raster pixels must be finite");
+            void* dev = nullptr;
+            hip_check(hipMalloc(&dev, n * sizeof(float)), "hipMalloc(raster)");
+            src->owned = dev;
+            hip_check(hipMemcpy(dev, px, n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy(raster)");
+            src->pixels = static_cast<const float*>(dev);
+        }
+        *out = src.release();
+    });
+}
+
+rawops_status rawops_source_create_raster(uint32_t w, uint32_t h, uint32_t stride, uint32_t space,
+                                          const float* host, rawops_source** out) {
+    return raster_common(w, h, stride, space, host, false, out);
+}
+
+rawops_status rawops_source_wrap_raster_device(uint32_t w, uint32_t h, uint32_t stride, uint32_t space,
+                                               const float* dev, rawops_source** out) {
+    return raster_common(w, h, stride, space, dev, true, out);
+}
+
+rawops_status rawops_source_raw_metadata(const rawops_source* src, rawops_raw_metadata* out) {
+    return guarded([&] {
+        if (!src || !out || !src->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "RAW source required");
+        *out = src->meta;
+    });
+}
+
+void rawops_source_destroy(rawops_source* src) {
+    if (!src) return;
+    if (src->owned) (void)hipFree(src->owned);
+    delete src;
+}
+
+rawops_status rawops_plan_create(const rawops_plan_desc* desc, rawops_plan** out) {
+    return guarded([&] {
+        if (!desc || !out) fail(RAWOPS_INVALID_ARGUMENT, "plan descriptor is required");
+        if (desc->demosaic > RAWOPS_DEMOSAIC_MENON) fail(RAWOPS_INVALID_ARGUMENT, "unsupported RAW demosaic algorithm");
+        auto plan = std::make_unique<rawops_plan>();
+        plan->demosaic = desc->demosaic;
+        plan->mip_anchor = desc->mip_anchor ? 1u : 0u;
+        compile_ops(*plan, desc->native_ops, desc->n_native, plan->native);
+        compile_ops(*plan, desc->reduced_ops, desc->n_reduced, plan->reduced);
+        const OpList lists[3] = {plan->native, plan->reduced, OpList{}};
+        hip_check(hipMalloc(reinterpret_cast<void**>(&plan->device), sizeof(lists)), "hipMalloc(plan)");
+        hip_check(hipMemcpy(plan->device, lists, sizeof(lists), hipMemcpyHostToDevice), "hipMemcpy(plan)");
+        *out = plan.release();
+    });
+}
+
+void rawops_plan_destroy(rawops_plan* plan) { delete plan; }
+
+rawops_status rawops_request_bounds(rawops_rect source, uint32_t mip, uint32_t quality, rawops_rect* out) {
+    return guarded([&] { *out = level_bounds(source, mip, quality); });
+}
+
+rawops_status rawops_render(const rawops_plan* plan, const rawops_source* src, rawops_rect viewport,
+                            uint32_t mip, uint32_t quality, float* dst, uint8_t* dst8, uint32_t dst_stride,
+                            uint32_t* flags, void* stream) {
+    return guarded([&] {
+        validate_render(plan, src, viewport, mip, quality);
+        if (!dst_stride) dst_stride = viewport.width;
+        if (dst_stride < viewport.width) fail(RAWOPS_INVALID_ARGUMENT, "output stride is smaller than the viewport");
+        if (!viewport.width || !viewport.height) return;
+        RenderLaunch L{};
+        L.sensor = sensor_args(*src);
+        L.demosaic = plan->demosaic;
+        // A singleton active-area axis routes Menon to bilinear (RawEngine.cpp:303).
+        if (src->meta.active_area.width <= 1 || src->meta.active_area.height <= 1) L.demosaic = RAWOPS_DEMOSAIC_BILINEAR;
+        L.mip = mip;
+        L.viewport = viewport;
+        L.out = dst;
+        L.out8 = dst8;
+        L.ostride = dst_stride;
+        L.flags = flags;
+        hip_check(launch_render(L, plan->device, plan->device + 1, static_cast<hipStream_t>(stream)), "render launch");
+    });
+}
+
+rawops_status rawops_check_flags(uint32_t* flags, void* stream) {
+    return guarded([&] {
+        if (!flags) return;
+        uint32_t host = 0;
+        auto s = static_cast<hipStream_t>(stream);
+        hip_check(hipMemcpyAsync(&host, flags, sizeof(host), hipMemcpyDeviceToHost, s), "flag readback");
+        hip_check(hipStreamSynchronize(s), "stream synchronize");
+        if (host) hip_check(hipMemsetAsync(flags, 0, sizeof(uint32_t), s), "flag reset");
+        if (host & FLAG_OVERFLOW) fail(RAWOPS_OVERFLOW_ERROR, "guided filter nonfinite intermediate");
+        if (host & FLAG_INVALID) fail(RAWOPS_INVALID_ARGUMENT, "extended curves/levels output exceeds finite float32");
+        if (host & FLAG_DOMAIN) fail(RAWOPS_DOMAIN_ERROR, "non-finite value at output boundary");
+    });
+}
+
+rawops_status rawops_render_tiles_host(const rawops_plan* plan, const rawops_source* src, rawops_rect viewport,
+                                       uint32_t mip, uint32_t quality, uint32_t tile_size,
+                                       rawops_tile_callback callback, void* user, const volatile int* cancel) {
+    return guarded([&] {
+        validate_render(plan, src, viewport, mip, quality);
+        if (!callback || !tile_size) fail(RAWOPS_INVALID_ARGUMENT, "callback and tile size are required");
+        if (cancel && *cancel) fail(RAWOPS_CANCELLED, "render cancelled");
+        if (!viewport.width || !viewport.height) return;
+        const size_t n = size_t(viewport.width) * viewport.height * 3;
+        float* dev = nullptr;
+        uint32_t* flags = nullptr;
+        float* host = nullptr;
+        struct Cleanup {
+            float*& d; uint32_t*& f; float*& h;
+            ~Cleanup() { if (d) (void)hipFree(d); if (f) (void)hipFree(f); if (h) (void)hipHostFree(h); }
+        } cleanup{dev, flags, host};
+        hip_check(hipMalloc(&dev, n * sizeof(float)), "hipMalloc(viewport)");
+        hip_check(hipMalloc(&flags, sizeof(uint32_t)), "hipMalloc(flags)");
+        hip_check(hipHostMalloc(&host, n * sizeof(float), hipHostMallocDefault), "hipHostMalloc(viewport)");
+        hip_check(hipMemset(flags, 0, sizeof(uint32_t)), "hipMemset(flags)");
+        const rawops_status st = rawops_render(plan, src, viewport, mip, quality, dev, nullptr, viewport.width,
+                                               flags, nullptr);
+        if (st != RAWOPS_OK) fail(st, g_error);
+        hip_check(hipMemcpy(host, dev, n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy(viewport)");
+        const rawops_status fs = rawops_check_flags(flags, nullptr);
+        if (fs != RAWOPS_OK) fail(fs, g_error);
+        std::vector<float> tile;
+        const uint64_t right = uint64_t(viewport.x) + viewport.width, bottom = uint64_t(viewport.y) + viewport.height;
+        for (uint64_t y = viewport.y; y < bottom; y += tile_size)
+            for (uint64_t x = viewport.x; x < right; x += tile_size) {
+                if (cancel && *cancel) fail(RAWOPS_CANCELLED, "render cancelled");
+                const rawops_rect r{uint32_t(x), uint32_t(y), uint32_t(std::min<uint64_t>(tile_size, right - x)),
+                                    uint32_t(std::min<uint64_t>(tile_size, bottom - y))};
+                tile.resize(size_t(r.width) * r.height * 3);
+                for (uint32_t row = 0; row < r.height; ++row)
+                    std::memcpy(tile.data() + size_t(row) * r.width * 3,
+                                host + ((size_t(r.y - viewport.y) + row) * viewport.width + (r.x - viewport.x)) * 3,
+                                size_t(r.width) * 3 * sizeof(float));
+                if (callback(user, r, tile.data())) fail(RAWOPS_CANCELLED, "render cancelled by callback");
+            }
+    });
+}
+
+rawops_status rawops_guided_workspace_size(uint32_t width, uint32_t height, rawops_rect viewport, uint32_t mip,
+                                           uint32_t radius, size_t* bytes) {
+    return guarded([&] {
+        if (!bytes) fail(RAWOPS_INVALID_ARGUMENT, "output pointer is null");
+        GuidedLaunch L{};
+        L.width = width; L.height = height; L.viewport = viewport; L.mip = mip; L.radius = radius;
+        *bytes = radius ? guided_workspace_bytes(L) : 0;
+    });
+}
+
+rawops_status rawops_guided_filter(const rawops_source* raster, rawops_rect viewport, uint32_t mip,
+                                   uint32_t radius, double epsilon, const rawops_plan* post, float* dst,
+                                   uint32_t dst_stride, void* workspace, size_t workspace_bytes,
+                                   uint32_t* flags, void* stream) {
+    return guarded([&] {
+        if (!raster || raster->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires a scene-linear raster");
+        if (radius > 32 || !std::isfinite(epsilon) || epsilon < 0x1p-24 || epsilon > 65536.0)
+            fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires radius 0..32 and finite epsilon [2^-24,65536]");
+        const rawops_rect image = level_bounds({0, 0, raster->width, raster->height}, mip,
+                                               mip ? RAWOPS_QUALITY_PREVIEW : RAWOPS_QUALITY_FINAL);
+        if (viewport.x < image.x || viewport.y < image.y ||
+            uint64_t(viewport.x) + viewport.width > uint64_t(image.x) + image.width ||
+            uint64_t(viewport.y) + viewport.height > uint64_t(image.y) + image.height)
+            fail(RAWOPS_OUT_OF_RANGE, "spatial viewport is outside image bounds");
+        if (!dst_stride) dst_stride = viewport.width;
+        if (!viewport.width || !viewport.height) return;
+        GuidedLaunch L{};
+        L.src = raster->pixels;
+        L.src_stride = raster->stride;
+        L.width = raster->width;
+        L.height = raster->height;
+        L.mip = mip;
+        L.viewport = viewport;
+        L.radius = radius;
+        L.epsilon = epsilon;
+        const auto w = working_y_weights(raster->space);
+        L.wr = w[0];
+        L.wb = w[1];
+        L.out = dst;
+        L.ostride = dst_stride;
+        L.flags = flags;
+        if (!post) fail(RAWOPS_INVALID_ARGUMENT, "guided filter needs a post-filter plan (may be empty)");
+        auto st = static_cast<hipStream_t>(stream);
+        const size_t need = radius ? guided_workspace_bytes(L) : 0;
+        void* owned = nullptr;
+        if (need) {
+            if (workspace) {
+                if (workspace_bytes < need) fail(RAWOPS_INVALID_ARGUMENT, "guided filter workspace is too small");
+                L.workspace = workspace;
+            } else {
+                hip_check(hipMallocAsync(&owned, need, st), "hipMallocAsync(guided workspace)");
+                L.workspace = owned;
+            }
+        }
+        const hipError_t e = launch_guided(L, post->device + 1, st);
+        if (owned) (void)hipFreeAsync(owned, st);
+        hip_check(e, "guided launch");
+    });
+}
+
+}  // extern "C"

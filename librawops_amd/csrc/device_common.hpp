@@ -1,0 +1,181 @@
+// Device-side building blocks shared by the render, guided-filter and adjoint
+// kernels: sensor description, the fused per-pixel op program and its
+// interpreter.  Compiled for gfx950 only, with FP contraction disabled so every
+// float/double operation rounds exactly where the reference's does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rawops {
+
+constexpr int kMaxOps = 12;
+constexpr int kMaxKnots = 256;
+
+enum OpKind : uint32_t {
+    OP_GAINS = 1, OP_SCALE = 2, OP_MATRIX = 3, OP_TONE = 4,
+    OP_CLIP = 5, OP_SRGB_ENCODE = 6, OP_CURVES = 7
+};
+
+// Error bits OR-ed into the caller's device flag word.
+enum : uint32_t { FLAG_DOMAIN = 1u, FLAG_INVALID = 2u, FLAG_OVERFLOW = 4u };
+
+// One prepared curve (ExtendedCurvesNode ctor, ToneOps.cpp:795-816).
+struct DevCurve {
+    uint32_t n, identity;
+    double x[kMaxKnots], y[kMaxKnots];
+    double slope[kMaxKnots], c0[kMaxKnots], c1[kMaxKnots];
+};
+struct DevCurves {
+    uint32_t cubic;
+    DevCurve c[4];  // master, R, G, B
+};
+
+struct DevOp {
+    uint32_t kind;
+    float v[9];
+    const DevCurves* curves;
+};
+
+struct OpList {
+    uint32_t n;
+    DevOp op[kMaxOps];
+};
+
+// Bayer sensor as the kernels see it (RawMetadata after normalisation).
+struct SensorArgs {
+    const uint16_t* samples;
+    uint32_t stride;          // samples per row
+    int ax, ay, aw, ah;       // active area, sensor coordinates
+    uint32_t colors;          // colour (0 R, 1 G, 2 B) of site s in bits 2s..2s+1
+    uint32_t phase_x, phase_y;
+    uint32_t ng_parity;       // bit b: sensor x parity of the non-green sites in rows of parity b
+    float black[4], denom[4]; // per site: black level, white - black
+};
+
+__device__ __forceinline__ uint32_t site_of(const SensorArgs& s, int x, int y) {
+    return ((((uint32_t)y & 1u) + s.phase_y) & 1u) * 2u + ((((uint32_t)x & 1u) + s.phase_x) & 1u);
+}
+
+__device__ __forceinline__ int color_of(const SensorArgs& s, int x, int y) {
+    return (int)((s.colors >> (2u * site_of(s, x, y))) & 3u);
+}
+
+__device__ __forceinline__ bool in_area(const SensorArgs& s, int x, int y) {
+    return x >= s.ax && y >= s.ay && x < s.ax + s.aw && y < s.ay + s.ah;
+}
+
+// (float(code) - black[site]) / (white - black), one IEEE division
+// (RawEngine.cpp:89-97).
+__device__ __forceinline__ float normalize_code(const SensorArgs& s, uint32_t code, uint32_t site) {
+    // Selects on the site bits (no indexed access, so the sensor copy stays in registers).
+    const bool hi = (site & 2u) != 0, odd = (site & 1u) != 0;
+    const float black = hi ? (odd ? s.black[3] : s.black[2]) : (odd ? s.black[1] : s.black[0]);
+    const float denom = hi ? (odd ? s.denom[3] : s.denom[2]) : (odd ? s.denom[1] : s.denom[0]);
+    return ((float)code - black) / denom;
+}
+
+// Whole-sample reflection into [origin, origin + size), size >= 2, for
+// coordinates a few samples outside (MenonDemosaic.cpp:60-66; repeated
+// reflection equals its periodic fold).
+__device__ __forceinline__ int reflect(int v, int origin, int size) {
+    const int last = origin + size - 1;
+    while (v < origin || v > last) v = v < origin ? 2 * origin - v : 2 * last - v;
+    return v;
+}
+
+__device__ __forceinline__ float clamp01(float v) {
+    // std::clamp(v, 0.0f, 1.0f): keeps -0.0f, as the reference does.
+    return v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v);
+}
+
+__device__ __forceinline__ bool finite(float v) { return __builtin_isfinite(v); }
+
+__device__ __forceinline__ float tone(float linear, float shoulder, float inverse_gamma) {
+    const float magnitude = fabsf(linear);
+    const float compressed = magnitude / (magnitude + shoulder);
+    const float mapped = inverse_gamma == 1.0f ? compressed : powf(compressed, inverse_gamma);
+    return copysignf(mapped, linear);
+}
+
+__device__ __forceinline__ float srgb_encode(float v) {
+    const float linear = clamp01(v);
+    return linear <= 0.0031308f ? 12.92f * linear : 1.055f * powf(linear, 1.0f / 2.4f) - 0.055f;
+}
+
+// curve_cubic_blend, ToneOps.cpp:788-792.
+__device__ __forceinline__ double cubic_blend(double a, double b, double t) {
+    if (a == b || t == 0) return a;
+    if (t == 1) return b;
+    return t <= .5 ? a + t * (b - a) : b + (1 - t) * (a - b);
+}
+
+// ExtendedCurvesNode::map, ToneOps.cpp:820-831.
+__device__ inline double curve_map(const DevCurve& c, uint32_t cubic, double v) {
+    if (c.identity) return v;
+    uint32_t lo = 0, hi = c.n;                 // upper_bound on x
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v < c.x[mid]) hi = mid; else lo = mid + 1;
+    }
+    const uint32_t left = lo == 0 ? 0u : lo - 1u;
+    const uint32_t i = left < c.n - 2u ? left : c.n - 2u;
+    if (v == c.x[left]) return c.y[left];
+    if (!cubic || v < c.x[0] || v > c.x[c.n - 1])
+        return c.slope[i] == 0 ? c.y[left] : c.y[left] + (v - c.x[left]) * c.slope[i];
+    const double t = (v - c.x[i]) / (c.x[i + 1] - c.x[i]);
+    const double a = cubic_blend(c.y[i], c.c0[i], t), b = cubic_blend(c.c0[i], c.c1[i], t),
+                 d = cubic_blend(c.c1[i], c.y[i + 1], t);
+    return cubic_blend(cubic_blend(a, b, t), cubic_blend(b, d, t), t);
+}
+
+__device__ inline float curves_channel(const DevCurves* cv, int channel, float value, uint32_t& err) {
+    if (cv->c[0].identity && cv->c[channel + 1].identity) return value;
+    if (!finite(value)) { err |= FLAG_INVALID; return value; }
+    const double master = curve_map(cv->c[0], cv->cubic, (double)value);
+    if (!__builtin_isfinite(master)) { err |= FLAG_INVALID; return value; }
+    const double mapped = curve_map(cv->c[channel + 1], cv->cubic, master);
+    if (!__builtin_isfinite(mapped) || fabs(mapped) > 3.4028234663852886e38) { err |= FLAG_INVALID; return value; }
+    return (float)mapped;
+}
+
+// The fused op program: each op rounds exactly like its reference node.
+__device__ __forceinline__ void apply_ops(const OpList& L, float& r, float& g, float& b, uint32_t& err) {
+    for (uint32_t i = 0; i < L.n; ++i) {
+        const DevOp& o = L.op[i];
+        switch (o.kind) {
+        case OP_GAINS: r *= o.v[0]; g *= o.v[1]; b *= o.v[2]; break;
+        case OP_SCALE: r *= o.v[0]; g *= o.v[0]; b *= o.v[0]; break;
+        case OP_MATRIX: {
+            const float R = o.v[0] * r + o.v[1] * g + o.v[2] * b;
+            const float G = o.v[3] * r + o.v[4] * g + o.v[5] * b;
+            const float B = o.v[6] * r + o.v[7] * g + o.v[8] * b;
+            r = R; g = G; b = B;
+            break;
+        }
+        case OP_TONE:
+            r = tone(r, o.v[0], o.v[1]); g = tone(g, o.v[0], o.v[1]); b = tone(b, o.v[0], o.v[1]);
+            break;
+        case OP_CLIP:
+            if (!finite(r) || !finite(g) || !finite(b)) err |= FLAG_DOMAIN;
+            r = clamp01(r); g = clamp01(g); b = clamp01(b);
+            break;
+        case OP_SRGB_ENCODE:
+            if (!finite(r) || !finite(g) || !finite(b)) err |= FLAG_DOMAIN;
+            r = srgb_encode(r); g = srgb_encode(g); b = srgb_encode(b);
+            break;
+        case OP_CURVES:
+            r = curves_channel(o.curves, 0, r, err);
+            g = curves_channel(o.curves, 1, g, err);
+            b = curves_channel(o.curves, 2, b, err);
+            break;
+        default: break;
+        }
+    }
+}
+
+__device__ __forceinline__ uint8_t quantize_u8(float v) {
+    return (uint8_t)(clamp01(v) * 255.0f + 0.5f);
+}
+
+}  // namespace rawops

@@ -1,0 +1,347 @@
+// Working-Y guided filter for gfx950 (SpatialOps.cpp:159-317), box-moment
+// formulation with true-edge actual counts.
+//
+// Algebra.  The reference's ordered centred form
+//   q_i = p_i + mean_{k in W_i} [ (p_k - p_i) + mp_k + a_k (Y_i - Y_k - mg_k) ]
+// with mp_k = mean_{j in W_k}(p_j - p_k), mg_k = mean(Y_j - Y_k) and
+// a_k = cov_k / (var_k + eps) equals  q_i = mean_{k in W_i}(a_k) Y_i +
+// mean_{k in W_i}(b_k),  b_k = mean_{W_k}(p) - a_k mean_{W_k}(Y).
+// Both box means are computed with sliding vertical sums and per-row
+// wavefront prefix scans (O(1) work per pixel for any radius), in FP64.  The
+// moments are accumulated on values shifted by a per-strip constant so that
+// E[Y^2] - E[Y]^2 does not cancel catastrophically.
+//
+// Two strip kernels: `coefficients` writes a_k, b_k (FP64) for every centre of
+// the viewport grown by r; `apply` box-averages them, forms q and runs the
+// fused post-filter op program (exposure, curves, ...).
+#include "device_common.hpp"
+#include "render_launch.hpp"
+
+#include <algorithm>
+
+namespace rawops {
+namespace {
+
+constexpr int kStripThreads = 256;
+constexpr int kBandRows = 64;
+
+struct GuidedArgs {
+    const float* src;
+    uint32_t src_stride;
+    int W, H;                // native raster size
+    int mip;
+    int LW, LH;              // level image size
+    int cx0, cy0, cw, ch;    // centre region (level coords) = viewport grown by r, clipped
+    int vx, vy, vw, vh;      // viewport (level coords)
+    int r;
+    double eps, wr, wb;
+    double* coef;            // [ch][cw][6]: a0 a1 a2 b0 b1 b2
+    float* out;
+    uint32_t ostride;
+    uint32_t* flags;
+    const OpList* __restrict__ post;
+};
+
+// Raster pixel at level coordinates; mip 1/2 average clipped native
+// footprints in FP64 (RasterSourceNode::render_level RawEngine.cpp:363-406).
+__device__ __forceinline__ void level_pixel(const GuidedArgs& g, int x, int y, float p[3]) {
+    if (g.mip == 0) {
+        const float* q = g.src + ((size_t)y * g.src_stride + (size_t)x) * 3;
+        p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+        return;
+    }
+    const int s = 1 << g.mip;
+    const int x0 = x * s, y0 = y * s;
+    const int x1 = min(x0 + s, g.W), y1 = min(y0 + s, g.H);
+    double sum[3] = {0, 0, 0};
+    for (int yy = y0; yy < y1; ++yy)
+        for (int xx = x0; xx < x1; ++xx) {
+            const float* q = g.src + ((size_t)yy * g.src_stride + (size_t)xx) * 3;
+            sum[0] += q[0]; sum[1] += q[1]; sum[2] += q[2];
+        }
+    const double count = (double)((x1 - x0) * (y1 - y0));
+    p[0] = (float)(sum[0] / count); p[1] = (float)(sum[1] / count); p[2] = (float)(sum[2] / count);
+}
+
+// Working Y of an RGB triple: (g + wr(r - g)) + wb(b - g) (SpatialOps.cpp:213-219).
+__device__ __forceinline__ double working_y(const GuidedArgs& g, double r, double gg, double b) {
+    return (gg + g.wr * (r - gg)) + g.wb * (b - gg);
+}
+
+// Inclusive prefix sum of N doubles per thread across the 256-thread strip:
+// wavefront shuffles, then wave totals through LDS.
+template <int N>
+__device__ void strip_scan(double (&v)[N], double* wave_totals /* [4][N] */) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+            const double o = __shfl_up(v[m], d, 64);
+            if (lane >= d) v[m] += o;
+        }
+    }
+    if (lane == 63)
+#pragma unroll
+        for (int m = 0; m < N; ++m) wave_totals[wave * N + m] = v[m];
+    __syncthreads();
+    for (int w = 0; w < wave; ++w)
+#pragma unroll
+        for (int m = 0; m < N; ++m) v[m] += wave_totals[w * N + m];
+}
+
+// Moments of one (shifted) pixel: Y, p0, p1, p2, YY, Yp0, Yp1, Yp2.
+__device__ __forceinline__ void moments(const GuidedArgs& g, const float p[3], const double shift[3],
+                                        double m[8], uint32_t& err) {
+    if (!finite(p[0]) || !finite(p[1]) || !finite(p[2])) err |= FLAG_INVALID;
+    const double d0 = (double)p[0] - shift[0], d1 = (double)p[1] - shift[1], d2 = (double)p[2] - shift[2];
+    const double y = working_y(g, d0, d1, d2);
+    m[0] = y; m[1] = d0; m[2] = d1; m[3] = d2;
+    m[4] = y * y; m[5] = y * d0; m[6] = y * d1; m[7] = y * d2;
+}
+
+__global__ void __launch_bounds__(kStripThreads) gf_coefficients(const GuidedArgs g) {
+    __shared__ double prefix[kStripThreads * 8];
+    __shared__ double totals[4 * 8];
+    const int span = kStripThreads - 2 * g.r;            // centres per strip
+    const int t = threadIdx.x;
+    const int strip_x = g.cx0 + (int)blockIdx.x * span;  // first centre of this strip
+    const int col = strip_x - g.r + t;                   // level column this thread sums
+    const bool col_ok = col >= 0 && col < g.LW;
+    const int k0 = g.cy0 + (int)blockIdx.y * kBandRows;
+    const int k1 = min(k0 + kBandRows, g.cy0 + g.ch);
+    uint32_t err = 0;
+
+    // Per-strip shift: the first centre pixel of the strip.
+    double shift[3];
+    {
+        float p[3];
+        level_pixel(g, min(strip_x, g.LW - 1), k0, p);
+        shift[0] = p[0]; shift[1] = p[1]; shift[2] = p[2];
+    }
+    const double shift_y = working_y(g, shift[0], shift[1], shift[2]);
+
+    double vs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (col_ok) {
+        const int y0 = max(0, k0 - g.r), y1 = min(g.LH, k0 + g.r + 1);
+        for (int y = y0; y < y1; ++y) {
+            float p[3];
+            double m[8];
+            level_pixel(g, col, y, p);
+            moments(g, p, shift, m, err);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) vs[i] += m[i];
+        }
+    }
+    for (int k = k0; k < k1; ++k) {
+        if (k > k0 && col_ok) {
+            const int enter = k + g.r, leave = k - g.r - 1;
+            if (enter < g.LH) {
+                float p[3];
+                double m[8];
+                level_pixel(g, col, enter, p);
+                moments(g, p, shift, m, err);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) vs[i] += m[i];
+            }
+            if (leave >= 0) {
+                float p[3];
+                double m[8];
+                level_pixel(g, col, leave, p);
+                moments(g, p, shift, m, err);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) vs[i] -= m[i];
+            }
+        }
+        double ps[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ps[i] = vs[i];
+        strip_scan<8>(ps, totals);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) prefix[i * kStripThreads + t] = ps[i];
+        __syncthreads();
+        const int kx = strip_x + t;
+        if (t < span && kx < g.cx0 + g.cw) {
+            const int lo = max(0, kx - g.r) - (strip_x - g.r);
+            const int hi = min(g.LW, kx + g.r + 1) - (strip_x - g.r);
+            const int rows = min(g.LH, k + g.r + 1) - max(0, k - g.r);
+            const double n = (double)((hi - lo) * rows);
+            double s[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                s[i] = prefix[i * kStripThreads + hi - 1] - (lo > 0 ? prefix[i * kStripThreads + lo - 1] : 0.0);
+            const double my = s[0] / n;
+            double var = s[4] / n - my * my;
+            if (var < 0) var = 0;
+            const double denom = var + g.eps;
+            double* c = g.coef + ((size_t)(k - g.cy0) * g.cw + (size_t)(kx - g.cx0)) * 6;
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const double mp = s[1 + ch] / n;
+                const double cov = s[5 + ch] / n - my * mp;
+                const double a = cov / denom;
+                // b in unshifted coordinates: mean(p) - a mean(Y).
+                const double b = (mp - a * my) + (shift[ch] - a * shift_y);
+                if (!__builtin_isfinite(a) || !__builtin_isfinite(b)) err |= FLAG_OVERFLOW;
+                c[ch] = a;
+                c[3 + ch] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (err && g.flags) atomicOr(g.flags, err);
+}
+
+__global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g) {
+    __shared__ double prefix[kStripThreads * 6];
+    __shared__ double totals[4 * 6];
+    const int span = kStripThreads - 2 * g.r;
+    const int t = threadIdx.x;
+    const int strip_x = g.vx + (int)blockIdx.x * span;   // first output column
+    const int col = strip_x - g.r + t;                   // centre column this thread sums
+    const bool col_ok = col >= g.cx0 && col < g.cx0 + g.cw;
+    const int i0 = g.vy + (int)blockIdx.y * kBandRows;
+    const int i1 = min(i0 + kBandRows, g.vy + g.vh);
+    uint32_t err = 0;
+
+    auto coef_row = [&](int k, double c[6]) {
+        const double* q = g.coef + ((size_t)(k - g.cy0) * g.cw + (size_t)(col - g.cx0)) * 6;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) c[m] = q[m];
+    };
+    double vs[6] = {0, 0, 0, 0, 0, 0};
+    if (col_ok) {
+        const int y0 = max(0, i0 - g.r), y1 = min(g.LH, i0 + g.r + 1);
+        for (int k = y0; k < y1; ++k) {
+            double c[6];
+            coef_row(k, c);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) vs[m] += c[m];
+        }
+    }
+    for (int i = i0; i < i1; ++i) {
+        if (i > i0 && col_ok) {
+            const int enter = i + g.r, leave = i - g.r - 1;
+            if (enter < g.LH) {
+                double c[6];
+                coef_row(enter, c);
+#pragma unroll
+                for (int m = 0; m < 6; ++m) vs[m] += c[m];
+            }
+            if (leave >= 0) {
+                double c[6];
+                coef_row(leave, c);
+#pragma unroll
+                for (int m = 0; m < 6; ++m) vs[m] -= c[m];
+            }
+        }
+        double ps[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) ps[m] = vs[m];
+        strip_scan<6>(ps, totals);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) prefix[m * kStripThreads + t] = ps[m];
+        __syncthreads();
+        const int x = strip_x + t;
+        if (t < span && x < g.vx + g.vw) {
+            const int lo = max(0, x - g.r) - (strip_x - g.r);
+            const int hi = min(g.LW, x + g.r + 1) - (strip_x - g.r);
+            const int rows = min(g.LH, i + g.r + 1) - max(0, i - g.r);
+            const double n = (double)((hi - lo) * rows);
+            float p[3];
+            level_pixel(g, x, i, p);
+            const double y = working_y(g, (double)p[0], (double)p[1], (double)p[2]);
+            float o[3];
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const double sa = prefix[ch * kStripThreads + hi - 1] - (lo > 0 ? prefix[ch * kStripThreads + lo - 1] : 0.0);
+                const double sb = prefix[(3 + ch) * kStripThreads + hi - 1] -
+                                  (lo > 0 ? prefix[(3 + ch) * kStripThreads + lo - 1] : 0.0);
+                const double q = (sa / n) * y + sb / n;
+                const double delta = q - (double)p[ch];
+                if (!__builtin_isfinite(delta)) err |= FLAG_OVERFLOW;
+                if (delta == 0) { o[ch] = p[ch]; continue; }   // exact copy, as the reference
+                const double value = (double)p[ch] + delta;
+                if (value < -3.4028234663852886e38 || value > 3.4028234663852886e38) err |= FLAG_OVERFLOW;
+                o[ch] = (float)value;
+            }
+            apply_ops(*g.post, o[0], o[1], o[2], err);
+            float* d = g.out + ((size_t)(i - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
+            d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
+        }
+        __syncthreads();
+    }
+    if (err && g.flags) atomicOr(g.flags, err);
+}
+
+// Radius 0: the filter is the identity (SpatialOps.cpp:180); only the
+// post-filter program runs.
+__global__ void gf_identity(const GuidedArgs g) {
+    const size_t n = (size_t)g.vw * g.vh;
+    uint32_t err = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / g.vw), x = (int)(i - (size_t)y * g.vw);
+        float p[3];
+        level_pixel(g, g.vx + x, g.vy + y, p);
+        if (!finite(p[0]) || !finite(p[1]) || !finite(p[2])) err |= FLAG_INVALID;
+        apply_ops(*g.post, p[0], p[1], p[2], err);
+        float* d = g.out + ((size_t)y * g.ostride + (size_t)x) * 3;
+        d[0] = p[0]; d[1] = p[1]; d[2] = p[2];
+    }
+    if (err && g.flags) atomicOr(g.flags, err);
+}
+
+}  // namespace
+
+size_t guided_workspace_bytes(const GuidedLaunch& L) {
+    const int s = 1 << L.mip;
+    const int LW = ((int)L.width + s - 1) / s, LH = ((int)L.height + s - 1) / s;
+    const int r = (int)L.radius;
+    const int cx0 = std::max(0, (int)L.viewport.x - r), cy0 = std::max(0, (int)L.viewport.y - r);
+    const int cx1 = std::min(LW, (int)(L.viewport.x + L.viewport.width) + r);
+    const int cy1 = std::min(LH, (int)(L.viewport.y + L.viewport.height) + r);
+    return (size_t)(cx1 - cx0) * (size_t)(cy1 - cy0) * 6 * sizeof(double);
+}
+
+hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t stream) {
+    GuidedArgs g{};
+    g.src = L.src;
+    g.src_stride = L.src_stride;
+    g.W = (int)L.width;
+    g.H = (int)L.height;
+    g.mip = (int)L.mip;
+    const int s = 1 << g.mip;
+    g.LW = (g.W + s - 1) / s;
+    g.LH = (g.H + s - 1) / s;
+    g.r = (int)L.radius;
+    g.eps = L.epsilon;
+    g.wr = L.wr;
+    g.wb = L.wb;
+    g.vx = (int)L.viewport.x; g.vy = (int)L.viewport.y;
+    g.vw = (int)L.viewport.width; g.vh = (int)L.viewport.height;
+    g.cx0 = std::max(0, g.vx - g.r); g.cy0 = std::max(0, g.vy - g.r);
+    g.cw = std::min(g.LW, g.vx + g.vw + g.r) - g.cx0;
+    g.ch = std::min(g.LH, g.vy + g.vh + g.r) - g.cy0;
+    g.out = L.out;
+    g.ostride = L.ostride;
+    g.flags = L.flags;
+    g.post = post;
+    if (!g.vw || !g.vh) return hipSuccess;
+    if (g.r == 0) {
+        const size_t n = (size_t)g.vw * g.vh;
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
+        hipLaunchKernelGGL(gf_identity, dim3(blocks), dim3(256), 0, stream, g);
+        return hipGetLastError();
+    }
+    g.coef = static_cast<double*>(L.workspace);
+    const int span = kStripThreads - 2 * g.r;
+    dim3 grid_c((unsigned)((g.cw + span - 1) / span), (unsigned)((g.ch + kBandRows - 1) / kBandRows));
+    hipLaunchKernelGGL(gf_coefficients, grid_c, dim3(kStripThreads), 0, stream, g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 grid_a((unsigned)((g.vw + span - 1) / span), (unsigned)((g.vh + kBandRows - 1) / kBandRows));
+    hipLaunchKernelGGL(gf_apply, grid_a, dim3(kStripThreads), 0, stream, g);
+    return hipGetLastError();
+}
+
+}  // namespace rawops

@@ -1,0 +1,114 @@
+"""ctypes wrapper over the compiled reference harness (oracle/_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY.  ``oracle/_ref/libref.so`` is compiled from the
+reference's own sources by ``make -C oracle ref`` wherever /root/reference is
+present; where it is absent (e.g. a machine that only has this repository) the
+tests that need it are skipped and the oracle restatement stands alone.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .oracle import Bayer, Chain, Rect, request_bounds
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ERRORS = {1: ValueError, 2: IndexError, 3: ArithmeticError, 4: MemoryError,
+          5: OverflowError, 6: RuntimeError, 7: RuntimeError}
+
+
+def available():
+    return os.path.exists(os.path.join(_HERE, "_ref", "libref.so"))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        L = ctypes.CDLL(os.path.join(_HERE, "_ref", "libref.so"))
+        P = ctypes.c_void_p
+        L.refh_last_error.restype = ctypes.c_char_p
+        L.refh_render_raw.argtypes = [P, P, ctypes.c_size_t, ctypes.c_int, P, ctypes.c_float,
+                                      ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                      ctypes.c_float, Rect, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, P]
+        L.refh_guided_chain.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                        ctypes.c_uint32, ctypes.c_double, ctypes.c_float,
+                                        P, P, P, ctypes.c_int, Rect, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_uint32, P]
+        L.refh_matrices.argtypes = [P, ctypes.c_int, P, P, P]
+        L.refh_exposure_multiplier.argtypes = [ctypes.c_float]
+        L.refh_exposure_multiplier.restype = ctypes.c_float
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _check(code):
+    if code:
+        raise ERRORS.get(code, RuntimeError)(lib().refh_last_error().decode())
+
+
+def render_raw(meta: Bayer, samples, chain: Chain, viewport, mip=0, quality=None, tile=256):
+    L = lib()
+    s = np.ascontiguousarray(samples, np.uint16)
+    out = np.zeros((viewport[3], viewport[2], 3), np.float32)
+    wb = np.asarray(chain.wb, np.float32)
+    cam = np.asarray(chain.camera_to_xyz if chain.camera_to_xyz is not None else [0.0] * 9, np.float64)
+    mode = {"linear": 0, "bounded": 1, "srgb": 2}[chain.output]
+    if quality is None:
+        quality = 1 if mip == 0 else 0
+    _check(L.refh_render_raw(ctypes.byref(meta.struct()), _p(s), s.size,
+                             int(chain.demosaic == "menon"), _p(wb), float(chain.exposure_stops),
+                             int(chain.camera_to_xyz is not None), _p(cam), chain.working_space,
+                             mode, float(chain.tone_shoulder), float(chain.tone_gamma),
+                             Rect(*viewport), mip, quality, tile, _p(out)))
+    return out
+
+
+def guided_chain(raster, space, radius, eps, viewport, stops=0.0, knots=None, cubic=True,
+                 mip=0, tile=256):
+    L = lib()
+    H, W = raster.shape[:2]
+    r = np.ascontiguousarray(raster, np.float32)
+    out = np.zeros((viewport[3], viewport[2], 3), np.float32)
+    xs = ys = off = None
+    if knots is not None:
+        xs_l, ys_l, off_l = [], [], [0]
+        for k in knots:
+            for px, py in k:
+                xs_l.append(px)
+                ys_l.append(py)
+            off_l.append(len(xs_l))
+        xs = np.asarray(xs_l, np.float64)
+        ys = np.asarray(ys_l, np.float64)
+        off = np.asarray(off_l, np.uint32)
+    quality = 1 if mip == 0 else 0
+    _check(L.refh_guided_chain(_p(r), W, H, space, radius, eps, float(stops),
+                               _p(xs), _p(ys), _p(off), int(cubic), Rect(*viewport), mip,
+                               quality, tile, _p(out)))
+    return out
+
+
+def matrices(camera_to_xyz, space):
+    L = lib()
+    cam = np.asarray(camera_to_xyz, np.float64)
+    a = np.zeros(9, np.float32)
+    b = np.zeros(9, np.float32)
+    c = np.zeros(9, np.float32)
+    _check(L.refh_matrices(_p(cam), space, _p(a), _p(b), _p(c)))
+    return a, b, c
+
+
+def exposure_multiplier(stops):
+    return np.float32(lib().refh_exposure_multiplier(float(np.float32(stops))))
+
+
+__all__ = ["available", "render_raw", "guided_chain", "matrices", "exposure_multiplier",
+           "Bayer", "Chain", "request_bounds"]

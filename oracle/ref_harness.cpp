@@ -1,0 +1,172 @@
+// Test-only harness over the compiled reference (tylermcm/librawops sources,
+// built from /root/reference by oracle/Makefile into oracle/_ref/).  It builds
+// the reference node chains for the render path and exposes them through a
+// small C interface so the tests can compare the oracle restatement (and the
+// HIP path) with the real reference.  Never part of the product.
+#include "RawEngine.hpp"
+#include "SpatialOps.hpp"
+#include "ToneOps.hpp"
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rawops_oracle.h"
+
+using namespace rawengine;
+
+namespace {
+thread_local std::string last_error;
+
+int fail(const std::exception& e, int code) {
+    last_error = e.what();
+    return code;
+}
+
+template <class F> int guarded(F body) {
+    try {
+        body();
+        return 0;
+    } catch (const std::invalid_argument& e) { return fail(e, 1); }
+    catch (const std::out_of_range& e) { return fail(e, 2); }
+    catch (const std::domain_error& e) { return fail(e, 3); }
+    catch (const std::length_error& e) { return fail(e, 4); }
+    catch (const std::overflow_error& e) { return fail(e, 5); }
+    catch (const RenderCancelled& e) { return fail(e, 6); }
+    catch (const std::exception& e) { return fail(e, 7); }
+}
+
+// Emits unit-vector pixels e_0, e_1, e_2 so a matrix node's output reveals
+// its float32 coefficients column by column.
+class UnitColumns final : public Node {
+public:
+    explicit UnitColumns(ImageDescriptor d) : d_(d) {}
+    Tile render(Rect b) const override {
+        Tile t{b, std::vector<float>(size_t(b.width) * b.height * 3), d_};
+        for (size_t i = 0; i < size_t(b.width) * b.height; ++i) t.rgb[i * 3 + (i % 3)] = 1.0f;
+        return t;
+    }
+    ImageDescriptor output_descriptor() const noexcept override { return d_; }
+private:
+    ImageDescriptor d_;
+};
+
+void columns_to_rows(const std::vector<float>& cols, float out[9]) {
+    for (int j = 0; j < 3; ++j)
+        for (int r = 0; r < 3; ++r) out[r * 3 + j] = cols[size_t(j) * 3 + r];
+}
+
+RawImage make_image(const ro_bayer* b, const uint16_t* samples, size_t count) {
+    RawMetadata m;
+    m.width = b->width; m.height = b->height; m.row_stride_samples = b->row_stride;
+    m.pattern = static_cast<BayerPattern>(b->pattern);
+    m.cfa_phase_x = uint8_t(b->phase_x); m.cfa_phase_y = uint8_t(b->phase_y);
+    m.active_area = {b->area.x, b->area.y, b->area.width, b->area.height};
+    for (int i = 0; i < 4; ++i) { m.black_levels[i] = b->black[i]; m.white_levels[i] = b->white[i]; }
+    return RawImage(m, std::vector<uint16_t>(samples, samples + count));
+}
+} // namespace
+
+extern "C" {
+
+const char* refh_last_error() { return last_error.c_str(); }
+
+// output_mode: 0 linear (ends at CameraToWorking, or Exposure without camera
+// colour), 1 LegacyBounded (tone + clip), 2 SrgbPreview (to-sRGB, tone, encode).
+int refh_render_raw(const ro_bayer* b, const uint16_t* samples, size_t count, int menon,
+                    const float wb[3], float stops, int has_camera, const double cam[9],
+                    int target, int output_mode, float shoulder, float gamma,
+                    ro_rect viewport, uint32_t mip, uint32_t quality, uint32_t tile,
+                    float* out) {
+    return guarded([&] {
+        const RawImage image = make_image(b, samples, count);
+        const Rect area = image.metadata().active_area;
+        RawDemosaicIdentity id;
+        if (menon) id = {"rawengine.menon_base", 1};
+        std::shared_ptr<const Node> n = std::make_shared<RawUnpackNode>(image, id);
+        n = std::make_shared<WhiteBalanceNode>(n, wb[0], wb[1], wb[2]);
+        n = std::make_shared<ExposureNode>(n, stops);
+        if (has_camera) {
+            CameraColorTransform t;
+            for (int i = 0; i < 9; ++i) t.camera_to_xyz_d50[i] = cam[i];
+            t.target = static_cast<WorkingSpace>(target);
+            n = std::make_shared<CameraToWorkingNode>(n, t, area);
+        }
+        if (output_mode == 1) {
+            n = std::make_shared<ToneCurveNode>(n, shoulder, gamma);
+            n = std::make_shared<OutputClipNode>(n);
+        } else if (output_mode == 2) {
+            n = std::make_shared<WorkingToSrgbNode>(n);
+            n = std::make_shared<ToneCurveNode>(n, shoulder, gamma);
+            n = std::make_shared<SrgbEncodeNode>(n);
+        }
+        RenderRequest req{Rect{viewport.x, viewport.y, viewport.width, viewport.height}, tile,
+                          RenderLevel{mip, static_cast<RenderQuality>(quality)}};
+        const Tile t = Renderer{}.render_image(*n, area, req);
+        std::memcpy(out, t.rgb.data(), t.rgb.size() * sizeof(float));
+    });
+}
+
+// Scene-linear raster -> guided filter -> exposure -> extended curves.
+int refh_guided_chain(const float* raster, uint32_t w, uint32_t h, int space,
+                      uint32_t radius, double epsilon, float stops,
+                      const double* xs, const double* ys, const uint32_t offsets[5], int cubic,
+                      ro_rect viewport, uint32_t mip, uint32_t quality, uint32_t tile, float* out) {
+    return guarded([&] {
+        RasterMetadata m{w, h, 0, static_cast<WorkingSpace>(space)};
+        const RasterImage image(m, std::vector<float>(raster, raster + size_t(w) * h * 3));
+        std::shared_ptr<const Node> n = std::make_shared<RasterSourceNode>(image);
+        n = std::make_shared<WorkingYGuidedFilterNode>(n, Rect{0, 0, w, h},
+                                                       WorkingYGuidedFilterSettings{radius, epsilon});
+        if (stops != 0.0f) n = std::make_shared<ExposureNode>(n, stops);
+        if (xs) {
+            ExtendedCurvesSettings s;
+            s.interpolation = cubic ? CurveInterpolation::ShapePreservingCubic : CurveInterpolation::Linear;
+            for (int c = 0; c < 4; ++c) {
+                std::vector<CurvePoint> k;
+                for (uint32_t i = offsets[c]; i < offsets[c + 1]; ++i) k.push_back({xs[i], ys[i]});
+                if (c == 0) s.master.knots = k; else s.channels[c - 1].knots = k;
+            }
+            n = std::make_shared<ExtendedCurvesNode>(n, s);
+        }
+        RenderRequest req{Rect{viewport.x, viewport.y, viewport.width, viewport.height}, tile,
+                          RenderLevel{mip, static_cast<RenderQuality>(quality)}};
+        const Tile t = Renderer{}.render_image(*n, Rect{0, 0, w, h}, req);
+        std::memcpy(out, t.rgb.data(), t.rgb.size() * sizeof(float));
+    });
+}
+
+// float32 coefficients the reference uses for camera->working (row-major),
+// working->linear sRGB, and working->other working space.
+int refh_matrices(const double cam[9], int target, float cam_to_working[9],
+                  float working_to_srgb[9], float working_to_other[9]) {
+    return guarded([&] {
+        CameraColorTransform t;
+        for (int i = 0; i < 9; ++i) t.camera_to_xyz_d50[i] = cam[i];
+        t.target = static_cast<WorkingSpace>(target);
+        auto cols = std::make_shared<UnitColumns>(ImageDescriptor::camera_linear());
+        columns_to_rows(CameraToWorkingNode(cols, t).render({0, 0, 3, 1}).rgb, cam_to_working);
+        const auto space = static_cast<WorkingSpace>(target);
+        auto scene = std::make_shared<UnitColumns>(ImageDescriptor::scene_linear(space));
+        columns_to_rows(WorkingToSrgbNode(scene).render({0, 0, 3, 1}).rgb, working_to_srgb);
+        const auto other = space == WorkingSpace::LinearProPhotoD50 ? WorkingSpace::LinearRec2020D65
+                                                                     : WorkingSpace::LinearProPhotoD50;
+        columns_to_rows(WorkingSpaceConvertNode(scene, other).render({0, 0, 3, 1}).rgb, working_to_other);
+    });
+}
+
+// Host-side exp2 the reference's ExposureNode uses for its multiplier.
+float refh_exposure_multiplier(float stops) {
+    const float one = 1.0f;
+    float v = one;
+    guarded([&] {
+        RasterMetadata m{1, 1, 0, WorkingSpace::LinearProPhotoD50};
+        auto src = std::make_shared<RasterSourceNode>(RasterImage(m, {1.0f, 1.0f, 1.0f}));
+        v = ExposureNode(src, stops).render({0, 0, 1, 1}).rgb[0];
+    });
+    return v;
+}
+
+} // extern "C"
