@@ -345,6 +345,10 @@ def request_bounds(area, mip):
 def render_raw(meta: Bayer, samples, chain: Chain, viewport, mip=0):
     """Whole-viewport render of ``chain`` (what Renderer::render_image returns)."""
     area = meta.area
+    b = request_bounds(area, mip)
+    if (viewport[0] < b[0] or viewport[1] < b[1] or viewport[0] + viewport[2] > b[0] + b[2]
+            or viewport[1] + viewport[3] > b[1] + b[3]):
+        raise IndexError("viewport is outside the active RAW area")   # RawEngine.cpp:22-29
 
     def native(rect):
         t = demosaic(meta, samples, rect, chain.demosaic)
