@@ -1,0 +1,786 @@
+// rawengine drop-in layer over the gfx950 C ABI.
+//
+// Node objects are immutable descriptions with the reference's validation and
+// level rules (RawEngine.cpp:223-758, 1215-1475; ToneOps.cpp:17-32, 795-844;
+// SpatialOps.cpp:159-317).  Rendering walks the chain from the requested node
+// to its source, compiles it into one device plan (demosaic + op program +
+// mip reduction, or guided filter + op program) and runs it with the source
+// resident in HBM.  A node type this library does not know is rendered with
+// its own render_level(); the built-in nodes above it then run on the GPU over
+// that uploaded tile.
+#include "../../../include/rawengine/RawEngine.hpp"
+#include "../../../include/rawengine/SpatialOps.hpp"
+#include "../../../include/rawengine/ToneOps.hpp"
+#include "../../../include/rawops_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace rawengine {
+namespace detail {
+
+struct DeviceSource {
+    rawops_source* handle = nullptr;
+    ~DeviceSource() { rawops_source_destroy(handle); }
+};
+
+[[noreturn]] void raise(rawops_status s) {
+    const std::string m = rawops_last_error();
+    switch (s) {
+    case RAWOPS_INVALID_ARGUMENT: throw std::invalid_argument(m);
+    case RAWOPS_OUT_OF_RANGE: throw std::out_of_range(m);
+    case RAWOPS_DOMAIN_ERROR: throw std::domain_error(m);
+    case RAWOPS_LENGTH_ERROR: throw std::length_error(m);
+    case RAWOPS_OVERFLOW_ERROR: throw std::overflow_error(m);
+    case RAWOPS_CANCELLED: throw RenderCancelled();
+    case RAWOPS_LOGIC_ERROR: throw std::logic_error(m);
+    default: throw std::runtime_error(m);
+    }
+}
+
+void check(rawops_status s) {
+    if (s != RAWOPS_OK) raise(s);
+}
+
+void hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct Plan {
+    rawops_plan* handle = nullptr;
+    ~Plan() { rawops_plan_destroy(handle); }
+};
+
+struct Op {
+    rawops_op_desc desc{};
+    std::vector<double> xs, ys;  // curve knots backing desc.knots_x/y
+};
+
+// The compiled form of a node chain.
+struct Chain {
+    enum Kind { Raw, Raster, Foreign } kind = Raw;
+    const RawImage* raw = nullptr;
+    const RasterImage* raster = nullptr;
+    const Node* foreign = nullptr;
+    uint32_t demosaic = RAWOPS_DEMOSAIC_BILINEAR;
+    bool anchor = false;
+    const WorkingYGuidedFilterNode* guided = nullptr;
+    std::vector<Op> native, reduced;
+    std::unique_ptr<Plan> plan;
+};
+
+rawops_rect c(Rect r) { return {r.x, r.y, r.width, r.height}; }
+
+Op matrix_op(const std::array<float, 9>& m) {
+    Op o;
+    o.desc.kind = RAWOPS_OP_MATRIX;
+    for (int i = 0; i < 9; ++i) o.desc.v[i] = m[i];
+    return o;
+}
+
+// Op contributed by one built-in point node; false if `n` is not one.
+bool point_ops(const Node* n, std::vector<Op>& out) {
+    if (auto* wb = dynamic_cast<const WhiteBalanceNode*>(n)) {
+        Op o;
+        o.desc.kind = RAWOPS_OP_GAINS;
+        for (int i = 0; i < 3; ++i) o.desc.v[i] = wb->gains()[i];
+        out.push_back(o);
+    } else if (auto* ex = dynamic_cast<const ExposureNode*>(n)) {
+        Op o;
+        o.desc.kind = RAWOPS_OP_SCALE;
+        o.desc.v[0] = ex->multiplier();
+        out.push_back(o);
+    } else if (auto* cw = dynamic_cast<const CameraToWorkingNode*>(n)) {
+        out.push_back(matrix_op(cw->matrix()));
+    } else if (auto* ws = dynamic_cast<const WorkingSpaceConvertNode*>(n)) {
+        if (!ws->identity()) out.push_back(matrix_op(ws->matrix()));
+    } else if (auto* ts = dynamic_cast<const WorkingToSrgbNode*>(n)) {
+        out.push_back(matrix_op(ts->matrix()));
+    } else if (auto* tc = dynamic_cast<const ToneCurveNode*>(n)) {
+        Op o;
+        o.desc.kind = RAWOPS_OP_TONE;
+        o.desc.v[0] = tc->shoulder();
+        o.desc.v[1] = tc->inverse_gamma();
+        out.push_back(o);
+    } else if (dynamic_cast<const OutputClipNode*>(n)) {
+        Op o;
+        o.desc.kind = RAWOPS_OP_CLIP;
+        out.push_back(o);
+    } else if (dynamic_cast<const SrgbEncodeNode*>(n)) {
+        Op o;
+        o.desc.kind = RAWOPS_OP_SRGB_ENCODE;
+        out.push_back(o);
+    } else if (auto* cv = dynamic_cast<const ExtendedCurvesNode*>(n)) {
+        Op o;
+        o.desc.kind = RAWOPS_OP_CURVES;
+        const auto& s = cv->settings();
+        o.desc.knot_offsets[0] = 0;
+        for (int k = 0; k < 4; ++k) {
+            const auto& knots = k == 0 ? s.master.knots : s.channels[k - 1].knots;
+            for (const auto& p : knots) { o.xs.push_back(p.x); o.ys.push_back(p.y); }
+            o.desc.knot_offsets[k + 1] = uint32_t(o.xs.size());
+        }
+        o.desc.cubic = s.interpolation == CurveInterpolation::ShapePreservingCubic ? 1u : 0u;
+        out.push_back(std::move(o));
+    } else {
+        return false;
+    }
+    return true;
+}
+
+std::unique_ptr<Plan> make_plan(uint32_t demosaic, bool anchor, std::vector<Op>& native, std::vector<Op>& reduced) {
+    std::vector<rawops_op_desc> a, b;
+    for (auto& o : native) {
+        if (o.desc.kind == RAWOPS_OP_CURVES) { o.desc.knots_x = o.xs.data(); o.desc.knots_y = o.ys.data(); }
+        a.push_back(o.desc);
+    }
+    for (auto& o : reduced) {
+        if (o.desc.kind == RAWOPS_OP_CURVES) { o.desc.knots_x = o.xs.data(); o.desc.knots_y = o.ys.data(); }
+        b.push_back(o.desc);
+    }
+    rawops_plan_desc d{};
+    d.demosaic = demosaic;
+    d.mip_anchor = anchor ? 1u : 0u;
+    d.n_native = uint32_t(a.size());
+    d.n_reduced = uint32_t(b.size());
+    d.native_ops = a.data();
+    d.reduced_ops = b.data();
+    auto p = std::make_unique<Plan>();
+    check(rawops_plan_create(&d, &p->handle));
+    return p;
+}
+
+std::unique_ptr<Chain> compile(const Node& output) {
+    std::vector<const Node*> nodes;
+    for (const Node* n = &output; n; n = n->input_node()) nodes.push_back(n);
+    std::reverse(nodes.begin(), nodes.end());
+    auto ch = std::make_unique<Chain>();
+    size_t first = 1;
+    if (auto* ru = dynamic_cast<const RawUnpackNode*>(nodes[0])) {
+        ch->kind = Chain::Raw;
+        ch->raw = &ru->image();
+        ch->demosaic = ru->raw_demosaic_identity()->algorithm == "rawengine.menon_base" ? RAWOPS_DEMOSAIC_MENON
+                                                                                         : RAWOPS_DEMOSAIC_BILINEAR;
+    } else if (auto* rs = dynamic_cast<const RasterSourceNode*>(nodes[0])) {
+        ch->kind = Chain::Raster;
+        ch->raster = &rs->image();
+    } else {
+        // Render the last foreign node with its own code; run the rest on the GPU.
+        size_t last = 0;
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            std::vector<Op> probe;
+            const bool known = point_ops(nodes[i], probe) || dynamic_cast<const DeviceNode*>(nodes[i]);
+            if (!known) last = i;
+        }
+        ch->kind = Chain::Foreign;
+        ch->foreign = nodes[last];
+        first = last + 1;
+    }
+    for (size_t i = first; i < nodes.size(); ++i) {
+        const Node* n = nodes[i];
+        if (auto* gf = dynamic_cast<const WorkingYGuidedFilterNode*>(n)) {
+            if (ch->kind != Chain::Raster || ch->guided || !ch->reduced.empty())
+                throw std::invalid_argument("guided filter is fused only directly over a raster source");
+            ch->guided = gf;
+            continue;
+        }
+        auto& list = ch->kind == Chain::Raw && !ch->anchor ? ch->native : ch->reduced;
+        if (!point_ops(n, list)) throw std::invalid_argument("node type cannot be compiled into a device plan");
+        if (auto* cw = dynamic_cast<const CameraToWorkingNode*>(n))
+            if (ch->kind == Chain::Raw && cw->native_input_bounds().width) ch->anchor = true;
+    }
+    ch->plan = make_plan(ch->demosaic, ch->anchor, ch->native, ch->reduced);
+    return ch;
+}
+
+// Device scratch for one render call.
+struct DeviceBuffer {
+    void* p = nullptr;
+    explicit DeviceBuffer(size_t bytes) { if (bytes) hip(hipMalloc(&p, bytes), "hipMalloc"); }
+    ~DeviceBuffer() { if (p) (void)hipFree(p); }
+};
+
+void run(const Chain& ch, Rect viewport, RenderLevel level, float* dst, uint32_t* flags, hipStream_t stream) {
+    const uint32_t quality = static_cast<uint32_t>(level.quality);
+    if (ch.kind == Chain::Raw) {
+        check(rawops_render(ch.plan->handle, ch.raw->device().handle, c(viewport), level.mip, quality, dst, nullptr,
+                            viewport.width, flags, stream));
+    } else if (ch.kind == Chain::Raster) {
+        const uint32_t r = ch.guided ? ch.guided->settings().radius : 0;
+        const double eps = ch.guided ? ch.guided->settings().epsilon : 1.0;
+        check(rawops_guided_filter(ch.raster->device().handle, c(viewport), level.mip, r, eps, ch.plan->handle, dst,
+                                   viewport.width, nullptr, 0, flags, stream));
+    } else {
+        // Foreign input: its own render, uploaded, then the device op program.
+        const Tile t = ch.foreign->render_level(viewport, ch.foreign->input_level(level));
+        if (t.rgb.size() != size_t(viewport.width) * viewport.height * 3)
+            throw std::domain_error("upstream node returned an invalid tile");
+        DeviceSource src;
+        check(rawops_source_create_raster(viewport.width, viewport.height, 0, 0, t.rgb.data(), &src.handle));
+        check(rawops_guided_filter(src.handle, {0, 0, viewport.width, viewport.height}, 0, 0, 1.0, ch.plan->handle,
+                                   dst, viewport.width, nullptr, 0, flags, stream));
+        hip(hipStreamSynchronize(stream), "stream synchronize");
+    }
+}
+
+Rect request_bounds(Rect source, RenderLevel level) {
+    rawops_rect out{};
+    check(rawops_request_bounds(c(source), level.mip, static_cast<uint32_t>(level.quality), &out));
+    return {out.x, out.y, out.width, out.height};
+}
+
+void validate_rect(Rect area, Rect r) {
+    if (r.x < area.x || r.y < area.y || uint64_t(r.x) + r.width > uint64_t(area.x) + area.width ||
+        uint64_t(r.y) + r.height > uint64_t(area.y) + area.height)
+        throw std::out_of_range("viewport is outside the active RAW area");
+}
+
+const Node* source_of(const Node& n) {
+    const Node* s = &n;
+    while (s->input_node()) s = s->input_node();
+    return s;
+}
+
+bool compilable(const Node& n) { return dynamic_cast<const DeviceNode*>(&n) != nullptr; }
+
+}  // namespace detail
+
+using detail::check;
+
+// ------------------------------------------------------------------ images --
+
+RawImage::RawImage(RawMetadata metadata, std::vector<std::uint16_t> samples)
+    : metadata_(std::move(metadata)),
+      bayer_(std::make_shared<const std::vector<std::uint16_t>>(std::move(samples))) {
+    auto& m = metadata_;
+    if (!m.width || !m.height || static_cast<int>(m.pattern) < 0 || static_cast<int>(m.pattern) > 3 ||
+        m.cfa_phase_x > 1 || m.cfa_phase_y > 1)
+        throw std::invalid_argument("invalid RAW dimensions, Bayer pattern, or CFA phase");
+    if (!m.row_stride_samples) m.row_stride_samples = m.width;
+    if (m.row_stride_samples < m.width) throw std::invalid_argument("RAW row stride is smaller than width");
+    if (!m.active_area.x && !m.active_area.y && !m.active_area.width && !m.active_area.height)
+        m.active_area = {0, 0, m.width, m.height};
+    const auto a = m.active_area;
+    if (!a.width || !a.height || uint64_t(a.x) + a.width > m.width || uint64_t(a.y) + a.height > m.height)
+        throw std::invalid_argument("RAW active area is outside the sensor");
+    for (int s = 0; s < 4; ++s)
+        if (m.white_levels[s] <= m.black_levels[s])
+            throw std::invalid_argument("RAW white level must exceed black level at every site");
+    if (bayer_->size() != size_t(m.row_stride_samples) * m.height)
+        throw std::invalid_argument("Bayer sample count does not match stride and height");
+}
+
+static RawMetadata uniform(std::uint32_t w, std::uint32_t h, BayerPattern p, std::uint16_t b, std::uint16_t wl) {
+    RawMetadata m;
+    m.width = w;
+    m.height = h;
+    m.pattern = p;
+    m.black_levels.fill(b);
+    m.white_levels.fill(wl);
+    return m;
+}
+
+RawImage::RawImage(std::uint32_t width, std::uint32_t height, std::vector<std::uint16_t> samples,
+                   BayerPattern pattern, std::uint16_t black, std::uint16_t white)
+    : RawImage(uniform(width, height, pattern, black, white), std::move(samples)) {}
+
+const detail::DeviceSource& RawImage::device() const {
+    std::call_once(*upload_once_, [&] {
+        auto src = std::make_shared<detail::DeviceSource>();
+        rawops_raw_metadata m{};
+        m.width = metadata_.width; m.height = metadata_.height; m.row_stride_samples = metadata_.row_stride_samples;
+        m.pattern = uint32_t(metadata_.pattern);
+        m.cfa_phase_x = metadata_.cfa_phase_x; m.cfa_phase_y = metadata_.cfa_phase_y;
+        m.active_area = detail::c(metadata_.active_area);
+        for (int i = 0; i < 4; ++i) { m.black_levels[i] = metadata_.black_levels[i]; m.white_levels[i] = metadata_.white_levels[i]; }
+        check(rawops_source_create_raw(&m, bayer_->data(), bayer_->size(), &src->handle));
+        *device_ = src;
+    });
+    return **device_;
+}
+
+RasterImage::RasterImage(RasterMetadata metadata, std::vector<float> pixels)
+    : metadata_(metadata), pixels_(std::make_shared<const std::vector<float>>(std::move(pixels))) {
+    if (!metadata_.width || !metadata_.height ||
+        (metadata_.working_space != WorkingSpace::LinearProPhotoD50 &&
+         metadata_.working_space != WorkingSpace::LinearRec2020D65))
+        throw std::invalid_argument("invalid scene-linear raster metadata");
+    if (!metadata_.row_stride_pixels) metadata_.row_stride_pixels = metadata_.width;
+    if (metadata_.row_stride_pixels < metadata_.width ||
+        pixels_->size() != size_t(metadata_.row_stride_pixels) * metadata_.height * 3)
+        throw std::invalid_argument("raster float count does not match stride and height");
+    if (!std::all_of(pixels_->begin(), pixels_->end(), [](float v) { return std::isfinite(v); }))
+        throw std::invalid_argument("scene-linear raster pixels must be finite");
+}
+
+const detail::DeviceSource& RasterImage::device() const {
+    std::call_once(*upload_once_, [&] {
+        auto src = std::make_shared<detail::DeviceSource>();
+        check(rawops_source_create_raster(metadata_.width, metadata_.height, metadata_.row_stride_pixels,
+                                          uint32_t(metadata_.working_space), pixels_->data(), &src->handle));
+        *device_ = src;
+    });
+    return **device_;
+}
+
+void validate_raw_demosaic(const RawDemosaicIdentity& identity) {
+    if ((identity.algorithm != "rawengine.bilinear" && identity.algorithm != "rawengine.menon_base") ||
+        identity.processing_version != 1)
+        throw std::invalid_argument("unsupported RAW demosaic algorithm or processing version");
+}
+
+// ------------------------------------------------------------------- nodes --
+
+Tile DeviceNode::render(Rect bounds) const { return render_level(bounds, {}); }
+
+Tile DeviceNode::render_level(Rect bounds, RenderLevel level) const {
+    if (!supports_level(level)) throw std::invalid_argument("node does not support this render level");
+    const Node* src = detail::source_of(*this);
+    const auto sb = src->source_bounds();
+    Rect source = sb ? *sb : bounds;
+    if (!sb && level.mip) throw std::invalid_argument("node has no source bounds for reduced levels");
+    const uint32_t tile = std::max<uint32_t>(1, std::max(bounds.width, bounds.height));
+    return Renderer{}.render_image(*this, source, RenderRequest{bounds, tile, level});
+}
+
+RawUnpackNode::RawUnpackNode(RawImage image) : RawUnpackNode(std::move(image), RawDemosaicIdentity{}) {}
+
+RawUnpackNode::RawUnpackNode(RawImage image, RawDemosaicIdentity demosaic)
+    : image_(std::move(image)), demosaic_(std::move(demosaic)) {
+    validate_raw_demosaic(demosaic_);
+}
+
+Rect RawUnpackNode::input_region(Rect output, Rect source_bounds) const {
+    const auto a = image_.metadata().active_area;
+    if (source_bounds.x != a.x || source_bounds.y != a.y || source_bounds.width != a.width ||
+        source_bounds.height != a.height)
+        throw std::invalid_argument("RAW source bounds differ from active area");
+    detail::validate_rect(a, output);
+    const uint32_t r = demosaic_.algorithm == "rawengine.menon_base" && a.width > 1 && a.height > 1 ? 6u : 1u;
+    const auto left = std::max<int64_t>(a.x, int64_t(output.x) - r);
+    const auto top = std::max<int64_t>(a.y, int64_t(output.y) - r);
+    const auto right = std::min<uint64_t>(uint64_t(a.x) + a.width, uint64_t(output.x) + output.width + r);
+    const auto bottom = std::min<uint64_t>(uint64_t(a.y) + a.height, uint64_t(output.y) + output.height + r);
+    return {uint32_t(left), uint32_t(top), uint32_t(right - left), uint32_t(bottom - top)};
+}
+
+RasterSourceNode::RasterSourceNode(RasterImage image) : image_(std::move(image)) {}
+
+WhiteBalanceNode::WhiteBalanceNode(std::shared_ptr<const Node> input, float red, float green, float blue)
+    : input_(std::move(input)), gains_{red, green, blue} {
+    if (!input_) throw std::invalid_argument("white balance input is null");
+    if (input_->output_descriptor() != ImageDescriptor::camera_linear())
+        throw std::invalid_argument("white balance requires camera-linear RGB input");
+    for (float g : gains_)
+        if (!std::isfinite(g) || g <= 0.0f || g > 65536.0f)
+            throw std::invalid_argument("white balance gains must be finite and in (0, 65536]");
+}
+
+ExposureNode::ExposureNode(std::shared_ptr<const Node> input, float stops) : input_(std::move(input)) {
+    if (!input_ || !std::isfinite(stops) || stops < -32.0f || stops > 32.0f)
+        throw std::invalid_argument("exposure stops must be finite and in [-32, 32]");
+    check(rawops_exposure_multiplier(stops, &multiplier_));
+    descriptor_ = input_->output_descriptor();
+    if (descriptor_ != ImageDescriptor::camera_linear() &&
+        descriptor_ != ImageDescriptor::scene_linear(WorkingSpace::LinearProPhotoD50) &&
+        descriptor_ != ImageDescriptor::scene_linear(WorkingSpace::LinearRec2020D65))
+        throw std::invalid_argument("exposure requires linear RGB input");
+}
+
+bool ExposureNode::supports_level(RenderLevel level) const noexcept {
+    if (level.mip == 0 && level.quality == RenderQuality::Final) return true;
+    return level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview &&
+           descriptor_.domain == PixelDomain::SceneLinearRGB && input_->supports_level(level);
+}
+
+CameraToWorkingNode::CameraToWorkingNode(std::shared_ptr<const Node> input, CameraColorTransform transform,
+                                         Rect native_input_bounds)
+    : input_(std::move(input)), descriptor_(ImageDescriptor::scene_linear(transform.target)),
+      native_input_bounds_(native_input_bounds) {
+    if (!input_ || input_->output_descriptor() != ImageDescriptor::camera_linear())
+        throw std::invalid_argument("color transform requires camera-linear RGB input");
+    const auto b = native_input_bounds_;
+    if ((b.width == 0) != (b.height == 0) || (!b.width && (b.x || b.y)) ||
+        uint64_t(b.x) + b.width > uint64_t(std::numeric_limits<uint32_t>::max()) + 1 ||
+        uint64_t(b.y) + b.height > uint64_t(std::numeric_limits<uint32_t>::max()) + 1)
+        throw std::invalid_argument("invalid camera preview input bounds");
+    check(rawops_camera_to_working_matrix(transform.camera_to_xyz_d50.data(), uint32_t(transform.target),
+                                          matrix_.data()));
+}
+
+bool CameraToWorkingNode::supports_level(RenderLevel level) const noexcept {
+    if (level.mip == 0 && level.quality == RenderQuality::Final) return input_->supports_level(level);
+    return level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview &&
+           native_input_bounds_.width && native_input_bounds_.height && input_->supports_level({});
+}
+
+RenderLevel CameraToWorkingNode::input_level(RenderLevel level) const {
+    if (!supports_level(level)) throw std::invalid_argument("camera calibration has no mapping for this level");
+    return {};
+}
+
+Rect CameraToWorkingNode::input_region_level(Rect output, Rect input_bounds, RenderLevel level) const {
+    if (!supports_level(level)) throw std::invalid_argument("camera calibration has no mapping for this level");
+    if (!level.mip) return output;
+    const auto a = native_input_bounds_;
+    if (input_bounds.x != a.x || input_bounds.y != a.y || input_bounds.width != a.width ||
+        input_bounds.height != a.height)
+        throw std::invalid_argument("camera preview input bounds differ from construction");
+    detail::validate_rect(detail::request_bounds(a, level), output);
+    const auto s = 1u << level.mip;
+    const auto left = std::min<uint64_t>(uint64_t(output.x) * s, a.width);
+    const auto top = std::min<uint64_t>(uint64_t(output.y) * s, a.height);
+    const auto right = std::min<uint64_t>((uint64_t(output.x) + output.width) * s, a.width);
+    const auto bottom = std::min<uint64_t>((uint64_t(output.y) + output.height) * s, a.height);
+    return {uint32_t(a.x + left), uint32_t(a.y + top), uint32_t(right - left), uint32_t(bottom - top)};
+}
+
+static bool scene_linear(ImageDescriptor d, WorkingSpace* space) {
+    if (d == ImageDescriptor::scene_linear(WorkingSpace::LinearProPhotoD50)) { *space = WorkingSpace::LinearProPhotoD50; return true; }
+    if (d == ImageDescriptor::scene_linear(WorkingSpace::LinearRec2020D65)) { *space = WorkingSpace::LinearRec2020D65; return true; }
+    return false;
+}
+
+WorkingSpaceConvertNode::WorkingSpaceConvertNode(std::shared_ptr<const Node> input, WorkingSpace target)
+    : input_(std::move(input)), descriptor_(ImageDescriptor::scene_linear(target)) {
+    if (!input_) throw std::invalid_argument("working-space conversion input is null");
+    WorkingSpace source;
+    if (!scene_linear(input_->output_descriptor(), &source))
+        throw std::invalid_argument("working-space conversion requires declared scene-linear RGB");
+    identity_ = source == target;
+    check(rawops_working_to_working_matrix(uint32_t(source), uint32_t(target), matrix_.data()));
+}
+
+bool WorkingSpaceConvertNode::supports_level(RenderLevel level) const noexcept {
+    if (level.mip == 0 && level.quality == RenderQuality::Final) return true;
+    return level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview && input_->supports_level(level);
+}
+
+WorkingToSrgbNode::WorkingToSrgbNode(std::shared_ptr<const Node> input) : input_(std::move(input)) {
+    if (!input_) throw std::invalid_argument("working-to-sRGB input is null");
+    WorkingSpace source;
+    if (!scene_linear(input_->output_descriptor(), &source))
+        throw std::invalid_argument("sRGB conversion requires declared scene-linear working RGB");
+    check(rawops_working_to_srgb_matrix(uint32_t(source), matrix_.data()));
+}
+
+bool WorkingToSrgbNode::supports_level(RenderLevel level) const noexcept {
+    if (level.mip == 0 && level.quality == RenderQuality::Final) return true;
+    return level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview && input_->supports_level(level);
+}
+
+ToneCurveNode::ToneCurveNode(std::shared_ptr<const Node> input, float shoulder, float gamma)
+    : input_(std::move(input)), shoulder_(shoulder), inverse_gamma_(1.0f / gamma) {
+    if (!input_ || !std::isfinite(shoulder) || shoulder <= 0.0f || !std::isfinite(gamma) || gamma <= 0.0f ||
+        !std::isfinite(inverse_gamma_))
+        throw std::invalid_argument("tone shoulder and gamma must be finite and positive");
+    const auto source = input_->output_descriptor();
+    WorkingSpace ws;
+    if (source != ImageDescriptor::camera_linear() && !scene_linear(source, &ws) &&
+        source != ImageDescriptor::linear_srgb())
+        throw std::invalid_argument("tone curve requires linear RGB input");
+    if (source == ImageDescriptor::linear_srgb()) {
+        descriptor_ = ImageDescriptor::display_linear_srgb();
+    } else {
+        descriptor_ = ImageDescriptor::tone_mapped();
+        descriptor_.primaries = source.primaries;
+        descriptor_.white_point = source.white_point;
+    }
+}
+
+bool ToneCurveNode::supports_level(RenderLevel level) const noexcept {
+    if (level.mip == 0 && level.quality == RenderQuality::Final) return true;
+    return level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview &&
+           input_->output_descriptor() == ImageDescriptor::linear_srgb() && input_->supports_level(level);
+}
+
+OutputClipNode::OutputClipNode(std::shared_ptr<const Node> input) : input_(std::move(input)) {
+    if (!input_ || input_->output_descriptor().domain != PixelDomain::ToneMappedUnmanagedRGB)
+        throw std::invalid_argument("output clip requires tone-mapped RGB input");
+    descriptor_ = input_->output_descriptor();
+    descriptor_.domain = PixelDomain::BoundedUnmanagedRGB;
+}
+
+SrgbEncodeNode::SrgbEncodeNode(std::shared_ptr<const Node> input) : input_(std::move(input)) {
+    if (!input_) throw std::invalid_argument("sRGB encode input is null");
+    if (input_->output_descriptor() != ImageDescriptor::display_linear_srgb())
+        throw std::invalid_argument("sRGB encode requires display-linear sRGB input");
+}
+
+bool SrgbEncodeNode::supports_level(RenderLevel level) const noexcept {
+    if (level.mip == 0 && level.quality == RenderQuality::Final) return true;
+    return level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview && input_->supports_level(level);
+}
+
+void validate_extended_curves_settings(const ExtendedCurvesSettings& settings) {
+    if (settings.interpolation != CurveInterpolation::Linear &&
+        settings.interpolation != CurveInterpolation::ShapePreservingCubic)
+        throw std::invalid_argument("extended curves interpolation is unsupported");
+    auto validate = [](const CurveKnots& c) {
+        if (c.knots.size() < 2 || c.knots.size() > 256)
+            throw std::invalid_argument("curve knot count exceeds selected operation bounds");
+        for (size_t i = 0; i < c.knots.size(); ++i) {
+            const auto& p = c.knots[i];
+            if (!std::isfinite(p.x) || !std::isfinite(p.y) || std::abs(p.x) > 65536 || std::abs(p.y) > 65536)
+                throw std::invalid_argument("curve coordinates must be finite with magnitude <=65536");
+            if (i) {
+                const auto& q = c.knots[i - 1];
+                if (!(p.x > q.x)) throw std::invalid_argument("curve x coordinates must increase strictly");
+                const double slope = (p.y - q.y) / (p.x - q.x);
+                if (!std::isfinite(slope) || std::abs(slope) > 65536)
+                    throw std::invalid_argument("curve slope must be finite with magnitude <=65536");
+            }
+        }
+    };
+    validate(settings.master);
+    for (const auto& c : settings.channels) validate(c);
+}
+
+ExtendedCurvesNode::ExtendedCurvesNode(std::shared_ptr<const Node> input, ExtendedCurvesSettings settings)
+    : input_(std::move(input)), settings_(std::move(settings)) {
+    validate_extended_curves_settings(settings_);
+    WorkingSpace ws;
+    if (!input_ || !scene_linear(input_->output_descriptor(), &ws))
+        throw std::invalid_argument("extended curves/levels require scene-linear working RGB");
+}
+
+bool ExtendedCurvesNode::supports_level(RenderLevel level) const noexcept {
+    return ((level.mip == 0 && (level.quality == RenderQuality::Final || level.quality == RenderQuality::Preview)) ||
+            (level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview)) &&
+           input_->supports_level(level);
+}
+
+Rect ExtendedCurvesNode::input_region_level(Rect r, Rect, RenderLevel level) const {
+    if (!supports_level(level)) throw std::invalid_argument("extended curves do not support this render level");
+    return r;
+}
+
+void validate_working_y_guided_filter_settings(const WorkingYGuidedFilterSettings& s) {
+    if (s.radius > 8 || !std::isfinite(s.epsilon) || s.epsilon < 0x1p-24 || s.epsilon > 65536.0)
+        throw std::invalid_argument("guided filter requires radius 0..8 and finite epsilon [2^-24,65536]");
+}
+
+Rect working_y_guided_filter_region(Rect output, Rect image, std::uint32_t radius) {
+    if (radius > 8) throw std::invalid_argument("guided filter radius must be 0..8");
+    if (output.x < image.x || output.y < image.y || uint64_t(output.x) + output.width > uint64_t(image.x) + image.width ||
+        uint64_t(output.y) + output.height > uint64_t(image.y) + image.height)
+        throw std::out_of_range("spatial viewport is outside image bounds");
+    const uint32_t r = 2 * radius;
+    const auto left = std::max<int64_t>(image.x, int64_t(output.x) - r);
+    const auto top = std::max<int64_t>(image.y, int64_t(output.y) - r);
+    const auto right = std::min<uint64_t>(uint64_t(image.x) + image.width, uint64_t(output.x) + output.width + r);
+    const auto bottom = std::min<uint64_t>(uint64_t(image.y) + image.height, uint64_t(output.y) + output.height + r);
+    return {uint32_t(left), uint32_t(top), uint32_t(right - left), uint32_t(bottom - top)};
+}
+
+WorkingYGuidedFilterNode::WorkingYGuidedFilterNode(std::shared_ptr<const Node> input, Rect native_bounds,
+                                                   WorkingYGuidedFilterSettings settings)
+    : input_(std::move(input)), native_bounds_(native_bounds), settings_(settings) {
+    if (!native_bounds_.width || !native_bounds_.height ||
+        uint64_t(native_bounds_.x) + native_bounds_.width > (uint64_t{1} << 32) ||
+        uint64_t(native_bounds_.y) + native_bounds_.height > (uint64_t{1} << 32))
+        throw std::invalid_argument("spatial operation requires nonempty addressable bounds");
+    validate_working_y_guided_filter_settings(settings_);
+    if (!input_) throw std::invalid_argument("guided filter requires input");
+    WorkingSpace ws;
+    descriptor_ = input_->output_descriptor();
+    if (!scene_linear(descriptor_, &ws)) throw std::invalid_argument("guided filter requires scene-linear working RGB");
+}
+
+bool WorkingYGuidedFilterNode::supports_level(RenderLevel level) const noexcept {
+    return ((level.mip == 0 && (level.quality == RenderQuality::Final || level.quality == RenderQuality::Preview)) ||
+            (level.mip >= 1 && level.mip <= 2 && level.quality == RenderQuality::Preview)) &&
+           input_->supports_level(level);
+}
+
+Rect WorkingYGuidedFilterNode::input_region(Rect output, Rect source_bounds) const {
+    return input_region_level(output, source_bounds, {});
+}
+
+Rect WorkingYGuidedFilterNode::input_region_level(Rect output, Rect source_bounds, RenderLevel level) const {
+    const auto image = detail::request_bounds(native_bounds_, level);
+    if (source_bounds.x != image.x || source_bounds.y != image.y || source_bounds.width != image.width ||
+        source_bounds.height != image.height || !supports_level(level))
+        throw std::invalid_argument("guided filter source bounds or level differ");
+    return working_y_guided_filter_region(output, image, settings_.radius);
+}
+
+// ------------------------------------------------------------------- graph --
+
+ImageGraph::ImageGraph(RawImage image, GraphRecipe recipe)
+    : ImageGraph(std::move(image), recipe, RawDemosaicIdentity{}, false) {}
+
+ImageGraph::ImageGraph(RawImage image, GraphRecipe recipe, RawDemosaicIdentity demosaic, bool linear_output)
+    : raw_image_(std::move(image)), source_bounds_(raw_image_->metadata().active_area), recipe_(recipe) {
+    std::shared_ptr<const Node> n = std::make_shared<RawUnpackNode>(*raw_image_, std::move(demosaic));
+    n = std::make_shared<WhiteBalanceNode>(n, recipe.red_gain, recipe.green_gain, recipe.blue_gain);
+    n = std::make_shared<ExposureNode>(n, recipe.exposure_stops);
+    if (recipe.camera_color) n = std::make_shared<CameraToWorkingNode>(n, *recipe.camera_color, source_bounds_);
+    finish(n, linear_output);
+}
+
+ImageGraph::ImageGraph(RasterImage image, GraphRecipe recipe)
+    : ImageGraph(std::make_shared<RasterSourceNode>(image), Rect{0, 0, image.width(), image.height()}, recipe) {}
+
+ImageGraph::ImageGraph(std::shared_ptr<const Node> source, Rect bounds, GraphRecipe recipe)
+    : source_bounds_(bounds), recipe_(recipe) {
+    if (!source || !bounds.width || !bounds.height)
+        throw std::invalid_argument("scene-linear source and valid bounds are required");
+    WorkingSpace ws;
+    if (!scene_linear(source->output_descriptor(), &ws))
+        throw std::invalid_argument("graph source must output a supported scene-linear space");
+    if (recipe.red_gain != 1.0f || recipe.green_gain != 1.0f || recipe.blue_gain != 1.0f || recipe.camera_color)
+        throw std::invalid_argument("raster input does not accept RAW calibration controls");
+    finish(std::make_shared<ExposureNode>(source, recipe.exposure_stops), false);
+}
+
+void ImageGraph::finish(std::shared_ptr<const Node> linear, bool linear_output) {
+    if (recipe_.output_mode == OutputMode::IccDisplay)
+        throw std::invalid_argument("ICC display output is not available in this build");
+    if (recipe_.output_mode != OutputMode::LegacyBounded && recipe_.output_mode != OutputMode::SrgbPreview)
+        throw std::invalid_argument("unknown output mode");
+    if (linear_output) {
+        output_ = linear;
+        return;
+    }
+    if (recipe_.output_mode == OutputMode::SrgbPreview) linear = std::make_shared<WorkingToSrgbNode>(linear);
+    auto tone = std::make_shared<ToneCurveNode>(linear, recipe_.tone_shoulder, recipe_.tone_gamma);
+    output_ = recipe_.output_mode == OutputMode::SrgbPreview
+                  ? std::static_pointer_cast<const Node>(std::make_shared<SrgbEncodeNode>(tone))
+                  : std::static_pointer_cast<const Node>(std::make_shared<OutputClipNode>(tone));
+}
+
+const RawImage& ImageGraph::image() const {
+    if (!raw_image_) throw std::logic_error("graph has a raster source, not a RAW source");
+    return *raw_image_;
+}
+
+// ---------------------------------------------------------------- renderer --
+
+void Renderer::render_tiles(const ImageGraph& graph, RenderRequest request, const TileCallback& callback,
+                            const CancellationToken* cancellation) const {
+    render_tiles(graph.output(), graph.source_bounds(), request, callback, cancellation);
+}
+
+void Renderer::render_tiles(const Node& output, Rect source_bounds, RenderRequest request,
+                            const TileCallback& callback, const CancellationToken* cancellation) const {
+    detail::validate_rect(detail::request_bounds(source_bounds, request.level), request.viewport);
+    if (!output.supports_level(request.level)) throw std::invalid_argument("node does not support this render level");
+    if (!callback || !request.tile_size) throw std::invalid_argument("callback and tile size are required");
+    if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
+    const auto v = request.viewport;
+    const uint64_t right = uint64_t(v.x) + v.width, bottom = uint64_t(v.y) + v.height;
+    if (!detail::compilable(output)) {
+        // A foreign output node renders through its own code, tile by tile.
+        for (uint64_t y = v.y; y < bottom; y += request.tile_size)
+            for (uint64_t x = v.x; x < right; x += request.tile_size) {
+                if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
+                const Rect r{uint32_t(x), uint32_t(y), uint32_t(std::min<uint64_t>(request.tile_size, right - x)),
+                             uint32_t(std::min<uint64_t>(request.tile_size, bottom - y))};
+                callback(output.render_level(r, request.level));
+            }
+        return;
+    }
+    if (!v.width || !v.height) return;
+    // One device render per row of tiles: the band is produced by one launch,
+    // copied to pinned host memory, then handed out tile by tile.
+    const auto chain = detail::compile(output);
+    const uint32_t band_h = std::min<uint32_t>(request.tile_size, v.height);
+    detail::DeviceBuffer dev(size_t(v.width) * band_h * 3 * sizeof(float));
+    detail::DeviceBuffer flags(sizeof(uint32_t));
+    float* host = nullptr;
+    detail::hip(hipHostMalloc(reinterpret_cast<void**>(&host), size_t(v.width) * band_h * 3 * sizeof(float),
+                              hipHostMallocDefault), "hipHostMalloc");
+    struct Free { float* p; ~Free() { (void)hipHostFree(p); } } free_host{host};
+    detail::hip(hipMemset(flags.p, 0, sizeof(uint32_t)), "hipMemset");
+    const auto desc = output.output_descriptor();
+    for (uint64_t y = v.y; y < bottom; y += request.tile_size) {
+        if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
+        const Rect band{v.x, uint32_t(y), v.width, uint32_t(std::min<uint64_t>(request.tile_size, bottom - y))};
+        detail::run(*chain, band, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), nullptr);
+        detail::hip(hipMemcpy(host, dev.p, size_t(band.width) * band.height * 3 * sizeof(float),
+                              hipMemcpyDeviceToHost), "hipMemcpy");
+        check(rawops_check_flags(static_cast<uint32_t*>(flags.p), nullptr));
+        for (uint64_t x = v.x; x < right; x += request.tile_size) {
+            if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
+            Tile t{Rect{uint32_t(x), band.y, uint32_t(std::min<uint64_t>(request.tile_size, right - x)), band.height},
+                   {}, desc};
+            t.rgb.resize(size_t(t.bounds.width) * t.bounds.height * 3);
+            for (uint32_t row = 0; row < t.bounds.height; ++row)
+                std::memcpy(t.rgb.data() + size_t(row) * t.bounds.width * 3,
+                            host + (size_t(row) * band.width + (t.bounds.x - v.x)) * 3,
+                            size_t(t.bounds.width) * 3 * sizeof(float));
+            callback(t);
+        }
+    }
+}
+
+void Renderer::render_tiles(const ImageGraph& graph, Rect viewport, const TileCallback& callback,
+                            std::uint32_t tile_size, const CancellationToken* cancellation) const {
+    render_tiles(graph.output(), graph.source_bounds(), RenderRequest{viewport, tile_size, {}}, callback, cancellation);
+}
+
+void Renderer::render_tiles(const Node& output, Rect source_bounds, Rect viewport, const TileCallback& callback,
+                            std::uint32_t tile_size, const CancellationToken* cancellation) const {
+    render_tiles(output, source_bounds, RenderRequest{viewport, tile_size, {}}, callback, cancellation);
+}
+
+Tile Renderer::render_image(const ImageGraph& graph, RenderRequest request,
+                            const CancellationToken* cancellation) const {
+    return render_image(graph.output(), graph.source_bounds(), request, cancellation);
+}
+
+Tile Renderer::render_image(const Node& output, Rect source_bounds, RenderRequest request,
+                            const CancellationToken* cancellation) const {
+    detail::validate_rect(detail::request_bounds(source_bounds, request.level), request.viewport);
+    if (!output.supports_level(request.level)) throw std::invalid_argument("node does not support this render level");
+    if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
+    const auto v = request.viewport;
+    Tile image{v, std::vector<float>(size_t(v.width) * v.height * 3), output.output_descriptor()};
+    if (!v.width || !v.height) return image;
+    if (!detail::compilable(output)) {
+        render_tiles(output, source_bounds, request, [&](const Tile& t) {
+            for (uint32_t row = 0; row < t.bounds.height; ++row)
+                std::memcpy(image.rgb.data() + (size_t(t.bounds.y - v.y + row) * v.width + t.bounds.x - v.x) * 3,
+                            t.rgb.data() + size_t(row) * t.bounds.width * 3, size_t(t.bounds.width) * 3 * sizeof(float));
+        }, cancellation);
+        return image;
+    }
+    const auto chain = detail::compile(output);
+    detail::DeviceBuffer dev(image.rgb.size() * sizeof(float));
+    detail::DeviceBuffer flags(sizeof(uint32_t));
+    detail::hip(hipMemset(flags.p, 0, sizeof(uint32_t)), "hipMemset");
+    detail::run(*chain, v, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), nullptr);
+    detail::hip(hipMemcpy(image.rgb.data(), dev.p, image.rgb.size() * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+    check(rawops_check_flags(static_cast<uint32_t*>(flags.p), nullptr));
+    return image;
+}
+
+Tile Renderer::render_image(const ImageGraph& graph, Rect viewport, std::uint32_t tile_size,
+                            const CancellationToken* cancellation) const {
+    return render_image(graph.output(), graph.source_bounds(), RenderRequest{viewport, tile_size, {}}, cancellation);
+}
+
+Tile Renderer::render_image(const Node& output, Rect source_bounds, Rect viewport, std::uint32_t tile_size,
+                            const CancellationToken* cancellation) const {
+    return render_image(output, source_bounds, RenderRequest{viewport, tile_size, {}}, cancellation);
+}
+
+std::vector<float> Renderer::render_roi(const ImageGraph& graph, Rect viewport, std::uint32_t tile_size,
+                                        const CancellationToken* cancellation) const {
+    return std::move(render_image(graph, viewport, tile_size, cancellation).rgb);
+}
+
+void Renderer::render_device(const Node& output, Rect source_bounds, RenderRequest request, float* device_rgb,
+                             void* stream) const {
+    detail::validate_rect(detail::request_bounds(source_bounds, request.level), request.viewport);
+    if (!output.supports_level(request.level)) throw std::invalid_argument("node does not support this render level");
+    const auto chain = detail::compile(output);
+    detail::run(*chain, request.viewport, request.level, device_rgb, nullptr, static_cast<hipStream_t>(stream));
+}
+
+}  // namespace rawengine

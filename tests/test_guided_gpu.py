@@ -1,0 +1,97 @@
+"""GPU parity of the working-Y guided filter (+ exposure + 16-point monotone
+curve) against the CPU oracle (ordered centred FP64, SpatialOps.cpp:168-287),
+the rational impulse KAT and exact-bit special cases.  Tolerance: 1e-5 of
+max(1, |value|) (box-moment FP64 accumulation vs ordered sums; values reach
+several hundred, where one float32 ulp exceeds 1e-5 absolute)."""
+import numpy as np
+import pytest
+import torch
+
+import librawops_amd as L
+from librawops_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_guided(raster, space, radius, eps, viewport, stops=0.0, knots=None, mip=0, cubic=True):
+    H, W = raster.shape[:2]
+    img = L.RasterImage(L.RasterMetadata(W, H, 0, L.WorkingSpace(space)), raster)
+    n = L.RasterSourceNode(img)
+    n = L.WorkingYGuidedFilterNode(n, L.Rect(0, 0, W, H), L.WorkingYGuidedFilterSettings(radius, eps), max_radius=32)
+    if stops != 0.0:
+        n = L.ExposureNode(n, stops)
+    if knots is not None:
+        n = L.ExtendedCurvesNode(n, L.ExtendedCurvesSettings(knots[0], knots[1:], L.CurveInterpolation(int(cubic))))
+    q = L.RenderQuality.Final if mip == 0 else L.RenderQuality.Preview
+    return L.Renderer().render_image(n, L.Rect(0, 0, W, H), L.RenderRequest(L.Rect(*viewport), 256, L.RenderLevel(mip, q))).rgb
+
+
+def close(a, b, tol=1e-5):
+    return np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))
+
+
+def test_impulse_kat():
+    for space in (0, 1):
+        img = np.zeros((1, 5, 3), np.float32)
+        img[0, 2] = 1
+        out = gpu_guided(img, space, 1, 1.0, (0, 0, 5, 1))
+        exp = np.array([3 / 22, 2 / 11, 5 / 11, 2 / 11, 3 / 22])
+        assert np.max(np.abs(out[0, :, 0] - exp)) <= 1e-7
+        assert np.max(np.abs(out[0, :, 1] - exp)) <= 1e-7
+
+
+def test_constant_and_extreme_exact_bits():
+    fmax = np.finfo(np.float32).max
+    tiny = np.float32(1.4e-45)
+    for space in (0, 1):
+        for vals in ([-0.0, 0.0, -0.0], [fmax, fmax, fmax], [-fmax, fmax, -fmax], [tiny, tiny, tiny], [-.5, .25, 2.0]):
+            img = np.array(vals, np.float32).reshape(1, 1, 3)
+            for radius in (0, 1, 8):
+                for eps in (2.0 ** -24, 65536.0):
+                    out = gpu_guided(img, space, radius, eps, (0, 0, 1, 1))
+                    assert out.tobytes() == img.tobytes(), (vals, radius, eps)
+
+
+@pytest.mark.parametrize("radius", [1, 4, 8, 16])
+@pytest.mark.parametrize("space", [0, 1])
+def test_against_oracle(radius, space):
+    raster = synth.lognormal_raster(97, 83, seed=radius + 10 * space)
+    knots = synth.monotone_knots(16, seed=radius)
+    ident = [(0.0, 0.0), (1.0, 1.0)]
+    for eps in (1e-4, 1e-3, 1e-2):
+        for mip in (0, 1, 2):
+            b = O.request_bounds((0, 0, 97, 83), mip)
+            for vp in (b, (b[2] // 5, b[3] // 4, b[2] // 2, b[3] // 2)):
+                got = gpu_guided(raster, space, radius, eps, vp, stops=0.7, knots=[knots, ident, ident, ident], mip=mip)
+                exp = O.guided_chain(raster, space, radius, eps, vp, stops=0.7, knots=[knots, ident, ident, ident],
+                                     mip=mip, threads=8)
+                err = close(got, exp)
+                assert err <= 1e-5, (eps, mip, vp, err)
+
+
+def test_config3_full_frame_crops():
+    """6000x4000 lognormal raster, r = 8, eps 1e-3, +0.7 EV, 16-point curve:
+    the full-frame GPU render checked on crops the oracle computes with their
+    complete 2r halo."""
+    W, H = 6000, 4000
+    raster = synth.lognormal_raster(W, H, seed=3)
+    knots = synth.monotone_knots(16, seed=3)
+    ident = [(0.0, 0.0), (1.0, 1.0)]
+    got = gpu_guided(raster, 1, 8, 1e-3, (0, 0, W, H), stops=0.7, knots=[knots, ident, ident, ident])
+    rng = np.random.default_rng(0)
+    for x, y in [(0, 0), (W - 48, H - 48), (0, H - 48)] + [tuple(rng.integers(0, [W - 48, H - 48])) for _ in range(4)]:
+        vp = (int(x), int(y), 48, 48)
+        exp = O.guided_chain(raster, 1, 8, 1e-3, vp, stops=0.7, knots=[knots, ident, ident, ident], threads=8)
+        err = close(got[y:y + 48, x:x + 48], exp)
+        assert err <= 1e-5, (vp, err)
+
+
+def test_non_finite_input_rejected():
+    img = np.ones((4, 4, 3), np.float32)
+    dev = torch.from_numpy(img).cuda()
+    dev[1, 1, 1] = float("nan")
+    raster = L.RasterImage(L.RasterMetadata(4, 4, 0, L.WorkingSpace.LinearProPhotoD50), dev)
+    n = L.WorkingYGuidedFilterNode(L.RasterSourceNode(raster), L.Rect(0, 0, 4, 4), L.WorkingYGuidedFilterSettings(1, 1e-3))
+    with pytest.raises(L.RawOpsError):
+        L.Renderer().render_image(n, L.Rect(0, 0, 4, 4), L.RenderRequest(L.Rect(0, 0, 4, 4)))
