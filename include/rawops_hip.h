@@ -170,6 +170,54 @@ rawops_status rawops_render_tiles_host(const rawops_plan* plan, const rawops_sou
                                        uint32_t tile_size, rawops_tile_callback callback,
                                        void* user, const volatile int* cancel);
 
+/* ---- adjoints (no reference counterpart: the reference has no derivatives) ---- */
+/* Reverse mode of the render chain out = M * (m * (g (.) demosaic(raw)))
+ * (WhiteBalanceNode -> ExposureNode -> CameraToWorkingNode after the demosaic,
+ * RawEngine.cpp:408-548) at mip 0.  Given dL/dout over `viewport` (device,
+ * float32 RGB, row stride g_out_stride_pixels), writes dL/d(normalised raw
+ * sample) into g_raw (device float,
+ * active_width x active_height, row-major) and ADDS dL/d(gains[3],
+ * multiplier, matrix[9]) into g_params (device double[13]).  Only photosites
+ * within 4 of the viewport (the adjoint's reach) are written, so a caller
+ * that needs the whole plane zero-fills it first; g_raw may be NULL when only
+ * the parameter gradients are wanted.  Bilinear: exact
+ * transpose (remosaic scatter).  Menon: transpose of the branch the forward
+ * classifier takes (decisions recomputed, not differentiated). */
+rawops_status rawops_render_backward(const rawops_source* source, uint32_t demosaic, rawops_rect viewport,
+                                     const float gains[3], float multiplier, const float matrix[9],
+                                     const float* g_out, uint32_t g_out_stride_pixels, float* g_raw,
+                                     double* g_params, void* stream);
+
+/* Reverse mode of the working-Y guided filter (radius 1..32, level 0) of a
+ * raster source: given dL/dq over `viewport` (device float32 RGB, row stride
+ * g_out_stride_pixels), writes dL/dp for every raster pixel into g_in (device
+ * float32, width x height x 3) and ADDS dL/d(eps) into *g_eps (device
+ * double).  The per-centre box statistics are recomputed from the input in
+ * FP64; workspace is device scratch of rawops_guided_backward_workspace_size()
+ * bytes (NULL: allocated stream-ordered by the call). */
+rawops_status rawops_guided_backward_workspace_size(uint32_t raster_width, uint32_t raster_height, size_t* bytes);
+rawops_status rawops_guided_backward(const rawops_source* raster, uint32_t radius, double epsilon,
+                                     rawops_rect viewport, const float* g_out, uint32_t g_out_stride_pixels,
+                                     float* g_in, double* g_eps, void* workspace, size_t workspace_bytes,
+                                     void* stream);
+
+/* Element-wise forward/adjoint kernels (device pointers, n floats). */
+/* sRGB encode of clamp01(x) (SrgbEncodeNode, RawEngine.cpp:1266-1274) and its derivative. */
+rawops_status rawops_srgb_encode(const float* x, float* y, size_t n, void* stream);
+rawops_status rawops_srgb_encode_backward(const float* x, const float* g_y, float* g_x, size_t n, void* stream);
+/* y = m x with m = 2^stops: g_x = m g_y, *g_stops += ln2 * sum(g_y * m x). */
+rawops_status rawops_exposure_backward(const float* x, const float* g_y, float multiplier, float* g_x,
+                                       double* g_stops, size_t n, void* stream);
+/* Detail boost y = x + s (x - base) and its adjoint (*g_s accumulated). */
+rawops_status rawops_detail_boost(const float* x, const float* base, float strength, float* y, size_t n, void* stream);
+rawops_status rawops_detail_boost_backward(const float* x, const float* base, const float* g_y, float strength,
+                                           float* g_x, float* g_base, double* g_s, size_t n, void* stream);
+/* Adjoint of the master curve of a plan whose reduced_ops[0] is RAWOPS_OP_CURVES
+ * (ExtendedCurvesNode::map, ToneOps.cpp:820-831): g_x and the knot-ordinate
+ * gradients g_knots_y (device double[knot count], accumulated). */
+rawops_status rawops_curve_backward(const rawops_plan* curve_plan, const float* x, const float* g_y, size_t n,
+                                    float* g_x, double* g_knots_y, void* stream);
+
 /* ---- guided filter (SpatialOps.cpp:159-317) + fused point epilogue ---- */
 /* Working-Y guided filter of a raster source at level `mip`, output rect
  * `viewport` (level coordinates), radius 0..32 (the reference admits 0..8,

@@ -123,6 +123,15 @@ def lib():
             "rawops_render": ([P, P, Rect, u32, u32, P, P, u32, P, P], S),
             "rawops_check_flags": ([P, P], S),
             "rawops_render_tiles_host": ([P, P, Rect, u32, u32, u32, TILE_CALLBACK, P, P], S),
+            "rawops_render_backward": ([P, u32, Rect, P, ctypes.c_float, P, P, u32, P, P, P], S),
+            "rawops_guided_backward_workspace_size": ([u32, u32, P], S),
+            "rawops_guided_backward": ([P, u32, ctypes.c_double, Rect, P, u32, P, P, P, ctypes.c_size_t, P], S),
+            "rawops_srgb_encode": ([P, P, ctypes.c_size_t, P], S),
+            "rawops_srgb_encode_backward": ([P, P, P, ctypes.c_size_t, P], S),
+            "rawops_exposure_backward": ([P, P, ctypes.c_float, P, P, ctypes.c_size_t, P], S),
+            "rawops_detail_boost": ([P, P, ctypes.c_float, P, ctypes.c_size_t, P], S),
+            "rawops_detail_boost_backward": ([P, P, P, ctypes.c_float, P, P, P, ctypes.c_size_t, P], S),
+            "rawops_curve_backward": ([P, P, P, ctypes.c_size_t, P, P, P], S),
             "rawops_guided_workspace_size": ([u32, u32, Rect, u32, u32, P], S),
             "rawops_guided_filter": ([P, Rect, u32, u32, ctypes.c_double, P, P, u32, P,
                                       ctypes.c_size_t, P, P], S),

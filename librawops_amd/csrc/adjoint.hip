@@ -1,0 +1,351 @@
+// Adjoint (reverse-mode) kernels of the render chain
+//     out = M * (m * (g (.) demosaic(raw)))
+// (WhiteBalanceNode, ExposureNode, CameraToWorkingNode applied after the
+// demosaic, RawEngine.cpp:408-548) for gfx950.
+//
+// Given dL/dout over a viewport, one launch produces dL/d(normalised raw) over
+// the whole active area and accumulates dL/d(gains, multiplier, matrix).
+//   * Bilinear: the transpose of the same-colour mean, i.e. a remosaic
+//     scatter, restricted per tile so each workgroup owns its raw outputs.
+//   * Menon: the transpose of the branch actually taken.  With the classifier
+//     decisions frozen (recomputed here exactly as the forward makes them),
+//     the demosaic is linear in the raw samples; its adjoint walks the stages
+//     backwards: final chroma -> green-site chroma -> directional green.
+//     Reads that the forward mirrors at the active-area edge scatter their
+//     adjoint to the mirrored sample.
+// A workgroup recomputes the forward over its tile grown by 4 (the adjoint's
+// reach), scatters into LDS accumulators with LDS float atomics, and writes
+// only its own tile, so no global atomics touch the raw gradient.
+#include "device_common.hpp"
+#include "menon_tile.hpp"
+#include "render_launch.hpp"
+
+#include <algorithm>
+
+namespace rawops {
+namespace {
+
+constexpr int kNT = 256;
+
+struct BackwardArgs {
+    SensorArgs s;
+    int vx, vy, vw, vh;        // viewport (native sensor coordinates)
+    float g[3], m, M[9];
+    const float* gout;         // viewport RGB
+    uint32_t gstride;          // pixels per gout row
+    float* graw;               // active-area plane, aw x ah
+    double* gparams;           // 13: gains[3], multiplier, matrix[9] (row-major)
+    int tiles_x;               // tiles per grid row
+    int tile0_x, tile0_y;      // first tile (in tiles from the area origin)
+};
+
+// dL/dout at a sensor pixel (zero outside the viewport).
+__device__ __forceinline__ bool load_gout(const BackwardArgs& a, int sx, int sy, float go[3]) {
+    const int lx = sx - a.vx, ly = sy - a.vy;
+    if (lx < 0 || ly < 0 || lx >= a.vw || ly >= a.vh) return false;
+    const float* p = a.gout + ((size_t)ly * a.gstride + (size_t)lx) * 3;
+    go[0] = p[0]; go[1] = p[1]; go[2] = p[2];
+    return true;
+}
+
+// gd_c = g_c * (m * (M^T gout)_c): gradient at the demosaic output.
+__device__ __forceinline__ void chain_back(const BackwardArgs& a, const float go[3], float u[3], float gd[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        u[c] = a.M[c] * go[0] + a.M[3 + c] * go[1] + a.M[6 + c] * go[2];
+        gd[c] = a.g[c] * (a.m * u[c]);
+    }
+}
+
+// Parameter gradients of one pixel with demosaic output d.
+__device__ __forceinline__ void param_grads(const BackwardArgs& a, const float d[3], const float go[3],
+                                            const float u[3], double acc[13]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        acc[c] += (double)a.m * d[c] * u[c];                  // d out / d g_c
+        acc[3] += (double)a.g[c] * d[c] * u[c];               // d out / d m
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[4 + r * 3 + c] += (double)go[r] * ((double)a.m * a.g[c] * d[c]);
+}
+
+__device__ void flush_params(const BackwardArgs& a, double acc[13], double* red /* [4][13] LDS */) {
+#pragma unroll
+    for (int i = 0; i < 13; ++i)
+        for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int i = 0; i < 13; ++i) red[wave * 13 + i] = acc[i];
+    __syncthreads();
+    if (threadIdx.x < 13) {
+        double t = 0;
+        for (int w = 0; w < kNT / 64; ++w) t += red[w * 13 + threadIdx.x];
+        atomicAdd(a.gparams + threadIdx.x, t);
+    }
+}
+
+__device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty, int TW, int TH) {
+    tx = a.s.ax + (a.tile0_x + (int)(blockIdx.x % a.tiles_x)) * TW;
+    ty = a.s.ay + (a.tile0_y + (int)(blockIdx.x / a.tiles_x)) * TH;
+}
+
+// --------------------------------------------------------------- Menon --
+
+template <int TW, int TH, bool BORDER>
+__device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx, int ty) {
+    constexpr int EW = TW + 8, EH = TH + 8;  // forward recompute region: tile +-4
+    using L = MenonLds<EW, EH>;
+    const int ex = tx - 4, ey = ty - 4;     // origin of the extended tile
+    MenonTile<EW, EH, BORDER, kNT> t{s, ex, ey, 0,
+                                     reinterpret_cast<float*>(lds + L::OFF_RAW),
+                                     reinterpret_cast<double*>(lds + L::OFF_GH),
+                                     reinterpret_cast<double*>(lds + L::OFF_GV),
+                                     reinterpret_cast<double*>(lds + L::OFF_DH),
+                                     reinterpret_cast<double*>(lds + L::OFF_DV),
+                                     reinterpret_cast<double*>(lds + L::OFF_GREEN),
+                                     reinterpret_cast<double*>(lds + L::OFF_GH),
+                                     reinterpret_cast<double*>(lds + L::OFF_GV),
+                                     reinterpret_cast<uint8_t*>(lds + L::OFF_HZ)};
+    for (int b = 0; b < 2; ++b) {
+        const uint32_t sensor = (s.ng_parity >> (((uint32_t)ey + (uint32_t)b) & 1u)) & 1u;
+        t.ngr |= ((sensor ^ ((uint32_t)ex & 1u)) & 1u) << b;
+    }
+    // Accumulators over dead dh/dv storage: red/blue adjoints at green sites
+    // of the extended tile, green adjoint at non-green sites of tile +-2, raw
+    // adjoint of the tile itself.
+    constexpr int RB_N = (EW / 2) * EH, GA_HW = (EW - 4) / 2, GA_N = GA_HW * (EH - 4), GR_N = TW * TH;
+    static_assert((2 * RB_N + GA_N + GR_N) * 4 <= L::DH_BYTES + L::DV_BYTES, "adjoint accumulators do not fit");
+    float* adj_r = reinterpret_cast<float*>(lds + L::OFF_DH);
+    float* adj_b = adj_r + RB_N;
+    float* adj_g = adj_b + RB_N;
+    float* adj_raw = adj_g + GA_N;
+
+    load_raw<EW, EH, 6, BORDER, kNT>(s, ex, ey, t.raw);
+    __syncthreads();
+    t.directional();
+    __syncthreads();
+    t.gradients();
+    __syncthreads();
+    t.classify();
+    __syncthreads();
+    t.chroma();
+    for (int i = threadIdx.x; i < 2 * RB_N + GA_N + GR_N; i += kNT) adj_r[i] = 0.0f;
+    __syncthreads();
+
+    // Extended-tile coordinates (u, v) = sensor - (ex, ey); the tile is [4, 4+TW) x [4, 4+TH).
+    auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
+    auto mirror_uv = [&](int& u, int& v) {
+        if (BORDER) {
+            const int sx = ex + u, sy = ey + v;
+            if (!in_area(s, sx, sy)) { u = reflect(sx, s.ax, s.aw) - ex; v = reflect(sy, s.ay, s.ah) - ey; }
+        }
+    };
+    auto add_raw = [&](int u, int v, float w) {
+        mirror_uv(u, v);
+        if (in_tile(u, v)) atomicAdd(adj_raw + (v - 4) * TW + (u - 4), w);
+    };
+    auto add_rb = [&](float* plane, int u, int v, float w) {   // green-site adjoint
+        mirror_uv(u, v);
+        if (u >= 0 && u < EW && v >= 0 && v < EH) atomicAdd(plane + v * (EW / 2) + (u >> 1), w);
+    };
+    auto add_g = [&](int u, int v, float w) {                  // non-green green adjoint
+        mirror_uv(u, v);
+        if (u >= 2 && u < EW - 2 && v >= 2 && v < EH - 2) atomicAdd(adj_g + (v - 2) * GA_HW + ((u - 2) >> 1), w);
+    };
+
+    double acc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // (a) outputs of the extended tile: direct terms, final-stage chroma.
+    for (int i = threadIdx.x; i < EW * EH; i += kNT) {
+        const int v = i / EW, u = i - v * EW;
+        const int sx = ex + u, sy = ey + v;
+        if (!t.inside(u, v)) continue;
+        float go[3];
+        if (!load_gout(a, sx, sy, go)) continue;
+        float uu[3], gd[3];
+        chain_back(a, go, uu, gd);
+        if (in_tile(u, v)) {
+            float d[3];
+            t.finish(u, v, d[0], d[1], d[2]);
+            param_grads(a, d, go, uu, acc);
+        }
+        const int own = color_of(s, sx, sy);
+        if (own == 1) {
+            add_rb(adj_r, u, v, gd[0]);
+            add_rb(adj_b, u, v, gd[2]);
+            add_raw(u, v, gd[1]);
+            continue;
+        }
+        add_raw(u, v, gd[0] + gd[2]);
+        add_g(u, v, gd[1]);
+        const bool horizontal = t.hz[v * (EW / 2) + (u >> 1)] != 0;
+        const int dx = horizontal ? 1 : 0, dy = horizontal ? 0 : 1;
+        // R site: B = r + mean(blue) - mean(red); B site: R = r + mean(red) - mean(blue).
+        const float w = own == 0 ? gd[2] : gd[0];
+        float* plus = own == 0 ? adj_b : adj_r;
+        float* minus = own == 0 ? adj_r : adj_b;
+        add_rb(plus, u - dx, v - dy, 0.5f * w);
+        add_rb(plus, u + dx, v + dy, 0.5f * w);
+        add_rb(minus, u - dx, v - dy, -0.5f * w);
+        add_rb(minus, u + dx, v + dy, -0.5f * w);
+    }
+    __syncthreads();
+    // (c) red/blue at green sites: Rg = G + mean(raw) - mean(G) along the red axis.
+    {
+        constexpr int HW = EW / 2, N = HW * EH;
+        for (int i = threadIdx.x; i < N; i += kNT) {
+            const int v = i / HW, k = i - v * HW;
+            const int u = 2 * k + (t.ng_parity(v) ^ 1);
+            if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1 || !t.inside(u, v)) continue;
+            const float ar = adj_r[i], ab = adj_b[i];
+            if (ar == 0.0f && ab == 0.0f) continue;
+            const bool red_horizontal = color_of(s, ex + u + 1, ey + v) == 0;
+            const int dx = red_horizontal ? 1 : 0, dy = red_horizontal ? 0 : 1;
+            add_raw(u, v, ar + ab);
+            add_raw(u - dx, v - dy, 0.5f * ar);
+            add_raw(u + dx, v + dy, 0.5f * ar);
+            add_g(u - dx, v - dy, -0.5f * ar);
+            add_g(u + dx, v + dy, -0.5f * ar);
+            add_raw(u - dy, v - dx, 0.5f * ab);
+            add_raw(u + dy, v + dx, 0.5f * ab);
+            add_g(u - dy, v - dx, -0.5f * ab);
+            add_g(u + dy, v + dx, -0.5f * ab);
+        }
+    }
+    __syncthreads();
+    // (d) directional green along the taken axis: 0.5(r-1 + r+1) + 0.5 r - 0.25(r-2 + r+2).
+    {
+        constexpr int N = GA_HW * (EH - 4);
+        for (int i = threadIdx.x; i < N; i += kNT) {
+            const int row = i / GA_HW, k = i - row * GA_HW;
+            const int v = row + 2;
+            const int u = 2 + 2 * k + t.ng_parity(v);
+            if (!t.inside(u, v)) continue;
+            const float w = adj_g[i];
+            if (w == 0.0f) continue;
+            const bool horizontal = t.hz[v * (EW / 2) + (u >> 1)] != 0;
+            const int dx = horizontal ? 1 : 0, dy = horizontal ? 0 : 1;
+            add_raw(u, v, 0.5f * w);
+            add_raw(u - dx, v - dy, 0.5f * w);
+            add_raw(u + dx, v + dy, 0.5f * w);
+            add_raw(u - 2 * dx, v - 2 * dy, -0.25f * w);
+            add_raw(u + 2 * dx, v + 2 * dy, -0.25f * w);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TW * TH; i += kNT) {
+        const int v = i / TW, u = i - v * TW;
+        const int sx = tx + u, sy = ty + v;
+        if (a.graw && sx < s.ax + s.aw && sy < s.ay + s.ah)
+            a.graw[(size_t)(sy - s.ay) * s.aw + (size_t)(sx - s.ax)] = adj_raw[i];
+    }
+    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GH));
+}
+
+template <int TW, int TH>
+__global__ void __launch_bounds__(kNT) menon_backward(const BackwardArgs a) {
+    __shared__ __attribute__((aligned(16))) char lds[MenonLds<TW + 8, TH + 8>::BYTES];
+    int tx, ty;
+    tile_of(a, tx, ty, TW, TH);
+    const SensorArgs s = a.s;
+    const bool interior = tx - 10 >= s.ax && ty - 10 >= s.ay && tx + TW + 10 <= s.ax + s.aw &&
+                          ty + TH + 10 <= s.ay + s.ah;
+    if (interior) menon_back_tile<TW, TH, false>(a, s, lds, tx, ty);
+    else menon_back_tile<TW, TH, true>(a, s, lds, tx, ty);
+}
+
+// ------------------------------------------------------------- bilinear --
+
+template <int TW, int TH>
+__global__ void __launch_bounds__(kNT) bilinear_backward(const BackwardArgs a) {
+    __shared__ float raw[(TW + 2) * (TH + 2)];
+    __shared__ float adj[TW * TH];
+    __shared__ double red[4 * 13];
+    int tx, ty;
+    tile_of(a, tx, ty, TW, TH);
+    const SensorArgs s = a.s;
+    load_raw<TW, TH, 1, true, kNT>(s, tx, ty, raw);
+    for (int i = threadIdx.x; i < TW * TH; i += kNT) adj[i] = 0.0f;
+    __syncthreads();
+    double acc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int EW = TW + 2, EH = TH + 2;
+    for (int i = threadIdx.x; i < EW * EH; i += kNT) {
+        const int v = i / EW, u = i - v * EW;       // tile coordinates + 1
+        const int sx = tx + u - 1, sy = ty + v - 1;
+        if (!in_area(s, sx, sy)) continue;
+        float go[3];
+        if (!load_gout(a, sx, sy, go)) continue;
+        float uu[3], gd[3];
+        chain_back(a, go, uu, gd);
+        const bool mine = u >= 1 && u <= TW && v >= 1 && v <= TH;
+        if (mine) {
+            float d[3];
+            bilinear_pixel<TW, TH, true>(s, raw, tx, ty, u - 1, v - 1, d[0], d[1], d[2]);
+            param_grads(a, d, go, uu, acc);
+        }
+        const int own = color_of(s, sx, sy);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (own == c) {
+                if (mine) atomicAdd(adj + (v - 1) * TW + (u - 1), gd[c]);
+                continue;
+            }
+            int count = 0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx)
+                    if (in_area(s, sx + dx, sy + dy) && color_of(s, sx + dx, sy + dy) == c) ++count;
+            if (!count) continue;
+            const float w = gd[c] / (float)count;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int px = u - 1 + dx, py = v - 1 + dy;
+                    if (px < 0 || px >= TW || py < 0 || py >= TH) continue;
+                    if (!in_area(s, sx + dx, sy + dy) || color_of(s, sx + dx, sy + dy) != c) continue;
+                    atomicAdd(adj + py * TW + px, w);
+                }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TW * TH; i += kNT) {
+        const int v = i / TW, u = i - v * TW;
+        const int sx = tx + u, sy = ty + v;
+        if (a.graw && sx < s.ax + s.aw && sy < s.ay + s.ah)
+            a.graw[(size_t)(sy - s.ay) * s.aw + (size_t)(sx - s.ax)] = adj[i];
+    }
+    flush_params(a, acc, red);
+}
+
+}  // namespace
+
+hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
+    BackwardArgs a{};
+    a.s = L.sensor;
+    a.vx = (int)L.viewport.x; a.vy = (int)L.viewport.y;
+    a.vw = (int)L.viewport.width; a.vh = (int)L.viewport.height;
+    for (int i = 0; i < 3; ++i) a.g[i] = L.gains[i];
+    a.m = L.multiplier;
+    for (int i = 0; i < 9; ++i) a.M[i] = L.matrix[i];
+    a.gout = L.g_out;
+    a.gstride = L.g_stride;
+    a.graw = L.g_raw;
+    a.gparams = L.g_params;
+    constexpr int TW = 64, TH = 32;
+    // Only raw samples within 4 of the viewport receive gradient (the
+    // adjoint's reach); tiles are anchored at the active-area origin.
+    const int x0 = std::max(0, a.vx - 4 - a.s.ax), y0 = std::max(0, a.vy - 4 - a.s.ay);
+    const int x1 = std::min(a.s.aw, a.vx + a.vw + 4 - a.s.ax), y1 = std::min(a.s.ah, a.vy + a.vh + 4 - a.s.ay);
+    a.tile0_x = x0 / TW;
+    a.tile0_y = y0 / TH;
+    a.tiles_x = (x1 + TW - 1) / TW - a.tile0_x;
+    const int tiles_y = (y1 + TH - 1) / TH - a.tile0_y;
+    if (a.vw <= 0 || a.vh <= 0 || a.tiles_x <= 0 || tiles_y <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
+    if (L.demosaic == RAWOPS_DEMOSAIC_MENON)
+        hipLaunchKernelGGL((menon_backward<TW, TH>), dim3(blocks), dim3(kNT), 0, stream, a);
+    else
+        hipLaunchKernelGGL((bilinear_backward<TW, TH>), dim3(blocks), dim3(kNT), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace rawops

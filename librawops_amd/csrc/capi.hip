@@ -31,6 +31,7 @@ struct rawops_plan {
     uint32_t demosaic = 0, mip_anchor = 0;
     OpList native{}, reduced{};
     OpList* device = nullptr;       // device copies: [0] native, [1] reduced, [2] empty
+    uint32_t curve_cubic = 0;       // interpolation of the first curve op
     std::vector<void*> tables;      // device curve tables
     ~rawops_plan() {
         for (void* t : tables) (void)hipFree(t);
@@ -197,6 +198,7 @@ void compile_ops(rawops_plan& plan, const rawops_op_desc* ops, uint32_t n, OpLis
             plan.tables.push_back(dev);
             hip_check(hipMemcpy(dev, host.get(), sizeof(DevCurves), hipMemcpyHostToDevice), "hipMemcpy(curves)");
             o.curves = static_cast<const DevCurves*>(dev);
+            plan.curve_cubic = d.cubic ? 1u : 0u;
             break;
         }
         default:
@@ -455,6 +457,123 @@ rawops_status rawops_render_tiles_host(const rawops_plan* plan, const rawops_sou
                                 size_t(r.width) * 3 * sizeof(float));
                 if (callback(user, r, tile.data())) fail(RAWOPS_CANCELLED, "render cancelled by callback");
             }
+    });
+}
+
+rawops_status rawops_render_backward(const rawops_source* src, uint32_t demosaic, rawops_rect viewport,
+                                     const float gains[3], float multiplier, const float matrix[9],
+                                     const float* g_out, uint32_t g_stride, float* g_raw, double* g_params,
+                                     void* stream) {
+    return guarded([&] {
+        if (!src || !src->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "backward needs a RAW source");
+        if (demosaic > RAWOPS_DEMOSAIC_MENON) fail(RAWOPS_INVALID_ARGUMENT, "unsupported RAW demosaic algorithm");
+        if (!gains || !matrix || !g_out || !g_params) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        inside_or_throw(src->meta.active_area, viewport);
+        if (!g_stride) g_stride = viewport.width;
+        BackwardLaunch L{};
+        L.sensor = sensor_args(*src);
+        L.demosaic = demosaic;
+        if (src->meta.active_area.width <= 1 || src->meta.active_area.height <= 1) L.demosaic = RAWOPS_DEMOSAIC_BILINEAR;
+        L.viewport = viewport;
+        for (int i = 0; i < 3; ++i) L.gains[i] = gains[i];
+        L.multiplier = multiplier;
+        for (int i = 0; i < 9; ++i) L.matrix[i] = matrix[i];
+        L.g_out = g_out;
+        L.g_stride = g_stride;
+        L.g_raw = g_raw;
+        L.g_params = g_params;
+        hip_check(launch_render_backward(L, static_cast<hipStream_t>(stream)), "backward launch");
+    });
+}
+
+rawops_status rawops_srgb_encode(const float* x, float* y, size_t n, void* stream) {
+    return guarded([&] { hip_check(launch_srgb(x, y, n, static_cast<hipStream_t>(stream)), "srgb launch"); });
+}
+
+rawops_status rawops_srgb_encode_backward(const float* x, const float* gy, float* gx, size_t n, void* stream) {
+    return guarded([&] {
+        hip_check(launch_srgb_backward(x, gy, gx, n, static_cast<hipStream_t>(stream)), "srgb backward launch");
+    });
+}
+
+rawops_status rawops_exposure_backward(const float* x, const float* gy, float m, float* gx, double* g_stops, size_t n,
+                                       void* stream) {
+    return guarded([&] {
+        hip_check(launch_exposure_backward(x, gy, m, gx, g_stops, n, static_cast<hipStream_t>(stream)),
+                  "exposure backward launch");
+    });
+}
+
+rawops_status rawops_detail_boost(const float* x, const float* base, float strength, float* y, size_t n, void* stream) {
+    return guarded([&] {
+        hip_check(launch_boost(x, base, strength, y, n, static_cast<hipStream_t>(stream)), "boost launch");
+    });
+}
+
+rawops_status rawops_detail_boost_backward(const float* x, const float* base, const float* gy, float strength,
+                                           float* gx, float* gbase, double* gs, size_t n, void* stream) {
+    return guarded([&] {
+        hip_check(launch_boost_backward(x, base, gy, strength, gx, gbase, gs, n, static_cast<hipStream_t>(stream)),
+                  "boost backward launch");
+    });
+}
+
+rawops_status rawops_curve_backward(const rawops_plan* plan, const float* x, const float* gy, size_t n, float* gx,
+                                    double* gys, void* stream) {
+    return guarded([&] {
+        if (!plan || plan->reduced.n < 1 || plan->reduced.op[0].kind != OP_CURVES)
+            fail(RAWOPS_INVALID_ARGUMENT, "curve backward needs a plan whose first reduced op is a curve");
+        const DevCurves* t = plan->reduced.op[0].curves;
+        hip_check(launch_curve_backward(x, gy, n, &t->c[0], plan->curve_cubic, gx, gys, static_cast<hipStream_t>(stream)),
+                  "curve backward launch");
+    });
+}
+
+rawops_status rawops_guided_backward_workspace_size(uint32_t width, uint32_t height, size_t* bytes) {
+    return guarded([&] {
+        if (!bytes) fail(RAWOPS_INVALID_ARGUMENT, "output pointer is null");
+        *bytes = guided_backward_workspace_bytes(width, height);
+    });
+}
+
+rawops_status rawops_guided_backward(const rawops_source* raster, uint32_t radius, double epsilon, rawops_rect viewport,
+                                     const float* g_out, uint32_t g_stride, float* g_in, double* g_eps,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
+    return guarded([&] {
+        if (!raster || raster->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires a scene-linear raster");
+        if (radius < 1 || radius > 32 || !std::isfinite(epsilon) || epsilon < 0x1p-24 || epsilon > 65536.0)
+            fail(RAWOPS_INVALID_ARGUMENT, "guided backward requires radius 1..32 and finite epsilon [2^-24,65536]");
+        inside_or_throw({0, 0, raster->width, raster->height}, viewport);
+        if (!g_out || !g_in || !g_eps) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        if (!g_stride) g_stride = viewport.width;
+        GuidedBackwardLaunch L{};
+        L.src = raster->pixels;
+        L.src_stride = raster->stride;
+        L.width = raster->width;
+        L.height = raster->height;
+        L.radius = radius;
+        L.epsilon = epsilon;
+        const auto w = working_y_weights(raster->space);
+        L.wr = w[0];
+        L.wb = w[1];
+        L.g_out = g_out;
+        L.g_stride = g_stride;
+        L.viewport = viewport;
+        L.g_in = g_in;
+        L.g_eps = g_eps;
+        auto st = static_cast<hipStream_t>(stream);
+        const size_t need = guided_backward_workspace_bytes(raster->width, raster->height);
+        void* owned = nullptr;
+        if (workspace) {
+            if (workspace_bytes < need) fail(RAWOPS_INVALID_ARGUMENT, "guided backward workspace is too small");
+            L.workspace = workspace;
+        } else {
+            hip_check(hipMallocAsync(&owned, need, st), "hipMallocAsync(guided backward workspace)");
+            L.workspace = owned;
+        }
+        const hipError_t e = launch_guided_backward(L, st);
+        if (owned) (void)hipFreeAsync(owned, st);
+        hip_check(e, "guided backward launch");
     });
 }
 

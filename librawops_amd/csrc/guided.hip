@@ -291,6 +291,184 @@ __global__ void gf_identity(const GuidedArgs g) {
     if (err && g.flags) atomicOr(g.flags, err);
 }
 
+
+// ------------------------------------------------------------- backward --
+//
+// Reverse mode of q_i = mean_{k in W_i}(a_k) Y_i + mean_{k in W_i}(b_k) with
+// the per-centre statistics recomputed from the input (not stored by the
+// forward).  Pass 1 (per centre k): box sums of the 8 input moments and of
+// gq_i Y_i / n_i, gq_i / n_i give a_k, b_k and their adjoints, pushed back to
+// the moments: P_k = dmu_p / n_k, Q_k = dE[Yp] / n_k, R_k = dmu_Y / n_k,
+// S_k = dE[Y^2] / n_k (and a_k for the direct Y term).  Pass 2 (per pixel
+// j): box sums of those 11 values give dL/dp_j.  Moments are unshifted FP64.
+
+struct GuidedBackArgs {
+    const float* p;          // raster, level 0 (W x H, stride)
+    uint32_t stride;
+    int W, H, r;
+    double eps, wr, wb;
+    const float* gq;         // dL/dq over the viewport
+    uint32_t gstride;
+    int vx, vy, vw, vh;
+    double* ws;              // [H][W][11] per-centre adjoint terms
+    float* gp;               // dL/dp, W x H x 3 (written in full)
+    double* geps;            // accumulated dL/deps
+};
+
+template <int N, class Src, class Sink>
+__device__ __forceinline__ void strip_box(int LW, int LH, int r, int strip_x, int y0, int y1, double* prefix,
+                                          double* totals, Src src, Sink sink) {
+    const int t = threadIdx.x;
+    const int span = kStripThreads - 2 * r;
+    const int col = strip_x - r + t;
+    const bool col_ok = col >= 0 && col < LW;
+    double vs[N];
+#pragma unroll
+    for (int m = 0; m < N; ++m) vs[m] = 0.0;
+    if (col_ok)
+        for (int y = max(0, y0 - r); y < min(LH, y0 + r + 1); ++y) {
+            double v[N];
+            src(col, y, v);
+#pragma unroll
+            for (int m = 0; m < N; ++m) vs[m] += v[m];
+        }
+    for (int k = y0; k < y1; ++k) {
+        if (k > y0 && col_ok) {
+            if (k + r < LH) {
+                double v[N];
+                src(col, k + r, v);
+#pragma unroll
+                for (int m = 0; m < N; ++m) vs[m] += v[m];
+            }
+            if (k - r - 1 >= 0) {
+                double v[N];
+                src(col, k - r - 1, v);
+#pragma unroll
+                for (int m = 0; m < N; ++m) vs[m] -= v[m];
+            }
+        }
+        double ps[N];
+#pragma unroll
+        for (int m = 0; m < N; ++m) ps[m] = vs[m];
+        strip_scan<N>(ps, totals);
+#pragma unroll
+        for (int m = 0; m < N; ++m) prefix[m * kStripThreads + t] = ps[m];
+        __syncthreads();
+        const int x = strip_x + t;
+        if (t < span && x < LW) {
+            const int lo = max(0, x - r) - (strip_x - r);
+            const int hi = min(LW, x + r + 1) - (strip_x - r);
+            const int rows = min(LH, k + r + 1) - max(0, k - r);
+            double S[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+                S[m] = prefix[m * kStripThreads + hi - 1] - (lo > 0 ? prefix[m * kStripThreads + lo - 1] : 0.0);
+            sink(x, k, S, (double)((hi - lo) * rows));
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ double window_count(int x, int y, int W, int H, int r) {
+    return (double)((min(W, x + r + 1) - max(0, x - r)) * (min(H, y + r + 1) - max(0, y - r)));
+}
+
+__global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBackArgs g) {
+    __shared__ double prefix[kStripThreads * 14];
+    __shared__ double totals[4 * 14];
+    __shared__ double eps_part[kStripThreads / 64];
+    const int span = kStripThreads - 2 * g.r;
+    const int strip_x = (int)blockIdx.x * span;
+    const int y0 = (int)blockIdx.y * kBandRows, y1 = min(y0 + kBandRows, g.H);
+    double geps = 0.0;
+    auto src = [&](int x, int y, double v[14]) {
+        const float* q = g.p + ((size_t)y * g.stride + (size_t)x) * 3;
+        const double pr = q[0], pg = q[1], pb = q[2];
+        const double Y = (pg + g.wr * (pr - pg)) + g.wb * (pb - pg);
+        v[0] = Y; v[1] = pr; v[2] = pg; v[3] = pb;
+        v[4] = Y * Y; v[5] = Y * pr; v[6] = Y * pg; v[7] = Y * pb;
+        const int lx = x - g.vx, ly = y - g.vy;
+        if (lx >= 0 && ly >= 0 && lx < g.vw && ly < g.vh) {
+            const float* d = g.gq + ((size_t)ly * g.gstride + (size_t)lx) * 3;
+            const double n = window_count(x, y, g.W, g.H, g.r);
+            for (int c = 0; c < 3; ++c) { v[8 + c] = (double)d[c] * Y / n; v[11 + c] = (double)d[c] / n; }
+        } else {
+            for (int c = 0; c < 6; ++c) v[8 + c] = 0.0;
+        }
+    };
+    auto sink = [&](int x, int y, const double S[14], double n) {
+        const double my = S[0] / n, eyy = S[4] / n;
+        const double var = eyy - my * my, D = (var < 0 ? 0 : var) + g.eps;
+        double a[3], mp[3], ga[3], gmp[3], gcov[3];
+        double gmy = 0, gD = 0;
+        for (int c = 0; c < 3; ++c) {
+            mp[c] = S[1 + c] / n;
+            const double cov = S[5 + c] / n - my * mp[c];
+            a[c] = cov / D;
+            const double GA = S[8 + c], GB = S[11 + c];
+            gmp[c] = GB;
+            ga[c] = GA - my * GB;
+            gmy -= a[c] * GB;
+            gcov[c] = ga[c] / D;
+            gD -= ga[c] * a[c] / D;
+        }
+        for (int c = 0; c < 3; ++c) {
+            gmy -= gcov[c] * mp[c];
+            gmp[c] -= gcov[c] * my;
+        }
+        gmy -= 2.0 * my * gD;
+        geps += gD;
+        double* w = g.ws + ((size_t)y * g.W + (size_t)x) * 11;
+        for (int c = 0; c < 3; ++c) { w[c] = gmp[c] / n; w[3 + c] = gcov[c] / n; w[8 + c] = a[c]; }
+        w[6] = gmy / n;
+        w[7] = gD / n;
+    };
+    strip_box<14>(g.W, g.H, g.r, strip_x, y0, y1, prefix, totals, src, sink);
+    for (int off = 32; off > 0; off >>= 1) geps += __shfl_xor(geps, off, 64);
+    if ((threadIdx.x & 63) == 0) eps_part[threadIdx.x >> 6] = geps;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kStripThreads / 64; ++w) t += eps_part[w];
+        atomicAdd(g.geps, t);
+    }
+}
+
+__global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBackArgs g) {
+    __shared__ double prefix[kStripThreads * 11];
+    __shared__ double totals[4 * 11];
+    const int span = kStripThreads - 2 * g.r;
+    const int strip_x = (int)blockIdx.x * span;
+    const int y0 = (int)blockIdx.y * kBandRows, y1 = min(y0 + kBandRows, g.H);
+    auto src = [&](int x, int y, double v[11]) {
+        const double* w = g.ws + ((size_t)y * g.W + (size_t)x) * 11;
+#pragma unroll
+        for (int m = 0; m < 11; ++m) v[m] = w[m];
+    };
+    auto sink = [&](int x, int y, const double S[11], double n) {
+        const float* q = g.p + ((size_t)y * g.stride + (size_t)x) * 3;
+        const double p[3] = {q[0], q[1], q[2]};
+        const double Y = (p[1] + g.wr * (p[0] - p[1])) + g.wb * (p[2] - p[1]);
+        double gY = S[6] + 2.0 * S[7] * Y;
+        double gp[3];
+        for (int c = 0; c < 3; ++c) {
+            gp[c] = S[c] + S[3 + c] * Y;
+            gY += S[3 + c] * p[c];
+        }
+        const int lx = x - g.vx, ly = y - g.vy;
+        if (lx >= 0 && ly >= 0 && lx < g.vw && ly < g.vh) {
+            const float* d = g.gq + ((size_t)ly * g.gstride + (size_t)lx) * 3;
+            for (int c = 0; c < 3; ++c) gY += (double)d[c] * (S[8 + c] / n);
+        }
+        gp[0] += gY * g.wr;
+        gp[1] += gY * (1.0 - g.wr - g.wb);
+        gp[2] += gY * g.wb;
+        float* o = g.gp + ((size_t)y * g.W + (size_t)x) * 3;
+        o[0] = (float)gp[0]; o[1] = (float)gp[1]; o[2] = (float)gp[2];
+    };
+    strip_box<11>(g.W, g.H, g.r, strip_x, y0, y1, prefix, totals, src, sink);
+}
+
 }  // namespace
 
 size_t guided_workspace_bytes(const GuidedLaunch& L) {
@@ -301,6 +479,36 @@ size_t guided_workspace_bytes(const GuidedLaunch& L) {
     const int cx1 = std::min(LW, (int)(L.viewport.x + L.viewport.width) + r);
     const int cy1 = std::min(LH, (int)(L.viewport.y + L.viewport.height) + r);
     return (size_t)(cx1 - cx0) * (size_t)(cy1 - cy0) * 6 * sizeof(double);
+}
+
+size_t guided_backward_workspace_bytes(uint32_t width, uint32_t height) {
+    return (size_t)width * height * 11 * sizeof(double);
+}
+
+hipError_t launch_guided_backward(const GuidedBackwardLaunch& L, hipStream_t stream) {
+    GuidedBackArgs g{};
+    g.p = L.src;
+    g.stride = L.src_stride;
+    g.W = (int)L.width;
+    g.H = (int)L.height;
+    g.r = (int)L.radius;
+    g.eps = L.epsilon;
+    g.wr = L.wr;
+    g.wb = L.wb;
+    g.gq = L.g_out;
+    g.gstride = L.g_stride;
+    g.vx = (int)L.viewport.x; g.vy = (int)L.viewport.y;
+    g.vw = (int)L.viewport.width; g.vh = (int)L.viewport.height;
+    g.ws = static_cast<double*>(L.workspace);
+    g.gp = L.g_in;
+    g.geps = L.g_eps;
+    const int span = kStripThreads - 2 * g.r;
+    dim3 grid((unsigned)((g.W + span - 1) / span), (unsigned)((g.H + kBandRows - 1) / kBandRows));
+    hipLaunchKernelGGL(gf_back_centres, grid, dim3(kStripThreads), 0, stream, g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(gf_back_pixels, grid, dim3(kStripThreads), 0, stream, g);
+    return hipGetLastError();
 }
 
 hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t stream) {

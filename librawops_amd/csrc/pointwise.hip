@@ -1,0 +1,174 @@
+// Per-element forward/adjoint kernels of the differentiable chain: sRGB
+// encode (SrgbEncodeNode, RawEngine.cpp:1266-1274), exposure
+// (ExposureNode, RawEngine.cpp:494-498), the guided-filter detail boost
+// y = x + s (x - GF(x)), and the master tone curve of ExtendedCurvesNode
+// (ToneOps.cpp:820-831) with gradients for its knot ordinates.
+#include "device_common.hpp"
+#include "render_launch.hpp"
+
+#include <algorithm>
+
+namespace rawops {
+namespace {
+
+constexpr int kPW = 256;
+
+__device__ __forceinline__ void block_add(double v, double* lds_part, double* out) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) lds_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kPW / 64; ++w) t += lds_part[w];
+        atomicAdd(out, t);
+    }
+}
+
+__global__ void srgb_fwd(const float* x, float* y, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kPW + threadIdx.x; i < n; i += (size_t)gridDim.x * kPW)
+        y[i] = srgb_encode(x[i]);
+}
+
+// d/dx of clamp01 then 12.92 x | 1.055 x^(1/2.4) - 0.055.
+__global__ void srgb_bwd(const float* x, const float* gy, float* gx, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kPW + threadIdx.x; i < n; i += (size_t)gridDim.x * kPW) {
+        const float v = x[i];
+        float d = 0.0f;
+        if (v > 0.0f && v < 1.0f) d = v <= 0.0031308f ? 12.92f : 1.055f / 2.4f * powf(v, 1.0f / 2.4f - 1.0f);
+        gx[i] = gy[i] * d;
+    }
+}
+
+__global__ void exposure_bwd(const float* x, const float* gy, float m, float* gx, double* g_stops, size_t n) {
+    __shared__ double part[kPW / 64];
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)kPW + threadIdx.x; i < n; i += (size_t)gridDim.x * kPW) {
+        gx[i] = gy[i] * m;
+        acc += (double)gy[i] * (double)x[i] * (double)m;
+    }
+    block_add(acc * 0.69314718055994530942, part, g_stops);   // d(2^s)/ds = ln2 * 2^s
+}
+
+__global__ void boost_fwd(const float* x, const float* base, float s, float* y, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kPW + threadIdx.x; i < n; i += (size_t)gridDim.x * kPW)
+        y[i] = x[i] + s * (x[i] - base[i]);
+}
+
+__global__ void boost_bwd(const float* x, const float* base, const float* gy, float s, float* gx, float* gbase,
+                          double* gs, size_t n) {
+    __shared__ double part[kPW / 64];
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)kPW + threadIdx.x; i < n; i += (size_t)gridDim.x * kPW) {
+        const float g = gy[i];
+        gx[i] = g * (1.0f + s);
+        gbase[i] = -s * g;
+        acc += (double)g * ((double)x[i] - (double)base[i]);
+    }
+    block_add(acc, part, gs);
+}
+
+// Master-curve adjoint.  For the segment the value falls in, the cubic is the
+// Bezier B(t) = sum_k b_k(t) P_k with P = (y_i, y_i + h T_i/3, y_i+1 - h T_i+1/3,
+// y_i+1); tangents T are the minmod of the adjacent secant slopes.  dL/dy_j
+// collects the Bernstein weights through P and T; dL/dx uses B'(t)/h.
+__global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevCurve* c, uint32_t cubic, float* gx,
+                          double* gys) {
+    __shared__ double acc[kMaxKnots];
+    for (int k = threadIdx.x; k < kMaxKnots; k += kPW) acc[k] = 0.0;
+    __syncthreads();
+    const uint32_t nk = c->n;
+    for (size_t e = blockIdx.x * (size_t)kPW + threadIdx.x; e < n; e += (size_t)gridDim.x * kPW) {
+        const double v = xv[e], g = gy[e];
+        uint32_t lo = 0, hi = nk;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (v < c->x[mid]) hi = mid; else lo = mid + 1;
+        }
+        const uint32_t left = lo == 0 ? 0u : lo - 1u;
+        const uint32_t i = left < nk - 2u ? left : nk - 2u;
+        const double h = c->x[i + 1] - c->x[i];
+        if (v == c->x[left]) {            // exact knot: value y_left
+            atomicAdd(&acc[left], g);
+            double slope = c->slope[left < nk - 1 ? left : nk - 2];   // linear: right-hand secant
+            if (cubic) {                                                // cubic: the knot tangent
+                slope = left < nk - 1 ? 3.0 * (c->c0[left] - c->y[left]) / (c->x[left + 1] - c->x[left])
+                                      : 3.0 * (c->y[left] - c->c1[left - 1]) / (c->x[left] - c->x[left - 1]);
+            }
+            gx[e] = (float)(g * slope);
+            continue;
+        }
+        if (!cubic || v < c->x[0] || v > c->x[nk - 1]) {
+            // y = y_left + (v - x_left) * s_i,  s_i = (y_i+1 - y_i) / h
+            const double dv = v - c->x[left];
+            atomicAdd(&acc[left], g);
+            atomicAdd(&acc[i + 1], g * dv / h);
+            atomicAdd(&acc[i], -g * dv / h);
+            gx[e] = (float)(g * c->slope[i]);
+            continue;
+        }
+        const double t = (v - c->x[i]) / h, u = 1.0 - t;
+        const double b0 = u * u * u, b1 = 3 * u * u * t, b2 = 3 * u * t * t, b3 = t * t * t;
+        const double P0 = c->y[i], P1 = c->c0[i], P2 = c->c1[i], P3 = c->y[i + 1];
+        gx[e] = (float)(g * 3.0 * (u * u * (P1 - P0) + 2 * u * t * (P2 - P1) + t * t * (P3 - P2)) / h);
+        atomicAdd(&acc[i], g * (b0 + b1));
+        atomicAdd(&acc[i + 1], g * (b2 + b3));
+        // Tangent T_j -> secant slopes (minmod picks one of them, or none).
+        auto tangent = [&](uint32_t j, double w) {   // w = dL/dT_j
+            if (j == 0) { atomicAdd(&acc[1], w / (c->x[1] - c->x[0])); atomicAdd(&acc[0], -w / (c->x[1] - c->x[0])); return; }
+            if (j == nk - 1) {
+                const double hh = c->x[nk - 1] - c->x[nk - 2];
+                atomicAdd(&acc[nk - 1], w / hh); atomicAdd(&acc[nk - 2], -w / hh); return;
+            }
+            const double a = c->slope[j - 1], b = c->slope[j];
+            if (a == 0 || b == 0 || signbit(a) != signbit(b)) return;
+            const uint32_t k = fabs(a) <= fabs(b) ? j - 1 : j;
+            const double hh = c->x[k + 1] - c->x[k];
+            atomicAdd(&acc[k + 1], w / hh);
+            atomicAdd(&acc[k], -w / hh);
+        };
+        tangent(i, g * b1 * h / 3.0);
+        tangent(i + 1, -g * b2 * h / 3.0);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (int)nk; k += kPW) atomicAdd(gys + k, acc[k]);
+}
+
+unsigned blocks_for(size_t n) { return (unsigned)std::min<size_t>((n + kPW - 1) / kPW, 8192); }
+
+}  // namespace
+
+hipError_t launch_srgb(const float* x, float* y, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(srgb_fwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, y, n);
+    return hipGetLastError();
+}
+hipError_t launch_srgb_backward(const float* x, const float* gy, float* gx, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(srgb_bwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, gx, n);
+    return hipGetLastError();
+}
+hipError_t launch_exposure_backward(const float* x, const float* gy, float m, float* gx, double* g_stops, size_t n,
+                                    hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(exposure_bwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, m, gx, g_stops, n);
+    return hipGetLastError();
+}
+hipError_t launch_boost(const float* x, const float* base, float strength, float* y, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(boost_fwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, base, strength, y, n);
+    return hipGetLastError();
+}
+hipError_t launch_boost_backward(const float* x, const float* base, const float* gy, float strength, float* gx,
+                                 float* gbase, double* gs, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(boost_bwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, base, gy, strength, gx, gbase, gs, n);
+    return hipGetLastError();
+}
+hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t cubic,
+                                 float* gx, double* gys, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(curve_bwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
+    return hipGetLastError();
+}
+
+}  // namespace rawops

@@ -24,6 +24,20 @@ struct RenderLaunch {
 hipError_t launch_render(const RenderLaunch& launch, const OpList* native, const OpList* reduced,
                          hipStream_t stream);
 
+// Adjoint of out = M * (m * (g (.) demosaic(raw))) over a native viewport.
+struct BackwardLaunch {
+    SensorArgs sensor;
+    uint32_t demosaic;
+    rawops_rect viewport;
+    float gains[3], multiplier, matrix[9];
+    const float* g_out;      // dL/dout, viewport RGB
+    uint32_t g_stride;       // pixels per g_out row
+    float* g_raw;            // dL/d(normalised raw), active-area plane (written in full)
+    double* g_params;        // 13 accumulated: gains[3], multiplier, matrix[9]
+};
+
+hipError_t launch_render_backward(const BackwardLaunch& launch, hipStream_t stream);
+
 struct GuidedLaunch {
     const float* src;        // raster pixels (native resolution)
     uint32_t src_stride;     // pixels per row
@@ -40,6 +54,33 @@ struct GuidedLaunch {
 };
 
 size_t guided_workspace_bytes(const GuidedLaunch& launch);
+
+// Reverse mode of the working-Y guided filter at level 0 over the whole
+// raster (radius >= 1).
+struct GuidedBackwardLaunch {
+    const float* src;        // raster p
+    uint32_t src_stride, width, height;
+    uint32_t radius;
+    double epsilon, wr, wb;
+    const float* g_out;      // dL/dq over viewport
+    uint32_t g_stride;
+    rawops_rect viewport;
+    float* g_in;             // dL/dp, width x height x 3
+    double* g_eps;           // accumulated
+    void* workspace;         // guided_backward_workspace_bytes()
+};
+size_t guided_backward_workspace_bytes(uint32_t width, uint32_t height);
+hipError_t launch_guided_backward(const GuidedBackwardLaunch& launch, hipStream_t stream);
 hipError_t launch_guided(const GuidedLaunch& launch, const OpList* post, hipStream_t stream);
+
+hipError_t launch_srgb(const float* x, float* y, size_t n, hipStream_t s);
+hipError_t launch_srgb_backward(const float* x, const float* gy, float* gx, size_t n, hipStream_t s);
+hipError_t launch_exposure_backward(const float* x, const float* gy, float m, float* gx, double* g_stops, size_t n,
+                                    hipStream_t s);
+hipError_t launch_boost(const float* x, const float* base, float strength, float* y, size_t n, hipStream_t s);
+hipError_t launch_boost_backward(const float* x, const float* base, const float* gy, float strength, float* gx,
+                                 float* gbase, double* gs, size_t n, hipStream_t s);
+hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t cubic,
+                                 float* gx, double* gys, hipStream_t s);
 
 }  // namespace rawops

@@ -408,3 +408,53 @@ def guided_chain(raster, space, radius, eps, viewport, stops=0.0, knots=None, cu
     if knots is not None:
         curves(out, knots, cubic)
     return out
+
+
+# ------------------------------------------------------- FP64 test entries --
+
+def _lib64():
+    L = lib()
+    if not getattr(L, "_f64_ready", False):
+        P = ctypes.c_void_p
+        L.ro_menon_f64.argtypes = [P, P, Rect, P, P, P]
+        L.ro_menon_f64.restype = ctypes.c_int
+        L.ro_guided_filter_f64.argtypes = [P, Rect, Rect, Rect, ctypes.c_uint32, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, P]
+        L.ro_curve_map_f64.argtypes = [P, P, ctypes.c_uint32, ctypes.c_int, P, P, ctypes.c_size_t]
+        L._f64_ready = True
+    return L
+
+
+def menon_f64(meta: Bayer, raw, rect, hz=None):
+    """Menon on an FP64 normalised-sample plane (area-relative, ah x aw);
+    returns (out (h, w, 3) float64, decisions (ah, aw) uint8)."""
+    L = _lib64()
+    raw = np.ascontiguousarray(raw, np.float64)
+    out = np.zeros((rect[3], rect[2], 3), np.float64)
+    dec = np.zeros((meta.area[3], meta.area[2]), np.uint8)
+    hz_in = np.ascontiguousarray(hz, np.uint8) if hz is not None else None
+    L.ro_menon_f64(ctypes.byref(meta.struct()), _p(raw), Rect(*rect),
+                   _p(hz_in) if hz_in is not None else None, _p(dec), _p(out))
+    return out, dec
+
+
+def guided_filter_f64(inp, radius, eps, space, viewport=None):
+    """FP64 ordered guided filter over a whole (H, W, 3) float64 image."""
+    L = _lib64()
+    H, W = inp.shape[:2]
+    wr, wb = WEIGHTS[space]
+    vp = viewport or (0, 0, W, H)
+    out = np.zeros((vp[3], vp[2], 3), np.float64)
+    L.ro_guided_filter_f64(_p(np.ascontiguousarray(inp, np.float64)), Rect(0, 0, W, H), Rect(*vp),
+                           Rect(0, 0, W, H), radius, eps, wr, wb, _p(out))
+    return out
+
+
+def curve_map_f64(knots, values, cubic=True):
+    L = _lib64()
+    xs = np.asarray([k[0] for k in knots], np.float64)
+    ys = np.asarray([k[1] for k in knots], np.float64)
+    v = np.ascontiguousarray(values, np.float64)
+    out = np.zeros_like(v)
+    L.ro_curve_map_f64(_p(xs), _p(ys), len(xs), int(cubic), _p(v), _p(out), v.size)
+    return out

@@ -116,7 +116,16 @@ static dplane mkplane(box b) {
 /* One output block, MenonDemosaic.cpp:89-174.  The raw plane is stored as
  * float values widened to double: the reference keeps it as float and widens
  * on every read, which is the same value. */
-static void menon_block(const ro_bayer* m, const uint16_t* s, box r, ro_rect out_rect, float* out) {
+/* FP64 test entry inputs: raw values and frozen decisions, area-relative planes. */
+typedef struct {
+    const double* raw;      /* normalised samples (aw x ah) replacing the codes, or NULL */
+    const uint8_t* hz_in;   /* classifier decisions to use (1 = horizontal), or NULL */
+    uint8_t* hz_out;        /* decisions taken, or NULL */
+    double* outd;           /* FP64 output (request-relative), or NULL for float out */
+} menon_io;
+
+static void menon_block(const ro_bayer* m, const uint16_t* s, box r, ro_rect out_rect, float* out,
+                        const menon_io* io) {
     const ro_rect a = m->area;
     dplane raw = mkplane(grow(r, a, 6, 6, 6, 6));
     const box dirb = grow(r, a, 4, 4, 4, 4);
@@ -129,7 +138,9 @@ static void menon_block(const ro_bayer* m, const uint16_t* s, box r, ro_rect out
     int64_t x, y;
 
     for (y = raw.b.y; y < raw.b.y + raw.b.h; ++y)
-        for (x = raw.b.x; x < raw.b.x + raw.b.w; ++x) *dat(&raw, x, y) = (double)normalized(m, s, x, y);
+        for (x = raw.b.x; x < raw.b.x + raw.b.w; ++x)
+            *dat(&raw, x, y) = io && io->raw ? io->raw[(size_t)(y - a.y) * a.width + (size_t)(x - a.x)]
+                                             : (double)normalized(m, s, x, y);
 
     for (y = dirb.y; y < dirb.y + dirb.h; ++y)
         for (x = dirb.x; x < dirb.x + dirb.w; ++x) {
@@ -171,7 +182,10 @@ static void menon_block(const ro_bayer* m, const uint16_t* s, box r, ro_rect out
                 h += w * (inside(&a, hx, hy) ? *dat(&dh, hx, hy) : 0.0);
                 v += w * (inside(&a, vx, vy) ? *dat(&dv, vx, vy) : 0.0);
             }
-            const int hz = v >= h;
+            int hz = v >= h;
+            const size_t ai = (size_t)(y - a.y) * a.width + (size_t)(x - a.x);
+            if (io && io->hz_in) hz = io->hz_in[ai] != 0;
+            if (io && io->hz_out) io->hz_out[ai] = (uint8_t)hz;
             *dat(&horiz, x, y) = hz ? 1.0 : 0.0;
             *dat(&green, x, y) = hz ? *dat(&gh, x, y) : *dat(&gv, x, y);
         }
@@ -198,10 +212,14 @@ static void menon_block(const ro_bayer* m, const uint16_t* s, box r, ro_rect out
             double rr = *dat(&red, x, y), bb = *dat(&blue, x, y);
             if (own == 2) rr = bb + MEAN(&red, x, y, dx, dy) - MEAN(&blue, x, y, dx, dy);
             if (own == 0) bb = rr + MEAN(&blue, x, y, dx, dy) - MEAN(&red, x, y, dx, dy);
-            float* px = out + ((size_t)(y - out_rect.y) * out_rect.width + (size_t)(x - out_rect.x)) * 3;
-            px[0] = (float)rr;
-            px[1] = (float)*dat(&green, x, y);
-            px[2] = (float)bb;
+            const size_t o = ((size_t)(y - out_rect.y) * out_rect.width + (size_t)(x - out_rect.x)) * 3;
+            if (io && io->outd) {
+                io->outd[o] = rr; io->outd[o + 1] = *dat(&green, x, y); io->outd[o + 2] = bb;
+                continue;
+            }
+            out[o] = (float)rr;
+            out[o + 1] = (float)*dat(&green, x, y);
+            out[o + 2] = (float)bb;
         }
 #undef MEAN
     free(raw.v); free(gh.v); free(gv.v); free(ch.v); free(cv.v);
@@ -209,7 +227,8 @@ static void menon_block(const ro_bayer* m, const uint16_t* s, box r, ro_rect out
 }
 
 /* render_menon_base MenonDemosaic.cpp:177-218: 256x256 output blocks. */
-static void menon_blocks(const ro_bayer* m, const uint16_t* s, ro_rect r, ro_rect out_rect, float* out) {
+static void menon_blocks(const ro_bayer* m, const uint16_t* s, ro_rect r, ro_rect out_rect, float* out,
+                         const menon_io* io) {
     const uint32_t B = 256;
     for (uint64_t by = r.y; by < (uint64_t)r.y + r.height; by += B)
         for (uint64_t bx = r.x; bx < (uint64_t)r.x + r.width; bx += B) {
@@ -217,14 +236,23 @@ static void menon_blocks(const ro_bayer* m, const uint16_t* s, ro_rect r, ro_rec
             b.x = (int64_t)bx; b.y = (int64_t)by;
             b.w = (int64_t)(((uint64_t)r.x + r.width - bx) < B ? ((uint64_t)r.x + r.width - bx) : B);
             b.h = (int64_t)(((uint64_t)r.y + r.height - by) < B ? ((uint64_t)r.y + r.height - by) : B);
-            menon_block(m, s, b, out_rect, out);
+            menon_block(m, s, b, out_rect, out, io);
         }
 }
 
 int ro_demosaic_menon(const ro_bayer* m, const uint16_t* s, ro_rect r, float* out) {
     if (m->area.width <= 1 || m->area.height <= 1) return -1;
     if (!r.width || !r.height) return 0;
-    menon_blocks(m, s, r, r, out);
+    menon_blocks(m, s, r, r, out, NULL);
+    return 0;
+}
+
+int ro_menon_f64(const ro_bayer* m, const double* raw, ro_rect r, const uint8_t* hz_in, uint8_t* hz_out,
+                 double* out) {
+    if (m->area.width <= 1 || m->area.height <= 1) return -1;
+    if (!r.width || !r.height) return 0;
+    menon_io io = {raw, hz_in, hz_out, out};
+    menon_blocks(m, NULL, r, r, NULL, &io);
     return 0;
 }
 
@@ -241,7 +269,7 @@ int ro_demosaic_menon_bands(const ro_bayer* m, const uint16_t* s, ro_rect r, flo
         if (y0 >= r.height) continue;
         const uint32_t h = r.height - y0 < band ? r.height - y0 : band;
         ro_rect sub = {r.x, r.y + y0, r.width, h};
-        menon_blocks(m, s, sub, r, out);
+        menon_blocks(m, s, sub, r, out, NULL);
     }
     return 0;
 }
@@ -501,6 +529,79 @@ static int gf_rows(const float* in, ro_rect needed, ro_rect output, ro_rect imag
 #undef YDELTA
     free(table);
     return status;
+}
+
+/* FP64 I/O variant for finite differences: same ordered algorithm, the input
+ * rounded to float32 is replaced by double values and the output is not
+ * rounded (SpatialOps.cpp:168-287 arithmetic, without the float32 casts). */
+int ro_guided_filter_f64(const double* in, ro_rect needed, ro_rect output, ro_rect image,
+                         uint32_t rad, double eps, double wr, double wb, double* out) {
+    const ro_rect centers = clip_grow(output, image, rad);
+    gf_coef* table = (gf_coef*)malloc(sizeof(gf_coef) * (size_t)centers.width * centers.height);
+#define PIXD(X, Y) (in + ((size_t)((Y) - needed.y) * needed.width + (size_t)((X) - needed.x)) * 3)
+#define YD(D) (((D)[1] + wr * ((D)[0] - (D)[1])) + wb * ((D)[2] - (D)[1]))
+    for (uint64_t ky = centers.y; ky < (uint64_t)centers.y + centers.height; ++ky)
+        for (uint64_t kx = centers.x; kx < (uint64_t)centers.x + centers.width; ++kx) {
+            ro_rect one = {(uint32_t)kx, (uint32_t)ky, 1, 1};
+            const ro_rect w = clip_grow(one, image, rad);
+            const double count = (double)((uint64_t)w.width * w.height);
+            const double* pk = PIXD(kx, ky);
+            double sg = 0, sp[3] = {0, 0, 0};
+            for (uint64_t y = w.y; y < (uint64_t)w.y + w.height; ++y)
+                for (uint64_t x = w.x; x < (uint64_t)w.x + w.width; ++x) {
+                    const double* pj = PIXD(x, y);
+                    double d[3] = {pj[0] - pk[0], pj[1] - pk[1], pj[2] - pk[2]};
+                    sg += YD(d);
+                    for (int c = 0; c < 3; ++c) sp[c] += d[c];
+                }
+            gf_coef co;
+            co.my = sg / count;
+            for (int c = 0; c < 3; ++c) co.mp[c] = sp[c] / count;
+            double vv = 0, cov[3] = {0, 0, 0};
+            for (uint64_t y = w.y; y < (uint64_t)w.y + w.height; ++y)
+                for (uint64_t x = w.x; x < (uint64_t)w.x + w.width; ++x) {
+                    const double* pj = PIXD(x, y);
+                    double d[3] = {pj[0] - pk[0], pj[1] - pk[1], pj[2] - pk[2]};
+                    const double dy = YD(d) - co.my;
+                    vv += dy * dy;
+                    for (int c = 0; c < 3; ++c) cov[c] += dy * (d[c] - co.mp[c]);
+                }
+            const double denominator = vv / count + eps;
+            for (int c = 0; c < 3; ++c) co.slope[c] = (cov[c] / count) / denominator;
+            table[(size_t)(ky - centers.y) * centers.width + (size_t)(kx - centers.x)] = co;
+        }
+    for (uint64_t y = output.y; y < (uint64_t)output.y + output.height; ++y)
+        for (uint64_t x = output.x; x < (uint64_t)output.x + output.width; ++x) {
+            ro_rect one = {(uint32_t)x, (uint32_t)y, 1, 1};
+            const ro_rect w = clip_grow(one, image, rad);
+            const double count = (double)((uint64_t)w.width * w.height);
+            const double* pi = PIXD(x, y);
+            double corr[3] = {0, 0, 0};
+            for (uint64_t ky = w.y; ky < (uint64_t)w.y + w.height; ++ky)
+                for (uint64_t kx = w.x; kx < (uint64_t)w.x + w.width; ++kx) {
+                    const gf_coef* co = &table[(size_t)(ky - centers.y) * centers.width + (size_t)(kx - centers.x)];
+                    const double* pk = PIXD(kx, ky);
+                    double rev[3] = {pi[0] - pk[0], pi[1] - pk[1], pi[2] - pk[2]};
+                    const double guide = YD(rev) - co->my;
+                    for (int c = 0; c < 3; ++c) corr[c] += (pk[c] - pi[c]) + co->mp[c] + co->slope[c] * guide;
+                }
+            double* po = out + ((size_t)(y - output.y) * output.width + (size_t)(x - output.x)) * 3;
+            for (int c = 0; c < 3; ++c) po[c] = pi[c] + corr[c] / count;
+        }
+#undef PIXD
+#undef YD
+    free(table);
+    return 0;
+}
+
+/* Master curve map in FP64 (ExtendedCurvesNode::map ToneOps.cpp:820-831). */
+void ro_curve_map_f64(const double* xs, const double* ys, uint32_t n, int cubic, const double* in, double* out,
+                      size_t count) {
+    curve* c = (curve*)malloc(sizeof(curve));
+    curve_prepare(c, xs, ys, n, cubic);
+    c->identity = 0;
+    for (size_t i = 0; i < count; ++i) out[i] = curve_map(c, cubic, in[i]);
+    free(c);
 }
 
 int ro_guided_filter(const float* in, ro_rect needed, ro_rect output, ro_rect image,

@@ -51,6 +51,13 @@ int ro_demosaic_menon(const ro_bayer* m, const uint16_t* samples, ro_rect r, flo
 int ro_demosaic_menon_bands(const ro_bayer* m, const uint16_t* samples, ro_rect r,
                             float* out, int threads);
 
+/* FP64 test entry: Menon on double normalised samples (area-relative plane),
+ * optionally with frozen classifier decisions (area-relative, 1 =
+ * horizontal) and recording the decisions taken; FP64 output.  The taken
+ * branch is linear in the samples, which finite differences rely on. */
+int ro_menon_f64(const ro_bayer* m, const double* raw, ro_rect r, const uint8_t* hz_in, uint8_t* hz_out,
+                 double* out);
+
 /* WhiteBalanceNode::render RawEngine.cpp:417-431. */
 void ro_scale3(float* rgb, size_t pixels, const float gains[3]);
 /* ExposureNode::render_level RawEngine.cpp:488-500. */
@@ -87,6 +94,12 @@ int ro_curves(float* rgb, size_t pixels, const double* xs, const double* ys,
  * overflow/invalid conditions. */
 int ro_guided_filter(const float* input, ro_rect needed, ro_rect output, ro_rect image,
                      uint32_t radius, double epsilon, double wr, double wb, float* out);
+/* FP64 I/O variant (no float32 rounding of input or output) for finite differences. */
+int ro_guided_filter_f64(const double* input, ro_rect needed, ro_rect output, ro_rect image,
+                         uint32_t radius, double epsilon, double wr, double wb, double* out);
+/* Master curve in FP64 for finite differences (ToneOps.cpp:820-831). */
+void ro_curve_map_f64(const double* xs, const double* ys, uint32_t n, int cubic, const double* in, double* out,
+                      size_t count);
 /* Band-parallel variant of the above for timing (threads OpenMP workers). */
 int ro_guided_filter_bands(const float* input, ro_rect needed, ro_rect output, ro_rect image,
                            uint32_t radius, double epsilon, double wr, double wb, float* out,
