@@ -51,6 +51,11 @@ def cpu_baseline(meta, samples, cam, threads, rows=512):
                       f"{threads} OpenMP row bands, oracle C restatement (gcc -O2)"}
 
 
+def compiled_matrix(node):
+    """float32 camera->working coefficients the CameraToWorkingNode uses."""
+    return node.matrix
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +121,38 @@ def main():
     kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
     assert int(flags.item()) == 0
 
+    # fwd+bwd: the forward render plus the hand-written adjoint of the same
+    # chain (dL/d normalised raw over the whole active area + dL/d gains,
+    # exposure, matrix) for a synthetic upstream gradient.
+    from librawops_amd import _capi as C
+    import ctypes
+    gout = torch.randn((area.height, area.width, 3), dtype=torch.float32, device="cuda", generator=None)
+    graw = torch.zeros((area.height, area.width), dtype=torch.float32, device="cuda")
+    gpar = torch.zeros(13, dtype=torch.float64, device="cuda")
+    g3 = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
+    mm = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(compiled_matrix(node), np.float32)])
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def backward():
+        C.check(C.lib().rawops_render_backward(img._handle, C.DEMOSAIC_MENON, area.c(), g3, 1.0, mm,
+                                               ctypes.c_void_p(gout.data_ptr()), area.width,
+                                               ctypes.c_void_p(graw.data_ptr()), ctypes.c_void_p(gpar.data_ptr()), sp))
+    for _ in range(max(1, args.warmup)):
+        backward()
+    torch.cuda.synchronize()
+    bsteps = max(1, args.steps // 2)
+    b0 = [torch.cuda.Event(enable_timing=True) for _ in range(bsteps)]
+    b1 = [torch.cuda.Event(enable_timing=True) for _ in range(bsteps)]
+    t1 = time.perf_counter()
+    for i in range(bsteps):
+        DeviceRender.run(compiled, area, level, out=out, flags=flags, stream=stream)
+        b0[i].record(stream)
+        backward()
+        b1[i].record(stream)
+    torch.cuda.synchronize()
+    fb_wall = time.perf_counter() - t1
+    bwd_ms = sum(a.elapsed_time(b) for a, b in zip(b0, b1)) / bsteps
+
     px = area.width * area.height
     ms_per_step = wall * 1e3 / args.steps
     value = px * args.steps / wall / 1e6
@@ -143,8 +180,12 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_PX * px},
         "cpu_baseline": cpu,
     }
+    line["fwd_bwd"] = {"value": round(px * bsteps / fb_wall / 1e6, 2), "unit": "Mpix/s",
+                       "ms_per_step": round(fb_wall * 1e3 / bsteps, 4), "backward_kernel_ms": round(bwd_ms, 4),
+                       "backward": "menon_backward<64,32>: dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        line["fwd_bwd"]["gpu_fwd_bwd_over_cpu_fwd"] = round(line["fwd_bwd"]["value"] / cpu["value"], 1)
     print(json.dumps(line))
 
 

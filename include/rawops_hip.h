@@ -170,6 +170,42 @@ rawops_status rawops_render_tiles_host(const rawops_plan* plan, const rawops_sou
                                        uint32_t tile_size, rawops_tile_callback callback,
                                        void* user, const volatile int* cancel);
 
+/* ---- batched heterogeneous render (the TileScheduler caller's job,
+ * TileScheduler.cpp:19-246, as one persistent device launch) ---- */
+typedef struct {
+    const rawops_plan* plan;
+    const rawops_source* source;       /* RAW source */
+    rawops_rect viewport;              /* level coordinates */
+    uint32_t mip, quality;
+    uint32_t linear_ops;               /* reduced ops applied before dst_rgb; the rest precede dst_u8 */
+    float* dst_rgb;                    /* device, may be NULL */
+    uint8_t* dst_u8;                   /* device, may be NULL */
+    uint32_t dst_stride_pixels;        /* 0 = viewport width */
+} rawops_batch_request;
+
+typedef struct rawops_batch rawops_batch;
+
+/* Validates every request (as rawops_render does), builds the (request, tile)
+ * work list, picks the launch configuration (shape_index <
+ * rawops_batch_shape_count(): tile shape x resident workgroups per CU, or
+ * UINT32_MAX to autotune: every configuration is timed on this batch and the
+ * fastest kept) and captures reset + persistent kernel into a hipGraph
+ * (captured on a private stream, replayed on the caller's).  Requests are
+ * served smallest first. */
+rawops_status rawops_batch_create(const rawops_batch_request* requests, size_t count, uint32_t shape_index,
+                                  void* stream, rawops_batch** out);
+/* Replays the captured graph (asynchronous on `stream`). */
+rawops_status rawops_batch_run(rawops_batch* batch, void* stream);
+/* After the stream has completed a run: per-request completion latency (ms)
+ * from the start of the launch, in request order; and non-finite status. */
+rawops_status rawops_batch_latencies(rawops_batch* batch, double* latency_ms);
+rawops_status rawops_batch_check(rawops_batch* batch, void* stream);
+/* Chosen tile shape and its autotune time (ms per batch, 0 if not tuned). */
+rawops_status rawops_batch_shape(const rawops_batch* batch, uint32_t* tile_width, uint32_t* tile_height,
+                                 double* tuned_ms);
+uint32_t rawops_batch_shape_count(void);
+void rawops_batch_destroy(rawops_batch* batch);
+
 /* ---- adjoints (no reference counterpart: the reference has no derivatives) ---- */
 /* Reverse mode of the render chain out = M * (m * (g (.) demosaic(raw)))
  * (WhiteBalanceNode -> ExposureNode -> CameraToWorkingNode after the demosaic,

@@ -90,6 +90,12 @@ class PlanDesc(ctypes.Structure):
                 ("native_ops", ctypes.POINTER(OpDesc)), ("reduced_ops", ctypes.POINTER(OpDesc))]
 
 
+class BatchRequestC(ctypes.Structure):
+    _fields_ = [("plan", ctypes.c_void_p), ("source", ctypes.c_void_p), ("viewport", Rect),
+                ("mip", ctypes.c_uint32), ("quality", ctypes.c_uint32), ("linear_ops", ctypes.c_uint32),
+                ("dst_rgb", ctypes.c_void_p), ("dst_u8", ctypes.c_void_p), ("dst_stride_pixels", ctypes.c_uint32)]
+
+
 TILE_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, Rect, ctypes.POINTER(ctypes.c_float))
 
 
@@ -123,6 +129,13 @@ def lib():
             "rawops_render": ([P, P, Rect, u32, u32, P, P, u32, P, P], S),
             "rawops_check_flags": ([P, P], S),
             "rawops_render_tiles_host": ([P, P, Rect, u32, u32, u32, TILE_CALLBACK, P, P], S),
+            "rawops_batch_create": ([P, ctypes.c_size_t, u32, P, PP], S),
+            "rawops_batch_run": ([P, P], S),
+            "rawops_batch_latencies": ([P, P], S),
+            "rawops_batch_check": ([P, P], S),
+            "rawops_batch_shape": ([P, P, P, P], S),
+            "rawops_batch_shape_count": ([], u32),
+            "rawops_batch_destroy": ([P], None),
             "rawops_render_backward": ([P, u32, Rect, P, ctypes.c_float, P, P, u32, P, P, P], S),
             "rawops_guided_backward_workspace_size": ([u32, u32, P], S),
             "rawops_guided_backward": ([P, u32, ctypes.c_double, Rect, P, u32, P, P, P, ctypes.c_size_t, P], S),

@@ -1,0 +1,33 @@
+// Host/device description of one request of a batched launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace rawops {
+
+struct BatchItem {
+    SensorArgs sensor;
+    const OpList* native;
+    const OpList* reduced;
+    uint32_t tap;            // reduced ops before the FP32 output
+    int demosaic, mip;
+    int vx, vy, vw, vh, ox0, oy0;
+    int tiles_x, tiles, first;   // tiles of this request; first item number
+    float* out;
+    uint8_t* out8;
+    uint32_t ostride;
+    uint32_t* flags;
+};
+
+struct BatchShape { int tw, th, per_cu; };
+constexpr int kBatchShapeCount = 2;
+extern const BatchShape kBatchShapes[kBatchShapeCount];
+
+size_t batch_state_bytes();
+int batch_grid(int shape);
+hipError_t launch_batch(int shape, const BatchItem* items, int n, int total, void* state, uint32_t* remaining,
+                        unsigned long long* done, int grid, hipStream_t stream);
+
+}  // namespace rawops

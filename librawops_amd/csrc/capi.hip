@@ -15,6 +15,10 @@
 #include "../../include/rawops_hip.h"
 #include "color.hpp"
 #include "render_launch.hpp"
+#include "batch_launch.hpp"
+
+#include <algorithm>
+#include <numeric>
 
 using namespace rawops;
 
@@ -36,6 +40,27 @@ struct rawops_plan {
     ~rawops_plan() {
         for (void* t : tables) (void)hipFree(t);
         if (device) (void)hipFree(device);
+    }
+};
+
+struct rawops_batch {
+    std::vector<rawops::BatchItem> items;      // served order
+    std::vector<size_t> order;                 // served position -> request index
+    rawops::BatchItem* dev_items = nullptr;
+    void* state = nullptr;
+    uint32_t* remaining = nullptr;
+    unsigned long long* done = nullptr;
+    uint32_t* flags = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int shape = 0, total = 0, grid = 0;
+    double tuned_ms = 0.0;
+    ~rawops_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : {static_cast<void*>(dev_items), state, static_cast<void*>(remaining), static_cast<void*>(done),
+                        static_cast<void*>(flags)})
+            if (p) (void)hipFree(p);
     }
 };
 
@@ -459,6 +484,157 @@ rawops_status rawops_render_tiles_host(const rawops_plan* plan, const rawops_sou
             }
     });
 }
+
+static void batch_layout(rawops_batch& b, int shape) {
+    const auto sh = kBatchShapes[shape];
+    int first = 0;
+    for (auto& it : b.items) {
+        const int scale = 1 << it.mip;
+        const int lw = sh.tw / scale, lh = sh.th / scale;
+        it.tiles_x = (it.vw + lw - 1) / lw;
+        it.tiles = it.tiles_x * ((it.vh + lh - 1) / lh);
+        it.first = first;
+        first += it.tiles;
+    }
+    b.total = first;
+    b.shape = shape;
+    b.grid = batch_grid(shape);
+    hip_check(hipMemcpy(b.dev_items, b.items.data(), b.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice),
+              "hipMemcpy(batch)");
+}
+
+rawops_status rawops_batch_create(const rawops_batch_request* reqs, size_t count, uint32_t shape_index, void* stream,
+                                  rawops_batch** out) {
+    return guarded([&] {
+        if (!out || (!reqs && count)) fail(RAWOPS_INVALID_ARGUMENT, "batch requests are required");
+        if (!count) fail(RAWOPS_INVALID_ARGUMENT, "a batch needs at least one request");
+        if (shape_index != UINT32_MAX && shape_index >= uint32_t(kBatchShapeCount))
+            fail(RAWOPS_INVALID_ARGUMENT, "unknown batch tile shape");
+        auto b = std::make_unique<rawops_batch>();
+        b->order.resize(count);
+        std::iota(b->order.begin(), b->order.end(), size_t(0));
+        std::stable_sort(b->order.begin(), b->order.end(), [&](size_t x, size_t y) {
+            return uint64_t(reqs[x].viewport.width) * reqs[x].viewport.height <
+                   uint64_t(reqs[y].viewport.width) * reqs[y].viewport.height;
+        });
+        hip_check(hipMalloc(reinterpret_cast<void**>(&b->flags), sizeof(uint32_t)), "hipMalloc(flags)");
+        hip_check(hipMemset(b->flags, 0, sizeof(uint32_t)), "hipMemset(flags)");
+        for (size_t k = 0; k < count; ++k) {
+            const rawops_batch_request& r = reqs[b->order[k]];
+            validate_render(r.plan, r.source, r.viewport, r.mip, r.quality);
+            if (!r.viewport.width || !r.viewport.height) fail(RAWOPS_INVALID_ARGUMENT, "batch viewports must be nonempty");
+            BatchItem it{};
+            it.sensor = sensor_args(*r.source);
+            it.native = r.plan->device;
+            it.reduced = r.plan->device + 1;
+            it.tap = r.linear_ops;
+            it.demosaic = int(r.plan->demosaic);
+            if (r.source->meta.active_area.width <= 1 || r.source->meta.active_area.height <= 1)
+                it.demosaic = RAWOPS_DEMOSAIC_BILINEAR;
+            it.mip = int(r.mip);
+            it.vx = int(r.viewport.x); it.vy = int(r.viewport.y);
+            it.vw = int(r.viewport.width); it.vh = int(r.viewport.height);
+            const int scale = 1 << it.mip;
+            it.ox0 = it.mip ? it.sensor.ax + it.vx * scale : it.vx;
+            it.oy0 = it.mip ? it.sensor.ay + it.vy * scale : it.vy;
+            it.out = r.dst_rgb;
+            it.out8 = r.dst_u8;
+            it.ostride = r.dst_stride_pixels ? r.dst_stride_pixels : r.viewport.width;
+            if (it.ostride < r.viewport.width) fail(RAWOPS_INVALID_ARGUMENT, "output stride is smaller than the viewport");
+            it.flags = b->flags;
+            b->items.push_back(it);
+        }
+        const size_t n = b->items.size();
+        hip_check(hipMalloc(reinterpret_cast<void**>(&b->dev_items), n * sizeof(BatchItem)), "hipMalloc(items)");
+        hip_check(hipMalloc(&b->state, batch_state_bytes()), "hipMalloc(state)");
+        hip_check(hipMalloc(reinterpret_cast<void**>(&b->remaining), n * sizeof(uint32_t)), "hipMalloc(remaining)");
+        hip_check(hipMalloc(reinterpret_cast<void**>(&b->done), n * sizeof(unsigned long long)), "hipMalloc(done)");
+        auto st = static_cast<hipStream_t>(stream);
+        int shape = shape_index == UINT32_MAX ? 0 : int(shape_index);
+        if (shape_index == UINT32_MAX) {
+            // Autotune: time every compiled tile shape on this batch.
+            hipEvent_t e0, e1;
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            double best = 1e30;
+            for (int c = 0; c < kBatchShapeCount; ++c) {
+                batch_layout(*b, c);
+                hip_check(launch_batch(c, b->dev_items, int(n), b->total, b->state, b->remaining, b->done, b->grid, st),
+                          "batch launch");
+                hip_check(hipEventRecord(e0, st), "hipEventRecord");
+                for (int rep = 0; rep < 3; ++rep)
+                    hip_check(launch_batch(c, b->dev_items, int(n), b->total, b->state, b->remaining, b->done, b->grid, st),
+                              "batch launch");
+                hip_check(hipEventRecord(e1, st), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                if (ms / 3.0 < best) { best = ms / 3.0; shape = c; }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            b->tuned_ms = best;
+        }
+        batch_layout(*b, shape);
+        // Capture on a private stream (the legacy null stream cannot be
+        // captured); the graph is replayed on the caller's stream.
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        hipStream_t cap = nullptr;
+        hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        struct Destroy { hipStream_t s; ~Destroy() { (void)hipStreamDestroy(s); } } destroy{cap};
+        hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const hipError_t le = launch_batch(shape, b->dev_items, int(n), b->total, b->state, b->remaining, b->done,
+                                           b->grid, cap);
+        const hipError_t ce = hipStreamEndCapture(cap, &b->graph);
+        hip_check(le, "batch launch (capture)");
+        hip_check(ce, "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        *out = b.release();
+    });
+}
+
+rawops_status rawops_batch_run(rawops_batch* b, void* stream) {
+    return guarded([&] {
+        if (!b) fail(RAWOPS_INVALID_ARGUMENT, "batch is null");
+        hip_check(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+    });
+}
+
+rawops_status rawops_batch_latencies(rawops_batch* b, double* ms) {
+    return guarded([&] {
+        if (!b || !ms) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        const size_t n = b->items.size();
+        std::vector<unsigned long long> done(n);
+        struct { uint32_t next, pad; unsigned long long t0; } state{};
+        hip_check(hipMemcpy(done.data(), b->done, n * sizeof(unsigned long long), hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(&state, b->state, sizeof(state), hipMemcpyDeviceToHost), "hipMemcpy");
+        int dev = 0, khz = 100000;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+        for (size_t k = 0; k < n; ++k) ms[b->order[k]] = double(done[k] - state.t0) / double(khz);
+    });
+}
+
+rawops_status rawops_batch_check(rawops_batch* b, void* stream) {
+    return guarded([&] {
+        if (!b) fail(RAWOPS_INVALID_ARGUMENT, "batch is null");
+        const rawops_status st = rawops_check_flags(b->flags, stream);
+        if (st != RAWOPS_OK) fail(st, g_error);
+    });
+}
+
+rawops_status rawops_batch_shape(const rawops_batch* b, uint32_t* tw, uint32_t* th, double* tuned_ms) {
+    return guarded([&] {
+        if (!b) fail(RAWOPS_INVALID_ARGUMENT, "batch is null");
+        if (tw) *tw = uint32_t(kBatchShapes[b->shape].tw);
+        if (th) *th = uint32_t(kBatchShapes[b->shape].th);
+        if (tuned_ms) *tuned_ms = b->tuned_ms;
+    });
+}
+
+uint32_t rawops_batch_shape_count(void) { return uint32_t(kBatchShapeCount); }
+
+void rawops_batch_destroy(rawops_batch* b) { delete b; }
 
 rawops_status rawops_render_backward(const rawops_source* src, uint32_t demosaic, rawops_rect viewport,
                                      const float gains[3], float multiplier, const float matrix[9],

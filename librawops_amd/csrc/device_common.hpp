@@ -140,8 +140,9 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
 }
 
 // The fused op program: each op rounds exactly like its reference node.
-__device__ __forceinline__ void apply_ops(const OpList& L, float& r, float& g, float& b, uint32_t& err) {
-    for (uint32_t i = 0; i < L.n; ++i) {
+__device__ __forceinline__ void apply_ops(const OpList& L, uint32_t begin, uint32_t end, float& r, float& g, float& b,
+                                          uint32_t& err) {
+    for (uint32_t i = begin; i < end; ++i) {
         const DevOp& o = L.op[i];
         switch (o.kind) {
         case OP_GAINS: r *= o.v[0]; g *= o.v[1]; b *= o.v[2]; break;
@@ -172,6 +173,10 @@ __device__ __forceinline__ void apply_ops(const OpList& L, float& r, float& g, f
         default: break;
         }
     }
+}
+
+__device__ __forceinline__ void apply_ops(const OpList& L, float& r, float& g, float& b, uint32_t& err) {
+    apply_ops(L, 0, L.n, r, g, b, err);
 }
 
 __device__ __forceinline__ uint8_t quantize_u8(float v) {
