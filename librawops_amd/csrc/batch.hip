@@ -10,7 +10,7 @@
 // replayed per batch.  The host-side autotuner times the compiled launch
 // configurations (tile shape x resident workgroups per CU) on the actual
 // batch and keeps the fastest.
-#include "render_tile.hpp"
+#include "render_tile_legacy.hpp"
 #include "batch_launch.hpp"
 
 namespace rawops {
@@ -18,7 +18,7 @@ namespace {
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // One LDS allocation large enough for both demosaicers at the 64x32 tile.
-constexpr int kBatchLds = cmax(Lds<RAWOPS_DEMOSAIC_MENON, 64, 32>::BYTES, Lds<RAWOPS_DEMOSAIC_BILINEAR, 64, 32>::BYTES);
+constexpr int kBatchLds = cmax(LegacyLds<RAWOPS_DEMOSAIC_MENON, 64, 32>::BYTES, LegacyLds<RAWOPS_DEMOSAIC_BILINEAR, 64, 32>::BYTES);
 
 struct BatchState {
     uint32_t next;
@@ -41,12 +41,12 @@ __device__ __forceinline__ void run_tile(const RenderArgs& a, int demosaic, int 
     const SensorArgs s = a.s;
     if (demosaic == RAWOPS_DEMOSAIC_MENON) {
         const bool interior = tx - 6 >= s.ax && ty - 6 >= s.ay && tx + TW + 6 <= s.ax + s.aw && ty + TH + 6 <= s.ay + s.ah;
-        if (interior) render_tile<RAWOPS_DEMOSAIC_MENON, TW, TH, false>(a, s, lds, tx, ty, lx0, ly0);
-        else render_tile<RAWOPS_DEMOSAIC_MENON, TW, TH, true>(a, s, lds, tx, ty, lx0, ly0);
+        if (interior) render_tile_legacy<RAWOPS_DEMOSAIC_MENON, TW, TH, false>(a, s, lds, tx, ty, lx0, ly0);
+        else render_tile_legacy<RAWOPS_DEMOSAIC_MENON, TW, TH, true>(a, s, lds, tx, ty, lx0, ly0);
     } else {
         const bool interior = tx - 1 >= s.ax && ty - 1 >= s.ay && tx + TW + 1 <= s.ax + s.aw && ty + TH + 1 <= s.ay + s.ah;
-        if (interior) render_tile<RAWOPS_DEMOSAIC_BILINEAR, TW, TH, false>(a, s, lds, tx, ty, lx0, ly0);
-        else render_tile<RAWOPS_DEMOSAIC_BILINEAR, TW, TH, true>(a, s, lds, tx, ty, lx0, ly0);
+        if (interior) render_tile_legacy<RAWOPS_DEMOSAIC_BILINEAR, TW, TH, false>(a, s, lds, tx, ty, lx0, ly0);
+        else render_tile_legacy<RAWOPS_DEMOSAIC_BILINEAR, TW, TH, true>(a, s, lds, tx, ty, lx0, ly0);
     }
 }
 

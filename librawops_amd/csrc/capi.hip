@@ -121,6 +121,19 @@ rawops_raw_metadata normalise(const rawops_raw_metadata* in, size_t count) {
     return m;
 }
 
+// RN(1/d) in float: the float nearest to 1/d (the product y * d is exact in
+// double for d < 2^16, so the comparison is exact).
+float nearest_reciprocal(uint32_t d) {
+    const float y = float(1.0 / double(d));
+    float best = y;
+    double err = std::abs(double(y) * d - 1.0);
+    for (const float c : {std::nextafter(y, 0.0f), std::nextafter(y, 2.0f)}) {
+        const double e = std::abs(double(c) * d - 1.0);
+        if (e < err) { err = e; best = c; }
+    }
+    return best;
+}
+
 SensorArgs sensor_args(const rawops_source& src) {
     static const int cells[4][4] = {{0, 1, 1, 2}, {2, 1, 1, 0}, {1, 0, 2, 1}, {1, 2, 0, 1}};
     const auto& m = src.meta;
@@ -140,6 +153,7 @@ SensorArgs sensor_args(const rawops_source& src) {
     for (int site = 0; site < 4; ++site) {
         s.black[site] = float(m.black_levels[site]);
         s.denom[site] = float(int(m.white_levels[site]) - int(m.black_levels[site]));
+        s.rcp[site] = nearest_reciprocal(uint32_t(int(m.white_levels[site]) - int(m.black_levels[site])));
     }
     return s;
 }

@@ -51,6 +51,7 @@ struct SensorArgs {
     uint32_t phase_x, phase_y;
     uint32_t ng_parity;       // bit b: sensor x parity of the non-green sites in rows of parity b
     float black[4], denom[4]; // per site: black level, white - black
+    float rcp[4];             // per site: 1 / (white - black), correctly rounded to float
 };
 
 __device__ __forceinline__ uint32_t site_of(const SensorArgs& s, int x, int y) {
@@ -65,14 +66,29 @@ __device__ __forceinline__ bool in_area(const SensorArgs& s, int x, int y) {
     return x >= s.ax && y >= s.ay && x < s.ax + s.aw && y < s.ay + s.ah;
 }
 
-// (float(code) - black[site]) / (white - black), one IEEE division
-// (RawEngine.cpp:89-97).
-__device__ __forceinline__ float normalize_code(const SensorArgs& s, uint32_t code, uint32_t site) {
-    // Selects on the site bits (no indexed access, so the sensor copy stays in registers).
+// Per-site value by select (no indexed access, so the sensor copy stays in
+// registers).
+__device__ __forceinline__ float site_value(const float (&v)[4], uint32_t site) {
     const bool hi = (site & 2u) != 0, odd = (site & 1u) != 0;
-    const float black = hi ? (odd ? s.black[3] : s.black[2]) : (odd ? s.black[1] : s.black[0]);
-    const float denom = hi ? (odd ? s.denom[3] : s.denom[2]) : (odd ? s.denom[1] : s.denom[0]);
-    return ((float)code - black) / denom;
+    return hi ? (odd ? v[3] : v[2]) : (odd ? v[1] : v[0]);
+}
+
+// (code - black) / (white - black) rounded exactly as the IEEE float
+// division of RawEngine.cpp:89-97 rounds it: n = code - black and
+// d = white - black are integers with |n| <= 65535 and 1 <= d <= 65535, and
+// for every such pair the corrected product q0 + (n - q0 d) y, with
+// y = RN(1/d), q0 = RN(n y) and both corrections fused, equals RN(n / d)
+// (checked exhaustively over every such pair by the oracle's
+// ro_division_mismatches).
+__device__ __forceinline__ float normalize_exact(float code, float black, float rcp, float denom) {
+    const float n = code - black;
+    const float q0 = n * rcp;
+    const float r = __builtin_fmaf(-q0, denom, n);
+    return __builtin_fmaf(r, rcp, q0);
+}
+
+__device__ __forceinline__ float normalize_code(const SensorArgs& s, uint32_t code, uint32_t site) {
+    return normalize_exact((float)code, site_value(s.black, site), site_value(s.rcp, site), site_value(s.denom, site));
 }
 
 // Whole-sample reflection into [origin, origin + size), size >= 2, for
@@ -129,6 +145,24 @@ __device__ inline double curve_map(const DevCurve& c, uint32_t cubic, double v) 
     return cubic_blend(cubic_blend(a, b, t), cubic_blend(b, d, t), t);
 }
 
+struct CurveOut {
+    float v;
+    uint32_t err;
+};
+
+// Out of line: the op program unrolls over a thread's pixels, and the curve
+// lookup is long.
+static __device__ __noinline__ CurveOut curves_channel_v(const DevCurves* cv, int channel, float value) {
+    if (cv->c[0].identity && cv->c[channel + 1].identity) return {value, 0u};
+    if (!finite(value)) return {value, FLAG_INVALID};
+    const double master = curve_map(cv->c[0], cv->cubic, (double)value);
+    if (!__builtin_isfinite(master)) return {value, FLAG_INVALID};
+    const double mapped = curve_map(cv->c[channel + 1], cv->cubic, master);
+    if (!__builtin_isfinite(mapped) || fabs(mapped) > 3.4028234663852886e38) return {value, FLAG_INVALID};
+    return {(float)mapped, 0u};
+}
+
+// Inline form for the per-pixel op program.
 __device__ inline float curves_channel(const DevCurves* cv, int channel, float value, uint32_t& err) {
     if (cv->c[0].identity && cv->c[channel + 1].identity) return value;
     if (!finite(value)) { err |= FLAG_INVALID; return value; }
@@ -177,6 +211,78 @@ __device__ __forceinline__ void apply_ops(const OpList& L, uint32_t begin, uint3
 
 __device__ __forceinline__ void apply_ops(const OpList& L, float& r, float& g, float& b, uint32_t& err) {
     apply_ops(L, 0, L.n, r, g, b, err);
+}
+
+// The op program over a thread's pixels, op by op: one (uniform) dispatch
+// per op for all NPX pixels.  err[p] collects pixel p's error bits.
+template <int NPX>
+__device__ __forceinline__ void apply_ops_px(const OpList& L, uint32_t begin, uint32_t end, float (&px)[NPX][3],
+                                             uint32_t (&err)[NPX]) {
+    for (uint32_t i = begin; i < end; ++i) {
+        const DevOp& o = L.op[i];
+        switch (o.kind) {
+        case OP_GAINS: {
+            const float a = o.v[0], b = o.v[1], c = o.v[2];
+#pragma unroll
+            for (int p = 0; p < NPX; ++p) { px[p][0] *= a; px[p][1] *= b; px[p][2] *= c; }
+            break;
+        }
+        case OP_SCALE: {
+            const float a = o.v[0];
+#pragma unroll
+            for (int p = 0; p < NPX; ++p) { px[p][0] *= a; px[p][1] *= a; px[p][2] *= a; }
+            break;
+        }
+        case OP_MATRIX: {
+            float m[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) m[k] = o.v[k];
+#pragma unroll
+            for (int p = 0; p < NPX; ++p) {
+                const float r = px[p][0], g = px[p][1], b = px[p][2];
+                px[p][0] = m[0] * r + m[1] * g + m[2] * b;
+                px[p][1] = m[3] * r + m[4] * g + m[5] * b;
+                px[p][2] = m[6] * r + m[7] * g + m[8] * b;
+            }
+            break;
+        }
+        case OP_TONE: {
+            const float sh = o.v[0], ig = o.v[1];
+#pragma unroll
+            for (int p = 0; p < NPX; ++p)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) px[p][c] = tone(px[p][c], sh, ig);
+            break;
+        }
+        case OP_CLIP:
+#pragma unroll
+            for (int p = 0; p < NPX; ++p) {
+                if (!finite(px[p][0]) || !finite(px[p][1]) || !finite(px[p][2])) err[p] |= FLAG_DOMAIN;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) px[p][c] = clamp01(px[p][c]);
+            }
+            break;
+        case OP_SRGB_ENCODE:
+#pragma unroll
+            for (int p = 0; p < NPX; ++p) {
+                if (!finite(px[p][0]) || !finite(px[p][1]) || !finite(px[p][2])) err[p] |= FLAG_DOMAIN;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) px[p][c] = srgb_encode(px[p][c]);
+            }
+            break;
+        case OP_CURVES:
+#pragma unroll
+            for (int p = 0; p < NPX; ++p)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const CurveOut r = curves_channel_v(o.curves, c, px[p][c]);
+                    px[p][c] = r.v;
+                    err[p] |= r.err;
+                }
+            break;
+        default: break;
+        }
+    }
 }
 
 __device__ __forceinline__ uint8_t quantize_u8(float v) {

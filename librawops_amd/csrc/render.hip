@@ -30,12 +30,8 @@ __global__ void __launch_bounds__(kThreads) render_kernel(const RenderArgs a) {
     __shared__ __attribute__((aligned(16))) char lds[Lds<DEMOSAIC, TW, TH>::BYTES];
     int tx, ty, lx0, ly0;
     tile_origin<TW, TH>(a, (int)blockIdx.x, tx, ty, lx0, ly0);
-    constexpr int H = DEMOSAIC == RAWOPS_DEMOSAIC_MENON ? 6 : 1;
     const SensorArgs s = a.s;
-    const bool interior = tx - H >= s.ax && ty - H >= s.ay &&
-                          tx + TW + H <= s.ax + s.aw && ty + TH + H <= s.ay + s.ah;
-    if (interior) render_tile<DEMOSAIC, TW, TH, false>(a, s, lds, tx, ty, lx0, ly0);
-    else render_tile<DEMOSAIC, TW, TH, true>(a, s, lds, tx, ty, lx0, ly0);
+    render_any_tile<DEMOSAIC, TW, TH>(a, s, lds, tx, ty, lx0, ly0);
 }
 
 template <int DEMOSAIC, int TW, int TH>

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "menon_fwd.hpp"
 #include "menon_tile.hpp"
 
 namespace rawops {
@@ -24,8 +25,14 @@ struct RenderArgs {
     uint32_t tap;            // reduced ops applied before the FP32 output; the rest precede the u8 output
 };
 
+// Tile-relative x parity of the non-green sites in even tile rows for a
+// render whose tiles start at native (ox0 + i TW, oy0 + j TH), TW, TH even.
+__host__ __device__ inline int tile_n0(const SensorArgs& s, int ox0, int oy0) {
+    return (int)(((s.ng_parity >> ((uint32_t)oy0 & 1u)) & 1u) ^ ((uint32_t)ox0 & 1u));
+}
+
 // Finish one level pixel: reduced ops up to the tap, FP32 store, remaining
-// ops, uint8 store.
+// ops, uint8 store (mip reduction path).
 __device__ __forceinline__ void emit_pixel(const RenderArgs& a, int lx, int ly, float r, float g, float b,
                                            uint32_t& err) {
     const OpList& L = *a.reduced;
@@ -50,7 +57,7 @@ struct BilinearLds {
 
 template <int DEMOSAIC, int TW, int TH>
 struct Lds;
-template <int TW, int TH> struct Lds<RAWOPS_DEMOSAIC_MENON, TW, TH> { static constexpr int BYTES = MenonLds<TW, TH>::BYTES; static constexpr int OFF_NATIVE = MenonLds<TW, TH>::OFF_NATIVE; };
+template <int TW, int TH> struct Lds<RAWOPS_DEMOSAIC_MENON, TW, TH> { static constexpr int BYTES = MenonFwdLds<TW, TH>::BYTES; static constexpr int OFF_NATIVE = MenonFwdLds<TW, TH>::OFF_NATIVE; };
 template <int TW, int TH> struct Lds<RAWOPS_DEMOSAIC_BILINEAR, TW, TH> { static constexpr int BYTES = BilinearLds<TW, TH>::BYTES; static constexpr int OFF_NATIVE = BilinearLds<TW, TH>::OFF_NATIVE; };
 
 // Level pixels per tile side and native tile origin.
@@ -64,34 +71,21 @@ __device__ __forceinline__ void tile_origin(const RenderArgs& a, int tile, int& 
     ty = a.oy0 + by * TH;
 }
 
-template <int DEMOSAIC, int TW, int TH, bool BORDER>
+// One tile.  Each thread owns NQ 2x2 cells of the native tile: pixel p =
+// (2 j + b) 2 + e of px is (2 qx + e, 2 qy + b) of cell threadIdx.x + j kThreads.
+template <int DEMOSAIC, int TW, int TH, bool BORDER, int N0>
 __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArgs& s, char* lds, int tx, int ty, int lx0, int ly0) {
-    uint32_t err = 0;
-    // Native pixels this thread finishes: pairs of horizontally adjacent
-    // pixels, so every lane handles one green and one non-green site.
-    constexpr int PAIRS = (TW / 2) * TH;
-    static_assert(PAIRS % kThreads == 0, "tile pairs must divide evenly over the workgroup");
-    constexpr int PER_THREAD = PAIRS / kThreads;
-    float px[PER_THREAD][2][3];
+    constexpr int NQ = (TW / 2) * (TH / 2) / kThreads;
+    static_assert(NQ * kThreads == (TW / 2) * (TH / 2), "tile cells must divide evenly over the workgroup");
+    constexpr int NPX = NQ * 4;
+    float px[NPX][3];
+    uint32_t err[NPX];
+#pragma unroll
+    for (int p = 0; p < NPX; ++p) err[p] = 0;
 
     if constexpr (DEMOSAIC == RAWOPS_DEMOSAIC_MENON) {
-        using L = MenonLds<TW, TH>;
-        MenonTile<TW, TH, BORDER> t{s, tx, ty, 0,
-                                    reinterpret_cast<float*>(lds + L::OFF_RAW),
-                                    reinterpret_cast<double*>(lds + L::OFF_GH),
-                                    reinterpret_cast<double*>(lds + L::OFF_GV),
-                                    reinterpret_cast<double*>(lds + L::OFF_DH),
-                                    reinterpret_cast<double*>(lds + L::OFF_DV),
-                                    reinterpret_cast<double*>(lds + L::OFF_GREEN),
-                                    reinterpret_cast<double*>(lds + L::OFF_GH),
-                                    reinterpret_cast<double*>(lds + L::OFF_GV),
-                                    reinterpret_cast<uint8_t*>(lds + L::OFF_HZ)};
-        // Tile-relative non-green parity for both row parities.
-        for (int b = 0; b < 2; ++b) {
-            const uint32_t sensor = (s.ng_parity >> (((uint32_t)ty + (uint32_t)b) & 1u)) & 1u;
-            t.ngr |= ((sensor ^ ((uint32_t)tx & 1u)) & 1u) << b;
-        }
-        load_raw<TW, TH, 6, BORDER>(s, tx, ty, t.raw);
+        MenonFwd<TW, TH, kThreads, N0, BORDER> t(s, lds, tx, ty);
+        t.load();
         __syncthreads();
         t.directional();
         __syncthreads();
@@ -101,67 +95,112 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         __syncthreads();
         t.chroma();
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PER_THREAD; ++j) {
-            const int p = threadIdx.x + j * kThreads;
-            const int ry = p / (TW / 2), rx = 2 * (p - ry * (TW / 2));
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                float r = 0.f, g = 0.f, b = 0.f;
-                if (t.inside(rx + e, ry)) {
-                    t.finish(rx + e, ry, r, g, b);
-                    apply_ops(*a.native, r, g, b, err);
-                }
-                px[j][e][0] = r; px[j][e][1] = g; px[j][e][2] = b;
-            }
-        }
+        t.finish(px);
     } else {
         float* raw = reinterpret_cast<float*>(lds);
         load_raw<TW, TH, 1, BORDER>(s, tx, ty, raw);
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < PER_THREAD; ++j) {
-            const int p = threadIdx.x + j * kThreads;
-            const int ry = p / (TW / 2), rx = 2 * (p - ry * (TW / 2));
+        for (int j = 0; j < NQ; ++j) {
+            const int it = (int)threadIdx.x + j * kThreads;
+            const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                float r = 0.f, g = 0.f, b = 0.f;
-                if (!BORDER || in_area(s, tx + rx + e, ty + ry)) {
-                    bilinear_pixel<TW, TH, BORDER>(s, raw, tx, ty, rx + e, ry, r, g, b);
-                    apply_ops(*a.native, r, g, b, err);
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    float* v = px[(2 * j + b) * 2 + e];
+                    v[0] = v[1] = v[2] = 0.f;
+                    const int rx = 2 * qx + e, ry = 2 * qy + b;
+                    if (!BORDER || in_area(s, tx + rx, ty + ry)) bilinear_pixel<TW, TH, BORDER>(s, raw, tx, ty, rx, ry, v[0], v[1], v[2]);
                 }
-                px[j][e][0] = r; px[j][e][1] = g; px[j][e][2] = b;
-            }
         }
     }
 
+    apply_ops_px(*a.native, 0, a.native->n, px, err);
+
     if (a.mip == 0) {
+        const OpList& R = *a.reduced;
+        const uint32_t tap = a.tap < R.n ? a.tap : R.n;
+        apply_ops_px(R, 0, tap, px, err);
+        if (a.out) {
 #pragma unroll
-        for (int j = 0; j < PER_THREAD; ++j) {
-            const int p = threadIdx.x + j * kThreads;
-            const int ry = p / (TW / 2), rx = 2 * (p - ry * (TW / 2));
-            const int ly = ty + ry - a.oy0;
+            for (int j = 0; j < NQ; ++j) {
+                const int it = (int)threadIdx.x + j * kThreads;
+                const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int lx = tx + rx + e - a.ox0;
-                if (lx >= a.vw || ly >= a.vh) continue;
-                emit_pixel(a, lx, ly, px[j][e][0], px[j][e][1], px[j][e][2], err);
+                for (int b = 0; b < 2; ++b) {
+                    const int lx = tx + 2 * qx - a.ox0, ly = ty + 2 * qy + b - a.oy0;
+                    if (ly >= a.vh || lx >= a.vw) continue;
+                    float* o = a.out + ((size_t)ly * a.ostride + (size_t)lx) * 3;
+                    const float* p0 = px[(2 * j + b) * 2];
+                    const float* p1 = px[(2 * j + b) * 2 + 1];
+                    if (lx + 1 < a.vw) {
+                        o[0] = p0[0]; o[1] = p0[1]; o[2] = p0[2];
+                        o[3] = p1[0]; o[4] = p1[1]; o[5] = p1[2];
+                    } else {
+                        o[0] = p0[0]; o[1] = p0[1]; o[2] = p0[2];
+                    }
+                }
             }
         }
-    } else {
+        if (a.out8) {
+            apply_ops_px(R, tap, R.n, px, err);
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                const int it = (int)threadIdx.x + j * kThreads;
+                const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int lx = tx + 2 * qx + e - a.ox0, ly = ty + 2 * qy + b - a.oy0;
+                        if (ly >= a.vh || lx >= a.vw) continue;
+                        uint8_t* o = a.out8 + ((size_t)ly * a.ostride + (size_t)lx) * 3;
+                        const float* v = px[(2 * j + b) * 2 + e];
+                        o[0] = quantize_u8(v[0]); o[1] = quantize_u8(v[1]); o[2] = quantize_u8(v[2]);
+                    }
+            }
+        }
+    }
+    // Errors count only for pixels the request covers: inside the area and
+    // inside the viewport (through the mip footprint).
+    uint32_t e_all = 0;
+#pragma unroll
+    for (int p = 0; p < NPX; ++p) e_all |= err[p];
+    if (e_all) {
+        e_all = 0;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            const int it = (int)threadIdx.x + j * kThreads;
+            const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int sx = tx + 2 * qx + e, sy = ty + 2 * qy + b;
+                    if (in_area(s, sx, sy) && ((sx - a.ox0) >> a.mip) < a.vw && ((sy - a.oy0) >> a.mip) < a.vh)
+                        e_all |= err[(2 * j + b) * 2 + e];
+                }
+        }
+    }
+
+    if (a.mip != 0) {
         // Footprint reduction anchored at the active-area origin
         // (CameraToWorkingNode::render_level RawEngine.cpp:580-601): FP64
         // sums in row-major footprint order, clipped to the area.
         float* native = reinterpret_cast<float*>(lds + Lds<DEMOSAIC, TW, TH>::OFF_NATIVE);
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < PER_THREAD; ++j) {
-            const int p = threadIdx.x + j * kThreads;
-            const int ry = p / (TW / 2), rx = 2 * (p - ry * (TW / 2));
+        for (int j = 0; j < NQ; ++j) {
+            const int it = (int)threadIdx.x + j * kThreads;
+            const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
-            for (int e = 0; e < 2; ++e)
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) native[(ry * TW + rx + e) * 3 + c] = px[j][e][c];
+                for (int e = 0; e < 2; ++e)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        native[((2 * qy + b) * TW + 2 * qx + e) * 3 + c] = px[(2 * j + b) * 2 + e][c];
         }
         __syncthreads();
         const int scale = 1 << a.mip;
@@ -179,10 +218,32 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
                     sr += q[0]; sg += q[1]; sb += q[2];
                     ++count;
                 }
-            emit_pixel(a, lx, ly, (float)(sr / count), (float)(sg / count), (float)(sb / count), err);
+            emit_pixel(a, lx, ly, (float)(sr / count), (float)(sg / count), (float)(sb / count), e_all);
         }
     }
-    if (err && a.flags) atomicOr(a.flags, err);
+    if (e_all && a.flags) atomicOr(a.flags, e_all);
+}
+
+// Interior or border form of a tile, for either non-green phase.
+template <int DEMOSAIC, int TW, int TH>
+__device__ __forceinline__ void render_any_tile(const RenderArgs& a, const SensorArgs& s, char* lds, int tx, int ty,
+                                                int lx0, int ly0) {
+    const int n0 = tile_n0(s, a.ox0, a.oy0);
+    constexpr int H = DEMOSAIC == RAWOPS_DEMOSAIC_MENON ? 6 : 1;
+    const bool interior = tx - H >= s.ax && ty - H >= s.ay && tx + TW + H <= s.ax + s.aw && ty + TH + H <= s.ay + s.ah;
+    if constexpr (DEMOSAIC == RAWOPS_DEMOSAIC_MENON) {
+        if (n0) {
+            if (interior) render_tile<DEMOSAIC, TW, TH, false, 1>(a, s, lds, tx, ty, lx0, ly0);
+            else render_tile<DEMOSAIC, TW, TH, true, 1>(a, s, lds, tx, ty, lx0, ly0);
+        } else {
+            if (interior) render_tile<DEMOSAIC, TW, TH, false, 0>(a, s, lds, tx, ty, lx0, ly0);
+            else render_tile<DEMOSAIC, TW, TH, true, 0>(a, s, lds, tx, ty, lx0, ly0);
+        }
+    } else {
+        (void)n0;
+        if (interior) render_tile<DEMOSAIC, TW, TH, false, 0>(a, s, lds, tx, ty, lx0, ly0);
+        else render_tile<DEMOSAIC, TW, TH, true, 0>(a, s, lds, tx, ty, lx0, ly0);
+    }
 }
 
 }  // namespace rawops

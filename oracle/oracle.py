@@ -65,8 +65,16 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, P]
         L.ro_guided_filter_bands.argtypes = [P, Rect, Rect, Rect, ctypes.c_uint32, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, P, ctypes.c_int]
+        L.ro_division_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        L.ro_division_mismatches.restype = ctypes.c_longlong
         _LIB = L
     return _LIB
+
+
+def division_mismatches(d_lo=1, d_hi=65535, planted=False):
+    """Operand pairs where the kernels' divide-free sample normalisation
+    differs from the IEEE float quotient (RawEngine.cpp:89-97)."""
+    return int(lib().ro_division_mismatches(d_lo, d_hi, 1 if planted else 0))
 
 
 def _p(a):

@@ -628,3 +628,44 @@ int ro_guided_filter_bands(const float* in, ro_rect needed, ro_rect output, ro_r
     }
     return status ? -1 : 0;
 }
+
+/* ----------------------------------------------------- exact division -- */
+
+static float nearest_recip(uint32_t d) {
+    const float y = (float)(1.0 / (double)d);
+    float best = y;
+    double err = fabs((double)y * d - 1.0);
+    const float c[2] = {nextafterf(y, 0.0f), nextafterf(y, 2.0f)};
+    for (int i = 0; i < 2; ++i) {
+        const double e = fabs((double)c[i] * d - 1.0);
+        if (e < err) { err = e; best = c[i]; }
+    }
+    return best;
+}
+
+#define DIV_BODY                                                            \
+    long long bad = 0;                                                      \
+    _Pragma("omp parallel for schedule(dynamic, 64) reduction(+ : bad)")    \
+    for (long long d = d_lo; d <= (long long)d_hi; ++d) {                   \
+        const float fd = (float)d, y = nearest_recip((uint32_t)d);          \
+        for (uint32_t n = 0; n <= 65535u; ++n) {                            \
+            const float fn = (float)n;                                      \
+            volatile float q = fn / fd;                                     \
+            const float q0 = fn * y;                                        \
+            const float q1 = planted ? q0 : fmaf(fmaf(-q0, fd, fn), y, q0); \
+            bad += q1 != q;                                                 \
+        }                                                                   \
+    }                                                                       \
+    return bad;
+
+__attribute__((target("fma"))) static long long div_mismatches_fma(uint32_t d_lo, uint32_t d_hi, int planted) {
+    DIV_BODY
+}
+static long long div_mismatches_libm(uint32_t d_lo, uint32_t d_hi, int planted) { DIV_BODY }
+
+long long ro_division_mismatches(uint32_t d_lo, uint32_t d_hi, int planted) {
+    if (d_lo < 1) d_lo = 1;
+    if (d_hi > 65535) d_hi = 65535;
+    if (__builtin_cpu_supports("fma")) return div_mismatches_fma(d_lo, d_hi, planted);
+    return div_mismatches_libm(d_lo, d_hi, planted);
+}

@@ -19,6 +19,8 @@
  *                         + compiled reference
  *   ro_guided_filter      exact rational impulse KAT [3/22,2/11,5/11,2/11,3/22]
  *                         + brute-force rational kernel + compiled reference (r<=8)
+ *   ro_division_mismatches  exhaustive by construction (every operand pair);
+ *                         a planted uncorrected quotient fails it
  *   ro_srgb_u8            parity unpinned (no reference implementation exists);
  *                         closed form round(clamp(v,0,1)*255) only.
  */
@@ -77,6 +79,14 @@ int ro_clip(float* values, size_t count);
 int ro_srgb_encode(float* values, size_t count);
 /* uint8 quantisation of encoded sRGB (no reference counterpart). */
 void ro_srgb_u8(const float* encoded, size_t count, uint8_t* out);
+
+/* Sample normalisation without a divide: for integers n in [0, 65535] and d
+ * in [d_lo, d_hi] (1 <= d <= 65535) counts the pairs where
+ * q0 + fma(-q0, d, n) * y (fused), q0 = RN(n * y), y = RN(1 / d), differs
+ * from the IEEE float quotient n / d that RawEngine.cpp:89-97 takes.  The
+ * recipe is odd in n, so n >= 0 covers the normalised range |n| <= 65535.
+ * planted != 0 drops the correction (the count must then be non-zero). */
+long long ro_division_mismatches(uint32_t d_lo, uint32_t d_hi, int planted);
 
 /* std::exp2 on a float, as ExposureNode computes its multiplier on the host
  * (RawEngine.cpp:465-466). */

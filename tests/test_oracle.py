@@ -42,6 +42,14 @@ def test_menon_plant_detected():
     assert _bits(out, r["expected"])
 
 
+def test_divide_free_normalisation_is_exact():
+    """The kernels normalise samples as q0 + fma(-q0, d, n) y (y = RN(1/d)),
+    not with a divide; over every integer numerator 0..65535 and denominator
+    1..65535 (all white - black levels) it must round like n / d."""
+    assert O.division_mismatches() == 0
+    assert O.division_mismatches(1, 64, planted=True) > 0
+
+
 def test_unpack_kats():
     """core_tests.cpp:134-229: signed values, stride/phase/per-site levels."""
     s = np.full(16, 500, np.uint16)
