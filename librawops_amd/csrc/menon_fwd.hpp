@@ -109,32 +109,50 @@ struct MenonFwd {
         const int64_t stride = s.stride;
         const uint16_t* base = s.samples + ((int64_t)(ty - 6) * stride + (tx - 6));
         const bool pairs = !BORDER && ((reinterpret_cast<uintptr_t>(base) | (uintptr_t)(stride * 2)) & 3) == 0;
+        constexpr int J = (N + NT - 1) / NT;
+        auto put = [&](int qr, int qk, int b, int e, uint32_t code) {
+            raw[(2 * qr + b) * RW + 2 * qk + e] = normalize_exact((float)code, blk[b][e], rcp[b][e], den[b][e]);
+        };
+        if (pairs) {
+            // Interior, 4-byte aligned cells: issue every load first, then convert.
+            uint32_t w[J][2];
 #pragma unroll
-        for (int j = 0; j < (N + NT - 1) / NT; ++j) {
+            for (int j = 0; j < J; ++j) {
+                const int it = (int)threadIdx.x + j * NT;
+                const int qr = it / QK, qk = it - qr * QK;
+                const uint16_t* p = base + ((int64_t)(2 * qr) * stride + 2 * qk);
+                if (it < N) {
+                    w[j][0] = *reinterpret_cast<const uint32_t*>(p);
+                    w[j][1] = *reinterpret_cast<const uint32_t*>(p + stride);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int it = (int)threadIdx.x + j * NT;
+                const int qr = it / QK, qk = it - qr * QK;
+                if (it < N) {
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        put(qr, qk, b, 0, w[j][b] & 0xffffu);
+                        put(qr, qk, b, 1, w[j][b] >> 16);
+                    }
+                }
+            }
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
             const int it = (int)threadIdx.x + j * NT;
             if (it >= N) break;
             const int qr = it / QK, qk = it - qr * QK;
             const uint16_t* p = base + ((int64_t)(2 * qr) * stride + 2 * qk);
-            float* dst = raw + (2 * qr) * RW + 2 * qk;
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                uint32_t c[2];
-                bool ok[2] = {true, true};
-                if (pairs) {
-                    const uint32_t w = *reinterpret_cast<const uint32_t*>(p + b * stride);
-                    c[0] = w & 0xffffu;
-                    c[1] = w >> 16;
-                } else {
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-                    for (int e = 0; e < 2; ++e) {
-                        ok[e] = !BORDER || in_area(s, tx - 6 + 2 * qk + e, ty - 6 + 2 * qr + b);
-                        c[e] = ok[e] ? p[b * stride + e] : 0u;
-                    }
+                for (int e = 0; e < 2; ++e) {
+                    const bool ok = !BORDER || in_area(s, tx - 6 + 2 * qk + e, ty - 6 + 2 * qr + b);
+                    if (ok) put(qr, qk, b, e, p[b * stride + e]);
                 }
-#pragma unroll
-                for (int e = 0; e < 2; ++e)
-                    if (ok[e]) dst[b * RW + e] = normalize_exact((float)c[e], blk[b][e], rcp[b][e], den[b][e]);
-            }
         }
     }
 

@@ -26,12 +26,12 @@ namespace rawops {
 namespace {
 
 template <int DEMOSAIC, int TW, int TH>
-__global__ void __launch_bounds__(kThreads) render_kernel(const RenderArgs a) {
+__global__ void __launch_bounds__(kRenderThreads) render_kernel(const RenderArgs a) {
     __shared__ __attribute__((aligned(16))) char lds[Lds<DEMOSAIC, TW, TH>::BYTES];
     int tx, ty, lx0, ly0;
     tile_origin<TW, TH>(a, (int)blockIdx.x, tx, ty, lx0, ly0);
     const SensorArgs s = a.s;
-    render_any_tile<DEMOSAIC, TW, TH>(a, s, lds, tx, ty, lx0, ly0);
+    render_any_tile<DEMOSAIC, TW, TH, kRenderThreads>(a, s, lds, tx, ty, lx0, ly0);
 }
 
 template <int DEMOSAIC, int TW, int TH>
@@ -56,7 +56,7 @@ hipError_t launch(const RenderLaunch& L, const OpList* native, const OpList* red
     a.tap = 0xffffffffu;
     const unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL((render_kernel<DEMOSAIC, TW, TH>), dim3(blocks), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL((render_kernel<DEMOSAIC, TW, TH>), dim3(blocks), dim3(kRenderThreads), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -8,7 +8,8 @@
 
 namespace rawops {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;        // batch kernel workgroup
+constexpr int kRenderThreads = 512;  // render kernel workgroup
 
 struct RenderArgs {
     SensorArgs s;
@@ -71,12 +72,13 @@ __device__ __forceinline__ void tile_origin(const RenderArgs& a, int tile, int& 
     ty = a.oy0 + by * TH;
 }
 
-// One tile.  Each thread owns NQ 2x2 cells of the native tile: pixel p =
-// (2 j + b) 2 + e of px is (2 qx + e, 2 qy + b) of cell threadIdx.x + j kThreads.
-template <int DEMOSAIC, int TW, int TH, bool BORDER, int N0>
+// One tile on a workgroup of NT threads.  Each thread owns NQ 2x2 cells of
+// the native tile: pixel p = (2 j + b) 2 + e of px is (2 qx + e, 2 qy + b) of
+// cell threadIdx.x + j NT.
+template <int DEMOSAIC, int TW, int TH, int NT, bool BORDER, int N0>
 __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArgs& s, char* lds, int tx, int ty, int lx0, int ly0) {
-    constexpr int NQ = (TW / 2) * (TH / 2) / kThreads;
-    static_assert(NQ * kThreads == (TW / 2) * (TH / 2), "tile cells must divide evenly over the workgroup");
+    constexpr int NQ = (TW / 2) * (TH / 2) / NT;
+    static_assert(NQ * NT == (TW / 2) * (TH / 2), "tile cells must divide evenly over the workgroup");
     constexpr int NPX = NQ * 4;
     float px[NPX][3];
     uint32_t err[NPX];
@@ -84,7 +86,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
     for (int p = 0; p < NPX; ++p) err[p] = 0;
 
     if constexpr (DEMOSAIC == RAWOPS_DEMOSAIC_MENON) {
-        MenonFwd<TW, TH, kThreads, N0, BORDER> t(s, lds, tx, ty);
+        MenonFwd<TW, TH, NT, N0, BORDER> t(s, lds, tx, ty);
         t.load();
         __syncthreads();
         t.directional();
@@ -98,11 +100,11 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         t.finish(px);
     } else {
         float* raw = reinterpret_cast<float*>(lds);
-        load_raw<TW, TH, 1, BORDER>(s, tx, ty, raw);
+        load_raw<TW, TH, 1, BORDER, NT>(s, tx, ty, raw);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
-            const int it = (int)threadIdx.x + j * kThreads;
+            const int it = (int)threadIdx.x + j * NT;
             const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
             for (int b = 0; b < 2; ++b)
@@ -125,7 +127,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         if (a.out) {
 #pragma unroll
             for (int j = 0; j < NQ; ++j) {
-                const int it = (int)threadIdx.x + j * kThreads;
+                const int it = (int)threadIdx.x + j * NT;
                 const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
@@ -147,7 +149,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
             apply_ops_px(R, tap, R.n, px, err);
 #pragma unroll
             for (int j = 0; j < NQ; ++j) {
-                const int it = (int)threadIdx.x + j * kThreads;
+                const int it = (int)threadIdx.x + j * NT;
                 const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
@@ -171,7 +173,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         e_all = 0;
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
-            const int it = (int)threadIdx.x + j * kThreads;
+            const int it = (int)threadIdx.x + j * NT;
             const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
             for (int b = 0; b < 2; ++b)
@@ -192,7 +194,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
-            const int it = (int)threadIdx.x + j * kThreads;
+            const int it = (int)threadIdx.x + j * NT;
             const int qy = it / (TW / 2), qx = it - qy * (TW / 2);
 #pragma unroll
             for (int b = 0; b < 2; ++b)
@@ -206,7 +208,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         const int scale = 1 << a.mip;
         const int ow = TW / scale, oh = TH / scale;
         const int nx_end = s.ax + s.aw - tx, ny_end = s.ay + s.ah - ty;   // area end, tile-relative
-        for (int i = threadIdx.x; i < ow * oh; i += kThreads) {
+        for (int i = threadIdx.x; i < ow * oh; i += NT) {
             const int oy = i / ow, ox = i - oy * ow;
             const int lx = lx0 + ox - a.vx, ly = ly0 + oy - a.vy;
             if (lx >= a.vw || ly >= a.vh) continue;
@@ -225,7 +227,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
 }
 
 // Interior or border form of a tile, for either non-green phase.
-template <int DEMOSAIC, int TW, int TH>
+template <int DEMOSAIC, int TW, int TH, int NT>
 __device__ __forceinline__ void render_any_tile(const RenderArgs& a, const SensorArgs& s, char* lds, int tx, int ty,
                                                 int lx0, int ly0) {
     const int n0 = tile_n0(s, a.ox0, a.oy0);
@@ -233,16 +235,16 @@ __device__ __forceinline__ void render_any_tile(const RenderArgs& a, const Senso
     const bool interior = tx - H >= s.ax && ty - H >= s.ay && tx + TW + H <= s.ax + s.aw && ty + TH + H <= s.ay + s.ah;
     if constexpr (DEMOSAIC == RAWOPS_DEMOSAIC_MENON) {
         if (n0) {
-            if (interior) render_tile<DEMOSAIC, TW, TH, false, 1>(a, s, lds, tx, ty, lx0, ly0);
-            else render_tile<DEMOSAIC, TW, TH, true, 1>(a, s, lds, tx, ty, lx0, ly0);
+            if (interior) render_tile<DEMOSAIC, TW, TH, NT, false, 1>(a, s, lds, tx, ty, lx0, ly0);
+            else render_tile<DEMOSAIC, TW, TH, NT, true, 1>(a, s, lds, tx, ty, lx0, ly0);
         } else {
-            if (interior) render_tile<DEMOSAIC, TW, TH, false, 0>(a, s, lds, tx, ty, lx0, ly0);
-            else render_tile<DEMOSAIC, TW, TH, true, 0>(a, s, lds, tx, ty, lx0, ly0);
+            if (interior) render_tile<DEMOSAIC, TW, TH, NT, false, 0>(a, s, lds, tx, ty, lx0, ly0);
+            else render_tile<DEMOSAIC, TW, TH, NT, true, 0>(a, s, lds, tx, ty, lx0, ly0);
         }
     } else {
         (void)n0;
-        if (interior) render_tile<DEMOSAIC, TW, TH, false, 0>(a, s, lds, tx, ty, lx0, ly0);
-        else render_tile<DEMOSAIC, TW, TH, true, 0>(a, s, lds, tx, ty, lx0, ly0);
+        if (interior) render_tile<DEMOSAIC, TW, TH, NT, false, 0>(a, s, lds, tx, ty, lx0, ly0);
+        else render_tile<DEMOSAIC, TW, TH, NT, true, 0>(a, s, lds, tx, ty, lx0, ly0);
     }
 }
 
