@@ -48,7 +48,235 @@ def cpu_baseline(meta, samples, cam, threads, rows=512):
     px = rect[2] * rect[3] * reps
     return {"value": px / el / 1e6, "unit": "Mpix/s", "cores": threads, "kind": "port",
             "sample": f"{reps}x Menon+WB+exposure+3x3 on a {rect[2]}x{rect[3]} band of the same frame, "
-                      f"{threads} OpenMP row bands, oracle C restatement (gcc -O2)"}
+                      f"{threads} OpenMP row bands, oracle C restatement (gcc -O2)"}, nat
+
+
+def _timed(fn, steps, stream):
+    """Average ms of fn() over `steps` runs, HIP events on `stream`."""
+    import torch
+    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    e1 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for i in range(steps):
+        e0[i].record(stream)
+        fn()
+        e1[i].record(stream)
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in zip(e0, e1)) / steps
+
+
+def leg_c1_bilinear(steps):
+    """1024x768 RGGB bilinear -> WB (2, 1, 1.5) -> identity camera->XYZ ->
+    ProPhoto, FP32 linear, mip 0 (the reference's CPU-runnable case)."""
+    import numpy as np
+    import torch
+    import librawops_amd as L
+    from librawops_amd import synth
+    from librawops_amd.graph import DeviceRender, compile_chain
+    s = synth.gradient_bayer(1024, 768, 0, 512, 16383, 20.0, seed=1)
+    md = L.RawMetadata(1024, 768, 1024, L.BayerPattern(0), 0, 0, L.Rect(0, 0, 0, 0), (512,) * 4, (16383,) * 4)
+    img = L.RawImage(md, torch.from_numpy(s.view(np.int16)).cuda())
+    area = img.metadata.active_area
+    node = L.CameraToWorkingNode(L.ExposureNode(L.WhiteBalanceNode(L.RawUnpackNode(img, L.BILINEAR), 2.0, 1.0, 1.5), 0.0),
+                                 L.CameraColorTransform((1, 0, 0, 0, 1, 0, 0, 0, 1), L.WorkingSpace.LinearProPhotoD50),
+                                 area)
+    c = compile_chain(node)
+    out = torch.empty((768, 1024, 3), dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream()
+    run = lambda: DeviceRender.run(c, area, L.RenderLevel(), out=out, stream=st)
+    for _ in range(3):
+        run()
+    ms = _timed(run, steps, st)
+    px = 1024 * 768
+    return {"workload": "1024x768 RGGB bilinear -> WB(2,1,1.5) -> identity cam -> ProPhoto, FP32 linear",
+            "kernel_ms": round(ms, 4), "mpix_s": round(px / ms / 1e3, 1),
+            "achieved_gbs": round(BYTES_PER_PX * px / ms / 1e6, 1)}
+
+
+def leg_c3_guided(steps, radius=8, eps=1e-3):
+    """Working-Y guided filter + 0.7 EV + 16-point monotone curve on a
+    6000x4000 FP32 lognormal raster (Rec.2020 guide): fused forward, and
+    the backward chain (curve, exposure, guided filter) for dL/dp, dL/deps,
+    dL/dknots."""
+    import ctypes
+    import torch
+    import librawops_amd as L
+    from librawops_amd import _capi as C, synth
+    from librawops_amd.autograd import _multiplier
+    from librawops_amd.graph import DeviceRender, Program, compile_chain
+    W, H = 6000, 4000
+    p = torch.from_numpy(synth.lognormal_raster(W, H, seed=3)).cuda()
+    knots = synth.monotone_knots(16, seed=3)
+    img = L.RasterImage(L.RasterMetadata(W, H, 0, L.WorkingSpace.LinearRec2020D65), p)
+    node = L.WorkingYGuidedFilterNode(L.RasterSourceNode(img), L.Rect(0, 0, W, H),
+                                      L.WorkingYGuidedFilterSettings(radius, eps), max_radius=32)
+    node = L.ExposureNode(node, 0.7)
+    ident = [(0.0, 0.0), (1.0, 1.0)]
+    node = L.ExtendedCurvesNode(node, L.ExtendedCurvesSettings(knots, [ident, ident, ident],
+                                                               L.CurveInterpolation.ShapePreservingCubic))
+    c = compile_chain(node)
+    vp = L.Rect(0, 0, W, H)
+    n = ctypes.c_size_t()
+    C.check(C.lib().rawops_guided_workspace_size(W, H, vp.c(), 0, radius, ctypes.byref(n)))
+    ws = torch.empty(max(1, n.value), dtype=torch.uint8, device="cuda")
+    q = torch.empty_like(p)
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    fwd = lambda: DeviceRender.run(c, vp, L.RenderLevel(), out=q, stream=st, workspace=ws)
+    for _ in range(2):
+        fwd()
+    fwd_ms = _timed(fwd, steps, st)
+    # backward: upstream gradient through the curve, the exposure, the filter
+    gq = torch.randn_like(p)
+    m = _multiplier(0.7)
+    base = torch.empty_like(p)
+    empty = Program.from_ops(C.DEMOSAIC_BILINEAR, False, [], [])
+    C.check(C.lib().rawops_guided_filter(img._handle, vp.c(), 0, radius, eps, empty.handle,
+                                         ctypes.c_void_p(base.data_ptr()), W, ctypes.c_void_p(ws.data_ptr()),
+                                         ws.numel(), None, sp))
+    lin = base * m
+    kx = [k[0] for k in knots]
+    X = (ctypes.c_double * 22)(*(kx + [0.0, 1.0] * 3))
+    Y = (ctypes.c_double * 22)(*([k[1] for k in knots] + [0.0, 1.0] * 3))
+    plan = Program.from_ops(C.DEMOSAIC_BILINEAR, False, [], [(C.OP_CURVES, (), (X, Y, [0, 16, 18, 20, 22], 1))])
+    g1, g2, gp = torch.empty_like(p), torch.empty_like(p), torch.empty_like(p)
+    gk = torch.zeros(16, dtype=torch.float64, device="cuda")
+    gs = torch.zeros(1, dtype=torch.float64, device="cuda")
+    ge = torch.zeros(1, dtype=torch.float64, device="cuda")
+    nb = ctypes.c_size_t()
+    C.check(C.lib().rawops_guided_backward_workspace_size(W, H, ctypes.byref(nb)))
+    wsb = torch.empty(max(1, nb.value), dtype=torch.uint8, device="cuda")
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    N = p.numel()
+
+    def bwd():
+        C.check(C.lib().rawops_curve_backward(plan.handle, P(lin), P(gq), N, P(g1), P(gk), sp))
+        C.check(C.lib().rawops_exposure_backward(P(base), P(g1), m, P(g2), P(gs), N, sp))
+        C.check(C.lib().rawops_guided_backward(img._handle, radius, eps, vp.c(), P(g2), W, P(gp), P(ge),
+                                               P(wsb), wsb.numel(), sp))
+    for _ in range(2):
+        bwd()
+    bwd_ms = _timed(bwd, max(1, steps // 2), st)
+
+    def gf_only():
+        C.check(C.lib().rawops_guided_backward(img._handle, radius, eps, vp.c(), P(g2), W, P(gp), P(ge),
+                                               P(wsb), wsb.numel(), sp))
+    gfb_ms = _timed(gf_only, max(1, steps // 2), st)
+    px = W * H
+    return {"workload": f"6000x4000 FP32 lognormal raster, Rec.2020 guide, r={radius}, eps={eps}, +0.7 EV, "
+                        "16-point cubic curve; fwd fused, bwd = curve + exposure + guided filter",
+            "forward_ms": round(fwd_ms, 4), "forward_gbs": round(24 * px / fwd_ms / 1e6, 1),
+            "backward_chain_ms": round(bwd_ms, 4), "guided_backward_ms": round(gfb_ms, 4),
+            "guided_backward_gbs": round(36 * px / gfb_ms / 1e6, 1),
+            "fwd_bwd_mpix_s": round(px / (fwd_ms + bwd_ms) / 1e3, 1)}
+
+
+def leg_c4_batch(replays, seed=4):
+    """64 heterogeneous requests over 8 Bayer sources (2000x1500 .. 6048x4024,
+    RGGB/GRBG): viewports 256^2 .. full frame, (mip 0, Final) / (mip 1 or 2,
+    Preview), bilinear and Menon, FP32 linear + uint8 sRGB preview, one
+    captured persistent launch (autotuned launch shape)."""
+    import numpy as np
+    import torch
+    import librawops_amd as L
+    from librawops_amd import synth
+    from librawops_amd.batch import Batch, BatchRequest
+    rng = np.random.default_rng(seed)
+    sizes = [(int(w) // 2 * 2, int(h) // 2 * 2) for w, h in zip(np.linspace(2000, 6048, 8), np.linspace(1500, 4024, 8))]
+    cam = tuple(synth.camera_to_xyz_for(synth.camera_to_prophoto_seeded(seed)))
+    graphs = []
+    for i, (w, h) in enumerate(sizes):
+        pat = 0 if i % 2 == 0 else 3
+        s = synth.gradient_bayer(w, h, pat, 512, 16383, 20.0, seed=seed + i)
+        md = L.RawMetadata(w, h, w, L.BayerPattern(pat), 0, 0, L.Rect(0, 0, 0, 0), (512,) * 4, (16383,) * 4)
+        img = L.RawImage(md, torch.from_numpy(s.view(np.int16)).cuda())
+        rec = L.GraphRecipe(red_gain=1.9, blue_gain=1.4, exposure_stops=0.3,
+                            camera_color=L.CameraColorTransform(cam, L.WorkingSpace.LinearProPhotoD50),
+                            output_mode=L.OutputMode.SrgbPreview)
+        graphs.append([L.ImageGraph(img, rec, demosaic=d) for d in (L.BILINEAR, L.MENON)])
+    reqs, native_px, level_px = [], 0, 0
+    for k in range(64):
+        i = int(rng.integers(0, 8))
+        g = graphs[i][k % 2]
+        mip = int(rng.integers(0, 3))
+        level = L.RenderLevel(mip, L.RenderQuality.Final if mip == 0 else L.RenderQuality.Preview)
+        b = L.request_bounds(g.source_bounds(), level)
+        lo = max(1, 256 >> mip)
+        area = float(np.exp(rng.uniform(np.log(lo * lo), np.log(b.width * b.height))))
+        aspect = b.width / b.height
+        w = int(min(b.width, max(lo, round((area * aspect) ** 0.5))))
+        h = int(min(b.height, max(lo, round(area / w))))
+        vp = L.Rect(b.x + int(rng.integers(0, b.width - w + 1)), b.y + int(rng.integers(0, b.height - h + 1)), w, h)
+        reqs.append(BatchRequest(g.output(), vp, level, 0,
+                                 torch.empty((h, w, 3), dtype=torch.float32, device="cuda"),
+                                 torch.empty((h, w, 3), dtype=torch.uint8, device="cuda")))
+        level_px += w * h
+        native_px += w * h << (2 * mip)
+    batch = Batch(reqs)
+    st = torch.cuda.current_stream()
+    batch.run()
+    batch.check()
+    lat, total = [], 0.0
+    for _ in range(replays):
+        total += _timed(batch.run, 1, st)
+        lat.append(batch.latencies_ms())
+    lat = np.asarray(lat)
+    tw, th, tuned = batch.shape()
+    ms = total / replays
+    return {"workload": "64 requests, 8 sources 2000x1500..6048x4024 RGGB/GRBG, viewports 256^2..full, mip 0/1/2, "
+                        "bilinear+Menon, FP32 linear + uint8 sRGB preview, one hipGraph persistent launch",
+            "batch_ms": round(ms, 3), "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
+            "aggregate_output_mpix_s": round(level_px / ms / 1e3, 1),
+            "aggregate_native_mpix_s": round(native_px / ms / 1e3, 1),
+            "tile": f"{tw}x{th}", "autotune_ms": round(tuned, 3)}
+
+
+def leg_c5_chain(steps, seed=5):
+    """16 crops of 512x512 from a 4000x3000 RGGB scene through Menon -> WB ->
+    3x3 -> exposure -> guided-filter detail boost (r=8, eps=1e-3) -> sRGB,
+    fitted to a seeded target render with Adam (gains, matrix, exposure,
+    detail strength)."""
+    import numpy as np
+    import torch
+    import librawops_amd as L
+    from librawops_amd import autograd as A, synth
+    W, H = 4000, 3000
+    s = synth.scene_bayer(W, H, 0, 0, 0, W, (0, 0, W, H), (512,) * 4, 16383, seed=seed)
+    md = L.RawMetadata(W, H, W, L.BayerPattern(0), 0, 0, L.Rect(0, 0, 0, 0), (512,) * 4, (16383,) * 4)
+    img = L.RawImage(md, torch.from_numpy(s.view(np.int16)).cuda())
+    rng = np.random.default_rng(seed)
+    crops = [L.Rect(int(rng.integers(0, W - 512)) // 2 * 2, int(rng.integers(0, H - 512)) // 2 * 2, 512, 512)
+             for _ in range(16)]
+    M0 = torch.tensor(np.asarray(synth.camera_to_prophoto_seeded(seed), np.float32), device="cuda")
+    with torch.no_grad():
+        targets = [A.full_chain(img, vp, torch.tensor([1.5, 1.0, 1.25], device="cuda"), torch.tensor(0.4, device="cuda"),
+                                M0, torch.tensor(0.6, device="cuda")) for vp in crops]
+    gains = torch.tensor([1.0, 1.0, 1.0], device="cuda", requires_grad=True)
+    stops = torch.tensor(0.0, device="cuda", requires_grad=True)
+    mat = (M0 * 0.9).clone().requires_grad_(True)
+    strength = torch.tensor(0.2, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([gains, stops, mat, strength], lr=0.01)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = 0.0
+        for vp, tgt in zip(crops, targets):
+            y = A.full_chain(img, vp, gains, stops, mat, strength)
+            l = torch.mean((y - tgt) ** 2)
+            l.backward()
+            loss += float(l.detach())
+        opt.step()
+        losses.append(loss / len(crops))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    px = 16 * 512 * 512 * steps
+    return {"workload": f"16 x 512^2 crops of 4000x3000 RGGB, Menon->WB->3x3->exposure->detail boost(r=8,eps=1e-3)"
+                        f"->sRGB, Adam {steps} steps",
+            "fwd_bwd_mpix_s": round(px / el / 1e6, 1), "seconds": round(el, 2),
+            "loss_first": round(losses[0], 6), "loss_last": round(losses[-1], 6),
+            "gradient_check": "<=1e-4 relative vs FP64 finite differences (tests/test_autograd_gpu.py)"}
 
 
 def compiled_matrix(node):
@@ -62,6 +290,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--main-only", action="store_true", help="skip the C1/C3/C4/C5 legs")
+    ap.add_argument("--c5-steps", type=int, default=200, help="Adam steps of the differentiable-chain leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -157,7 +387,20 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = px * args.steps / wall / 1e6
     achieved = BYTES_PER_PX * px / (kernel_ms * 1e-3) / 1e9
-    cpu = None if args.no_cpu else cpu_baseline(meta, samples, cam_xyz, threads=min(16, os.cpu_count() or 1))
+    cpu, flip = None, None
+    if not args.no_cpu:
+        cpu, band = cpu_baseline(meta, samples, cam_xyz, threads=min(16, os.cpu_count() or 1))
+        got = out[:band.shape[0]].cpu().numpy()
+        diff = np.any(got.view(np.uint32) != band.view(np.uint32), axis=2)
+        # Output bits differ wherever a classifier decision flips (a flip changes the
+        # selected green and the chroma built on it); equal bits bound the flip rate.
+        flip = {"rate": float(diff.mean()), "pixels_compared": int(diff.size),
+                "basis": "bitwise GPU vs CPU oracle output on the CPU-baseline band (FP64, reference order)"}
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "menon_render_24mp.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_launch")
     line = {
         "metric": "Mpix/s 24MP Menon render (fwd, fwd+bwd) + achieved HBM GB/s vs peak",
         "value": round(value, 2),
@@ -175,7 +418,7 @@ def main():
                                "per-site black, seeded camera->ProPhoto 3x3, FP32 linear out",
                    "pixels": px, "tile": "64x32 fused LDS tile, one launch per frame"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": "render_kernel<Menon,64,32>", "kernel_ms": round(kernel_ms, 4),
                      "algorithmic_bytes_per_launch": BYTES_PER_PX * px},
         "cpu_baseline": cpu,
@@ -186,6 +429,21 @@ def main():
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["fwd_bwd"]["gpu_fwd_bwd_over_cpu_fwd"] = round(line["fwd_bwd"]["value"] / cpu["value"], 1)
+        line["menon_classifier_flip_rate"] = flip
+    if not args.main_only:
+        del gout, graw
+        torch.cuda.empty_cache()
+        legs = (("c1_bilinear", lambda: leg_c1_bilinear(20)), ("c3_guided", lambda: leg_c3_guided(6)),
+                ("c4_batch", lambda: leg_c4_batch(5)), ("c5_chain", lambda: leg_c5_chain(args.c5_steps)))
+        for name, fn in legs:
+            print(f"bench: {name}", file=sys.stderr, flush=True)
+            t = time.perf_counter()
+            try:
+                line[name] = fn()
+            except Exception as e:  # reported in the line, the headline still prints
+                line[name] = {"error": repr(e)[:400]}
+            line[name]["leg_seconds"] = round(time.perf_counter() - t, 1)
+            torch.cuda.empty_cache()
     print(json.dumps(line))
 
 
