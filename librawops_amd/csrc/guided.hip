@@ -11,9 +11,11 @@
 // moments are accumulated on values shifted by a per-strip constant so that
 // E[Y^2] - E[Y]^2 does not cancel catastrophically.
 //
-// Two strip kernels: `coefficients` writes a_k, b_k (FP64) for every centre of
-// the viewport grown by r; `apply` box-averages them, forms q and runs the
-// fused post-filter op program (exposure, curves, ...).
+// Two strip kernels: `coefficients` writes a_k, b_k for every centre of the
+// viewport grown by r (computed in FP64, stored as FP32 planes: the stored
+// values carry a relative error of 2^-24, far inside the filter's 1e-5
+// tolerance, and halve the workspace traffic); `apply` box-averages them in
+// FP64, forms q and runs the fused post-filter op program (exposure, curves).
 #include "device_common.hpp"
 #include "render_launch.hpp"
 
@@ -23,7 +25,9 @@ namespace rawops {
 namespace {
 
 constexpr int kStripThreads = 256;
-constexpr int kBandRows = 64;
+constexpr int kCols = 1;                          // columns per thread
+constexpr int kStripCols = kStripThreads * kCols; // columns per strip (centres + 2r halo)
+constexpr int kBandRows = 32;
 
 struct GuidedArgs {
     const float* src;
@@ -35,7 +39,7 @@ struct GuidedArgs {
     int vx, vy, vw, vh;      // viewport (level coords)
     int r;
     double eps, wr, wb;
-    double* coef;            // [ch][cw][6]: a0 a1 a2 b0 b1 b2
+    float* coef;             // 6 planes [ch][cw] (a0 a1 a2 b0 b1 b2), plane stride cw * ch
     float* out;
     uint32_t ostride;
     uint32_t* flags;
@@ -68,26 +72,113 @@ __device__ __forceinline__ double working_y(const GuidedArgs& g, double r, doubl
     return (gg + g.wr * (r - gg)) + g.wb * (b - gg);
 }
 
-// Inclusive prefix sum of N doubles per thread across the 256-thread strip:
-// wavefront shuffles, then wave totals through LDS.
-template <int N>
-__device__ void strip_scan(double (&v)[N], double* wave_totals /* [4][N] */) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Box sums over a strip of kStripCols columns and a band of rows, O(1) work
+// per pixel for any radius.  Thread t owns columns strip_x - r + t kCols + j
+// (j < kCols) and keeps their vertical window sums (rows k-r..k+r, clipped)
+// of the N values src(col, row) produces, sliding them down one row at a
+// time.  Per row the strip's inclusive prefix along x is formed (serial over
+// a thread's columns, then a wavefront shuffle scan of the thread totals and
+// the wave totals through LDS), so each centre's (2r+1)^2 box is two prefix
+// reads; sink(x, row, S, count) receives the clipped box sums and the true
+// window size.  Centres x in [strip_x, min(xend, strip_x + kStripCols - 2r))
+// are produced, kCols per thread.  Windows are clipped to [0, LW) x [0, LH);
+// src is only asked for columns in [c_lo, c_hi), which must hold every
+// clipped window of those centres.
+template <int N, class Src, class Sink>
+__device__ __forceinline__ void strip_box(int LW, int LH, int c_lo, int c_hi, int r, int strip_x, int xend, int y0, int y1,
+                                          double* prefix /* [N][kStripCols] */, double* totals /* [4][N] */,
+                                          Src src, Sink sink) {
+    constexpr int K = kCols;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int span = kStripCols - 2 * r;
+    const int base = strip_x - r;                 // column of prefix index 0
+    double vs[K][N];
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
+    for (int j = 0; j < K; ++j)
 #pragma unroll
-        for (int m = 0; m < N; ++m) {
-            const double o = __shfl_up(v[m], d, 64);
-            if (lane >= d) v[m] += o;
-        }
+        for (int m = 0; m < N; ++m) vs[j][m] = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int col = base + t * K + j;
+        if (col >= c_lo && col < c_hi)
+            for (int y = max(0, y0 - r); y < min(LH, y0 + r + 1); ++y) {
+                double v[N];
+                src(col, y, v);
+#pragma unroll
+                for (int m = 0; m < N; ++m) vs[j][m] += v[m];
+            }
     }
-    if (lane == 63)
+    for (int k = y0; k < y1; ++k) {
+        if (k > y0) {
 #pragma unroll
-        for (int m = 0; m < N; ++m) wave_totals[wave * N + m] = v[m];
-    __syncthreads();
-    for (int w = 0; w < wave; ++w)
+            for (int j = 0; j < K; ++j) {
+                const int col = base + t * K + j;
+                if (col < c_lo || col >= c_hi) continue;
+                if (k + r < LH) {
+                    double v[N];
+                    src(col, k + r, v);
 #pragma unroll
-        for (int m = 0; m < N; ++m) v[m] += wave_totals[w * N + m];
+                    for (int m = 0; m < N; ++m) vs[j][m] += v[m];
+                }
+                if (k - r - 1 >= 0) {
+                    double v[N];
+                    src(col, k - r - 1, v);
+#pragma unroll
+                    for (int m = 0; m < N; ++m) vs[j][m] -= v[m];
+                }
+            }
+        }
+        // Inclusive prefix along the strip.
+        double run[N], inc[N];
+#pragma unroll
+        for (int m = 0; m < N; ++m) run[m] = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+#pragma unroll
+            for (int m = 0; m < N; ++m) run[m] += vs[j][m];
+#pragma unroll
+        for (int m = 0; m < N; ++m) inc[m] = run[m];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+            for (int m = 0; m < N; ++m) {
+                const double o = __shfl_up(inc[m], d, 64);
+                if (lane >= d) inc[m] += o;
+            }
+        if (lane == 63)
+#pragma unroll
+            for (int m = 0; m < N; ++m) totals[wave * N + m] = inc[m];
+        __syncthreads();
+        double acc[N];
+#pragma unroll
+        for (int m = 0; m < N; ++m) acc[m] = inc[m] - run[m];   // exclusive within the wave
+        for (int w = 0; w < wave; ++w)
+#pragma unroll
+            for (int m = 0; m < N; ++m) acc[m] += totals[w * N + m];
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+#pragma unroll
+            for (int m = 0; m < N; ++m) {
+                acc[m] += vs[j][m];
+                prefix[m * kStripCols + t * K + j] = acc[m];
+            }
+        __syncthreads();
+        const int rows = min(LH, k + r + 1) - max(0, k - r);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int o = t * K + j;              // centre index within the strip
+            const int x = strip_x + o;
+            if (o >= span || x >= xend) continue;
+            const int lo = max(0, x - r) - base;
+            const int hi = min(LW, x + r + 1) - base;
+            double S[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+                S[m] = prefix[m * kStripCols + hi - 1] - (lo > 0 ? prefix[m * kStripCols + lo - 1] : 0.0);
+            sink(x, k, S, (double)((hi - lo) * rows));
+        }
+        __syncthreads();
+    }
 }
 
 // Moments of one (shifted) pixel: Y, p0, p1, p2, YY, Yp0, Yp1, Yp2.
@@ -101,17 +192,13 @@ __device__ __forceinline__ void moments(const GuidedArgs& g, const float p[3], c
 }
 
 __global__ void __launch_bounds__(kStripThreads) gf_coefficients(const GuidedArgs g) {
-    __shared__ double prefix[kStripThreads * 8];
+    __shared__ double prefix[kStripCols * 8];
     __shared__ double totals[4 * 8];
-    const int span = kStripThreads - 2 * g.r;            // centres per strip
-    const int t = threadIdx.x;
-    const int strip_x = g.cx0 + (int)blockIdx.x * span;  // first centre of this strip
-    const int col = strip_x - g.r + t;                   // level column this thread sums
-    const bool col_ok = col >= 0 && col < g.LW;
+    const int span = kStripCols - 2 * g.r;
+    const int strip_x = g.cx0 + (int)blockIdx.x * span;
     const int k0 = g.cy0 + (int)blockIdx.y * kBandRows;
     const int k1 = min(k0 + kBandRows, g.cy0 + g.ch);
     uint32_t err = 0;
-
     // Per-strip shift: the first centre pixel of the strip.
     double shift[3];
     {
@@ -120,157 +207,72 @@ __global__ void __launch_bounds__(kStripThreads) gf_coefficients(const GuidedArg
         shift[0] = p[0]; shift[1] = p[1]; shift[2] = p[2];
     }
     const double shift_y = working_y(g, shift[0], shift[1], shift[2]);
-
-    double vs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (col_ok) {
-        const int y0 = max(0, k0 - g.r), y1 = min(g.LH, k0 + g.r + 1);
-        for (int y = y0; y < y1; ++y) {
-            float p[3];
-            double m[8];
-            level_pixel(g, col, y, p);
-            moments(g, p, shift, m, err);
+    auto src = [&](int x, int y, double m[8]) {
+        float p[3];
+        level_pixel(g, x, y, p);
+        moments(g, p, shift, m, err);
+    };
+    auto sink = [&](int kx, int k, const double s[8], double n) {
+        const double inv_n = 1.0 / n;
+        const double my = s[0] * inv_n;
+        double var = s[4] * inv_n - my * my;
+        if (var < 0) var = 0;
+        const double inv_d = 1.0 / (var + g.eps);
+        const size_t plane = (size_t)g.cw * g.ch;
+        float* c = g.coef + (size_t)(k - g.cy0) * g.cw + (size_t)(kx - g.cx0);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) vs[i] += m[i];
+        for (int ch = 0; ch < 3; ++ch) {
+            const double mp = s[1 + ch] * inv_n;
+            const double cov = s[5 + ch] * inv_n - my * mp;
+            const double a = cov * inv_d;
+            // b in unshifted coordinates: mean(p) - a mean(Y).
+            const double b = (mp - a * my) + (shift[ch] - a * shift_y);
+            if (!__builtin_isfinite(a) || !__builtin_isfinite(b)) err |= FLAG_OVERFLOW;
+            c[ch * plane] = (float)a;
+            c[(3 + ch) * plane] = (float)b;
         }
-    }
-    for (int k = k0; k < k1; ++k) {
-        if (k > k0 && col_ok) {
-            const int enter = k + g.r, leave = k - g.r - 1;
-            if (enter < g.LH) {
-                float p[3];
-                double m[8];
-                level_pixel(g, col, enter, p);
-                moments(g, p, shift, m, err);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) vs[i] += m[i];
-            }
-            if (leave >= 0) {
-                float p[3];
-                double m[8];
-                level_pixel(g, col, leave, p);
-                moments(g, p, shift, m, err);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) vs[i] -= m[i];
-            }
-        }
-        double ps[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) ps[i] = vs[i];
-        strip_scan<8>(ps, totals);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) prefix[i * kStripThreads + t] = ps[i];
-        __syncthreads();
-        const int kx = strip_x + t;
-        if (t < span && kx < g.cx0 + g.cw) {
-            const int lo = max(0, kx - g.r) - (strip_x - g.r);
-            const int hi = min(g.LW, kx + g.r + 1) - (strip_x - g.r);
-            const int rows = min(g.LH, k + g.r + 1) - max(0, k - g.r);
-            const double n = (double)((hi - lo) * rows);
-            double s[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                s[i] = prefix[i * kStripThreads + hi - 1] - (lo > 0 ? prefix[i * kStripThreads + lo - 1] : 0.0);
-            const double my = s[0] / n;
-            double var = s[4] / n - my * my;
-            if (var < 0) var = 0;
-            const double denom = var + g.eps;
-            double* c = g.coef + ((size_t)(k - g.cy0) * g.cw + (size_t)(kx - g.cx0)) * 6;
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                const double mp = s[1 + ch] / n;
-                const double cov = s[5 + ch] / n - my * mp;
-                const double a = cov / denom;
-                // b in unshifted coordinates: mean(p) - a mean(Y).
-                const double b = (mp - a * my) + (shift[ch] - a * shift_y);
-                if (!__builtin_isfinite(a) || !__builtin_isfinite(b)) err |= FLAG_OVERFLOW;
-                c[ch] = a;
-                c[3 + ch] = b;
-            }
-        }
-        __syncthreads();
-    }
+    };
+    strip_box<8>(g.LW, g.LH, 0, g.LW, g.r, strip_x, g.cx0 + g.cw, k0, k1, prefix, totals, src, sink);
     if (err && g.flags) atomicOr(g.flags, err);
 }
 
 __global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g) {
-    __shared__ double prefix[kStripThreads * 6];
+    __shared__ double prefix[kStripCols * 6];
     __shared__ double totals[4 * 6];
-    const int span = kStripThreads - 2 * g.r;
-    const int t = threadIdx.x;
-    const int strip_x = g.vx + (int)blockIdx.x * span;   // first output column
-    const int col = strip_x - g.r + t;                   // centre column this thread sums
-    const bool col_ok = col >= g.cx0 && col < g.cx0 + g.cw;
+    const int span = kStripCols - 2 * g.r;
+    const int strip_x = g.vx + (int)blockIdx.x * span;
     const int i0 = g.vy + (int)blockIdx.y * kBandRows;
     const int i1 = min(i0 + kBandRows, g.vy + g.vh);
     uint32_t err = 0;
-
-    auto coef_row = [&](int k, double c[6]) {
-        const double* q = g.coef + ((size_t)(k - g.cy0) * g.cw + (size_t)(col - g.cx0)) * 6;
+    // Coefficients exist for centres in [cx0, cx0 + cw) x [cy0, cy0 + ch):
+    // exactly the clipped windows of the viewport's pixels.
+    const size_t plane = (size_t)g.cw * g.ch;
+    auto src = [&](int k_x, int k_y, double c[6]) {
+        const float* q = g.coef + (size_t)(k_y - g.cy0) * g.cw + (size_t)(k_x - g.cx0);
 #pragma unroll
-        for (int m = 0; m < 6; ++m) c[m] = q[m];
+        for (int m = 0; m < 6; ++m) c[m] = (double)q[m * plane];
     };
-    double vs[6] = {0, 0, 0, 0, 0, 0};
-    if (col_ok) {
-        const int y0 = max(0, i0 - g.r), y1 = min(g.LH, i0 + g.r + 1);
-        for (int k = y0; k < y1; ++k) {
-            double c[6];
-            coef_row(k, c);
+    auto sink = [&](int x, int i, const double S[6], double n) {
+        const double inv_n = 1.0 / n;
+        float p[3];
+        level_pixel(g, x, i, p);
+        const double y = working_y(g, (double)p[0], (double)p[1], (double)p[2]);
+        float o[3];
 #pragma unroll
-            for (int m = 0; m < 6; ++m) vs[m] += c[m];
+        for (int ch = 0; ch < 3; ++ch) {
+            const double q = (S[ch] * inv_n) * y + S[3 + ch] * inv_n;
+            const double delta = q - (double)p[ch];
+            if (!__builtin_isfinite(delta)) err |= FLAG_OVERFLOW;
+            if (delta == 0) { o[ch] = p[ch]; continue; }   // exact copy, as the reference
+            const double value = (double)p[ch] + delta;
+            if (value < -3.4028234663852886e38 || value > 3.4028234663852886e38) err |= FLAG_OVERFLOW;
+            o[ch] = (float)value;
         }
-    }
-    for (int i = i0; i < i1; ++i) {
-        if (i > i0 && col_ok) {
-            const int enter = i + g.r, leave = i - g.r - 1;
-            if (enter < g.LH) {
-                double c[6];
-                coef_row(enter, c);
-#pragma unroll
-                for (int m = 0; m < 6; ++m) vs[m] += c[m];
-            }
-            if (leave >= 0) {
-                double c[6];
-                coef_row(leave, c);
-#pragma unroll
-                for (int m = 0; m < 6; ++m) vs[m] -= c[m];
-            }
-        }
-        double ps[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) ps[m] = vs[m];
-        strip_scan<6>(ps, totals);
-#pragma unroll
-        for (int m = 0; m < 6; ++m) prefix[m * kStripThreads + t] = ps[m];
-        __syncthreads();
-        const int x = strip_x + t;
-        if (t < span && x < g.vx + g.vw) {
-            const int lo = max(0, x - g.r) - (strip_x - g.r);
-            const int hi = min(g.LW, x + g.r + 1) - (strip_x - g.r);
-            const int rows = min(g.LH, i + g.r + 1) - max(0, i - g.r);
-            const double n = (double)((hi - lo) * rows);
-            float p[3];
-            level_pixel(g, x, i, p);
-            const double y = working_y(g, (double)p[0], (double)p[1], (double)p[2]);
-            float o[3];
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                const double sa = prefix[ch * kStripThreads + hi - 1] - (lo > 0 ? prefix[ch * kStripThreads + lo - 1] : 0.0);
-                const double sb = prefix[(3 + ch) * kStripThreads + hi - 1] -
-                                  (lo > 0 ? prefix[(3 + ch) * kStripThreads + lo - 1] : 0.0);
-                const double q = (sa / n) * y + sb / n;
-                const double delta = q - (double)p[ch];
-                if (!__builtin_isfinite(delta)) err |= FLAG_OVERFLOW;
-                if (delta == 0) { o[ch] = p[ch]; continue; }   // exact copy, as the reference
-                const double value = (double)p[ch] + delta;
-                if (value < -3.4028234663852886e38 || value > 3.4028234663852886e38) err |= FLAG_OVERFLOW;
-                o[ch] = (float)value;
-            }
-            apply_ops(*g.post, o[0], o[1], o[2], err);
-            float* d = g.out + ((size_t)(i - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
-            d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
-        }
-        __syncthreads();
-    }
+        apply_ops(*g.post, o[0], o[1], o[2], err);
+        float* d = g.out + ((size_t)(i - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
+        d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
+    };
+    strip_box<6>(g.LW, g.LH, g.cx0, g.cx0 + g.cw, g.r, strip_x, g.vx + g.vw, i0, i1, prefix, totals, src, sink);
     if (err && g.flags) atomicOr(g.flags, err);
 }
 
@@ -300,7 +302,8 @@ __global__ void gf_identity(const GuidedArgs g) {
 // gq_i Y_i / n_i, gq_i / n_i give a_k, b_k and their adjoints, pushed back to
 // the moments: P_k = dmu_p / n_k, Q_k = dE[Yp] / n_k, R_k = dmu_Y / n_k,
 // S_k = dE[Y^2] / n_k (and a_k for the direct Y term).  Pass 2 (per pixel
-// j): box sums of those 11 values give dL/dp_j.  Moments are unshifted FP64.
+// j): box sums of those 11 values give dL/dp_j.  Moments are unshifted FP64;
+// the per-centre terms are stored as FP32 planes.
 
 struct GuidedBackArgs {
     const float* p;          // raster, level 0 (W x H, stride)
@@ -310,74 +313,20 @@ struct GuidedBackArgs {
     const float* gq;         // dL/dq over the viewport
     uint32_t gstride;
     int vx, vy, vw, vh;
-    double* ws;              // [H][W][11] per-centre adjoint terms
+    float* ws;               // 11 planes [H][W] of per-centre adjoint terms
     float* gp;               // dL/dp, W x H x 3 (written in full)
     double* geps;            // accumulated dL/deps
 };
-
-template <int N, class Src, class Sink>
-__device__ __forceinline__ void strip_box(int LW, int LH, int r, int strip_x, int y0, int y1, double* prefix,
-                                          double* totals, Src src, Sink sink) {
-    const int t = threadIdx.x;
-    const int span = kStripThreads - 2 * r;
-    const int col = strip_x - r + t;
-    const bool col_ok = col >= 0 && col < LW;
-    double vs[N];
-#pragma unroll
-    for (int m = 0; m < N; ++m) vs[m] = 0.0;
-    if (col_ok)
-        for (int y = max(0, y0 - r); y < min(LH, y0 + r + 1); ++y) {
-            double v[N];
-            src(col, y, v);
-#pragma unroll
-            for (int m = 0; m < N; ++m) vs[m] += v[m];
-        }
-    for (int k = y0; k < y1; ++k) {
-        if (k > y0 && col_ok) {
-            if (k + r < LH) {
-                double v[N];
-                src(col, k + r, v);
-#pragma unroll
-                for (int m = 0; m < N; ++m) vs[m] += v[m];
-            }
-            if (k - r - 1 >= 0) {
-                double v[N];
-                src(col, k - r - 1, v);
-#pragma unroll
-                for (int m = 0; m < N; ++m) vs[m] -= v[m];
-            }
-        }
-        double ps[N];
-#pragma unroll
-        for (int m = 0; m < N; ++m) ps[m] = vs[m];
-        strip_scan<N>(ps, totals);
-#pragma unroll
-        for (int m = 0; m < N; ++m) prefix[m * kStripThreads + t] = ps[m];
-        __syncthreads();
-        const int x = strip_x + t;
-        if (t < span && x < LW) {
-            const int lo = max(0, x - r) - (strip_x - r);
-            const int hi = min(LW, x + r + 1) - (strip_x - r);
-            const int rows = min(LH, k + r + 1) - max(0, k - r);
-            double S[N];
-#pragma unroll
-            for (int m = 0; m < N; ++m)
-                S[m] = prefix[m * kStripThreads + hi - 1] - (lo > 0 ? prefix[m * kStripThreads + lo - 1] : 0.0);
-            sink(x, k, S, (double)((hi - lo) * rows));
-        }
-        __syncthreads();
-    }
-}
 
 __device__ __forceinline__ double window_count(int x, int y, int W, int H, int r) {
     return (double)((min(W, x + r + 1) - max(0, x - r)) * (min(H, y + r + 1) - max(0, y - r)));
 }
 
 __global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBackArgs g) {
-    __shared__ double prefix[kStripThreads * 14];
+    __shared__ double prefix[kStripCols * 14];
     __shared__ double totals[4 * 14];
     __shared__ double eps_part[kStripThreads / 64];
-    const int span = kStripThreads - 2 * g.r;
+    const int span = kStripCols - 2 * g.r;
     const int strip_x = (int)blockIdx.x * span;
     const int y0 = (int)blockIdx.y * kBandRows, y1 = min(y0 + kBandRows, g.H);
     double geps = 0.0;
@@ -390,40 +339,52 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBac
         const int lx = x - g.vx, ly = y - g.vy;
         if (lx >= 0 && ly >= 0 && lx < g.vw && ly < g.vh) {
             const float* d = g.gq + ((size_t)ly * g.gstride + (size_t)lx) * 3;
-            const double n = window_count(x, y, g.W, g.H, g.r);
-            for (int c = 0; c < 3; ++c) { v[8 + c] = (double)d[c] * Y / n; v[11 + c] = (double)d[c] / n; }
+            const double inv_n = 1.0 / window_count(x, y, g.W, g.H, g.r);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { v[8 + c] = (double)d[c] * Y * inv_n; v[11 + c] = (double)d[c] * inv_n; }
         } else {
+#pragma unroll
             for (int c = 0; c < 6; ++c) v[8 + c] = 0.0;
         }
     };
     auto sink = [&](int x, int y, const double S[14], double n) {
-        const double my = S[0] / n, eyy = S[4] / n;
+        const double inv_n = 1.0 / n;
+        const double my = S[0] * inv_n, eyy = S[4] * inv_n;
         const double var = eyy - my * my, D = (var < 0 ? 0 : var) + g.eps;
+        const double inv_D = 1.0 / D;
         double a[3], mp[3], ga[3], gmp[3], gcov[3];
         double gmy = 0, gD = 0;
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            mp[c] = S[1 + c] / n;
-            const double cov = S[5 + c] / n - my * mp[c];
-            a[c] = cov / D;
+            mp[c] = S[1 + c] * inv_n;
+            const double cov = S[5 + c] * inv_n - my * mp[c];
+            a[c] = cov * inv_D;
             const double GA = S[8 + c], GB = S[11 + c];
             gmp[c] = GB;
             ga[c] = GA - my * GB;
             gmy -= a[c] * GB;
-            gcov[c] = ga[c] / D;
-            gD -= ga[c] * a[c] / D;
+            gcov[c] = ga[c] * inv_D;
+            gD -= ga[c] * a[c] * inv_D;
         }
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             gmy -= gcov[c] * mp[c];
             gmp[c] -= gcov[c] * my;
         }
         gmy -= 2.0 * my * gD;
         geps += gD;
-        double* w = g.ws + ((size_t)y * g.W + (size_t)x) * 11;
-        for (int c = 0; c < 3; ++c) { w[c] = gmp[c] / n; w[3 + c] = gcov[c] / n; w[8 + c] = a[c]; }
-        w[6] = gmy / n;
-        w[7] = gD / n;
+        const size_t plane = (size_t)g.W * g.H;
+        float* w = g.ws + (size_t)y * g.W + (size_t)x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            w[c * plane] = (float)(gmp[c] * inv_n);
+            w[(3 + c) * plane] = (float)(gcov[c] * inv_n);
+            w[(8 + c) * plane] = (float)a[c];
+        }
+        w[6 * plane] = (float)(gmy * inv_n);
+        w[7 * plane] = (float)(gD * inv_n);
     };
-    strip_box<14>(g.W, g.H, g.r, strip_x, y0, y1, prefix, totals, src, sink);
+    strip_box<14>(g.W, g.H, 0, g.W, g.r, strip_x, g.W, y0, y1, prefix, totals, src, sink);
     for (int off = 32; off > 0; off >>= 1) geps += __shfl_xor(geps, off, 64);
     if ((threadIdx.x & 63) == 0) eps_part[threadIdx.x >> 6] = geps;
     __syncthreads();
@@ -435,15 +396,16 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBac
 }
 
 __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBackArgs g) {
-    __shared__ double prefix[kStripThreads * 11];
+    __shared__ double prefix[kStripCols * 11];
     __shared__ double totals[4 * 11];
-    const int span = kStripThreads - 2 * g.r;
+    const int span = kStripCols - 2 * g.r;
     const int strip_x = (int)blockIdx.x * span;
     const int y0 = (int)blockIdx.y * kBandRows, y1 = min(y0 + kBandRows, g.H);
+    const size_t plane = (size_t)g.W * g.H;
     auto src = [&](int x, int y, double v[11]) {
-        const double* w = g.ws + ((size_t)y * g.W + (size_t)x) * 11;
+        const float* w = g.ws + (size_t)y * g.W + (size_t)x;
 #pragma unroll
-        for (int m = 0; m < 11; ++m) v[m] = w[m];
+        for (int m = 0; m < 11; ++m) v[m] = (double)w[m * plane];
     };
     auto sink = [&](int x, int y, const double S[11], double n) {
         const float* q = g.p + ((size_t)y * g.stride + (size_t)x) * 3;
@@ -451,6 +413,7 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBack
         const double Y = (p[1] + g.wr * (p[0] - p[1])) + g.wb * (p[2] - p[1]);
         double gY = S[6] + 2.0 * S[7] * Y;
         double gp[3];
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             gp[c] = S[c] + S[3 + c] * Y;
             gY += S[3 + c] * p[c];
@@ -458,7 +421,9 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBack
         const int lx = x - g.vx, ly = y - g.vy;
         if (lx >= 0 && ly >= 0 && lx < g.vw && ly < g.vh) {
             const float* d = g.gq + ((size_t)ly * g.gstride + (size_t)lx) * 3;
-            for (int c = 0; c < 3; ++c) gY += (double)d[c] * (S[8 + c] / n);
+            const double inv_n = 1.0 / n;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gY += (double)d[c] * (S[8 + c] * inv_n);
         }
         gp[0] += gY * g.wr;
         gp[1] += gY * (1.0 - g.wr - g.wb);
@@ -466,7 +431,7 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBack
         float* o = g.gp + ((size_t)y * g.W + (size_t)x) * 3;
         o[0] = (float)gp[0]; o[1] = (float)gp[1]; o[2] = (float)gp[2];
     };
-    strip_box<11>(g.W, g.H, g.r, strip_x, y0, y1, prefix, totals, src, sink);
+    strip_box<11>(g.W, g.H, 0, g.W, g.r, strip_x, g.W, y0, y1, prefix, totals, src, sink);
 }
 
 }  // namespace
@@ -478,11 +443,11 @@ size_t guided_workspace_bytes(const GuidedLaunch& L) {
     const int cx0 = std::max(0, (int)L.viewport.x - r), cy0 = std::max(0, (int)L.viewport.y - r);
     const int cx1 = std::min(LW, (int)(L.viewport.x + L.viewport.width) + r);
     const int cy1 = std::min(LH, (int)(L.viewport.y + L.viewport.height) + r);
-    return (size_t)(cx1 - cx0) * (size_t)(cy1 - cy0) * 6 * sizeof(double);
+    return (size_t)(cx1 - cx0) * (size_t)(cy1 - cy0) * 6 * sizeof(float);
 }
 
 size_t guided_backward_workspace_bytes(uint32_t width, uint32_t height) {
-    return (size_t)width * height * 11 * sizeof(double);
+    return (size_t)width * height * 11 * sizeof(float);
 }
 
 hipError_t launch_guided_backward(const GuidedBackwardLaunch& L, hipStream_t stream) {
@@ -499,10 +464,10 @@ hipError_t launch_guided_backward(const GuidedBackwardLaunch& L, hipStream_t str
     g.gstride = L.g_stride;
     g.vx = (int)L.viewport.x; g.vy = (int)L.viewport.y;
     g.vw = (int)L.viewport.width; g.vh = (int)L.viewport.height;
-    g.ws = static_cast<double*>(L.workspace);
+    g.ws = static_cast<float*>(L.workspace);
     g.gp = L.g_in;
     g.geps = L.g_eps;
-    const int span = kStripThreads - 2 * g.r;
+    const int span = kStripCols - 2 * g.r;
     dim3 grid((unsigned)((g.W + span - 1) / span), (unsigned)((g.H + kBandRows - 1) / kBandRows));
     hipLaunchKernelGGL(gf_back_centres, grid, dim3(kStripThreads), 0, stream, g);
     hipError_t e = hipGetLastError();
@@ -541,8 +506,8 @@ hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t 
         hipLaunchKernelGGL(gf_identity, dim3(blocks), dim3(256), 0, stream, g);
         return hipGetLastError();
     }
-    g.coef = static_cast<double*>(L.workspace);
-    const int span = kStripThreads - 2 * g.r;
+    g.coef = static_cast<float*>(L.workspace);
+    const int span = kStripCols - 2 * g.r;
     dim3 grid_c((unsigned)((g.cw + span - 1) / span), (unsigned)((g.ch + kBandRows - 1) / kBandRows));
     hipLaunchKernelGGL(gf_coefficients, grid_c, dim3(kStripThreads), 0, stream, g);
     hipError_t e = hipGetLastError();
