@@ -42,6 +42,16 @@ struct OpList {
     DevOp op[kMaxOps];
 };
 
+// Plan-owned tables (op lists, curves) are written by the host before a
+// launch and only read by kernels: reading them through the constant address
+// space lets uniform reads go through the scalar cache (no vector-memory wait).
+template <class T>
+using Const = __attribute__((address_space(4))) const T;
+template <class T>
+__device__ __forceinline__ Const<T>& as_const(const T& v) {
+    return *(Const<T>*)&v;
+}
+
 // Bayer sensor as the kernels see it (RawMetadata after normalisation).
 struct SensorArgs {
     const uint16_t* samples;
@@ -127,12 +137,22 @@ __device__ __forceinline__ double cubic_blend(double a, double b, double t) {
 }
 
 // ExtendedCurvesNode::map, ToneOps.cpp:820-831.
-__device__ inline double curve_map(const DevCurve& c, uint32_t cubic, double v) {
+__device__ inline double curve_map(const DevCurve& curve, uint32_t cubic, double v) {
+    Const<DevCurve>& c = as_const(curve);
     if (c.identity) return v;
-    uint32_t lo = 0, hi = c.n;                 // upper_bound on x
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (v < c.x[mid]) hi = mid; else lo = mid + 1;
+    // upper_bound on x (v is finite, x strictly increasing).  Short curves:
+    // count the knots <= v, so every knot load has the same address on all
+    // lanes and none waits on another; long curves: binary search.
+    uint32_t lo = 0;
+    if (c.n <= 32) {
+#pragma unroll 8
+        for (uint32_t k = 0; k < c.n; ++k) lo += v >= c.x[k] ? 1u : 0u;
+    } else {
+        uint32_t hi = c.n;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (v < c.x[mid]) hi = mid; else lo = mid + 1;
+        }
     }
     const uint32_t left = lo == 0 ? 0u : lo - 1u;
     const uint32_t i = left < c.n - 2u ? left : c.n - 2u;
@@ -174,10 +194,11 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
 }
 
 // The fused op program: each op rounds exactly like its reference node.
-__device__ __forceinline__ void apply_ops(const OpList& L, uint32_t begin, uint32_t end, float& r, float& g, float& b,
+__device__ __forceinline__ void apply_ops(const OpList& list, uint32_t begin, uint32_t end, float& r, float& g, float& b,
                                           uint32_t& err) {
+    Const<OpList>& L = as_const(list);
     for (uint32_t i = begin; i < end; ++i) {
-        const DevOp& o = L.op[i];
+        Const<DevOp>& o = L.op[i];
         switch (o.kind) {
         case OP_GAINS: r *= o.v[0]; g *= o.v[1]; b *= o.v[2]; break;
         case OP_SCALE: r *= o.v[0]; g *= o.v[0]; b *= o.v[0]; break;
@@ -210,16 +231,17 @@ __device__ __forceinline__ void apply_ops(const OpList& L, uint32_t begin, uint3
 }
 
 __device__ __forceinline__ void apply_ops(const OpList& L, float& r, float& g, float& b, uint32_t& err) {
-    apply_ops(L, 0, L.n, r, g, b, err);
+    apply_ops(L, 0, as_const(L).n, r, g, b, err);
 }
 
 // The op program over a thread's pixels, op by op: one (uniform) dispatch
 // per op for all NPX pixels.  err[p] collects pixel p's error bits.
 template <int NPX>
-__device__ __forceinline__ void apply_ops_px(const OpList& L, uint32_t begin, uint32_t end, float (&px)[NPX][3],
+__device__ __forceinline__ void apply_ops_px(const OpList& list, uint32_t begin, uint32_t end, float (&px)[NPX][3],
                                              uint32_t (&err)[NPX]) {
+    Const<OpList>& L = as_const(list);
     for (uint32_t i = begin; i < end; ++i) {
-        const DevOp& o = L.op[i];
+        Const<DevOp>& o = L.op[i];
         switch (o.kind) {
         case OP_GAINS: {
             const float a = o.v[0], b = o.v[1], c = o.v[2];
