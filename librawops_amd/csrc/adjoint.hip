@@ -25,7 +25,7 @@
 namespace rawops {
 namespace {
 
-constexpr int kNT = 256;
+constexpr int kNT = 512;
 
 struct BackwardArgs {
     SensorArgs s;
@@ -71,7 +71,7 @@ __device__ __forceinline__ void param_grads(const BackwardArgs& a, const float d
         for (int c = 0; c < 3; ++c) acc[4 + r * 3 + c] += (double)go[r] * ((double)a.m * a.g[c] * d[c]);
 }
 
-__device__ void flush_params(const BackwardArgs& a, double acc[13], double* red /* [4][13] LDS */) {
+__device__ void flush_params(const BackwardArgs& a, double acc[13], double* red /* [waves][13] LDS */) {
 #pragma unroll
     for (int i = 0; i < 13; ++i)
         for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
@@ -261,7 +261,7 @@ template <int TW, int TH>
 __global__ void __launch_bounds__(kNT) bilinear_backward(const BackwardArgs a) {
     __shared__ float raw[(TW + 2) * (TH + 2)];
     __shared__ float adj[TW * TH];
-    __shared__ double red[4 * 13];
+    __shared__ double red[(kNT / 64) * 13];
     int tx, ty;
     tile_of(a, tx, ty, TW, TH);
     const SensorArgs s = a.s;
