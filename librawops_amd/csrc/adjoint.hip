@@ -59,7 +59,7 @@ __device__ __forceinline__ void chain_back(const BackwardArgs& a, const float go
 
 // Parameter gradients of one pixel with demosaic output d.
 __device__ __forceinline__ void param_grads(const BackwardArgs& a, const float d[3], const float go[3],
-                                            const float u[3], double acc[13]) {
+                                            const float u[3], double (&acc)[13]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         acc[c] += (double)a.m * d[c] * u[c];                  // d out / d g_c
@@ -71,7 +71,7 @@ __device__ __forceinline__ void param_grads(const BackwardArgs& a, const float d
         for (int c = 0; c < 3; ++c) acc[4 + r * 3 + c] += (double)go[r] * ((double)a.m * a.g[c] * d[c]);
 }
 
-__device__ void flush_params(const BackwardArgs& a, double acc[13], double* red /* [waves][13] LDS */) {
+__device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc)[13], double* red /* [waves][13] LDS */) {
 #pragma unroll
     for (int i = 0; i < 13; ++i)
         for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(kNT) menon_backward(const BackwardArgs a) {
     __shared__ __attribute__((aligned(16))) char lds[MenonLds<TW + 8, TH + 8>::BYTES];
     int tx, ty;
     tile_of(a, tx, ty, TW, TH);
-    const SensorArgs s = a.s;
+    const SensorArgs& s = a.s;
     const bool interior = tx - 10 >= s.ax && ty - 10 >= s.ay && tx + TW + 10 <= s.ax + s.aw &&
                           ty + TH + 10 <= s.ay + s.ah;
     if (interior) menon_back_tile<TW, TH, false>(a, s, lds, tx, ty);
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(kNT) bilinear_backward(const BackwardArgs a) {
     __shared__ double red[(kNT / 64) * 13];
     int tx, ty;
     tile_of(a, tx, ty, TW, TH);
-    const SensorArgs s = a.s;
+    const SensorArgs& s = a.s;
     load_raw<TW, TH, 1, true, kNT>(s, tx, ty, raw);
     for (int i = threadIdx.x; i < TW * TH; i += kNT) adj[i] = 0.0f;
     __syncthreads();
@@ -330,9 +330,12 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     a.gstride = L.g_stride;
     a.graw = L.g_raw;
     a.gparams = L.g_params;
-    constexpr int TW = 64, TH = 32;
     // Only raw samples within 4 of the viewport receive gradient (the
-    // adjoint's reach); tiles are anchored at the active-area origin.
+    // adjoint's reach); tiles are anchored at the active-area origin.  The
+    // Menon tile is 64x24 so its recompute region (tile +-4) fits two
+    // workgroups per CU in LDS.
+    const bool menon = L.demosaic == RAWOPS_DEMOSAIC_MENON;
+    const int TW = 64, TH = menon ? 24 : 32;
     const int x0 = std::max(0, a.vx - 4 - a.s.ax), y0 = std::max(0, a.vy - 4 - a.s.ay);
     const int x1 = std::min(a.s.aw, a.vx + a.vw + 4 - a.s.ax), y1 = std::min(a.s.ah, a.vy + a.vh + 4 - a.s.ay);
     a.tile0_x = x0 / TW;
@@ -341,10 +344,10 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     const int tiles_y = (y1 + TH - 1) / TH - a.tile0_y;
     if (a.vw <= 0 || a.vh <= 0 || a.tiles_x <= 0 || tiles_y <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
-    if (L.demosaic == RAWOPS_DEMOSAIC_MENON)
-        hipLaunchKernelGGL((menon_backward<TW, TH>), dim3(blocks), dim3(kNT), 0, stream, a);
+    if (menon)
+        hipLaunchKernelGGL((menon_backward<64, 24>), dim3(blocks), dim3(kNT), 0, stream, a);
     else
-        hipLaunchKernelGGL((bilinear_backward<TW, TH>), dim3(blocks), dim3(kNT), 0, stream, a);
+        hipLaunchKernelGGL((bilinear_backward<64, 32>), dim3(blocks), dim3(kNT), 0, stream, a);
     return hipGetLastError();
 }
 
