@@ -93,6 +93,127 @@ __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty,
 
 // --------------------------------------------------------------- Menon --
 
+// Interior tiles: the same transposes as the scatter path below, written as
+// gathers (each stage's adjoint at a site is a fixed stencil over its
+// neighbours and their recorded decisions), so no LDS atomics are needed.
+// Extended-tile coordinates (u, v) = sensor - (ex, ey); the tile is
+// [4, 4+TW) x [4, 4+TH).  Planes over the extended tile are compacted per
+// site class: index v (EW/2) + (u >> 1).
+//   pass 1 (non-green sites): gd = dL/d demosaic output; keep the channel the
+//          final chroma step mixes (PO) and the green channel (PG)
+//   pass 2 (green sites): red/blue adjoints AR, AB (own gd plus the +-0.5
+//          terms of the non-green neighbours whose direction covers the site)
+//   pass 3 (non-green sites): green adjoint AG = PG - 0.5 sum of the 4
+//          neighbours' AR (red sites) or AB (blue sites)
+//   pass 4 (tile): raw adjoint through the directional-green stencil.
+template <int TW, int TH, class Tile>
+__device__ __forceinline__ void gather_tile(const BackwardArgs& a, const SensorArgs& s, char* lds, const Tile& t,
+                                            int ex, int ey, int tx, int ty) {
+    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
+    using L = MenonLds<EW, EH>;
+    static_assert(4 * NC * 4 <= L::DH_BYTES + L::DV_BYTES, "gather planes do not fit over dh/dv");
+    static_assert((NC + TW * TH) * 4 <= L::G2_BYTES, "gather planes do not fit over green");
+    float* PO = reinterpret_cast<float*>(lds + L::OFF_DH);
+    float* PG = PO + NC;
+    float* AR = PG + NC;
+    float* AB = AR + NC;
+    float* AG = reinterpret_cast<float*>(lds + L::OFF_GREEN);   // after pass 1: green is dead
+    float* RP = AG + NC;                                          // raw partials of the tile
+    auto ci = [](int u, int v) { return v * HWE + (u >> 1); };
+    auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
+    auto gd_at = [&](int u, int v, float gd[3], float go[3], float uu[3]) -> bool {
+        if (!load_gout(a, ex + u, ey + v, go)) { gd[0] = gd[1] = gd[2] = 0.f; return false; }
+        chain_back(a, go, uu, gd);
+        return true;
+    };
+    double acc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+    // pass 1: non-green sites of the extended tile
+    for (int i = threadIdx.x; i < NC; i += kNT) {
+        const int v = i / HWE, u = 2 * (i - v * HWE) + t.ng_parity(v);
+        float gd[3], go[3], uu[3];
+        const bool has = gd_at(u, v, gd, go, uu);
+        const bool red_site = color_of(s, ex + u, ey + v) == 0;
+        PO[i] = red_site ? gd[2] : gd[0];
+        PG[i] = gd[1];
+        if (has && in_tile(u, v)) {
+            float d[3];
+            t.finish(u, v, d[0], d[1], d[2]);
+            param_grads(a, d, go, uu, acc);
+        }
+    }
+    __syncthreads();
+    // pass 2: green sites (tile +-3 needs them; every green site with four
+    // neighbours inside the extended tile is done)
+    for (int i = threadIdx.x; i < NC; i += kNT) {
+        const int v = i / HWE, u = 2 * (i - v * HWE) + (t.ng_parity(v) ^ 1);
+        if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1) continue;
+        float gd[3], go[3], uu[3];
+        const bool has = gd_at(u, v, gd, go, uu);
+        float ar = gd[0], ab = gd[2];
+        auto take = [&](int nu, int nv, bool horizontal_needed) {
+            const bool horizontal = t.hz[nv * HWE + (nu >> 1)] != 0;
+            if (horizontal != horizontal_needed) return;
+            const float w = 0.5f * PO[ci(nu, nv)];
+            if (color_of(s, ex + nu, ey + nv) == 0) { ar -= w; ab += w; }   // R site: B = r + mean(B) - mean(R)
+            else { ar += w; ab -= w; }                                      // B site: R = b + mean(R) - mean(B)
+        };
+        take(u - 1, v, true);
+        take(u + 1, v, true);
+        take(u, v - 1, false);
+        take(u, v + 1, false);
+        AR[i] = ar;
+        AB[i] = ab;
+        if (in_tile(u, v)) {
+            RP[(v - 4) * TW + (u - 4)] = ar + ab + gd[1];
+            if (has) {
+                float d[3];
+                t.finish(u, v, d[0], d[1], d[2]);
+                param_grads(a, d, go, uu, acc);
+            }
+        }
+    }
+    __syncthreads();
+    // pass 3: non-green sites of tile +-2
+    for (int i = threadIdx.x; i < NC; i += kNT) {
+        const int v = i / HWE, u = 2 * (i - v * HWE) + t.ng_parity(v);
+        if (u < 2 || u >= EW - 2 || v < 2 || v >= EH - 2) continue;
+        const bool red_site = color_of(s, ex + u, ey + v) == 0;
+        const float* A = red_site ? AR : AB;
+        const float S = (A[ci(u - 1, v)] + A[ci(u + 1, v)]) + (A[ci(u, v - 1)] + A[ci(u, v + 1)]);
+        AG[i] = PG[i] - 0.5f * S;
+        if (in_tile(u, v)) {
+            float gd[3], go[3], uu[3];
+            gd_at(u, v, gd, go, uu);
+            RP[(v - 4) * TW + (u - 4)] = (gd[0] + gd[2]) + 0.5f * S;   // own sample feeds both r and b
+        }
+    }
+    __syncthreads();
+    // pass 4: raw adjoint of the tile through the directional-green stencil
+    for (int i = threadIdx.x; i < TW * TH; i += kNT) {
+        const int v = 4 + i / TW, u = 4 + (i - (i / TW) * TW);
+        const bool green_site = (u & 1) != t.ng_parity(v);
+        float r = RP[i];
+        auto hzat = [&](int nu, int nv) { return t.hz[nv * HWE + (nu >> 1)] != 0; };
+        if (green_site) {
+            if (hzat(u - 1, v)) r += 0.5f * AG[ci(u - 1, v)];
+            if (hzat(u + 1, v)) r += 0.5f * AG[ci(u + 1, v)];
+            if (!hzat(u, v - 1)) r += 0.5f * AG[ci(u, v - 1)];
+            if (!hzat(u, v + 1)) r += 0.5f * AG[ci(u, v + 1)];
+        } else {
+            r += 0.5f * AG[ci(u, v)];
+            if (hzat(u - 2, v)) r -= 0.25f * AG[ci(u - 2, v)];
+            if (hzat(u + 2, v)) r -= 0.25f * AG[ci(u + 2, v)];
+            if (!hzat(u, v - 2)) r -= 0.25f * AG[ci(u, v - 2)];
+            if (!hzat(u, v + 2)) r -= 0.25f * AG[ci(u, v + 2)];
+        }
+        const int sx = tx + (u - 4), sy = ty + (v - 4);
+        if (a.graw && sx < s.ax + s.aw && sy < s.ay + s.ah)
+            a.graw[(size_t)(sy - s.ay) * s.aw + (size_t)(sx - s.ax)] = r;
+    }
+    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GH));
+}
+
 template <int TW, int TH, bool BORDER>
 __device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx, int ty) {
     constexpr int EW = TW + 8, EH = TH + 8;  // forward recompute region: tile +-4
@@ -131,6 +252,11 @@ __device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const Sen
     t.classify();
     __syncthreads();
     t.chroma();
+    __syncthreads();
+    if constexpr (!BORDER) {
+        gather_tile<TW, TH>(a, s, lds, t, ex, ey, tx, ty);
+        return;
+    }
     for (int i = threadIdx.x; i < 2 * RB_N + GA_N + GR_N; i += kNT) adj_r[i] = 0.0f;
     __syncthreads();
 

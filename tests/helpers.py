@@ -70,3 +70,36 @@ def first_mismatch(a, b):
         return None
     i = tuple(idx[0])
     return i, float(np.asarray(a)[i]), float(np.asarray(b)[i]), int(idx.shape[0])
+
+
+def menon_adjoint_by_probing(om, dec, wgt, vp, gains, mult, M):
+    """dL/d(normalised raw) of L(r) = sum(wgt * (mult * M @ (gains * menon(r))))
+    over the viewport vp = (x, y, w, h) (sensor coordinates), with the Menon
+    decisions frozen to dec, using the FP64 oracle forward only.
+
+    With frozen decisions the demosaic is linear and homogeneous in the raw
+    samples, and a sample moves outputs within Chebyshev distance 4 only
+    (directional green 2, green-site chroma 1, final chroma 1; whole-sample
+    reflection at the area edge does not lengthen a read).  Samples 10 apart
+    therefore have disjoint 9x9 output windows: one forward per lattice
+    offset, over the indicator of that lattice, gives every lattice sample's
+    adjoint as the window sum of the per-pixel loss density."""
+    ax, ay, aw, ah = om.area
+    P, R = 10, 4
+    vx, vy, vw, vh = vp
+    lin = (np.asarray(M, np.float64) * np.asarray(gains, np.float64)[None, :]) * mult   # out = lin @ demosaic
+    out = np.zeros((ah, aw))
+    for oy in range(P):
+        for ox in range(P):
+            d = np.zeros((ah, aw))
+            d[oy::P, ox::P] = 1.0
+            y = O.menon_f64(om, d, om.area, hz=dec)[0][vy - ay:vy - ay + vh, vx - ax:vx - ax + vw]
+            lam = np.zeros((ah + 2 * R, aw + 2 * R))
+            lam[R + vy - ay:R + vy - ay + vh, R + vx - ax:R + vx - ax + vw] = np.einsum("hwr,rc,hwc->hw", wgt, lin, y)
+            S = np.zeros((ah + 2 * R + 1, aw + 2 * R + 1))
+            S[1:, 1:] = lam.cumsum(0).cumsum(1)
+            ys = np.arange(oy, ah, P)[:, None]
+            xs = np.arange(ox, aw, P)[None, :]
+            k = 2 * R + 1
+            out[oy::P, ox::P] = S[ys + k, xs + k] - S[ys, xs + k] - S[ys + k, xs] + S[ys, xs]
+    return out

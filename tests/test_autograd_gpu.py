@@ -12,7 +12,7 @@ import librawops_amd as L
 from librawops_amd import autograd as A
 from librawops_amd import synth
 from oracle import oracle as O
-from tests.helpers import gpu_image, meta_dict, oracle_meta, random_samples
+from tests.helpers import gpu_image, menon_adjoint_by_probing, meta_dict, oracle_meta, random_samples
 
 pytestmark = pytest.mark.gpu
 
@@ -125,6 +125,29 @@ def test_render_chain_gradients_vs_fd(demosaic, case):
             fM[r, c] = (loss64(raw, M=Mp) - loss64(raw, M=Mm)) / 2e-4
     assert rel(mat.grad.cpu().numpy(), fM) <= 1e-4
     assert d0.shape[2] == 3
+
+
+@pytest.mark.parametrize("vp_rel", [None, (37, 29, 201, 87)])
+def test_menon_raw_adjoint_large_frame(vp_rel):
+    """A 300x150 active area spans interior tiles (forward recompute fully
+    inside the area) as well as border tiles; the raw adjoint of every sample
+    is checked against the lattice-probed FP64 oracle adjoint, over the whole
+    area and over a viewport whose edges cut through interior tiles."""
+    w, h, stride, area = 310, 160, 320, (3, 2, 300, 150)
+    m = meta_dict(w, h, stride, 2, 0, 1, area, (500, 510, 505, 515), (9000,) * 4)
+    s = random_samples(h, stride, seed=11, lo=300, hi=9000)
+    img = gpu_image(m, s)
+    om = oracle_meta(m)
+    vp = (area[0], area[1], area[2], area[3]) if vp_rel is None else \
+        (area[0] + vp_rel[0], area[1] + vp_rel[1], vp_rel[2], vp_rel[3])
+    rng = np.random.default_rng(3)
+    wgt = rng.normal(size=(vp[3], vp[2], 3)).astype(np.float32)
+    gains, stops = [1.7, 1.0, 1.3], 0.4
+    graw, gp = A.render_backward(img, torch.from_numpy(wgt).cuda(), gains, stops, CAMW, "menon", L.Rect(*vp))
+    torch.cuda.synchronize()
+    _, dec = O.menon_f64(om, norm_plane(m, s), om.area)
+    ref = menon_adjoint_by_probing(om, dec, wgt.astype(np.float64), vp, gains, 2.0 ** 0.4, CAMW.astype(np.float64))
+    assert rel(graw.cpu().numpy(), ref) <= 1e-4
 
 
 @pytest.mark.parametrize("radius", [1, 2, 4])
