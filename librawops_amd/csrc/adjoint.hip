@@ -17,6 +17,7 @@
 // reach), scatters into LDS accumulators with LDS float atomics, and writes
 // only its own tile, so no global atomics touch the raw gradient.
 #include "device_common.hpp"
+#include "menon_fwd.hpp"
 #include "menon_tile.hpp"
 #include "render_launch.hpp"
 
@@ -93,12 +94,14 @@ __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty,
 
 // --------------------------------------------------------------- Menon --
 
-// Interior tiles: the same transposes as the scatter path below, written as
-// gathers (each stage's adjoint at a site is a fixed stencil over its
-// neighbours and their recorded decisions), so no LDS atomics are needed.
-// Extended-tile coordinates (u, v) = sensor - (ex, ey); the tile is
-// [4, 4+TW) x [4, 4+TH).  Planes over the extended tile are compacted per
-// site class: index v (EW/2) + (u >> 1).
+// Interior tiles (forward support of tile +-4 inside the active area, so no
+// read is mirrored): the forward is recomputed over the extended tile
+// (tile +-4) with the row-pair forward stages, then the same transposes as
+// the scatter path below are taken as gathers -- each stage's adjoint at a
+// site is a fixed stencil over its neighbours and their recorded decisions --
+// so no LDS atomics are needed.  Extended-tile coordinates (u, v) = sensor -
+// (ex, ey); the tile is [4, 4+TW) x [4, 4+TH).  Adjoint planes over the
+// extended tile are compacted per site class: index v (EW/2) + (u >> 1).
 //   pass 1 (non-green sites): gd = dL/d demosaic output; keep the channel the
 //          final chroma step mixes (PO) and the green channel (PG)
 //   pass 2 (green sites): red/blue adjoints AR, AB (own gd plus the +-0.5
@@ -106,21 +109,44 @@ __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty,
 //   pass 3 (non-green sites): green adjoint AG = PG - 0.5 sum of the 4
 //          neighbours' AR (red sites) or AB (blue sites)
 //   pass 4 (tile): raw adjoint through the directional-green stencil.
-template <int TW, int TH, class Tile>
-__device__ __forceinline__ void gather_tile(const BackwardArgs& a, const SensorArgs& s, char* lds, const Tile& t,
-                                            int ex, int ey, int tx, int ty) {
-    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
-    using L = MenonLds<EW, EH>;
-    static_assert(4 * NC * 4 <= L::DH_BYTES + L::DV_BYTES, "gather planes do not fit over dh/dv");
-    static_assert((NC + TW * TH) * 4 <= L::G2_BYTES, "gather planes do not fit over green");
-    float* PO = reinterpret_cast<float*>(lds + L::OFF_DH);
+// LDS reuse: PO/PG over dv (dead after the classifier), AR/AB and the tile's
+// green-site raw partials over gh (dead after pass 1), AG and the non-green
+// raw partials over raw (dead after pass 2), the parameter reduction over
+// gv (dead after pass 2).
+template <int TW, int TH, int N0>
+__device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx,
+                                                    int ty) {
+    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH, HT = TW / 2;
+    using F = MenonFwd<EW, EH, kNT, N0, false>;
+    using L = MenonFwdLds<EW, EH>;
+    static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "PO/PG do not fit over dv");
+    static_assert((2 * NC + HT * TH) * 4 <= L::NP * L::NR * 8, "AR/AB do not fit over gh");
+    static_assert((NC + HT * TH) * 4 <= L::OFF_GH, "AG does not fit over raw");
+    const int ex = tx - 4, ey = ty - 4;
+    F t(s, lds, ex, ey);
+    t.load();
+    __syncthreads();
+    t.directional();
+    __syncthreads();
+    t.gradients();
+    __syncthreads();
+    t.classify();
+    __syncthreads();
+    t.chroma();
+    __syncthreads();
+
+    float* PO = reinterpret_cast<float*>(lds + L::OFF_DV);
     float* PG = PO + NC;
-    float* AR = PG + NC;
+    float* AR = reinterpret_cast<float*>(lds + L::OFF_GH);
     float* AB = AR + NC;
-    float* AG = reinterpret_cast<float*>(lds + L::OFF_GREEN);   // after pass 1: green is dead
-    float* RP = AG + NC;                                          // raw partials of the tile
+    float* RPG = AB + NC;                                  // raw partials, tile green sites
+    float* AG = reinterpret_cast<float*>(lds + L::OFF_RAW);
+    float* RPN = AG + NC;                                  // raw partials, tile non-green sites
     auto ci = [](int u, int v) { return v * HWE + (u >> 1); };
+    auto ti = [](int u, int v) { return (v - 4) * HT + ((u - 4) >> 1); };
     auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
+    auto horiz = [&](int u, int v) { return t.hz[ci(u, v)] != 0; };
+    auto red_row = [&](int v) { return ((t.red_rows >> (v & 1)) & 1u) != 0; };
     auto gd_at = [&](int u, int v, float gd[3], float go[3], float uu[3]) -> bool {
         if (!load_gout(a, ex + u, ey + v, go)) { gd[0] = gd[1] = gd[2] = 0.f; return false; }
         chain_back(a, go, uu, gd);
@@ -130,33 +156,38 @@ __device__ __forceinline__ void gather_tile(const BackwardArgs& a, const SensorA
 
     // pass 1: non-green sites of the extended tile
     for (int i = threadIdx.x; i < NC; i += kNT) {
-        const int v = i / HWE, u = 2 * (i - v * HWE) + t.ng_parity(v);
+        const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
         float gd[3], go[3], uu[3];
         const bool has = gd_at(u, v, gd, go, uu);
-        const bool red_site = color_of(s, ex + u, ey + v) == 0;
-        PO[i] = red_site ? gd[2] : gd[0];
+        const bool rs = red_row(v);
+        PO[i] = rs ? gd[2] : gd[0];
         PG[i] = gd[1];
         if (has && in_tile(u, v)) {
-            float d[3];
-            t.finish(u, v, d[0], d[1], d[2]);
+            // final chroma (MenonDemosaic.cpp:162-173), as the forward takes it
+            const bool h = horiz(u, v);
+            const int ia = F::gi(u - (h ? 1 : 0), v - (h ? 0 : 1)), ic = F::gi(u + (h ? 1 : 0), v + (h ? 0 : 1));
+            const double mr = 0.5 * (t.red[ia] + t.red[ic]);
+            const double mb = 0.5 * (t.blue[ia] + t.blue[ic]);
+            const float self = t.raw[F::rawi(u, v)];
+            const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
+            const float d[3] = {rs ? self : other, (float)t.gh[F::ni(u, v)], rs ? other : self};
             param_grads(a, d, go, uu, acc);
         }
     }
     __syncthreads();
-    // pass 2: green sites (tile +-3 needs them; every green site with four
-    // neighbours inside the extended tile is done)
+    // pass 2: green sites of tile +-3 (all four neighbours inside the extended tile)
     for (int i = threadIdx.x; i < NC; i += kNT) {
-        const int v = i / HWE, u = 2 * (i - v * HWE) + (t.ng_parity(v) ^ 1);
+        const int v = i / HWE, u = 2 * (i - v * HWE) + (F::nx(v & 1) ^ 1);
         if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1) continue;
         float gd[3], go[3], uu[3];
         const bool has = gd_at(u, v, gd, go, uu);
         float ar = gd[0], ab = gd[2];
+        // R site n: B = r + mean(B) - mean(R); B site n: R = b + mean(R) - mean(B)
         auto take = [&](int nu, int nv, bool horizontal_needed) {
-            const bool horizontal = t.hz[nv * HWE + (nu >> 1)] != 0;
-            if (horizontal != horizontal_needed) return;
+            if (horiz(nu, nv) != horizontal_needed) return;
             const float w = 0.5f * PO[ci(nu, nv)];
-            if (color_of(s, ex + nu, ey + nv) == 0) { ar -= w; ab += w; }   // R site: B = r + mean(B) - mean(R)
-            else { ar += w; ab -= w; }                                      // B site: R = b + mean(R) - mean(B)
+            if (red_row(nv)) { ar -= w; ab += w; }
+            else { ar += w; ab -= w; }
         };
         take(u - 1, v, true);
         take(u + 1, v, true);
@@ -165,10 +196,10 @@ __device__ __forceinline__ void gather_tile(const BackwardArgs& a, const SensorA
         AR[i] = ar;
         AB[i] = ab;
         if (in_tile(u, v)) {
-            RP[(v - 4) * TW + (u - 4)] = ar + ab + gd[1];
+            RPG[ti(u, v)] = ar + ab + gd[1];
             if (has) {
-                float d[3];
-                t.finish(u, v, d[0], d[1], d[2]);
+                const int g = F::gi(u, v);
+                const float d[3] = {(float)t.red[g], t.raw[F::rawi(u, v)], (float)t.blue[g]};
                 param_grads(a, d, go, uu, acc);
             }
         }
@@ -176,42 +207,39 @@ __device__ __forceinline__ void gather_tile(const BackwardArgs& a, const SensorA
     __syncthreads();
     // pass 3: non-green sites of tile +-2
     for (int i = threadIdx.x; i < NC; i += kNT) {
-        const int v = i / HWE, u = 2 * (i - v * HWE) + t.ng_parity(v);
+        const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
         if (u < 2 || u >= EW - 2 || v < 2 || v >= EH - 2) continue;
-        const bool red_site = color_of(s, ex + u, ey + v) == 0;
-        const float* A = red_site ? AR : AB;
+        const float* A = red_row(v) ? AR : AB;
         const float S = (A[ci(u - 1, v)] + A[ci(u + 1, v)]) + (A[ci(u, v - 1)] + A[ci(u, v + 1)]);
         AG[i] = PG[i] - 0.5f * S;
         if (in_tile(u, v)) {
             float gd[3], go[3], uu[3];
             gd_at(u, v, gd, go, uu);
-            RP[(v - 4) * TW + (u - 4)] = (gd[0] + gd[2]) + 0.5f * S;   // own sample feeds both r and b
+            RPN[ti(u, v)] = (gd[0] + gd[2]) + 0.5f * S;   // the own sample feeds both r and b
         }
     }
     __syncthreads();
     // pass 4: raw adjoint of the tile through the directional-green stencil
     for (int i = threadIdx.x; i < TW * TH; i += kNT) {
         const int v = 4 + i / TW, u = 4 + (i - (i / TW) * TW);
-        const bool green_site = (u & 1) != t.ng_parity(v);
-        float r = RP[i];
-        auto hzat = [&](int nu, int nv) { return t.hz[nv * HWE + (nu >> 1)] != 0; };
-        if (green_site) {
-            if (hzat(u - 1, v)) r += 0.5f * AG[ci(u - 1, v)];
-            if (hzat(u + 1, v)) r += 0.5f * AG[ci(u + 1, v)];
-            if (!hzat(u, v - 1)) r += 0.5f * AG[ci(u, v - 1)];
-            if (!hzat(u, v + 1)) r += 0.5f * AG[ci(u, v + 1)];
+        float r;
+        if ((u & 1) != F::nx(v & 1)) {
+            r = RPG[ti(u, v)];
+            if (horiz(u - 1, v)) r += 0.5f * AG[ci(u - 1, v)];
+            if (horiz(u + 1, v)) r += 0.5f * AG[ci(u + 1, v)];
+            if (!horiz(u, v - 1)) r += 0.5f * AG[ci(u, v - 1)];
+            if (!horiz(u, v + 1)) r += 0.5f * AG[ci(u, v + 1)];
         } else {
-            r += 0.5f * AG[ci(u, v)];
-            if (hzat(u - 2, v)) r -= 0.25f * AG[ci(u - 2, v)];
-            if (hzat(u + 2, v)) r -= 0.25f * AG[ci(u + 2, v)];
-            if (!hzat(u, v - 2)) r -= 0.25f * AG[ci(u, v - 2)];
-            if (!hzat(u, v + 2)) r -= 0.25f * AG[ci(u, v + 2)];
+            r = RPN[ti(u, v)] + 0.5f * AG[ci(u, v)];
+            if (horiz(u - 2, v)) r -= 0.25f * AG[ci(u - 2, v)];
+            if (horiz(u + 2, v)) r -= 0.25f * AG[ci(u + 2, v)];
+            if (!horiz(u, v - 2)) r -= 0.25f * AG[ci(u, v - 2)];
+            if (!horiz(u, v + 2)) r -= 0.25f * AG[ci(u, v + 2)];
         }
         const int sx = tx + (u - 4), sy = ty + (v - 4);
-        if (a.graw && sx < s.ax + s.aw && sy < s.ay + s.ah)
-            a.graw[(size_t)(sy - s.ay) * s.aw + (size_t)(sx - s.ax)] = r;
+        if (a.graw) a.graw[(size_t)(sy - s.ay) * s.aw + (size_t)(sx - s.ax)] = r;
     }
-    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GH));
+    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GV));
 }
 
 template <int TW, int TH, bool BORDER>
@@ -253,10 +281,6 @@ __device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const Sen
     __syncthreads();
     t.chroma();
     __syncthreads();
-    if constexpr (!BORDER) {
-        gather_tile<TW, TH>(a, s, lds, t, ex, ey, tx, ty);
-        return;
-    }
     for (int i = threadIdx.x; i < 2 * RB_N + GA_N + GR_N; i += kNT) adj_r[i] = 0.0f;
     __syncthreads();
 
@@ -371,14 +395,22 @@ __device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const Sen
 
 template <int TW, int TH>
 __global__ void __launch_bounds__(kNT) menon_backward(const BackwardArgs a) {
-    __shared__ __attribute__((aligned(16))) char lds[MenonLds<TW + 8, TH + 8>::BYTES];
+    constexpr int LDS = std::max(MenonLds<TW + 8, TH + 8>::BYTES, MenonFwdLds<TW + 8, TH + 8>::BYTES);
+    __shared__ __attribute__((aligned(16))) char lds[LDS];
     int tx, ty;
     tile_of(a, tx, ty, TW, TH);
     const SensorArgs& s = a.s;
     const bool interior = tx - 10 >= s.ax && ty - 10 >= s.ay && tx + TW + 10 <= s.ax + s.aw &&
                           ty + TH + 10 <= s.ay + s.ah;
-    if (interior) menon_back_tile<TW, TH, false>(a, s, lds, tx, ty);
-    else menon_back_tile<TW, TH, true>(a, s, lds, tx, ty);
+    if (!interior) {
+        menon_back_tile<TW, TH, true>(a, s, lds, tx, ty);
+        return;
+    }
+    // x parity of the non-green sites in even tile rows: the same for every
+    // tile of the launch (tile origins step by even sizes)
+    const uint32_t n0 = ((s.ng_parity >> ((uint32_t)ty & 1u)) & 1u) ^ ((uint32_t)tx & 1u);
+    if (n0) menon_back_interior<TW, TH, 1>(a, s, lds, tx, ty);
+    else menon_back_interior<TW, TH, 0>(a, s, lds, tx, ty);
 }
 
 // ------------------------------------------------------------- bilinear --
