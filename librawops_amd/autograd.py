@@ -51,12 +51,13 @@ class _Raster:
         assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == 3 and t.shape[2] == 3
         self.t = t
         self.h = ctypes.c_void_p()
+        self._free = C.lib().rawops_source_destroy   # bound now: usable during interpreter exit
         H, W = t.shape[:2]
         C.check(C.lib().rawops_source_wrap_raster_device(W, H, 0, space, _ptr(t), ctypes.byref(self.h)))
 
     def __del__(self):
         if getattr(self, "h", None) is not None and self.h.value:
-            C.lib().rawops_source_destroy(self.h)
+            self._free(self.h)
 
 
 _EMPTY = None

@@ -54,6 +54,7 @@ class Batch:
         C.check(C.lib().rawops_batch_create(arr, len(self.requests), 0xFFFFFFFF if shape is None else shape,
                                             ctypes.c_void_p(s.cuda_stream), ctypes.byref(h)))
         self._h = h
+        self._free = C.lib().rawops_batch_destroy   # bound now: usable during interpreter exit
 
     def run(self, stream=None):
         s = stream or torch.cuda.current_stream()
@@ -76,5 +77,5 @@ class Batch:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            C.lib().rawops_batch_destroy(h)
+            self._free(h)
             self._h = None

@@ -58,32 +58,49 @@ __device__ __forceinline__ void chain_back(const BackwardArgs& a, const float go
     }
 }
 
-// Parameter gradients of one pixel with demosaic output d.
-__device__ __forceinline__ void param_grads(const BackwardArgs& a, const float d[3], const float go[3],
-                                            const float u[3], double (&acc)[13]) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        acc[c] += (double)a.m * d[c] * u[c];                  // d out / d g_c
-        acc[3] += (double)a.g[c] * d[c] * u[c];               // d out / d m
-    }
+// Parameter gradients.  With u = M^T gout and out = M (m g (.) d), every
+// parameter gradient is a fixed combination of the nine sums
+//     P[r][c] = sum over pixels of gout_r d_c :
+//     dL/dg_c    = m sum_r M[r][c] P[r][c]
+//     dL/dm      = sum_c g_c sum_r M[r][c] P[r][c]
+//     dL/dM[r][c] = m g_c P[r][c]
+// so a pixel adds nine exact FP64 products and the combinations are taken
+// once per workgroup.
+__device__ __forceinline__ void param_grads(const float d[3], const float go[3], double (&acc)[9]) {
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[4 + r * 3 + c] += (double)go[r] * ((double)a.m * a.g[c] * d[c]);
+        for (int c = 0; c < 3; ++c) acc[r * 3 + c] += (double)go[r] * (double)d[c];
 }
 
-__device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc)[13], double* red /* [waves][13] LDS */) {
+__device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc)[9], double* red /* [waves][9] LDS */) {
 #pragma unroll
-    for (int i = 0; i < 13; ++i)
+    for (int i = 0; i < 9; ++i)
         for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0)
-        for (int i = 0; i < 13; ++i) red[wave * 13 + i] = acc[i];
+        for (int i = 0; i < 9; ++i) red[wave * 9 + i] = acc[i];
     __syncthreads();
     if (threadIdx.x < 13) {
-        double t = 0;
-        for (int w = 0; w < kNT / 64; ++w) t += red[w * 13 + threadIdx.x];
-        atomicAdd(a.gparams + threadIdx.x, t);
+        double P[9];
+        for (int i = 0; i < 9; ++i) {
+            double t = 0;
+            for (int w = 0; w < kNT / 64; ++w) t += red[w * 9 + i];
+            P[i] = t;
+        }
+        const int k = threadIdx.x;
+        double v;
+        if (k < 3) {
+            v = (double)a.m * ((double)a.M[k] * P[k] + (double)a.M[3 + k] * P[3 + k] + (double)a.M[6 + k] * P[6 + k]);
+        } else if (k == 3) {
+            v = 0;
+            for (int c = 0; c < 3; ++c)
+                v += (double)a.g[c] * ((double)a.M[c] * P[c] + (double)a.M[3 + c] * P[3 + c] + (double)a.M[6 + c] * P[6 + c]);
+        } else {
+            const int c = (k - 4) % 3;
+            v = (double)a.m * (double)a.g[c] * P[k - 4];
+        }
+        atomicAdd(a.gparams + k, v);
     }
 }
 
@@ -152,7 +169,7 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
         chain_back(a, go, uu, gd);
         return true;
     };
-    double acc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // pass 1: non-green sites of the extended tile
     for (int i = threadIdx.x; i < NC; i += kNT) {
@@ -171,7 +188,7 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
             const float self = t.raw[F::rawi(u, v)];
             const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
             const float d[3] = {rs ? self : other, (float)t.gh[F::ni(u, v)], rs ? other : self};
-            param_grads(a, d, go, uu, acc);
+            param_grads(d, go, acc);
         }
     }
     __syncthreads();
@@ -200,7 +217,7 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
             if (has) {
                 const int g = F::gi(u, v);
                 const float d[3] = {(float)t.red[g], t.raw[F::rawi(u, v)], (float)t.blue[g]};
-                param_grads(a, d, go, uu, acc);
+                param_grads(d, go, acc);
             }
         }
     }
@@ -305,7 +322,7 @@ __device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const Sen
         if (u >= 2 && u < EW - 2 && v >= 2 && v < EH - 2) atomicAdd(adj_g + (v - 2) * GA_HW + ((u - 2) >> 1), w);
     };
 
-    double acc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     // (a) outputs of the extended tile: direct terms, final-stage chroma.
     for (int i = threadIdx.x; i < EW * EH; i += kNT) {
         const int v = i / EW, u = i - v * EW;
@@ -318,7 +335,7 @@ __device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const Sen
         if (in_tile(u, v)) {
             float d[3];
             t.finish(u, v, d[0], d[1], d[2]);
-            param_grads(a, d, go, uu, acc);
+            param_grads(d, go, acc);
         }
         const int own = color_of(s, sx, sy);
         if (own == 1) {
@@ -419,14 +436,14 @@ template <int TW, int TH>
 __global__ void __launch_bounds__(kNT) bilinear_backward(const BackwardArgs a) {
     __shared__ float raw[(TW + 2) * (TH + 2)];
     __shared__ float adj[TW * TH];
-    __shared__ double red[(kNT / 64) * 13];
+    __shared__ double red[(kNT / 64) * 9];
     int tx, ty;
     tile_of(a, tx, ty, TW, TH);
     const SensorArgs& s = a.s;
     load_raw<TW, TH, 1, true, kNT>(s, tx, ty, raw);
     for (int i = threadIdx.x; i < TW * TH; i += kNT) adj[i] = 0.0f;
     __syncthreads();
-    double acc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     constexpr int EW = TW + 2, EH = TH + 2;
     for (int i = threadIdx.x; i < EW * EH; i += kNT) {
         const int v = i / EW, u = i - v * EW;       // tile coordinates + 1
@@ -440,7 +457,7 @@ __global__ void __launch_bounds__(kNT) bilinear_backward(const BackwardArgs a) {
         if (mine) {
             float d[3];
             bilinear_pixel<TW, TH, true>(s, raw, tx, ty, u - 1, v - 1, d[0], d[1], d[2]);
-            param_grads(a, d, go, uu, acc);
+            param_grads(d, go, acc);
         }
         const int own = color_of(s, sx, sy);
 #pragma unroll

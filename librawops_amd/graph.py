@@ -244,6 +244,7 @@ class RawImage:
             C.check(C.lib().rawops_source_create_raw(ctypes.byref(m), s.ctypes.data_as(ctypes.c_void_p),
                                                      s.size, ctypes.byref(h)))
         self._handle = h
+        self._free = C.lib().rawops_source_destroy   # bound now: usable during interpreter exit
         out = C.RawMetadataC()
         C.check(C.lib().rawops_source_raw_metadata(h, ctypes.byref(out)))
         a = out.active_area
@@ -262,7 +263,7 @@ class RawImage:
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h is not None and h.value:
-            C.lib().rawops_source_destroy(h)
+            self._free(h)
             self._handle = None
 
 
@@ -288,6 +289,7 @@ class RasterImage:
                                                         int(M.working_space), p.ctypes.data_as(ctypes.c_void_p),
                                                         ctypes.byref(h)))
         self._handle = h
+        self._free = C.lib().rawops_source_destroy   # bound now: usable during interpreter exit
         self.metadata = RasterMetadata(M.width, M.height, M.row_stride_pixels or M.width, WorkingSpace(M.working_space))
 
     @property
@@ -301,7 +303,7 @@ class RasterImage:
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h is not None and h.value:
-            C.lib().rawops_source_destroy(h)
+            self._free(h)
             self._handle = None
 
 
@@ -678,6 +680,7 @@ class Program:
 
     def __init__(self, handle, keep):
         self._handle = handle
+        self._free = C.lib().rawops_plan_destroy     # bound now: usable during interpreter exit
         self._keep = keep
 
     @staticmethod
@@ -720,7 +723,7 @@ class Program:
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h is not None and h.value:
-            C.lib().rawops_plan_destroy(h)
+            self._free(h)
             self._handle = None
 
 
