@@ -36,8 +36,9 @@ struct BackwardArgs {
     uint32_t gstride;          // pixels per gout row
     float* graw;               // active-area plane, aw x ah
     double* gparams;           // 13: gains[3], multiplier, matrix[9] (row-major)
-    int tiles_x;               // tiles per grid row
+    int tiles_x, tiles_y;      // tiles of the launch
     int tile0_x, tile0_y;      // first tile (in tiles from the area origin)
+    int r0, r1, c0, c1;        // launch-relative tile rows / columns of the interior block
 };
 
 // dL/dout at a sensor pixel (zero outside the viewport).
@@ -104,6 +105,30 @@ __device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc
     }
 }
 
+// Border-first tile order: the tiles outside the interior block [r0, r1) x
+// [c0, c1) (slower: mirrored reads, scatter adjoint) take the first block
+// indices so they start with the launch instead of trailing it.
+__device__ __forceinline__ void tile_of_border_first(const BackwardArgs& a, int& tx, int& ty, int TW, int TH) {
+    const int TX = a.tiles_x, TY = a.tiles_y;
+    const int top = a.r0 * TX, bottom = (TY - a.r1) * TX, side = a.c0 + (TX - a.c1), middle = (a.r1 - a.r0) * side;
+    int b = (int)blockIdx.x, r, c;
+    if (b < top) {
+        r = b / TX; c = b - r * TX;
+    } else if ((b -= top) < bottom) {
+        r = a.r1 + b / TX; c = b - (b / TX) * TX;
+    } else if ((b -= bottom) < middle) {
+        r = a.r0 + b / side;
+        const int j = b - (b / side) * side;
+        c = j < a.c0 ? j : a.c1 + (j - a.c0);
+    } else {
+        b -= middle;
+        const int ci = a.c1 - a.c0;
+        r = a.r0 + b / ci; c = a.c0 + (b - (b / ci) * ci);
+    }
+    tx = a.s.ax + (a.tile0_x + c) * TW;
+    ty = a.s.ay + (a.tile0_y + r) * TH;
+}
+
 __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty, int TW, int TH) {
     tx = a.s.ax + (a.tile0_x + (int)(blockIdx.x % a.tiles_x)) * TW;
     ty = a.s.ay + (a.tile0_y + (int)(blockIdx.x / a.tiles_x)) * TH;
@@ -142,6 +167,21 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
     const int ex = tx - 4, ey = ty - 4;
     F t(s, lds, ex, ey);
     t.load();
+    // Upstream gradients of the thread's sites, fetched while the forward is
+    // recomputed: item j covers compacted column k of extended row v, i.e.
+    // one non-green site (pass 1, 3) and one green site (pass 2).
+    constexpr int J = (NC + kNT - 1) / kNT;
+    float goN[J][3], goG[J][3];
+    uint32_t hasN = 0, hasG = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int i = (int)threadIdx.x + j * kNT;
+        if (i < NC) {
+            const int v = i / HWE, un = 2 * (i - v * HWE) + F::nx(v & 1);
+            if (load_gout(a, ex + un, ey + v, goN[j])) hasN |= 1u << j;
+            if (load_gout(a, ex + (un ^ 1), ey + v, goG[j])) hasG |= 1u << j;
+        }
+    }
     __syncthreads();
     t.directional();
     __syncthreads();
@@ -164,18 +204,22 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
     auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
     auto horiz = [&](int u, int v) { return t.hz[ci(u, v)] != 0; };
     auto red_row = [&](int v) { return ((t.red_rows >> (v & 1)) & 1u) != 0; };
-    auto gd_at = [&](int u, int v, float gd[3], float go[3], float uu[3]) -> bool {
-        if (!load_gout(a, ex + u, ey + v, go)) { gd[0] = gd[1] = gd[2] = 0.f; return false; }
-        chain_back(a, go, uu, gd);
-        return true;
+    auto gd_of = [&](bool has, const float go[3], float gd[3]) {
+        float uu[3];
+        if (has) chain_back(a, go, uu, gd);
+        else gd[0] = gd[1] = gd[2] = 0.f;
     };
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // pass 1: non-green sites of the extended tile
-    for (int i = threadIdx.x; i < NC; i += kNT) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int i = (int)threadIdx.x + j * kNT;
+        if (i >= NC) break;
         const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
-        float gd[3], go[3], uu[3];
-        const bool has = gd_at(u, v, gd, go, uu);
+        const bool has = (hasN >> j) & 1u;
+        float gd[3];
+        gd_of(has, goN[j], gd);
         const bool rs = red_row(v);
         PO[i] = rs ? gd[2] : gd[0];
         PG[i] = gd[1];
@@ -188,16 +232,20 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
             const float self = t.raw[F::rawi(u, v)];
             const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
             const float d[3] = {rs ? self : other, (float)t.gh[F::ni(u, v)], rs ? other : self};
-            param_grads(d, go, acc);
+            param_grads(d, goN[j], acc);
         }
     }
     __syncthreads();
     // pass 2: green sites of tile +-3 (all four neighbours inside the extended tile)
-    for (int i = threadIdx.x; i < NC; i += kNT) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int i = (int)threadIdx.x + j * kNT;
+        if (i >= NC) break;
         const int v = i / HWE, u = 2 * (i - v * HWE) + (F::nx(v & 1) ^ 1);
         if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1) continue;
-        float gd[3], go[3], uu[3];
-        const bool has = gd_at(u, v, gd, go, uu);
+        const bool has = (hasG >> j) & 1u;
+        float gd[3];
+        gd_of(has, goG[j], gd);
         float ar = gd[0], ab = gd[2];
         // R site n: B = r + mean(B) - mean(R); B site n: R = b + mean(R) - mean(B)
         auto take = [&](int nu, int nv, bool horizontal_needed) {
@@ -217,21 +265,24 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
             if (has) {
                 const int g = F::gi(u, v);
                 const float d[3] = {(float)t.red[g], t.raw[F::rawi(u, v)], (float)t.blue[g]};
-                param_grads(d, go, acc);
+                param_grads(d, goG[j], acc);
             }
         }
     }
     __syncthreads();
     // pass 3: non-green sites of tile +-2
-    for (int i = threadIdx.x; i < NC; i += kNT) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int i = (int)threadIdx.x + j * kNT;
+        if (i >= NC) break;
         const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
         if (u < 2 || u >= EW - 2 || v < 2 || v >= EH - 2) continue;
         const float* A = red_row(v) ? AR : AB;
         const float S = (A[ci(u - 1, v)] + A[ci(u + 1, v)]) + (A[ci(u, v - 1)] + A[ci(u, v + 1)]);
         AG[i] = PG[i] - 0.5f * S;
         if (in_tile(u, v)) {
-            float gd[3], go[3], uu[3];
-            gd_at(u, v, gd, go, uu);
+            float gd[3];
+            gd_of((hasN >> j) & 1u, goN[j], gd);
             RPN[ti(u, v)] = (gd[0] + gd[2]) + 0.5f * S;   // the own sample feeds both r and b
         }
     }
@@ -415,7 +466,7 @@ __global__ void __launch_bounds__(kNT) menon_backward(const BackwardArgs a) {
     constexpr int LDS = std::max(MenonLds<TW + 8, TH + 8>::BYTES, MenonFwdLds<TW + 8, TH + 8>::BYTES);
     __shared__ __attribute__((aligned(16))) char lds[LDS];
     int tx, ty;
-    tile_of(a, tx, ty, TW, TH);
+    tile_of_border_first(a, tx, ty, TW, TH);
     const SensorArgs& s = a.s;
     const bool interior = tx - 10 >= s.ax && ty - 10 >= s.ay && tx + TW + 10 <= s.ax + s.aw &&
                           ty + TH + 10 <= s.ay + s.ah;
@@ -517,8 +568,23 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     a.tile0_y = y0 / TH;
     a.tiles_x = (x1 + TW - 1) / TW - a.tile0_x;
     const int tiles_y = (y1 + TH - 1) / TH - a.tile0_y;
+    a.tiles_y = tiles_y;
     if (a.vw <= 0 || a.vh <= 0 || a.tiles_x <= 0 || tiles_y <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
+    // Interior block of the Menon launch: tiles whose recompute support
+    // (tile +-10) lies inside the active area, a contiguous range of rows
+    // and of columns; empty when either range is.
+    auto interior_range = [](int first, int n, int T, int lo, int len, int& i0, int& i1) {
+        i0 = n; i1 = n;
+        for (int i = 0; i < n; ++i) {
+            const int t0 = lo + (first + i) * T;
+            if (t0 - 10 >= lo && t0 + T + 10 <= lo + len) { if (i0 == n) i0 = i; i1 = i + 1; }
+        }
+        if (i0 >= i1) i0 = i1 = 0;
+    };
+    interior_range(a.tile0_y, tiles_y, TH, a.s.ay, a.s.ah, a.r0, a.r1);
+    interior_range(a.tile0_x, a.tiles_x, TW, a.s.ax, a.s.aw, a.c0, a.c1);
+    if (a.r0 == a.r1 || a.c0 == a.c1) a.r0 = a.r1 = a.c0 = a.c1 = 0;
     if (menon)
         hipLaunchKernelGGL((menon_backward<64, 24>), dim3(blocks), dim3(kNT), 0, stream, a);
     else
