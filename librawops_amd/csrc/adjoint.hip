@@ -142,44 +142,55 @@ __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty,
 // the scatter path below are taken as gathers -- each stage's adjoint at a
 // site is a fixed stencil over its neighbours and their recorded decisions --
 // so no LDS atomics are needed.  Extended-tile coordinates (u, v) = sensor -
-// (ex, ey); the tile is [4, 4+TW) x [4, 4+TH).  Adjoint planes over the
-// extended tile are compacted per site class: index v (EW/2) + (u >> 1).
-//   pass 1 (non-green sites): gd = dL/d demosaic output; keep the channel the
-//          final chroma step mixes (PO) and the green channel (PG)
-//   pass 2 (green sites): red/blue adjoints AR, AB (own gd plus the +-0.5
-//          terms of the non-green neighbours whose direction covers the site)
-//   pass 3 (non-green sites): green adjoint AG = PG - 0.5 sum of the 4
-//          neighbours' AR (red sites) or AB (blue sites)
-//   pass 4 (tile): raw adjoint through the directional-green stencil.
-// LDS reuse: PO/PG over dv (dead after the classifier), AR/AB and the tile's
-// green-site raw partials over gh (dead after pass 1), AG and the non-green
-// raw partials over raw (dead after pass 2), the parameter reduction over
-// gv (dead after pass 2).
+// (ex, ey); the tile is [4, 4+TW) x [4, 4+TH).
+//
+// Work is handed out as row-pair items like the forward stages: item (qr, k)
+// holds, in each row v = 2 qr + b, the non-green site 2k + nx(b) and the
+// green site 2k + 1 - nx(b).  Both have compacted index c = v (EW/2) + k in
+// the adjoint planes, so every neighbour is a compile-time offset from c, and
+// a thread keeps its items' per-site values in registers across passes:
+//   pass 1: gd = dL/d demosaic output at every site; non-green sites publish
+//           the channel the final chroma step mixes, split by the site's
+//           decision into POh (horizontal) and POv (vertical); parameter
+//           gradients of the tile's pixels
+//   pass 2: green sites: red/blue adjoints AR, AB = own gd -+ 0.5 of the
+//           covering neighbours' PO (horizontal ones from POh, vertical ones
+//           from POv)
+//   pass 3: non-green sites: green adjoint AG = gd_g - 0.5 sum of the four
+//           neighbours' AR (red sites) or AB (blue sites), split into
+//           AGh / AGv by the site's decision
+//   pass 4: tile sites: raw adjoint through the directional-green stencil.
+// LDS reuse: POh/POv over dv (dead after the classifier), AR/AB over gh and
+// AGh/AGv over raw (both dead after pass 1), the parameter reduction over gv.
 template <int TW, int TH, int N0>
 __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx,
                                                     int ty) {
-    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH, HT = TW / 2;
+    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
+    constexpr int NI = HWE * (EH / 2), J = (NI + kNT - 1) / kNT;   // row-pair items
     using F = MenonFwd<EW, EH, kNT, N0, false>;
     using L = MenonFwdLds<EW, EH>;
-    static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "PO/PG do not fit over dv");
-    static_assert((2 * NC + HT * TH) * 4 <= L::NP * L::NR * 8, "AR/AB do not fit over gh");
-    static_assert((NC + HT * TH) * 4 <= L::OFF_GH, "AG does not fit over raw");
+    static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "POh/POv do not fit over dv");
+    static_assert(2 * NC * 4 <= L::NP * L::NR * 8, "AR/AB do not fit over gh");
+    static_assert(2 * NC * 4 <= L::OFF_GH, "AGh/AGv do not fit over raw");
+    static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles must cover whole Bayer cells");
     const int ex = tx - 4, ey = ty - 4;
     F t(s, lds, ex, ey);
     t.load();
     // Upstream gradients of the thread's sites, fetched while the forward is
-    // recomputed: item j covers compacted column k of extended row v, i.e.
-    // one non-green site (pass 1, 3) and one green site (pass 2).
-    constexpr int J = (NC + kNT - 1) / kNT;
-    float goN[J][3], goG[J][3];
-    uint32_t hasN = 0, hasG = 0;
+    // recomputed.
+    float goN[J][2][3], goG[J][2][3];
+    uint32_t hasN = 0, hasG = 0;    // bit 2j + b
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const int i = (int)threadIdx.x + j * kNT;
-        if (i < NC) {
-            const int v = i / HWE, un = 2 * (i - v * HWE) + F::nx(v & 1);
-            if (load_gout(a, ex + un, ey + v, goN[j])) hasN |= 1u << j;
-            if (load_gout(a, ex + (un ^ 1), ey + v, goG[j])) hasG |= 1u << j;
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it < NI) {
+            const int qr = it / HWE, k = it - qr * HWE;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int v = 2 * qr + b, un = 2 * k + F::nx(b);
+                if (load_gout(a, ex + un, ey + v, goN[j][b])) hasN |= 1u << (2 * j + b);
+                if (load_gout(a, ex + (un ^ 1), ey + v, goG[j][b])) hasG |= 1u << (2 * j + b);
+            }
         }
     }
     __syncthreads();
@@ -192,120 +203,125 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
     t.chroma();
     __syncthreads();
 
-    float* PO = reinterpret_cast<float*>(lds + L::OFF_DV);
-    float* PG = PO + NC;
+    float* POh = reinterpret_cast<float*>(lds + L::OFF_DV);
+    float* POv = POh + NC;
     float* AR = reinterpret_cast<float*>(lds + L::OFF_GH);
     float* AB = AR + NC;
-    float* RPG = AB + NC;                                  // raw partials, tile green sites
-    float* AG = reinterpret_cast<float*>(lds + L::OFF_RAW);
-    float* RPN = AG + NC;                                  // raw partials, tile non-green sites
-    auto ci = [](int u, int v) { return v * HWE + (u >> 1); };
-    auto ti = [](int u, int v) { return (v - 4) * HT + ((u - 4) >> 1); };
-    auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
-    auto horiz = [&](int u, int v) { return t.hz[ci(u, v)] != 0; };
-    auto red_row = [&](int v) { return ((t.red_rows >> (v & 1)) & 1u) != 0; };
-    auto gd_of = [&](bool has, const float go[3], float gd[3]) {
-        float uu[3];
-        if (has) chain_back(a, go, uu, gd);
-        else gd[0] = gd[1] = gd[2] = 0.f;
-    };
+    float* AGh = reinterpret_cast<float*>(lds + L::OFF_RAW);
+    float* AGv = AGh + NC;
+    const bool red0 = (t.red_rows & 1u) != 0, red1 = (t.red_rows & 2u) != 0;   // non-green sites of row parity b are red
+    auto red_row = [&](int b) { return b ? red1 : red0; };
+    auto tile_item = [](int qr, int k) { return qr >= 2 && qr < 2 + TH / 2 && k >= 2 && k < 2 + TW / 2; };
+    // per item and row: green-channel gd and gd_r + gd_b of the non-green
+    // site, gd of the green site, raw partials of both
+    float pg[J][2], rn[J][2], gg[J][2][3], rpg[J][2], rpn[J][2];
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
-    // pass 1: non-green sites of the extended tile
+    // pass 1
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const int i = (int)threadIdx.x + j * kNT;
-        if (i >= NC) break;
-        const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
-        const bool has = (hasN >> j) & 1u;
-        float gd[3];
-        gd_of(has, goN[j], gd);
-        const bool rs = red_row(v);
-        PO[i] = rs ? gd[2] : gd[0];
-        PG[i] = gd[1];
-        if (has && in_tile(u, v)) {
-            // final chroma (MenonDemosaic.cpp:162-173), as the forward takes it
-            const bool h = horiz(u, v);
-            const int ia = F::gi(u - (h ? 1 : 0), v - (h ? 0 : 1)), ic = F::gi(u + (h ? 1 : 0), v + (h ? 0 : 1));
-            const double mr = 0.5 * (t.red[ia] + t.red[ic]);
-            const double mb = 0.5 * (t.blue[ia] + t.blue[ic]);
-            const float self = t.raw[F::rawi(u, v)];
-            const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
-            const float d[3] = {rs ? self : other, (float)t.gh[F::ni(u, v)], rs ? other : self};
-            param_grads(d, goN[j], acc);
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+        const bool mine = tile_item(qr, k);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = F::nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
+            const bool rs = red_row(b);
+            const bool hn = (hasN >> (2 * j + b)) & 1u, hg = (hasG >> (2 * j + b)) & 1u;
+            float gd[3], uu[3];
+            if (hn) chain_back(a, goN[j][b], uu, gd);
+            else gd[0] = gd[1] = gd[2] = 0.f;
+            const bool h = t.hz[c] != 0;
+            const float po = rs ? gd[2] : gd[0];
+            POh[c] = h ? po : 0.f;
+            POv[c] = h ? 0.f : po;
+            pg[j][b] = gd[1];
+            rn[j][b] = gd[0] + gd[2];           // the own sample feeds both r and b
+            if (hg) chain_back(a, goG[j][b], uu, gg[j][b]);
+            else gg[j][b][0] = gg[j][b][1] = gg[j][b][2] = 0.f;
+            if (mine) {
+                if (hn) {
+                    // final chroma (MenonDemosaic.cpp:162-173), as the forward takes it
+                    const int ia = F::gi(un - (h ? 1 : 0), v - (h ? 0 : 1));
+                    const int ic = F::gi(un + (h ? 1 : 0), v + (h ? 0 : 1));
+                    const double mr = 0.5 * (t.red[ia] + t.red[ic]);
+                    const double mb = 0.5 * (t.blue[ia] + t.blue[ic]);
+                    const float self = t.raw[F::rawi(un, v)];
+                    const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
+                    const float d[3] = {rs ? self : other, (float)t.gh[F::ni(un, v)], rs ? other : self};
+                    param_grads(d, goN[j][b], acc);
+                }
+                if (hg) {
+                    const int g = F::gi(ug, v);
+                    const float d[3] = {(float)t.red[g], t.raw[F::rawi(ug, v)], (float)t.blue[g]};
+                    param_grads(d, goG[j][b], acc);
+                }
+            }
         }
     }
     __syncthreads();
-    // pass 2: green sites of tile +-3 (all four neighbours inside the extended tile)
+    // pass 2: green sites with all four neighbours inside the extended tile
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const int i = (int)threadIdx.x + j * kNT;
-        if (i >= NC) break;
-        const int v = i / HWE, u = 2 * (i - v * HWE) + (F::nx(v & 1) ^ 1);
-        if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1) continue;
-        const bool has = (hasG >> j) & 1u;
-        float gd[3];
-        gd_of(has, goG[j], gd);
-        float ar = gd[0], ab = gd[2];
-        // R site n: B = r + mean(B) - mean(R); B site n: R = b + mean(R) - mean(B)
-        auto take = [&](int nu, int nv, bool horizontal_needed) {
-            if (horiz(nu, nv) != horizontal_needed) return;
-            const float w = 0.5f * PO[ci(nu, nv)];
-            if (red_row(nv)) { ar -= w; ab += w; }
-            else { ar += w; ab -= w; }
-        };
-        take(u - 1, v, true);
-        take(u + 1, v, true);
-        take(u, v - 1, false);
-        take(u, v + 1, false);
-        AR[i] = ar;
-        AB[i] = ab;
-        if (in_tile(u, v)) {
-            RPG[ti(u, v)] = ar + ab + gd[1];
-            if (has) {
-                const int g = F::gi(u, v);
-                const float d[3] = {(float)t.red[g], t.raw[F::rawi(u, v)], (float)t.blue[g]};
-                param_grads(d, goG[j], acc);
-            }
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = F::nx(b), ug = 2 * k + 1 - nb, c = v * HWE + k;
+            if (ug < 1 || ug >= EW - 1 || v < 1 || v >= EH - 1) continue;
+            // R neighbour: B = r + mean(B) - mean(R); B neighbour: R = b + mean(R) - mean(B)
+            const float sh = 0.5f * (POh[c - nb] + POh[c + 1 - nb]);      // row neighbours (row parity b)
+            const float sv = 0.5f * (POv[c - HWE] + POv[c + HWE]);        // column neighbours (parity 1 - b)
+            const float th = red_row(b) ? sh : -sh, tv = red_row(b ^ 1) ? sv : -sv;
+            const float ar = gg[j][b][0] - th - tv;
+            const float ab = gg[j][b][2] + th + tv;
+            AR[c] = ar;
+            AB[c] = ab;
+            rpg[j][b] = ar + ab + gg[j][b][1];
         }
     }
     __syncthreads();
     // pass 3: non-green sites of tile +-2
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const int i = (int)threadIdx.x + j * kNT;
-        if (i >= NC) break;
-        const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
-        if (u < 2 || u >= EW - 2 || v < 2 || v >= EH - 2) continue;
-        const float* A = red_row(v) ? AR : AB;
-        const float S = (A[ci(u - 1, v)] + A[ci(u + 1, v)]) + (A[ci(u, v - 1)] + A[ci(u, v + 1)]);
-        AG[i] = PG[i] - 0.5f * S;
-        if (in_tile(u, v)) {
-            float gd[3];
-            gd_of((hasN >> j) & 1u, goN[j], gd);
-            RPN[ti(u, v)] = (gd[0] + gd[2]) + 0.5f * S;   // the own sample feeds both r and b
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = F::nx(b), un = 2 * k + nb, c = v * HWE + k;
+            if (un < 2 || un >= EW - 2 || v < 2 || v >= EH - 2) continue;
+            const float* A = red_row(b) ? AR : AB;
+            const float S = (A[c - 1 + nb] + A[c + nb]) + (A[c - HWE] + A[c + HWE]);
+            const float ag = pg[j][b] - 0.5f * S;
+            const bool h = t.hz[c] != 0;
+            AGh[c] = h ? ag : 0.f;
+            AGv[c] = h ? 0.f : ag;
+            rpn[j][b] = rn[j][b] + 0.5f * S;
         }
     }
     __syncthreads();
     // pass 4: raw adjoint of the tile through the directional-green stencil
-    for (int i = threadIdx.x; i < TW * TH; i += kNT) {
-        const int v = 4 + i / TW, u = 4 + (i - (i / TW) * TW);
-        float r;
-        if ((u & 1) != F::nx(v & 1)) {
-            r = RPG[ti(u, v)];
-            if (horiz(u - 1, v)) r += 0.5f * AG[ci(u - 1, v)];
-            if (horiz(u + 1, v)) r += 0.5f * AG[ci(u + 1, v)];
-            if (!horiz(u, v - 1)) r += 0.5f * AG[ci(u, v - 1)];
-            if (!horiz(u, v + 1)) r += 0.5f * AG[ci(u, v + 1)];
-        } else {
-            r = RPN[ti(u, v)] + 0.5f * AG[ci(u, v)];
-            if (horiz(u - 2, v)) r -= 0.25f * AG[ci(u - 2, v)];
-            if (horiz(u + 2, v)) r -= 0.25f * AG[ci(u + 2, v)];
-            if (!horiz(u, v - 2)) r -= 0.25f * AG[ci(u, v - 2)];
-            if (!horiz(u, v + 2)) r -= 0.25f * AG[ci(u, v + 2)];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+        if (!tile_item(qr, k)) continue;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = F::nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
+            const float rgv = rpg[j][b] + 0.5f * (AGh[c - nb] + AGh[c + 1 - nb]) + 0.5f * (AGv[c - HWE] + AGv[c + HWE]);
+            const float rnv = rpn[j][b] + 0.5f * (AGh[c] + AGv[c]) - 0.25f * (AGh[c - 1] + AGh[c + 1]) -
+                              0.25f * (AGv[c - 2 * HWE] + AGv[c + 2 * HWE]);
+            if (a.graw) {
+                float* row = a.graw + (size_t)(ty + v - 4 - s.ay) * s.aw + (size_t)(tx - 4 - s.ax);
+                row[un] = rnv;
+                row[ug] = rgv;
+            }
         }
-        const int sx = tx + (u - 4), sy = ty + (v - 4);
-        if (a.graw) a.graw[(size_t)(sy - s.ay) * s.aw + (size_t)(sx - s.ax)] = r;
     }
     flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GV));
 }
