@@ -11,11 +11,12 @@
 //     decisions frozen (recomputed here exactly as the forward makes them),
 //     the demosaic is linear in the raw samples; its adjoint walks the stages
 //     backwards: final chroma -> green-site chroma -> directional green.
-//     Reads that the forward mirrors at the active-area edge scatter their
-//     adjoint to the mirrored sample.
-// A workgroup recomputes the forward over its tile grown by 4 (the adjoint's
-// reach), scatters into LDS accumulators with LDS float atomics, and writes
-// only its own tile, so no global atomics touch the raw gradient.
+//     Each stage's transpose is taken as a gather (a fixed neighbour stencil
+//     over the recorded decisions); reads that the forward mirrors at the
+//     active-area edge are gathered from the mirrored positions.
+// A Menon workgroup recomputes the forward over its tile grown by 4 (the
+// adjoint's reach) and writes only its own tile, so neither LDS nor global
+// atomics touch the raw gradient.
 #include "device_common.hpp"
 #include "menon_fwd.hpp"
 #include "menon_tile.hpp"
@@ -326,36 +327,40 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
     flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GV));
 }
 
-template <int TW, int TH, bool BORDER>
-__device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx, int ty) {
-    constexpr int EW = TW + 8, EH = TH + 8;  // forward recompute region: tile +-4
+// Border tiles: some reads of the forward are mirrored at the active-area
+// edge (whole-sample reflection m, applied per coordinate).  A stage output
+// at p reads its input at m(p + k) for each tap k, so the adjoint reaching a
+// target q gathers over the virtual positions V(q) = {z : m(z) = q} that lie
+// within a tap's reach (2) of the area -- q itself and its reflections
+// across nearby edges -- and applies the interior stencil at each z, counting
+// only sources inside the area.  The forward is recomputed with the
+// per-site border stages (MenonTile); the passes are those of the interior
+// path, per site, with upstream gradients re-read where a pass needs them.
+// LDS reuse (MenonLds layout): POh/POv/PG and AR/AB over dh+dv, AGh/AGv over
+// raw, the tile's raw partials over green, the parameter reduction over gh.
+template <int TW, int TH>
+__device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx, int ty) {
+    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
     using L = MenonLds<EW, EH>;
-    const int ex = tx - 4, ey = ty - 4;     // origin of the extended tile
-    MenonTile<EW, EH, BORDER, kNT> t{s, ex, ey, 0,
-                                     reinterpret_cast<float*>(lds + L::OFF_RAW),
-                                     reinterpret_cast<double*>(lds + L::OFF_GH),
-                                     reinterpret_cast<double*>(lds + L::OFF_GV),
-                                     reinterpret_cast<double*>(lds + L::OFF_DH),
-                                     reinterpret_cast<double*>(lds + L::OFF_DV),
-                                     reinterpret_cast<double*>(lds + L::OFF_GREEN),
-                                     reinterpret_cast<double*>(lds + L::OFF_GH),
-                                     reinterpret_cast<double*>(lds + L::OFF_GV),
-                                     reinterpret_cast<uint8_t*>(lds + L::OFF_HZ)};
+    static_assert(5 * NC * 4 <= L::DH_BYTES + L::DV_BYTES, "adjoint planes do not fit over dh/dv");
+    static_assert(2 * NC * 4 <= L::RAW_BYTES, "AGh/AGv do not fit over raw");
+    static_assert(TW * TH * 4 <= L::G2_BYTES, "raw partials do not fit over green");
+    const int ex = tx - 4, ey = ty - 4;     // origin of the extended tile (tile +-4)
+    MenonTile<EW, EH, true, kNT> t{s, ex, ey, 0,
+                                   reinterpret_cast<float*>(lds + L::OFF_RAW),
+                                   reinterpret_cast<double*>(lds + L::OFF_GH),
+                                   reinterpret_cast<double*>(lds + L::OFF_GV),
+                                   reinterpret_cast<double*>(lds + L::OFF_DH),
+                                   reinterpret_cast<double*>(lds + L::OFF_DV),
+                                   reinterpret_cast<double*>(lds + L::OFF_GREEN),
+                                   reinterpret_cast<double*>(lds + L::OFF_GH),
+                                   reinterpret_cast<double*>(lds + L::OFF_GV),
+                                   reinterpret_cast<uint8_t*>(lds + L::OFF_HZ)};
     for (int b = 0; b < 2; ++b) {
         const uint32_t sensor = (s.ng_parity >> (((uint32_t)ey + (uint32_t)b) & 1u)) & 1u;
         t.ngr |= ((sensor ^ ((uint32_t)ex & 1u)) & 1u) << b;
     }
-    // Accumulators over dead dh/dv storage: red/blue adjoints at green sites
-    // of the extended tile, green adjoint at non-green sites of tile +-2, raw
-    // adjoint of the tile itself.
-    constexpr int RB_N = (EW / 2) * EH, GA_HW = (EW - 4) / 2, GA_N = GA_HW * (EH - 4), GR_N = TW * TH;
-    static_assert((2 * RB_N + GA_N + GR_N) * 4 <= L::DH_BYTES + L::DV_BYTES, "adjoint accumulators do not fit");
-    float* adj_r = reinterpret_cast<float*>(lds + L::OFF_DH);
-    float* adj_b = adj_r + RB_N;
-    float* adj_g = adj_b + RB_N;
-    float* adj_raw = adj_g + GA_N;
-
-    load_raw<EW, EH, 6, BORDER, kNT>(s, ex, ey, t.raw);
+    load_raw<EW, EH, 6, true, kNT>(s, ex, ey, t.raw);
     __syncthreads();
     t.directional();
     __syncthreads();
@@ -365,114 +370,139 @@ __device__ __forceinline__ void menon_back_tile(const BackwardArgs& a, const Sen
     __syncthreads();
     t.chroma();
     __syncthreads();
-    for (int i = threadIdx.x; i < 2 * RB_N + GA_N + GR_N; i += kNT) adj_r[i] = 0.0f;
-    __syncthreads();
 
-    // Extended-tile coordinates (u, v) = sensor - (ex, ey); the tile is [4, 4+TW) x [4, 4+TH).
+    float* POh = reinterpret_cast<float*>(lds + L::OFF_DH);
+    float* POv = POh + NC;
+    float* PG = POv + NC;
+    float* AR = PG + NC;
+    float* AB = AR + NC;
+    float* AGh = reinterpret_cast<float*>(lds + L::OFF_RAW);
+    float* AGv = AGh + NC;
+    float* RP = reinterpret_cast<float*>(lds + L::OFF_GREEN);
+    auto ci = [](int u, int v) { return v * HWE + (u >> 1); };
     auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
-    auto mirror_uv = [&](int& u, int& v) {
-        if (BORDER) {
-            const int sx = ex + u, sy = ey + v;
-            if (!in_area(s, sx, sy)) { u = reflect(sx, s.ax, s.aw) - ex; v = reflect(sy, s.ay, s.ah) - ey; }
-        }
+    auto in_ext = [](int u, int v) { return u >= 0 && u < EW && v >= 0 && v < EH; };
+    auto in_a = [&](int u, int v) { return in_area(s, ex + u, ey + v); };
+    auto src = [&](int u, int v) { return in_ext(u, v) && in_a(u, v); };    // a computed stage output
+    auto red_at = [&](int u, int v) { return color_of(s, ex + u, ey + v) == 0; };
+    auto gd_at = [&](int u, int v, float gd[3], float go[3]) -> bool {
+        float uu[3];
+        if (!load_gout(a, ex + u, ey + v, go)) { gd[0] = gd[1] = gd[2] = 0.f; return false; }
+        chain_back(a, go, uu, gd);
+        return true;
     };
-    auto add_raw = [&](int u, int v, float w) {
-        mirror_uv(u, v);
-        if (in_tile(u, v)) atomicAdd(adj_raw + (v - 4) * TW + (u - 4), w);
+    // Virtual positions of (u, v): per coordinate, the position itself and
+    // the outside positions within 2 of the area that reflect onto it.
+    auto axis = [](int q, int lo, int n, int (&z)[3]) {
+        int cnt = 0;
+        z[cnt++] = q;
+        const int cand[4] = {lo - 2, lo - 1, lo + n, lo + n + 1};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (reflect(cand[i], lo, n) == q && cnt < 3) z[cnt++] = cand[i];
+        return cnt;
     };
-    auto add_rb = [&](float* plane, int u, int v, float w) {   // green-site adjoint
-        mirror_uv(u, v);
-        if (u >= 0 && u < EW && v >= 0 && v < EH) atomicAdd(plane + v * (EW / 2) + (u >> 1), w);
+    auto virtuals = [&](int u, int v, auto&& fn) {
+        int zx[3], zy[3];
+        const int nx = axis(ex + u, s.ax, s.aw, zx), ny = axis(ey + v, s.ay, s.ah, zy);
+        for (int i = 0; i < ny; ++i)
+            for (int k = 0; k < nx; ++k) fn(zx[k] - ex, zy[i] - ey);
     };
-    auto add_g = [&](int u, int v, float w) {                  // non-green green adjoint
-        mirror_uv(u, v);
-        if (u >= 2 && u < EW - 2 && v >= 2 && v < EH - 2) atomicAdd(adj_g + (v - 2) * GA_HW + ((u - 2) >> 1), w);
-    };
-
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    // (a) outputs of the extended tile: direct terms, final-stage chroma.
+
+    // pass 1: every site of the extended tile inside the area
     for (int i = threadIdx.x; i < EW * EH; i += kNT) {
         const int v = i / EW, u = i - v * EW;
-        const int sx = ex + u, sy = ey + v;
-        if (!t.inside(u, v)) continue;
-        float go[3];
-        if (!load_gout(a, sx, sy, go)) continue;
-        float uu[3], gd[3];
-        chain_back(a, go, uu, gd);
-        if (in_tile(u, v)) {
+        if (!in_a(u, v)) continue;
+        float gd[3], go[3];
+        const bool has = gd_at(u, v, gd, go);
+        const bool green = ((u & 1) != t.ng_parity(v));
+        if (!green) {
+            const int c = ci(u, v);
+            const bool h = t.hz[c] != 0;
+            const float po = red_at(u, v) ? gd[2] : gd[0];
+            POh[c] = h ? po : 0.f;
+            POv[c] = h ? 0.f : po;
+            PG[c] = gd[1];
+        }
+        if (has && in_tile(u, v)) {
             float d[3];
             t.finish(u, v, d[0], d[1], d[2]);
             param_grads(d, go, acc);
         }
-        const int own = color_of(s, sx, sy);
-        if (own == 1) {
-            add_rb(adj_r, u, v, gd[0]);
-            add_rb(adj_b, u, v, gd[2]);
-            add_raw(u, v, gd[1]);
-            continue;
-        }
-        add_raw(u, v, gd[0] + gd[2]);
-        add_g(u, v, gd[1]);
-        const bool horizontal = t.hz[v * (EW / 2) + (u >> 1)] != 0;
-        const int dx = horizontal ? 1 : 0, dy = horizontal ? 0 : 1;
-        // R site: B = r + mean(blue) - mean(red); B site: R = r + mean(red) - mean(blue).
-        const float w = own == 0 ? gd[2] : gd[0];
-        float* plus = own == 0 ? adj_b : adj_r;
-        float* minus = own == 0 ? adj_r : adj_b;
-        add_rb(plus, u - dx, v - dy, 0.5f * w);
-        add_rb(plus, u + dx, v + dy, 0.5f * w);
-        add_rb(minus, u - dx, v - dy, -0.5f * w);
-        add_rb(minus, u + dx, v + dy, -0.5f * w);
     }
     __syncthreads();
-    // (c) red/blue at green sites: Rg = G + mean(raw) - mean(G) along the red axis.
-    {
-        constexpr int HW = EW / 2, N = HW * EH;
-        for (int i = threadIdx.x; i < N; i += kNT) {
-            const int v = i / HW, k = i - v * HW;
-            const int u = 2 * k + (t.ng_parity(v) ^ 1);
-            if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1 || !t.inside(u, v)) continue;
-            const float ar = adj_r[i], ab = adj_b[i];
-            if (ar == 0.0f && ab == 0.0f) continue;
-            const bool red_horizontal = color_of(s, ex + u + 1, ey + v) == 0;
-            const int dx = red_horizontal ? 1 : 0, dy = red_horizontal ? 0 : 1;
-            add_raw(u, v, ar + ab);
-            add_raw(u - dx, v - dy, 0.5f * ar);
-            add_raw(u + dx, v + dy, 0.5f * ar);
-            add_g(u - dx, v - dy, -0.5f * ar);
-            add_g(u + dx, v + dy, -0.5f * ar);
-            add_raw(u - dy, v - dx, 0.5f * ab);
-            add_raw(u + dy, v + dx, 0.5f * ab);
-            add_g(u - dy, v - dx, -0.5f * ab);
-            add_g(u + dy, v + dx, -0.5f * ab);
-        }
+    // pass 2: green sites inside the area with all four neighbours in the extended tile
+    for (int i = threadIdx.x; i < NC; i += kNT) {
+        const int v = i / HWE, u = 2 * (i - v * HWE) + (t.ng_parity(v) ^ 1);
+        if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1 || !in_a(u, v)) continue;
+        float gd[3], go[3];
+        gd_at(u, v, gd, go);
+        float ar = gd[0], ab = gd[2];
+        // R neighbour: B = r + mean(B) - mean(R); B neighbour: R = b + mean(R) - mean(B)
+        auto take = [&](int nu, int nv, const float* P) {
+            if (!src(nu, nv)) return;
+            const float w = 0.5f * P[ci(nu, nv)];
+            if (red_at(nu, nv)) { ar -= w; ab += w; }
+            else { ar += w; ab -= w; }
+        };
+        virtuals(u, v, [&](int zu, int zv) {
+            take(zu - 1, zv, POh);
+            take(zu + 1, zv, POh);
+            take(zu, zv - 1, POv);
+            take(zu, zv + 1, POv);
+        });
+        AR[i] = ar;
+        AB[i] = ab;
+        if (in_tile(u, v)) RP[(v - 4) * TW + (u - 4)] = ar + ab + gd[1];
     }
     __syncthreads();
-    // (d) directional green along the taken axis: 0.5(r-1 + r+1) + 0.5 r - 0.25(r-2 + r+2).
-    {
-        constexpr int N = GA_HW * (EH - 4);
-        for (int i = threadIdx.x; i < N; i += kNT) {
-            const int row = i / GA_HW, k = i - row * GA_HW;
-            const int v = row + 2;
-            const int u = 2 + 2 * k + t.ng_parity(v);
-            if (!t.inside(u, v)) continue;
-            const float w = adj_g[i];
-            if (w == 0.0f) continue;
-            const bool horizontal = t.hz[v * (EW / 2) + (u >> 1)] != 0;
-            const int dx = horizontal ? 1 : 0, dy = horizontal ? 0 : 1;
-            add_raw(u, v, 0.5f * w);
-            add_raw(u - dx, v - dy, 0.5f * w);
-            add_raw(u + dx, v + dy, 0.5f * w);
-            add_raw(u - 2 * dx, v - 2 * dy, -0.25f * w);
-            add_raw(u + 2 * dx, v + 2 * dy, -0.25f * w);
+    // pass 3: non-green sites of tile +-2 inside the area
+    for (int i = threadIdx.x; i < NC; i += kNT) {
+        const int v = i / HWE, u = 2 * (i - v * HWE) + t.ng_parity(v);
+        if (u < 2 || u >= EW - 2 || v < 2 || v >= EH - 2 || !in_a(u, v)) continue;
+        const float* A = red_at(u, v) ? AR : AB;
+        float S = 0.f;
+        virtuals(u, v, [&](int zu, int zv) {
+            if (src(zu - 1, zv)) S += A[ci(zu - 1, zv)];
+            if (src(zu + 1, zv)) S += A[ci(zu + 1, zv)];
+            if (src(zu, zv - 1)) S += A[ci(zu, zv - 1)];
+            if (src(zu, zv + 1)) S += A[ci(zu, zv + 1)];
+        });
+        const float ag = PG[i] - 0.5f * S;
+        const bool h = t.hz[i] != 0;
+        AGh[i] = h ? ag : 0.f;
+        AGv[i] = h ? 0.f : ag;
+        if (in_tile(u, v)) {
+            float gd[3], go[3];
+            gd_at(u, v, gd, go);
+            RP[(v - 4) * TW + (u - 4)] = (gd[0] + gd[2]) + 0.5f * S;   // the own sample feeds both r and b
         }
     }
     __syncthreads();
+    // pass 4: raw adjoint of the tile's samples inside the area
     for (int i = threadIdx.x; i < TW * TH; i += kNT) {
-        const int v = i / TW, u = i - v * TW;
-        const int sx = tx + u, sy = ty + v;
-        if (a.graw && sx < s.ax + s.aw && sy < s.ay + s.ah)
-            a.graw[(size_t)(sy - s.ay) * s.aw + (size_t)(sx - s.ax)] = adj_raw[i];
+        const int v = 4 + i / TW, u = 4 + (i - (i / TW) * TW);
+        if (!in_a(u, v)) continue;
+        float r = RP[i];
+        if ((u & 1) != t.ng_parity(v)) {
+            virtuals(u, v, [&](int zu, int zv) {
+                if (src(zu - 1, zv)) r += 0.5f * AGh[ci(zu - 1, zv)];
+                if (src(zu + 1, zv)) r += 0.5f * AGh[ci(zu + 1, zv)];
+                if (src(zu, zv - 1)) r += 0.5f * AGv[ci(zu, zv - 1)];
+                if (src(zu, zv + 1)) r += 0.5f * AGv[ci(zu, zv + 1)];
+            });
+        } else {
+            const int c = ci(u, v);
+            r += 0.5f * (AGh[c] + AGv[c]);
+            virtuals(u, v, [&](int zu, int zv) {
+                if (src(zu - 2, zv)) r -= 0.25f * AGh[ci(zu - 2, zv)];
+                if (src(zu + 2, zv)) r -= 0.25f * AGh[ci(zu + 2, zv)];
+                if (src(zu, zv - 2)) r -= 0.25f * AGv[ci(zu, zv - 2)];
+                if (src(zu, zv + 2)) r -= 0.25f * AGv[ci(zu, zv + 2)];
+            });
+        }
+        if (a.graw) a.graw[(size_t)(ty + v - 4 - s.ay) * s.aw + (size_t)(tx + u - 4 - s.ax)] = r;
     }
     flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GH));
 }
@@ -487,7 +517,7 @@ __global__ void __launch_bounds__(kNT) menon_backward(const BackwardArgs a) {
     const bool interior = tx - 10 >= s.ax && ty - 10 >= s.ay && tx + TW + 10 <= s.ax + s.aw &&
                           ty + TH + 10 <= s.ay + s.ah;
     if (!interior) {
-        menon_back_tile<TW, TH, true>(a, s, lds, tx, ty);
+        menon_back_border<TW, TH>(a, s, lds, tx, ty);
         return;
     }
     // x parity of the non-green sites in even tile rows: the same for every

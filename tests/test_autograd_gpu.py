@@ -66,7 +66,8 @@ def bilinear_f64(om, raw):
 
 
 @pytest.mark.parametrize("demosaic", ["bilinear", "menon"])
-@pytest.mark.parametrize("case", [(22, 18, 0, (0, 0, 0, 0), 0, (0, 0)), (29, 23, 32, (3, 2, 24, 19), 1, (1, 0))])
+@pytest.mark.parametrize("case", [(22, 18, 0, (0, 0, 0, 0), 0, (0, 0)), (29, 23, 32, (3, 2, 24, 19), 1, (1, 0)),
+                                  (8, 7, 0, (1, 1, 5, 4), 3, (0, 1))])
 def test_render_chain_gradients_vs_fd(demosaic, case):
     w, h, stride, area, pat, ph = case
     m = meta_dict(w, h, stride, pat, ph[0], ph[1], area, (500, 510, 505, 515), (9000,) * 4)
