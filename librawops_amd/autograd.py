@@ -273,6 +273,84 @@ def tone_curve(x, knots_y, knots_x, cubic=True):
     return ToneCurveFn.apply(x, knots_y, knots_x, cubic)
 
 
+class FullChainCropsFn(torch.autograd.Function):
+    """``full_chain`` over several equal-size viewports of one image as one
+    autograd node: the same kernels in the same order per crop, with the
+    parameters read to the host and the two op plans built once per call
+    (instead of once per crop and op), and the parameter gradients of all
+    crops accumulated on the device."""
+
+    @staticmethod
+    def forward(ctx, gains, stops, matrix, strength, image: RawImage, viewports, demosaic: int, radius: int,
+                eps: float):
+        host = torch.cat([gains.detach().float().reshape(-1), stops.detach().float().reshape(1),
+                          matrix.detach().float().reshape(-1), strength.detach().float().reshape(1)]).cpu().tolist()
+        g, st, M, sb = host[0:3], host[3], host[4:13], host[13]
+        m = _multiplier(st)
+        render = Program.from_ops(demosaic, False, [(C.OP_GAINS, g), (C.OP_SCALE, (1.0,)), (C.OP_MATRIX, M)], [])
+        expose = Program.from_ops(C.DEMOSAIC_BILINEAR, False, [], [(C.OP_SCALE, (m,))])
+        h, w = viewports[0].height, viewports[0].width
+        n = len(viewports)
+        lin = torch.empty((n, h, w, 3), dtype=torch.float32, device="cuda")
+        out = torch.empty_like(lin)
+        lin2, base, boosted = torch.empty_like(lin), torch.empty_like(lin), torch.empty_like(lin)
+        for i, vp in enumerate(viewports):
+            if (vp.width, vp.height) != (w, h):
+                raise ValueError("full_chain_crops needs viewports of one size")
+            C.check(C.lib().rawops_render(render.handle, image._handle, vp.c(), 0, C.QUALITY_FINAL, _ptr(lin[i]), None,
+                                          w, None, _stream()))
+            src = _Raster(lin[i], 0)
+            C.check(C.lib().rawops_guided_filter(src.h, C.Rect(0, 0, w, h), 0, 0, 1.0, expose.handle, _ptr(lin2[i]), w,
+                                                 None, 0, None, _stream()))
+            src2 = _Raster(lin2[i], 0)
+            C.check(C.lib().rawops_guided_filter(src2.h, C.Rect(0, 0, w, h), 0, radius, eps, _empty_plan().handle,
+                                                 _ptr(base[i]), w, None, 0, None, _stream()))
+            C.check(C.lib().rawops_detail_boost(_ptr(lin2[i]), _ptr(base[i]), sb, _ptr(boosted[i]), lin2[i].numel(),
+                                                _stream()))
+            C.check(C.lib().rawops_srgb_encode(_ptr(boosted[i]), _ptr(out[i]), boosted[i].numel(), _stream()))
+        ctx.save_for_backward(lin, lin2, base, boosted)
+        ctx.image, ctx.viewports, ctx.demosaic, ctx.radius, ctx.eps = image, viewports, demosaic, radius, eps
+        ctx.g, ctx.m, ctx.M, ctx.sb = g, m, M, sb
+        return out
+
+    @staticmethod
+    def backward(ctx, gy):
+        lin, lin2, base, boosted = ctx.saved_tensors
+        gy = gy.contiguous().float()
+        n, h, w = lin.shape[:3]
+        gp = torch.zeros(13, dtype=torch.float64, device="cuda")     # gains, (unused), matrix
+        gst = torch.zeros(1, dtype=torch.float64, device="cuda")
+        gsb = torch.zeros(1, dtype=torch.float64, device="cuda")
+        geps = torch.zeros(1, dtype=torch.float64, device="cuda")
+        g3 = (ctypes.c_float * 3)(*ctx.g)
+        M9 = (ctypes.c_float * 9)(*ctx.M)
+        t1, t2, t3 = torch.empty_like(lin[0]), torch.empty_like(lin[0]), torch.empty_like(lin[0])
+        for i, vp in enumerate(ctx.viewports):
+            N = lin[i].numel()
+            C.check(C.lib().rawops_srgb_encode_backward(_ptr(boosted[i]), _ptr(gy[i]), _ptr(t1), N, _stream()))
+            C.check(C.lib().rawops_detail_boost_backward(_ptr(lin2[i]), _ptr(base[i]), _ptr(t1), ctx.sb, _ptr(t2),
+                                                         _ptr(t3), _ptr(gsb), N, _stream()))
+            if ctx.radius:
+                src = _Raster(lin2[i], 0)
+                C.check(C.lib().rawops_guided_backward(src.h, ctx.radius, ctx.eps, C.Rect(0, 0, w, h), _ptr(t3), w,
+                                                       _ptr(t1), _ptr(geps), None, 0, _stream()))
+                t2.add_(t1)
+            else:
+                t2.add_(t3)
+            C.check(C.lib().rawops_exposure_backward(_ptr(lin[i]), _ptr(t2), ctx.m, _ptr(t1), _ptr(gst), N, _stream()))
+            C.check(C.lib().rawops_render_backward(ctx.image._handle, ctx.demosaic, vp.c(), g3, 1.0, M9, _ptr(t1), w,
+                                                   None, _ptr(gp), _stream()))
+        return (gp[0:3].float(), gst[0].float(), gp[4:13].float().reshape(3, 3), gsb[0].float(),
+                None, None, None, None, None)
+
+
+def full_chain_crops(image: RawImage, viewports, gains, stops, matrix, strength, demosaic="menon", radius=8,
+                     eps=1e-3):
+    """``full_chain`` of each viewport (all of one size), stacked (n, h, w, 3)."""
+    d = C.DEMOSAIC_MENON if demosaic in ("menon", C.DEMOSAIC_MENON) else C.DEMOSAIC_BILINEAR
+    return FullChainCropsFn.apply(gains, stops, matrix, strength, image, list(viewports), d, radius, float(eps))
+
+
 def full_chain(image: RawImage, viewport: Rect, gains, stops, matrix, strength, demosaic="menon", radius=8,
                eps=1e-3):
     """RAW -> demosaic -> WB -> exposure -> 3x3 -> guided-filter detail boost
