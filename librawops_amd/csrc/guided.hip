@@ -27,9 +27,20 @@ namespace {
 constexpr int kStripThreads = 256;
 constexpr int kCols = 1;                          // columns per thread
 constexpr int kStripCols = kStripThreads * kCols; // columns per strip (centres + 2r halo)
-constexpr int kBandRows = 32;
+constexpr int kBandRows = 32;                     // band height when the grid fills the GPU
+constexpr int kMinBandRows = 8;
+
+// Rows per band: kBandRows, halved (down to kMinBandRows) while the launch
+// would have fewer than ~4 workgroups per CU; a shorter band repeats the
+// 2r-row warm-up of the vertical window but keeps small images parallel.
+inline int band_rows(int strips, int rows) {
+    int band = kBandRows;
+    while (band > kMinBandRows && (long)strips * ((rows + band - 1) / band) < 1024) band /= 2;
+    return band;
+}
 
 struct GuidedArgs {
+    int band;                // rows per band (blockIdx.y)
     const float* src;
     uint32_t src_stride;
     int W, H;                // native raster size
@@ -196,8 +207,8 @@ __global__ void __launch_bounds__(kStripThreads) gf_coefficients(const GuidedArg
     __shared__ double totals[4 * 8];
     const int span = kStripCols - 2 * g.r;
     const int strip_x = g.cx0 + (int)blockIdx.x * span;
-    const int k0 = g.cy0 + (int)blockIdx.y * kBandRows;
-    const int k1 = min(k0 + kBandRows, g.cy0 + g.ch);
+    const int k0 = g.cy0 + (int)blockIdx.y * g.band;
+    const int k1 = min(k0 + g.band, g.cy0 + g.ch);
     uint32_t err = 0;
     // Per-strip shift: the first centre pixel of the strip.
     double shift[3];
@@ -241,8 +252,8 @@ __global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g) {
     __shared__ double totals[4 * 6];
     const int span = kStripCols - 2 * g.r;
     const int strip_x = g.vx + (int)blockIdx.x * span;
-    const int i0 = g.vy + (int)blockIdx.y * kBandRows;
-    const int i1 = min(i0 + kBandRows, g.vy + g.vh);
+    const int i0 = g.vy + (int)blockIdx.y * g.band;
+    const int i1 = min(i0 + g.band, g.vy + g.vh);
     uint32_t err = 0;
     // Coefficients exist for centres in [cx0, cx0 + cw) x [cy0, cy0 + ch):
     // exactly the clipped windows of the viewport's pixels.
@@ -306,6 +317,7 @@ __global__ void gf_identity(const GuidedArgs g) {
 // the per-centre terms are stored as FP32 planes.
 
 struct GuidedBackArgs {
+    int band;                // rows per band (blockIdx.y)
     const float* p;          // raster, level 0 (W x H, stride)
     uint32_t stride;
     int W, H, r;
@@ -328,7 +340,7 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBac
     __shared__ double eps_part[kStripThreads / 64];
     const int span = kStripCols - 2 * g.r;
     const int strip_x = (int)blockIdx.x * span;
-    const int y0 = (int)blockIdx.y * kBandRows, y1 = min(y0 + kBandRows, g.H);
+    const int y0 = (int)blockIdx.y * g.band, y1 = min(y0 + g.band, g.H);
     double geps = 0.0;
     auto src = [&](int x, int y, double v[14]) {
         const float* q = g.p + ((size_t)y * g.stride + (size_t)x) * 3;
@@ -400,7 +412,7 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBack
     __shared__ double totals[4 * 11];
     const int span = kStripCols - 2 * g.r;
     const int strip_x = (int)blockIdx.x * span;
-    const int y0 = (int)blockIdx.y * kBandRows, y1 = min(y0 + kBandRows, g.H);
+    const int y0 = (int)blockIdx.y * g.band, y1 = min(y0 + g.band, g.H);
     const size_t plane = (size_t)g.W * g.H;
     auto src = [&](int x, int y, double v[11]) {
         const float* w = g.ws + (size_t)y * g.W + (size_t)x;
@@ -468,7 +480,9 @@ hipError_t launch_guided_backward(const GuidedBackwardLaunch& L, hipStream_t str
     g.gp = L.g_in;
     g.geps = L.g_eps;
     const int span = kStripCols - 2 * g.r;
-    dim3 grid((unsigned)((g.W + span - 1) / span), (unsigned)((g.H + kBandRows - 1) / kBandRows));
+    const int strips = (g.W + span - 1) / span;
+    g.band = band_rows(strips, g.H);
+    dim3 grid((unsigned)strips, (unsigned)((g.H + g.band - 1) / g.band));
     hipLaunchKernelGGL(gf_back_centres, grid, dim3(kStripThreads), 0, stream, g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -508,11 +522,15 @@ hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t 
     }
     g.coef = static_cast<float*>(L.workspace);
     const int span = kStripCols - 2 * g.r;
-    dim3 grid_c((unsigned)((g.cw + span - 1) / span), (unsigned)((g.ch + kBandRows - 1) / kBandRows));
+    const int strips_c = (g.cw + span - 1) / span;
+    g.band = band_rows(strips_c, g.ch);
+    dim3 grid_c((unsigned)strips_c, (unsigned)((g.ch + g.band - 1) / g.band));
     hipLaunchKernelGGL(gf_coefficients, grid_c, dim3(kStripThreads), 0, stream, g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    dim3 grid_a((unsigned)((g.vw + span - 1) / span), (unsigned)((g.vh + kBandRows - 1) / kBandRows));
+    const int strips_a = (g.vw + span - 1) / span;
+    g.band = band_rows(strips_a, g.vh);
+    dim3 grid_a((unsigned)strips_a, (unsigned)((g.vh + g.band - 1) / g.band));
     hipLaunchKernelGGL(gf_apply, grid_a, dim3(kStripThreads), 0, stream, g);
     return hipGetLastError();
 }

@@ -134,6 +134,12 @@ __global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevC
 }
 
 unsigned blocks_for(size_t n) { return (unsigned)std::min<size_t>((n + kPW - 1) / kPW, 8192); }
+// Kernels that end in one FP64 atomic per block: at least 8 elements per
+// thread and at most 1024 blocks, so small inputs do not serialise thousands
+// of atomics on the one result.
+unsigned reduce_blocks_for(size_t n) {
+    return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 8 * kPW - 1) / (8 * kPW), 1024));
+}
 
 }  // namespace
 
@@ -150,7 +156,7 @@ hipError_t launch_srgb_backward(const float* x, const float* gy, float* gx, size
 hipError_t launch_exposure_backward(const float* x, const float* gy, float m, float* gx, double* g_stops, size_t n,
                                     hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(exposure_bwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, m, gx, g_stops, n);
+    hipLaunchKernelGGL(exposure_bwd, dim3(reduce_blocks_for(n)), dim3(kPW), 0, s, x, gy, m, gx, g_stops, n);
     return hipGetLastError();
 }
 hipError_t launch_boost(const float* x, const float* base, float strength, float* y, size_t n, hipStream_t s) {
@@ -161,7 +167,7 @@ hipError_t launch_boost(const float* x, const float* base, float strength, float
 hipError_t launch_boost_backward(const float* x, const float* base, const float* gy, float strength, float* gx,
                                  float* gbase, double* gs, size_t n, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(boost_bwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, base, gy, strength, gx, gbase, gs, n);
+    hipLaunchKernelGGL(boost_bwd, dim3(reduce_blocks_for(n)), dim3(kPW), 0, s, x, base, gy, strength, gx, gbase, gs, n);
     return hipGetLastError();
 }
 hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t cubic,
