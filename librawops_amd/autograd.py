@@ -294,24 +294,30 @@ class FullChainCropsFn(torch.autograd.Function):
         lin = torch.empty((n, h, w, 3), dtype=torch.float32, device="cuda")
         out = torch.empty_like(lin)
         lin2, base, boosted = torch.empty_like(lin), torch.empty_like(lin), torch.empty_like(lin)
+        if any((vp.width, vp.height) != (w, h) for vp in viewports):
+            raise ValueError("full_chain_crops needs viewports of one size")
         for i, vp in enumerate(viewports):
-            if (vp.width, vp.height) != (w, h):
-                raise ValueError("full_chain_crops needs viewports of one size")
-            C.check(C.lib().rawops_render(render.handle, image._handle, vp.c(), 0, C.QUALITY_FINAL, _ptr(lin[i]), None,
-                                          w, None, _stream()))
-            src = _Raster(lin[i], 0)
-            C.check(C.lib().rawops_guided_filter(src.h, C.Rect(0, 0, w, h), 0, 0, 1.0, expose.handle, _ptr(lin2[i]), w,
-                                                 None, 0, None, _stream()))
-            src2 = _Raster(lin2[i], 0)
-            C.check(C.lib().rawops_guided_filter(src2.h, C.Rect(0, 0, w, h), 0, radius, eps, _empty_plan().handle,
-                                                 _ptr(base[i]), w, None, 0, None, _stream()))
-            C.check(C.lib().rawops_detail_boost(_ptr(lin2[i]), _ptr(base[i]), sb, _ptr(boosted[i]), lin2[i].numel(),
-                                                _stream()))
-            C.check(C.lib().rawops_srgb_encode(_ptr(boosted[i]), _ptr(out[i]), boosted[i].numel(), _stream()))
+            FullChainCropsFn._forward_one(image, vp, render, expose, radius, eps, sb, lin[i], lin2[i], base[i],
+                                          boosted[i], out[i])
         ctx.save_for_backward(lin, lin2, base, boosted)
         ctx.image, ctx.viewports, ctx.demosaic, ctx.radius, ctx.eps = image, viewports, demosaic, radius, eps
         ctx.g, ctx.m, ctx.M, ctx.sb = g, m, M, sb
+        ctx.plans = (render, expose)
         return out
+
+    @staticmethod
+    def _forward_one(image, vp, render, expose, radius, eps, sb, lin, lin2, base, boosted, out):
+        h, w = lin.shape[:2]
+        C.check(C.lib().rawops_render(render.handle, image._handle, vp.c(), 0, C.QUALITY_FINAL, _ptr(lin), None, w,
+                                      None, _stream()))
+        src = _Raster(lin, 0)
+        C.check(C.lib().rawops_guided_filter(src.h, C.Rect(0, 0, w, h), 0, 0, 1.0, expose.handle, _ptr(lin2), w, None,
+                                             0, None, _stream()))
+        src2 = _Raster(lin2, 0)
+        C.check(C.lib().rawops_guided_filter(src2.h, C.Rect(0, 0, w, h), 0, radius, eps, _empty_plan().handle,
+                                             _ptr(base), w, None, 0, None, _stream()))
+        C.check(C.lib().rawops_detail_boost(_ptr(lin2), _ptr(base), sb, _ptr(boosted), lin2.numel(), _stream()))
+        C.check(C.lib().rawops_srgb_encode(_ptr(boosted), _ptr(out), boosted.numel(), _stream()))
 
     @staticmethod
     def backward(ctx, gy):
