@@ -162,7 +162,7 @@ __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty,
 //           AGh / AGv by the site's decision
 //   pass 4: tile sites: raw adjoint through the directional-green stencil.
 // LDS reuse: POh/POv over dv (dead after the classifier), AR/AB over gh and
-// AGh/AGv over raw (both dead after pass 1), the parameter reduction over gv.
+// AGh/AGv over gv (both dead after pass 1), the parameter reduction over raw.
 template <int TW, int TH, int N0>
 __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx,
                                                     int ty) {
@@ -172,7 +172,8 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
     using L = MenonFwdLds<EW, EH>;
     static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "POh/POv do not fit over dv");
     static_assert(2 * NC * 4 <= L::NP * L::NR * 8, "AR/AB do not fit over gh");
-    static_assert(2 * NC * 4 <= L::OFF_GH, "AGh/AGv do not fit over raw");
+    static_assert(2 * NC * 4 <= L::NP * L::NR * 8, "AGh/AGv do not fit over gv");
+    static_assert((kNT / 64) * 9 * 8 <= L::OFF_GH, "the parameter reduction does not fit over raw");
     static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles must cover whole Bayer cells");
     const int ex = tx - 4, ey = ty - 4;
     F t(s, lds, ex, ey);
@@ -208,7 +209,7 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
     float* POv = POh + NC;
     float* AR = reinterpret_cast<float*>(lds + L::OFF_GH);
     float* AB = AR + NC;
-    float* AGh = reinterpret_cast<float*>(lds + L::OFF_RAW);
+    float* AGh = reinterpret_cast<float*>(lds + L::OFF_GV);
     float* AGv = AGh + NC;
     const bool red0 = (t.red_rows & 1u) != 0, red1 = (t.red_rows & 2u) != 0;   // non-green sites of row parity b are red
     auto red_row = [&](int b) { return b ? red1 : red0; };
@@ -248,14 +249,14 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
                     const int ic = F::gi(un + (h ? 1 : 0), v + (h ? 0 : 1));
                     const double mr = 0.5 * (t.red[ia] + t.red[ic]);
                     const double mb = 0.5 * (t.blue[ia] + t.blue[ic]);
-                    const float self = t.raw[F::rawi(un, v)];
+                    const float self = t.Rn(F::rawi(un, v), b, nb);
                     const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
                     const float d[3] = {rs ? self : other, (float)t.gh[F::ni(un, v)], rs ? other : self};
                     param_grads(d, goN[j][b], acc);
                 }
                 if (hg) {
                     const int g = F::gi(ug, v);
-                    const float d[3] = {(float)t.red[g], t.raw[F::rawi(ug, v)], (float)t.blue[g]};
+                    const float d[3] = {(float)t.red[g], t.Rn(F::rawi(ug, v), b, nb ^ 1), (float)t.blue[g]};
                     param_grads(d, goG[j][b], acc);
                 }
             }
@@ -324,7 +325,7 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
             }
         }
     }
-    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GV));
+    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_RAW));
 }
 
 // Border tiles: some reads of the forward are mirrored at the active-area

@@ -10,7 +10,10 @@
 // (MenonDemosaic.cpp:89-174) and must be compiled with -ffp-contract=off.
 //
 // LDS planes (checkerboard-compacted: one value per two columns):
-//   raw    f32   tile +-6, all sites
+//   raw    u16   tile +-6, all sites: the sample codes, normalised where read
+//                (the per-site levels are per-tile constants by CFA parity,
+//                and the normalisation is exact, so every read returns the
+//                same float; 2 bytes a sample keep three workgroups per CU)
 //   gh     f64   tile +-4, non-green sites; overwritten in place by the
 //                selected green (classifier output)
 //   gv     f64   tile +-4, non-green sites; red at green sites afterwards
@@ -34,7 +37,7 @@ struct MenonFwdLds {
     static constexpr int DVP = (TW + 8) / 2, DVR = TH + 6;
     static constexpr int GP = (TW + 4) / 2, GR = TH + 4; // red / blue, origin (-2, -2)
     static constexpr int OFF_RAW = 0;
-    static constexpr int OFF_GH = (RW * RH * 4 + 15) / 16 * 16;
+    static constexpr int OFF_GH = (RW * RH * 2 + 15) / 16 * 16;
     static constexpr int OFF_GV = OFF_GH + NP * NR * 8;
     static constexpr int OFF_DH = OFF_GV + NP * NR * 8;
     static constexpr int OFF_DV = OFF_DH + DHP * DHR * 8;
@@ -56,12 +59,13 @@ struct MenonFwd {
     const SensorArgs& s;
     int tx, ty;             // tile origin (sensor)
     uint32_t red_rows;      // bit b: the non-green sites of tile rows with parity b are red
-    float* raw;
+    uint16_t* raw;
     double *gh, *gv, *dh, *dv, *red, *blue;
     uint8_t* hz;
+    float blk[2][2], rcp[2][2], den[2][2];   // levels by (row, column) parity relative to the tile origin
 
     __device__ __forceinline__ MenonFwd(const SensorArgs& s_, char* lds, int tx_, int ty_) : s(s_), tx(tx_), ty(ty_) {
-        raw = reinterpret_cast<float*>(lds + L::OFF_RAW);
+        raw = reinterpret_cast<uint16_t*>(lds + L::OFF_RAW);
         gh = reinterpret_cast<double*>(lds + L::OFF_GH);
         gv = reinterpret_cast<double*>(lds + L::OFF_GV);
         dh = reinterpret_cast<double*>(lds + L::OFF_DH);
@@ -71,6 +75,22 @@ struct MenonFwd {
         hz = reinterpret_cast<uint8_t*>(lds + L::OFF_HZ);
         red_rows = 0;
         for (int b = 0; b < 2; ++b) red_rows |= (color_of(s, tx + nx(b), ty + b) == 0 ? 1u : 0u) << b;
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const uint32_t site = site_of(s, tx + e, ty + b);
+                blk[b][e] = site_value(s.black, site);
+                rcp[b][e] = site_value(s.rcp, site);
+                den[b][e] = site_value(s.denom, site);
+            }
+    }
+
+    // Normalised sample (RawEngine.cpp:89-97) at LDS index i of a position
+    // with row parity py and column parity px (relative to the tile origin;
+    // whole-sample reflection keeps parity).
+    __device__ __forceinline__ float Rn(int i, int py, int px) const {
+        return normalize_exact((float)raw[i], blk[py][px], rcp[py][px], den[py][px]);
     }
 
     // x parity (tile-relative) of the non-green sites in rows of parity b.
@@ -96,23 +116,11 @@ struct MenonFwd {
     // their levels are per-tile constants.
     __device__ __forceinline__ void load() {
         constexpr int QK = (TW + 12) / 2, QR = (TH + 12) / 2, N = QK * QR;
-        float blk[2][2], rcp[2][2], den[2][2];
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const uint32_t site = site_of(s, tx - 6 + e, ty - 6 + b);
-                blk[b][e] = site_value(s.black, site);
-                rcp[b][e] = site_value(s.rcp, site);
-                den[b][e] = site_value(s.denom, site);
-            }
         const int64_t stride = s.stride;
         const uint16_t* base = s.samples + ((int64_t)(ty - 6) * stride + (tx - 6));
         const bool pairs = !BORDER && ((reinterpret_cast<uintptr_t>(base) | (uintptr_t)(stride * 2)) & 3) == 0;
         constexpr int J = (N + NT - 1) / NT;
-        auto put = [&](int qr, int qk, int b, int e, uint32_t code) {
-            raw[(2 * qr + b) * RW + 2 * qk + e] = normalize_exact((float)code, blk[b][e], rcp[b][e], den[b][e]);
-        };
+        auto put = [&](int qr, int qk, int b, int e, uint32_t code) { raw[(2 * qr + b) * RW + 2 * qk + e] = (uint16_t)code; };
         if (pairs) {
             // Interior, 4-byte aligned cells: issue every load first, then convert.
             uint32_t w[J][2];
@@ -132,10 +140,8 @@ struct MenonFwd {
                 const int qr = it / QK, qk = it - qr * QK;
                 if (it < N) {
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        put(qr, qk, b, 0, w[j][b] & 0xffffu);
-                        put(qr, qk, b, 1, w[j][b] >> 16);
-                    }
+                    for (int b = 0; b < 2; ++b)   // both codes of the cell row in one 4-byte store
+                        *reinterpret_cast<uint32_t*>(raw + (2 * qr + b) * RW + 2 * qk) = w[j][b];
                 }
             }
             return;
@@ -171,12 +177,13 @@ struct MenonFwd {
                 if (!inside(rx, ry)) continue;
                 const int r0 = rawi(rx, ry);
                 auto R = [&](int ox, int oy) -> double {
+                    const int py = (b + oy) & 1, px = (nx(b) + ox) & 1;
                     if constexpr (BORDER) {
                         int x = rx + ox, y = ry + oy;
                         mirror(x, y);
-                        return (double)raw[rawi(x, y)];
+                        return (double)Rn(rawi(x, y), py, px);
                     } else {
-                        return (double)raw[r0 + oy * RW + ox];
+                        return (double)Rn(r0 + oy * RW + ox, py, px);
                     }
                 };
                 const double own = R(0, 0);
@@ -204,16 +211,16 @@ struct MenonFwd {
                 const int ry = -4 + 2 * qr + b, rx = -4 + 2 * k + nx(b);
                 const bool do_h = k < DHP, do_v = 2 * qr + b < TH + 6;
                 if (!inside(rx, ry)) continue;
-                const int r0 = rawi(rx, ry), n0 = (2 * qr + b) * NP + k;
-                const double own = (double)raw[r0];
+                const int r0 = rawi(rx, ry), n0 = (2 * qr + b) * NP + k, nb = nx(b);
+                const double own = (double)Rn(r0, b, nb);
                 if (do_h) {
                     double there;
                     if constexpr (BORDER) {
                         int x = rx + 2, y = ry;
                         mirror(x, y);
-                        there = (double)raw[rawi(x, y)] - gh[ni(x, y)];
+                        there = (double)Rn(rawi(x, y), b, nb) - gh[ni(x, y)];
                     } else {
-                        there = (double)raw[r0 + 2] - gh[n0 + 1];
+                        there = (double)Rn(r0 + 2, b, nb) - gh[n0 + 1];
                     }
                     dh[(2 * qr + b) * DHP + k] = fabs((own - gh[n0]) - there);
                 }
@@ -222,9 +229,9 @@ struct MenonFwd {
                     if constexpr (BORDER) {
                         int x = rx, y = ry + 2;
                         mirror(x, y);
-                        there = (double)raw[rawi(x, y)] - gv[ni(x, y)];
+                        there = (double)Rn(rawi(x, y), b, nb) - gv[ni(x, y)];
                     } else {
-                        there = (double)raw[r0 + 2 * RW] - gv[n0 + 2 * NP];
+                        there = (double)Rn(r0 + 2 * RW, b, nb) - gv[n0 + 2 * NP];
                     }
                     dv[(2 * qr + b) * DVP + k] = fabs((own - gv[n0]) - there);
                 }
@@ -302,12 +309,13 @@ struct MenonFwd {
                 const int r0 = rawi(rx, ry);
                 const int g0 = (2 * qr + 3 + b) * NP + k;   // row of this site in the gh/green plane
                 auto R = [&](int ox, int oy) -> double {
+                    const int py = (p + oy) & 1, px = (1 - np + ox) & 1;   // rx = -1 + 2k + np
                     if constexpr (BORDER) {
                         int x = rx + ox, y = ry + oy;
                         mirror(x, y);
-                        return (double)raw[rawi(x, y)];
+                        return (double)Rn(rawi(x, y), py, px);
                     } else {
-                        return (double)raw[r0 + oy * RW + ox];
+                        return (double)Rn(r0 + oy * RW + ox, py, px);
                     }
                 };
                 auto GR = [&](int ox, int oy) -> double {
@@ -350,7 +358,7 @@ struct MenonFwd {
                 const int gbase = (2 * qy + 2 + b) * GP + qx;   // red/blue row of this pixel row
                 if (inside(xg, ry)) {
                     gr = (float)red[gbase + 1];
-                    gg = raw[rawi(xg, ry)];
+                    gg = Rn(rawi(xg, ry), b, 1 - nb);
                     gb = (float)blue[gbase + 1];
                 }
                 if (inside(xn, ry)) {
@@ -371,7 +379,7 @@ struct MenonFwd {
                     }
                     const double mr = 0.5 * (red[ia] + red[ic]);
                     const double mb = 0.5 * (blue[ia] + blue[ic]);
-                    const float self = raw[rawi(xn, ry)];
+                    const float self = Rn(rawi(xn, ry), b, nb);
                     const double own = (double)self;
                     const bool red_site = (red_rows >> b) & 1u;
                     // R site: bb = rr + mean(blue) - mean(red); B site: rr = bb + mean(red) - mean(blue).
