@@ -425,7 +425,7 @@ def main():
     }
     line["fwd_bwd"] = {"value": round(px * bsteps / fb_wall / 1e6, 2), "unit": "Mpix/s",
                        "ms_per_step": round(fb_wall * 1e3 / bsteps, 4), "backward_kernel_ms": round(bwd_ms, 4),
-                       "backward": "menon_backward<64,24>: dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
+                       "backward": "menon_backward<64,20>: dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["fwd_bwd"]["gpu_fwd_bwd_over_cpu_fwd"] = round(line["fwd_bwd"]["value"] / cpu["value"], 1)

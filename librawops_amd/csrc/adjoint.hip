@@ -334,34 +334,25 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
 // target q gathers over the virtual positions V(q) = {z : m(z) = q} that lie
 // within a tap's reach (2) of the area -- q itself and its reflections
 // across nearby edges -- and applies the interior stencil at each z, counting
-// only sources inside the area.  The forward is recomputed with the
-// per-site border stages (MenonTile); the passes are those of the interior
-// path, per site, with upstream gradients re-read where a pass needs them.
-// LDS reuse (MenonLds layout): POh/POv/PG and AR/AB over dh+dv, AGh/AGv over
-// raw, the tile's raw partials over green, the parameter reduction over gh.
-template <int TW, int TH>
+// only sources inside the area.  The forward is recomputed with the border
+// form of the row-pair stages; the passes are those of the interior path,
+// per site, with upstream gradients re-read where a pass needs them.
+// LDS reuse: POh/POv over dv (dead after the classifier), AR/AB over dh,
+// AGh/AGv over gv (blue and red, dead after pass 1), the tile's raw partials
+// over gh, the parameter reduction over raw.
+template <int TW, int TH, int N0>
 __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx, int ty) {
     constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
-    using L = MenonLds<EW, EH>;
-    static_assert(5 * NC * 4 <= L::DH_BYTES + L::DV_BYTES, "adjoint planes do not fit over dh/dv");
-    static_assert(2 * NC * 4 <= L::RAW_BYTES, "AGh/AGv do not fit over raw");
-    static_assert(TW * TH * 4 <= L::G2_BYTES, "raw partials do not fit over green");
+    using F = MenonFwd<EW, EH, kNT, N0, true>;
+    using L = MenonFwdLds<EW, EH>;
+    static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "POh/POv do not fit over dv");
+    static_assert(2 * NC * 4 <= L::DHP * L::DHR * 8, "AR/AB do not fit over dh");
+    static_assert(2 * NC * 4 <= L::NP * L::NR * 8, "AGh/AGv do not fit over gv");
+    static_assert(TW * TH * 4 <= L::NP * L::NR * 8, "raw partials do not fit over gh");
+    static_assert((kNT / 64) * 9 * 8 <= L::OFF_GH, "the parameter reduction does not fit over raw");
     const int ex = tx - 4, ey = ty - 4;     // origin of the extended tile (tile +-4)
-    MenonTile<EW, EH, true, kNT> t{s, ex, ey, 0,
-                                   reinterpret_cast<float*>(lds + L::OFF_RAW),
-                                   reinterpret_cast<double*>(lds + L::OFF_GH),
-                                   reinterpret_cast<double*>(lds + L::OFF_GV),
-                                   reinterpret_cast<double*>(lds + L::OFF_DH),
-                                   reinterpret_cast<double*>(lds + L::OFF_DV),
-                                   reinterpret_cast<double*>(lds + L::OFF_GREEN),
-                                   reinterpret_cast<double*>(lds + L::OFF_GH),
-                                   reinterpret_cast<double*>(lds + L::OFF_GV),
-                                   reinterpret_cast<uint8_t*>(lds + L::OFF_HZ)};
-    for (int b = 0; b < 2; ++b) {
-        const uint32_t sensor = (s.ng_parity >> (((uint32_t)ey + (uint32_t)b) & 1u)) & 1u;
-        t.ngr |= ((sensor ^ ((uint32_t)ex & 1u)) & 1u) << b;
-    }
-    load_raw<EW, EH, 6, true, kNT>(s, ex, ey, t.raw);
+    F t(s, lds, ex, ey);
+    t.load();
     __syncthreads();
     t.directional();
     __syncthreads();
@@ -372,20 +363,25 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
     t.chroma();
     __syncthreads();
 
-    float* POh = reinterpret_cast<float*>(lds + L::OFF_DH);
+    float* POh = reinterpret_cast<float*>(lds + L::OFF_DV);    // dv is dead after the classifier
     float* POv = POh + NC;
-    float* PG = POv + NC;
-    float* AR = PG + NC;
+    float* AR = reinterpret_cast<float*>(lds + L::OFF_DH);     // dh (blue) is dead after pass 1
     float* AB = AR + NC;
-    float* AGh = reinterpret_cast<float*>(lds + L::OFF_RAW);
+    float* AGh = reinterpret_cast<float*>(lds + L::OFF_GV);
     float* AGv = AGh + NC;
-    float* RP = reinterpret_cast<float*>(lds + L::OFF_GREEN);
+    float* RP = reinterpret_cast<float*>(lds + L::OFF_GH);
     auto ci = [](int u, int v) { return v * HWE + (u >> 1); };
     auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
     auto in_ext = [](int u, int v) { return u >= 0 && u < EW && v >= 0 && v < EH; };
-    auto in_a = [&](int u, int v) { return in_area(s, ex + u, ey + v); };
+    // Area bounds read once (the kernel arguments stay in the argument
+    // segment instead of being copied to scratch memory).
+    const int ax = s.ax, ay = s.ay, aw = s.aw, ah = s.ah;
+    auto in_a = [&](int u, int v) {
+        const int x = ex + u, y = ey + v;
+        return x >= ax && y >= ay && x < ax + aw && y < ay + ah;
+    };
     auto src = [&](int u, int v) { return in_ext(u, v) && in_a(u, v); };    // a computed stage output
-    auto red_at = [&](int u, int v) { return color_of(s, ex + u, ey + v) == 0; };
+    auto red_at = [&](int, int v) { return ((t.red_rows >> (v & 1)) & 1u) != 0; };   // non-green sites only
     auto gd_at = [&](int u, int v, float gd[3], float go[3]) -> bool {
         float uu[3];
         if (!load_gout(a, ex + u, ey + v, go)) { gd[0] = gd[1] = gd[2] = 0.f; return false; }
@@ -393,49 +389,95 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
         return true;
     };
     // Virtual positions of (u, v): per coordinate, the position itself and
-    // the outside positions within 2 of the area that reflect onto it.
-    auto axis = [](int q, int lo, int n, int (&z)[3]) {
-        int cnt = 0;
-        z[cnt++] = q;
-        const int cand[4] = {lo - 2, lo - 1, lo + n, lo + n + 1};
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (reflect(cand[i], lo, n) == q && cnt < 3) z[cnt++] = cand[i];
+    // the outside positions within 2 of the area that reflect onto it (at
+    // most two; kept in scalars, not an indexed array, so nothing goes to
+    // scratch memory).
+    auto axis = [](int q, int lo, int n, int& z1, int& z2) {
+        int cnt = 1;
+        z1 = z2 = q;
+        auto take = [&](int c) {
+            // whole-sample reflection as its periodic fold (period 2(n-1)), loop-free
+            const int period = 2 * (n - 1);
+            int t = period > 0 ? (c - lo) % period : 0;
+            if (t < 0) t += period;
+            const int m = lo + (t < n ? t : period - t);
+            if (cnt < 3 && m == q) {
+                if (cnt == 1) z1 = c;
+                else z2 = c;
+                ++cnt;
+            }
+        };
+        take(lo - 2);
+        take(lo - 1);
+        take(lo + n);
+        take(lo + n + 1);
         return cnt;
     };
     auto virtuals = [&](int u, int v, auto&& fn) {
-        int zx[3], zy[3];
-        const int nx = axis(ex + u, s.ax, s.aw, zx), ny = axis(ey + v, s.ay, s.ah, zy);
-        for (int i = 0; i < ny; ++i)
-            for (int k = 0; k < nx; ++k) fn(zx[k] - ex, zy[i] - ey);
+        const int qx = ex + u, qy = ey + v;
+        int x1, x2, y1, y2;
+        const int nx = axis(qx, ax, aw, x1, x2), ny = axis(qy, ay, ah, y1, y2);
+        for (int i = 0; i < ny; ++i) {
+            const int zy = i == 0 ? qy : (i == 1 ? y1 : y2);
+            for (int k = 0; k < nx; ++k) {
+                const int zx = k == 0 ? qx : (k == 1 ? x1 : x2);
+                fn(zx - ex, zy - ey);
+            }
+        }
     };
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
-    // pass 1: every site of the extended tile inside the area
-    for (int i = threadIdx.x; i < EW * EH; i += kNT) {
-        const int v = i / EW, u = i - v * EW;
-        if (!in_a(u, v)) continue;
-        float gd[3], go[3];
-        const bool has = gd_at(u, v, gd, go);
-        const bool green = ((u & 1) != t.ng_parity(v));
-        if (!green) {
-            const int c = ci(u, v);
-            const bool h = t.hz[c] != 0;
-            const float po = red_at(u, v) ? gd[2] : gd[0];
-            POh[c] = h ? po : 0.f;
-            POv[c] = h ? 0.f : po;
-            PG[c] = gd[1];
-        }
-        if (has && in_tile(u, v)) {
-            float d[3];
-            t.finish(u, v, d[0], d[1], d[2]);
-            param_grads(d, go, acc);
-        }
+    // pass 1: every site of the extended tile inside the area, as row-pair
+    // items (compile-time CFA parities: the per-site levels are then selected
+    // at compile time)
+    for (int it = threadIdx.x; it < NC / 2; it += kNT) {
+        const int qr = it / HWE, k = it - qr * HWE;
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int gsite = 0; gsite < 2; ++gsite) {
+                const int v = 2 * qr + b, pu = F::nx(b) ^ gsite, u = 2 * k + pu;   // pu: parity of u
+                if (!in_a(u, v)) continue;
+                float gd[3], go[3];
+                const bool has = gd_at(u, v, gd, go);
+                const bool rs = ((t.red_rows >> b) & 1u) != 0;
+                if (!gsite) {
+                    const int c = ci(u, v);
+                    const bool h = t.hz[c] != 0;
+                    const float po = rs ? gd[2] : gd[0];
+                    POh[c] = h ? po : 0.f;
+                    POv[c] = h ? 0.f : po;
+                }
+                if (has && in_tile(u, v)) {
+                    // demosaic output (MenonDemosaic.cpp:162-173), as the forward takes it
+                    float d[3];
+                    if (gsite) {
+                        const int g = F::gi(u, v);
+                        d[0] = (float)t.red[g];
+                        d[1] = t.Rn(F::rawi(u, v), b, pu);
+                        d[2] = (float)t.blue[g];
+                    } else {
+                        const bool h = t.hz[ci(u, v)] != 0;
+                        int xa = u - (h ? 1 : 0), ya = v - (h ? 0 : 1), xc = u + (h ? 1 : 0), yc = v + (h ? 0 : 1);
+                        t.mirror(xa, ya);
+                        t.mirror(xc, yc);
+                        const int ia = F::gi(xa, ya), ic = F::gi(xc, yc);
+                        const double mr = 0.5 * (t.red[ia] + t.red[ic]);
+                        const double mb = 0.5 * (t.blue[ia] + t.blue[ic]);
+                        const float self = t.Rn(F::rawi(u, v), b, pu);
+                        const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
+                        d[0] = rs ? self : other;
+                        d[1] = (float)t.gh[F::ni(u, v)];
+                        d[2] = rs ? other : self;
+                    }
+                    param_grads(d, go, acc);
+                }
+            }
     }
     __syncthreads();
     // pass 2: green sites inside the area with all four neighbours in the extended tile
     for (int i = threadIdx.x; i < NC; i += kNT) {
-        const int v = i / HWE, u = 2 * (i - v * HWE) + (t.ng_parity(v) ^ 1);
+        const int v = i / HWE, u = 2 * (i - v * HWE) + (F::nx(v & 1) ^ 1);
         if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1 || !in_a(u, v)) continue;
         float gd[3], go[3];
         gd_at(u, v, gd, go);
@@ -460,7 +502,7 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
     __syncthreads();
     // pass 3: non-green sites of tile +-2 inside the area
     for (int i = threadIdx.x; i < NC; i += kNT) {
-        const int v = i / HWE, u = 2 * (i - v * HWE) + t.ng_parity(v);
+        const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
         if (u < 2 || u >= EW - 2 || v < 2 || v >= EH - 2 || !in_a(u, v)) continue;
         const float* A = red_at(u, v) ? AR : AB;
         float S = 0.f;
@@ -470,62 +512,71 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
             if (src(zu, zv - 1)) S += A[ci(zu, zv - 1)];
             if (src(zu, zv + 1)) S += A[ci(zu, zv + 1)];
         });
-        const float ag = PG[i] - 0.5f * S;
+        float gd[3], go[3];
+        gd_at(u, v, gd, go);
+        const float ag = gd[1] - 0.5f * S;
         const bool h = t.hz[i] != 0;
         AGh[i] = h ? ag : 0.f;
         AGv[i] = h ? 0.f : ag;
-        if (in_tile(u, v)) {
-            float gd[3], go[3];
-            gd_at(u, v, gd, go);
-            RP[(v - 4) * TW + (u - 4)] = (gd[0] + gd[2]) + 0.5f * S;   // the own sample feeds both r and b
-        }
+        if (in_tile(u, v)) RP[(v - 4) * TW + (u - 4)] = (gd[0] + gd[2]) + 0.5f * S;   // the own sample feeds both r and b
     }
     __syncthreads();
     // pass 4: raw adjoint of the tile's samples inside the area
     for (int i = threadIdx.x; i < TW * TH; i += kNT) {
         const int v = 4 + i / TW, u = 4 + (i - (i / TW) * TW);
         if (!in_a(u, v)) continue;
+        const bool green = (u & 1) != F::nx(v & 1);
+        const int step = green ? 1 : 2;            // directional-green taps reaching this sample
+        const float w = green ? 0.5f : -0.25f;
         float r = RP[i];
-        if ((u & 1) != t.ng_parity(v)) {
-            virtuals(u, v, [&](int zu, int zv) {
-                if (src(zu - 1, zv)) r += 0.5f * AGh[ci(zu - 1, zv)];
-                if (src(zu + 1, zv)) r += 0.5f * AGh[ci(zu + 1, zv)];
-                if (src(zu, zv - 1)) r += 0.5f * AGv[ci(zu, zv - 1)];
-                if (src(zu, zv + 1)) r += 0.5f * AGv[ci(zu, zv + 1)];
-            });
-        } else {
+        if (!green) {
             const int c = ci(u, v);
             r += 0.5f * (AGh[c] + AGv[c]);
-            virtuals(u, v, [&](int zu, int zv) {
-                if (src(zu - 2, zv)) r -= 0.25f * AGh[ci(zu - 2, zv)];
-                if (src(zu + 2, zv)) r -= 0.25f * AGh[ci(zu + 2, zv)];
-                if (src(zu, zv - 2)) r -= 0.25f * AGv[ci(zu, zv - 2)];
-                if (src(zu, zv + 2)) r -= 0.25f * AGv[ci(zu, zv + 2)];
-            });
         }
-        if (a.graw) a.graw[(size_t)(ty + v - 4 - s.ay) * s.aw + (size_t)(tx + u - 4 - s.ax)] = r;
+        int x1, x2, y1, y2;
+        const int nxv = axis(ex + u, ax, aw, x1, x2), nyv = axis(ey + v, ay, ah, y1, y2);
+        for (int iy = 0; iy < nyv; ++iy) {
+            const int zv = (iy == 0 ? ey + v : (iy == 1 ? y1 : y2)) - ey;
+            for (int ix = 0; ix < nxv; ++ix) {
+                const int zu = (ix == 0 ? ex + u : (ix == 1 ? x1 : x2)) - ex;
+                if (src(zu - step, zv)) r += w * AGh[ci(zu - step, zv)];
+                if (src(zu + step, zv)) r += w * AGh[ci(zu + step, zv)];
+                if (src(zu, zv - step)) r += w * AGv[ci(zu, zv - step)];
+                if (src(zu, zv + step)) r += w * AGv[ci(zu, zv + step)];
+            }
+        }
+        if (a.graw) a.graw[(size_t)(ty + v - 4 - ay) * aw + (size_t)(tx + u - 4 - ax)] = r;
     }
-    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_GH));
+    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_RAW));
 }
 
 template <int TW, int TH>
-__global__ void __launch_bounds__(kNT) menon_backward(const BackwardArgs a) {
-    constexpr int LDS = std::max(MenonLds<TW + 8, TH + 8>::BYTES, MenonFwdLds<TW + 8, TH + 8>::BYTES);
-    __shared__ __attribute__((aligned(16))) char lds[LDS];
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6))) menon_backward(const BackwardArgs args) {
+    __shared__ __attribute__((aligned(16))) char lds[MenonFwdLds<TW + 8, TH + 8>::BYTES];
+    // The arguments are read where they lie, in the kernel-argument segment
+    // (the only parameter, at offset 0): a by-value copy that the optimiser
+    // cannot fold away would be materialised in scratch memory.
+    (void)args;
+    const BackwardArgs& a = *(const BackwardArgs*)__builtin_amdgcn_kernarg_segment_ptr();
     int tx, ty;
     tile_of_border_first(a, tx, ty, TW, TH);
+    // Uniform across the workgroup; said so, so that everything derived from
+    // the tile origin (CFA parities, per-site levels) stays scalar.
+    tx = __builtin_amdgcn_readfirstlane(tx);
+    ty = __builtin_amdgcn_readfirstlane(ty);
     const SensorArgs& s = a.s;
     const bool interior = tx - 10 >= s.ax && ty - 10 >= s.ay && tx + TW + 10 <= s.ax + s.aw &&
                           ty + TH + 10 <= s.ay + s.ah;
-    if (!interior) {
-        menon_back_border<TW, TH>(a, s, lds, tx, ty);
-        return;
-    }
     // x parity of the non-green sites in even tile rows: the same for every
     // tile of the launch (tile origins step by even sizes)
     const uint32_t n0 = ((s.ng_parity >> ((uint32_t)ty & 1u)) & 1u) ^ ((uint32_t)tx & 1u);
-    if (n0) menon_back_interior<TW, TH, 1>(a, s, lds, tx, ty);
-    else menon_back_interior<TW, TH, 0>(a, s, lds, tx, ty);
+    if (n0) {
+        if (interior) menon_back_interior<TW, TH, 1>(a, s, lds, tx, ty);
+        else menon_back_border<TW, TH, 1>(a, s, lds, tx, ty);
+    } else {
+        if (interior) menon_back_interior<TW, TH, 0>(a, s, lds, tx, ty);
+        else menon_back_border<TW, TH, 0>(a, s, lds, tx, ty);
+    }
 }
 
 // ------------------------------------------------------------- bilinear --
@@ -605,10 +656,10 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     a.gparams = L.g_params;
     // Only raw samples within 4 of the viewport receive gradient (the
     // adjoint's reach); tiles are anchored at the active-area origin.  The
-    // Menon tile is 64x24 so its recompute region (tile +-4) fits two
+    // Menon tile is 64x20 so its recompute region (tile +-4) fits three
     // workgroups per CU in LDS.
     const bool menon = L.demosaic == RAWOPS_DEMOSAIC_MENON;
-    const int TW = 64, TH = menon ? 24 : 32;
+    const int TW = 64, TH = menon ? 20 : 32;
     const int x0 = std::max(0, a.vx - 4 - a.s.ax), y0 = std::max(0, a.vy - 4 - a.s.ay);
     const int x1 = std::min(a.s.aw, a.vx + a.vw + 4 - a.s.ax), y1 = std::min(a.s.ah, a.vy + a.vh + 4 - a.s.ay);
     a.tile0_x = x0 / TW;
@@ -633,7 +684,7 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     interior_range(a.tile0_x, a.tiles_x, TW, a.s.ax, a.s.aw, a.c0, a.c1);
     if (a.r0 == a.r1 || a.c0 == a.c1) a.r0 = a.r1 = a.c0 = a.c1 = 0;
     if (menon)
-        hipLaunchKernelGGL((menon_backward<64, 24>), dim3(blocks), dim3(kNT), 0, stream, a);
+        hipLaunchKernelGGL((menon_backward<64, 20>), dim3(blocks), dim3(kNT), 0, stream, a);
     else
         hipLaunchKernelGGL((bilinear_backward<64, 32>), dim3(blocks), dim3(kNT), 0, stream, a);
     return hipGetLastError();

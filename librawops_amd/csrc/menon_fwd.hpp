@@ -75,14 +75,19 @@ struct MenonFwd {
         hz = reinterpret_cast<uint8_t*>(lds + L::OFF_HZ);
         red_rows = 0;
         for (int b = 0; b < 2; ++b) red_rows |= (color_of(s, tx + nx(b), ty + b) == 0 ? 1u : 0u) << b;
+        // The levels are uniform over the workgroup; readfirstlane keeps them
+        // opaque scalars, so a run-time parity select later cannot be folded
+        // into an indexed read of the kernel arguments (which would copy them
+        // to scratch memory).
+        auto uni = [](float v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); };
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const uint32_t site = site_of(s, tx + e, ty + b);
-                blk[b][e] = site_value(s.black, site);
-                rcp[b][e] = site_value(s.rcp, site);
-                den[b][e] = site_value(s.denom, site);
+                blk[b][e] = uni(site_value(s.black, site));
+                rcp[b][e] = uni(site_value(s.rcp, site));
+                den[b][e] = uni(site_value(s.denom, site));
             }
     }
 
@@ -90,7 +95,12 @@ struct MenonFwd {
     // with row parity py and column parity px (relative to the tile origin;
     // whole-sample reflection keeps parity).
     __device__ __forceinline__ float Rn(int i, int py, int px) const {
-        return normalize_exact((float)raw[i], blk[py][px], rcp[py][px], den[py][px]);
+        return normalize_exact((float)raw[i], lvl(blk, py, px), lvl(rcp, py, px), lvl(den, py, px));
+    }
+    // Selects rather than an indexed load, so a run-time parity does not put
+    // the tile state in scratch memory.
+    __device__ __forceinline__ static float lvl(const float (&v)[2][2], int py, int px) {
+        return py ? (px ? v[1][1] : v[1][0]) : (px ? v[0][1] : v[0][0]);
     }
 
     // x parity (tile-relative) of the non-green sites in rows of parity b.
