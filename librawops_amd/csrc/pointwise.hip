@@ -135,10 +135,10 @@ __global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevC
 
 unsigned blocks_for(size_t n) { return (unsigned)std::min<size_t>((n + kPW - 1) / kPW, 8192); }
 // Kernels that end in one FP64 atomic per block: at least 8 elements per
-// thread and at most 1024 blocks, so small inputs do not serialise thousands
-// of atomics on the one result.
+// thread, so small inputs do not serialise thousands of atomics on the one
+// result (large inputs keep the full streaming grid).
 unsigned reduce_blocks_for(size_t n) {
-    return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 8 * kPW - 1) / (8 * kPW), 1024));
+    return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 8 * kPW - 1) / (8 * kPW), 8192));
 }
 
 }  // namespace
