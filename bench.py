@@ -287,8 +287,9 @@ def compiled_matrix(node):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=200,
+                    help="untimed steps first (the GPU clocks ramp up over the first ~20 ms of work)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--main-only", action="store_true", help="skip the C1/C3/C4/C5 legs")
     ap.add_argument("--c5-steps", type=int, default=200, help="Adam steps of the differentiable-chain leg")
