@@ -821,6 +821,12 @@ rawops_status rawops_guided_filter(const rawops_source* raster, rawops_rect view
                 L.workspace = owned;
             }
         }
+        // Curve lookups are long-latency (data-dependent table reads, an FP64
+        // division): inside the strip pass they stall its per-row loop, so
+        // they run in a separate pointwise pass over the output.
+        L.post_ops = L.fused_ops = post->reduced.n;
+        for (uint32_t i = 0; i < post->reduced.n; ++i)
+            if (post->reduced.op[i].kind == OP_CURVES) { L.fused_ops = i; break; }
         const hipError_t e = launch_guided(L, post->device + 1, st);
         if (owned) (void)hipFreeAsync(owned, st);
         hip_check(e, "guided launch");

@@ -136,9 +136,33 @@ __device__ __forceinline__ double cubic_blend(double a, double b, double t) {
     return t <= .5 ? a + t * (b - a) : b + (1 - t) * (a - b);
 }
 
-// ExtendedCurvesNode::map, ToneOps.cpp:820-831.
-__device__ inline double curve_map(const DevCurve& curve, uint32_t cubic, double v) {
-    Const<DevCurve>& c = as_const(curve);
+// A curve's knot tables staged in LDS (same fields as DevCurve, n knots).
+struct LdsCurve {
+    uint32_t n, identity;
+    const double *x, *y, *slope, *c0, *c1;
+};
+
+// Stage curve c (n <= cap knots) into LDS at base (5 n doubles); the whole
+// workgroup copies, the caller synchronises.
+__device__ __forceinline__ LdsCurve stage_curve(const DevCurve* c, double* base) {
+    Const<DevCurve>& s = as_const(*c);
+    const uint32_t n = s.n;
+    LdsCurve l{n, s.identity, base, base + n, base + 2 * n, base + 3 * n, base + 4 * n};
+    if (!l.identity)
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            base[k] = s.x[k];
+            base[n + k] = s.y[k];
+            base[2 * n + k] = s.slope[k];
+            base[3 * n + k] = s.c0[k];
+            base[4 * n + k] = s.c1[k];
+        }
+    return l;
+}
+
+// ExtendedCurvesNode::map, ToneOps.cpp:820-831, over a DevCurve in the
+// constant address space or an LdsCurve.
+template <class C>
+__device__ inline double curve_map_in(const C& c, uint32_t cubic, double v) {
     if (c.identity) return v;
     // upper_bound on x (v is finite, x strictly increasing).  Short curves:
     // count the knots <= v, so every knot load has the same address on all
@@ -165,6 +189,10 @@ __device__ inline double curve_map(const DevCurve& curve, uint32_t cubic, double
     return cubic_blend(cubic_blend(a, b, t), cubic_blend(b, d, t), t);
 }
 
+__device__ inline double curve_map(const DevCurve& curve, uint32_t cubic, double v) {
+    return curve_map_in(as_const(curve), cubic, v);
+}
+
 struct CurveOut {
     float v;
     uint32_t err;
@@ -189,6 +217,19 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
     const double master = curve_map(cv->c[0], cv->cubic, (double)value);
     if (!__builtin_isfinite(master)) { err |= FLAG_INVALID; return value; }
     const double mapped = curve_map(cv->c[channel + 1], cv->cubic, master);
+    if (!__builtin_isfinite(mapped) || fabs(mapped) > 3.4028234663852886e38) { err |= FLAG_INVALID; return value; }
+    return (float)mapped;
+}
+
+// Master then channel curve with LDS-staged tables (curves[0] master,
+// curves[1 + channel]); same checks and rounding as curves_channel.
+__device__ inline float curves_channel_lds(const LdsCurve* curves, uint32_t cubic, int channel, float value,
+                                           uint32_t& err) {
+    if (curves[0].identity && curves[channel + 1].identity) return value;
+    if (!finite(value)) { err |= FLAG_INVALID; return value; }
+    const double master = curve_map_in(curves[0], cubic, (double)value);
+    if (!__builtin_isfinite(master)) { err |= FLAG_INVALID; return value; }
+    const double mapped = curve_map_in(curves[channel + 1], cubic, master);
     if (!__builtin_isfinite(mapped) || fabs(mapped) > 3.4028234663852886e38) { err |= FLAG_INVALID; return value; }
     return (float)mapped;
 }

@@ -41,6 +41,7 @@ inline int band_rows(int strips, int rows) {
 
 struct GuidedArgs {
     int band;                // rows per band (blockIdx.y)
+    uint32_t fused;          // post ops [0, fused) run in gf_apply, [fused, n) in gf_ops_tail
     const float* src;
     uint32_t src_stride;
     int W, H;                // native raster size
@@ -279,11 +280,46 @@ __global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g) {
             if (value < -3.4028234663852886e38 || value > 3.4028234663852886e38) err |= FLAG_OVERFLOW;
             o[ch] = (float)value;
         }
-        apply_ops(*g.post, o[0], o[1], o[2], err);
+        apply_ops(*g.post, 0, g.fused, o[0], o[1], o[2], err);
         float* d = g.out + ((size_t)(i - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
         d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
     };
     strip_box<6>(g.LW, g.LH, g.cx0, g.cx0 + g.cw, g.r, strip_x, g.vx + g.vw, i0, i1, prefix, totals, src, sink);
+    if (err && g.flags) atomicOr(g.flags, err);
+}
+
+// The post-filter ops from g.fused on (g.fused is a curve op), in place over
+// the filtered output.  The curve op's knot tables are staged in LDS when
+// every curve has at most kTailKnots knots: its lookups are data-dependent,
+// and from LDS they do not wait on divergent global loads.
+constexpr int kTailKnots = 64;
+__global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g) {
+    __shared__ double tables[4 * 5 * kTailKnots];
+    const size_t n = (size_t)g.vw * g.vh;
+    Const<OpList>& L = as_const(*g.post);
+    const uint32_t end = L.n;
+    const DevCurves* cv = L.op[g.fused].curves;
+    bool staged = true;
+    for (int c = 0; c < 4; ++c) staged = staged && (as_const(*cv).c[c].identity || as_const(*cv).c[c].n <= kTailKnots);
+    LdsCurve curves[4];
+    const uint32_t cubic = as_const(*cv).cubic;
+    if (staged) {
+        for (int c = 0; c < 4; ++c) curves[c] = stage_curve(&cv->c[c], tables + c * 5 * kTailKnots);
+        __syncthreads();
+    }
+    uint32_t err = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / g.vw), x = (int)(i - (size_t)y * g.vw);
+        float* d = g.out + ((size_t)y * g.ostride + (size_t)x) * 3;
+        float p[3] = {d[0], d[1], d[2]};
+        if (staged) {
+            for (int c = 0; c < 3; ++c) p[c] = curves_channel_lds(curves, cubic, c, p[c], err);
+            apply_ops(*g.post, g.fused + 1, end, p[0], p[1], p[2], err);
+        } else {
+            apply_ops(*g.post, g.fused, end, p[0], p[1], p[2], err);
+        }
+        d[0] = p[0]; d[1] = p[1]; d[2] = p[2];
+    }
     if (err && g.flags) atomicOr(g.flags, err);
 }
 
@@ -531,8 +567,17 @@ hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t 
     const int strips_a = (g.vw + span - 1) / span;
     g.band = band_rows(strips_a, g.vh);
     dim3 grid_a((unsigned)strips_a, (unsigned)((g.vh + g.band - 1) / g.band));
+    g.fused = L.fused_ops;
     hipLaunchKernelGGL(gf_apply, grid_a, dim3(kStripThreads), 0, stream, g);
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (L.fused_ops < L.post_ops) {
+        const size_t n = (size_t)g.vw * g.vh;
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
+        hipLaunchKernelGGL(gf_ops_tail, dim3(blocks), dim3(256), 0, stream, g);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 }  // namespace rawops

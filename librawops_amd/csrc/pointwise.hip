@@ -74,55 +74,65 @@ __global__ void boost_bwd(const float* x, const float* base, const float* gy, fl
 __global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevCurve* c, uint32_t cubic, float* gx,
                           double* gys) {
     __shared__ double acc[kMaxKnots];
+    __shared__ double tab[5 * kMaxKnots];   // the knot tables, staged: lookups are data-dependent
     for (int k = threadIdx.x; k < kMaxKnots; k += kPW) acc[k] = 0.0;
+    const uint32_t nk = as_const(*c).n;
+    for (uint32_t k = threadIdx.x; k < nk; k += kPW) {
+        tab[k] = c->x[k];
+        tab[nk + k] = c->y[k];
+        tab[2 * nk + k] = c->slope[k];
+        tab[3 * nk + k] = c->c0[k];
+        tab[4 * nk + k] = c->c1[k];
+    }
     __syncthreads();
-    const uint32_t nk = c->n;
+    const LdsCurve l{nk, 0u, tab, tab + nk, tab + 2 * nk, tab + 3 * nk, tab + 4 * nk};
+    const LdsCurve* cl = &l;
     for (size_t e = blockIdx.x * (size_t)kPW + threadIdx.x; e < n; e += (size_t)gridDim.x * kPW) {
         const double v = xv[e], g = gy[e];
         uint32_t lo = 0, hi = nk;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (v < c->x[mid]) hi = mid; else lo = mid + 1;
+            if (v < cl->x[mid]) hi = mid; else lo = mid + 1;
         }
         const uint32_t left = lo == 0 ? 0u : lo - 1u;
         const uint32_t i = left < nk - 2u ? left : nk - 2u;
-        const double h = c->x[i + 1] - c->x[i];
-        if (v == c->x[left]) {            // exact knot: value y_left
+        const double h = cl->x[i + 1] - cl->x[i];
+        if (v == cl->x[left]) {            // exact knot: value y_left
             atomicAdd(&acc[left], g);
-            double slope = c->slope[left < nk - 1 ? left : nk - 2];   // linear: right-hand secant
+            double slope = cl->slope[left < nk - 1 ? left : nk - 2];   // linear: right-hand secant
             if (cubic) {                                                // cubic: the knot tangent
-                slope = left < nk - 1 ? 3.0 * (c->c0[left] - c->y[left]) / (c->x[left + 1] - c->x[left])
-                                      : 3.0 * (c->y[left] - c->c1[left - 1]) / (c->x[left] - c->x[left - 1]);
+                slope = left < nk - 1 ? 3.0 * (cl->c0[left] - cl->y[left]) / (cl->x[left + 1] - cl->x[left])
+                                      : 3.0 * (cl->y[left] - cl->c1[left - 1]) / (cl->x[left] - cl->x[left - 1]);
             }
             gx[e] = (float)(g * slope);
             continue;
         }
-        if (!cubic || v < c->x[0] || v > c->x[nk - 1]) {
+        if (!cubic || v < cl->x[0] || v > cl->x[nk - 1]) {
             // y = y_left + (v - x_left) * s_i,  s_i = (y_i+1 - y_i) / h
-            const double dv = v - c->x[left];
+            const double dv = v - cl->x[left];
             atomicAdd(&acc[left], g);
             atomicAdd(&acc[i + 1], g * dv / h);
             atomicAdd(&acc[i], -g * dv / h);
-            gx[e] = (float)(g * c->slope[i]);
+            gx[e] = (float)(g * cl->slope[i]);
             continue;
         }
-        const double t = (v - c->x[i]) / h, u = 1.0 - t;
+        const double t = (v - cl->x[i]) / h, u = 1.0 - t;
         const double b0 = u * u * u, b1 = 3 * u * u * t, b2 = 3 * u * t * t, b3 = t * t * t;
-        const double P0 = c->y[i], P1 = c->c0[i], P2 = c->c1[i], P3 = c->y[i + 1];
+        const double P0 = cl->y[i], P1 = cl->c0[i], P2 = cl->c1[i], P3 = cl->y[i + 1];
         gx[e] = (float)(g * 3.0 * (u * u * (P1 - P0) + 2 * u * t * (P2 - P1) + t * t * (P3 - P2)) / h);
         atomicAdd(&acc[i], g * (b0 + b1));
         atomicAdd(&acc[i + 1], g * (b2 + b3));
         // Tangent T_j -> secant slopes (minmod picks one of them, or none).
         auto tangent = [&](uint32_t j, double w) {   // w = dL/dT_j
-            if (j == 0) { atomicAdd(&acc[1], w / (c->x[1] - c->x[0])); atomicAdd(&acc[0], -w / (c->x[1] - c->x[0])); return; }
+            if (j == 0) { atomicAdd(&acc[1], w / (cl->x[1] - cl->x[0])); atomicAdd(&acc[0], -w / (cl->x[1] - cl->x[0])); return; }
             if (j == nk - 1) {
-                const double hh = c->x[nk - 1] - c->x[nk - 2];
+                const double hh = cl->x[nk - 1] - cl->x[nk - 2];
                 atomicAdd(&acc[nk - 1], w / hh); atomicAdd(&acc[nk - 2], -w / hh); return;
             }
-            const double a = c->slope[j - 1], b = c->slope[j];
+            const double a = cl->slope[j - 1], b = cl->slope[j];
             if (a == 0 || b == 0 || signbit(a) != signbit(b)) return;
             const uint32_t k = fabs(a) <= fabs(b) ? j - 1 : j;
-            const double hh = c->x[k + 1] - c->x[k];
+            const double hh = cl->x[k + 1] - cl->x[k];
             atomicAdd(&acc[k + 1], w / hh);
             atomicAdd(&acc[k], -w / hh);
         };

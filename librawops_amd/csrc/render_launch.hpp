@@ -51,6 +51,8 @@ struct GuidedLaunch {
     uint32_t ostride;
     uint32_t* flags;
     void* workspace;         // guided_workspace_bytes() of device memory (r > 0)
+    uint32_t fused_ops;      // leading post-filter ops run inside the filter pass; the rest
+    uint32_t post_ops;       // (from the first curve op on, up to post_ops) run in a pass of their own
 };
 
 size_t guided_workspace_bytes(const GuidedLaunch& launch);
