@@ -364,23 +364,41 @@ def main():
     mm = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(compiled_matrix(node), np.float32)])
     sp = ctypes.c_void_p(stream.cuda_stream)
 
+    # The training forward records the classifier decisions (one byte per
+    # non-green site, 12 MB here) so the adjoint reads them instead of
+    # recomputing the FP64 classifier.
+    nbytes = ctypes.c_size_t()
+    C.check(C.lib().rawops_menon_decisions_size(img._handle, ctypes.byref(nbytes)))
+    decisions = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    dptr = ctypes.c_void_p(decisions.data_ptr())
+
+    def forward_record():
+        C.check(C.lib().rawops_render_record(compiled.program.handle, img._handle, area.c(),
+                                             ctypes.c_void_p(out.data_ptr()), area.width,
+                                             ctypes.c_void_p(flags.data_ptr()), dptr, sp))
+
     def backward():
-        C.check(C.lib().rawops_render_backward(img._handle, C.DEMOSAIC_MENON, area.c(), g3, 1.0, mm,
-                                               ctypes.c_void_p(gout.data_ptr()), area.width,
-                                               ctypes.c_void_p(graw.data_ptr()), ctypes.c_void_p(gpar.data_ptr()), sp))
+        C.check(C.lib().rawops_render_backward_recorded(img._handle, area.c(), g3, 1.0, mm,
+                                                        ctypes.c_void_p(gout.data_ptr()), area.width,
+                                                        ctypes.c_void_p(graw.data_ptr()),
+                                                        ctypes.c_void_p(gpar.data_ptr()), dptr, area.c(), sp))
     for _ in range(max(1, args.warmup)):
+        forward_record()
         backward()
     torch.cuda.synchronize()
     bsteps = max(1, args.steps // 2)
+    f0 = [torch.cuda.Event(enable_timing=True) for _ in range(bsteps)]
     b0 = [torch.cuda.Event(enable_timing=True) for _ in range(bsteps)]
     b1 = [torch.cuda.Event(enable_timing=True) for _ in range(bsteps)]
     t1 = time.perf_counter()
     for i in range(bsteps):
-        DeviceRender.run(compiled, area, level, out=out, flags=flags, stream=stream)
+        f0[i].record(stream)
+        forward_record()
         b0[i].record(stream)
         backward()
         b1[i].record(stream)
     torch.cuda.synchronize()
+    fwd_rec_ms = sum(a.elapsed_time(b) for a, b in zip(f0, b0)) / bsteps
     fb_wall = time.perf_counter() - t1
     bwd_ms = sum(a.elapsed_time(b) for a, b in zip(b0, b1)) / bsteps
 
@@ -426,7 +444,8 @@ def main():
     }
     line["fwd_bwd"] = {"value": round(px * bsteps / fb_wall / 1e6, 2), "unit": "Mpix/s",
                        "ms_per_step": round(fb_wall * 1e3 / bsteps, 4), "backward_kernel_ms": round(bwd_ms, 4),
-                       "backward": "menon_backward<64,20>: dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
+                       "forward_record_ms": round(fwd_rec_ms, 4),
+                       "backward": "menon_backward_recorded<64,20> (classifier decisions recorded by the forward) + menon_backward<64,20> on border tiles: dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["fwd_bwd"]["gpu_fwd_bwd_over_cpu_fwd"] = round(line["fwd_bwd"]["value"] / cpu["value"], 1)

@@ -224,6 +224,28 @@ rawops_status rawops_render_backward(const rawops_source* source, uint32_t demos
                                      const float* g_out, uint32_t g_out_stride_pixels, float* g_raw,
                                      double* g_params, void* stream);
 
+/* Menon classifier decisions recorded by the forward for the adjoint (no
+ * reference counterpart).  A decision plane holds one byte per non-green
+ * photosite of the active area, at plane[y * ((active_width + 1) / 2) + x / 2]
+ * for area-relative (x, y); 1 = horizontal.  Its size in bytes: */
+rawops_status rawops_menon_decisions_size(const rawops_source* source, size_t* bytes);
+/* rawops_render of a Menon plan at mip 0 (FP32 output only) that also writes
+ * the decision of every non-green photosite of `viewport` into `decisions`
+ * (device).  Output bits equal rawops_render's. */
+rawops_status rawops_render_record(const rawops_plan* plan, const rawops_source* source, rawops_rect viewport,
+                                   float* dst_rgb, uint32_t dst_stride_pixels, uint32_t* device_flags,
+                                   uint8_t* decisions, void* stream);
+/* rawops_render_backward (Menon) with the decisions a rawops_render_record of
+ * the viewport `recorded` wrote: tiles whose adjoint reach lies inside
+ * `recorded` and away from the area edge read them instead of recomputing the
+ * classifier (the rest recompute as rawops_render_backward does).  Same
+ * outputs as rawops_render_backward. */
+rawops_status rawops_render_backward_recorded(const rawops_source* source, rawops_rect viewport,
+                                              const float gains[3], float multiplier, const float matrix[9],
+                                              const float* g_out, uint32_t g_out_stride_pixels, float* g_raw,
+                                              double* g_params, const uint8_t* decisions, rawops_rect recorded,
+                                              void* stream);
+
 /* Reverse mode of the working-Y guided filter (radius 1..32, level 0) of a
  * raster source: given dL/dq over `viewport` (device float32 RGB, row stride
  * g_out_stride_pixels), writes dL/dp for every raster pixel into g_in (device

@@ -137,6 +137,9 @@ def lib():
             "rawops_batch_shape_count": ([], u32),
             "rawops_batch_destroy": ([P], None),
             "rawops_render_backward": ([P, u32, Rect, P, ctypes.c_float, P, P, u32, P, P, P], S),
+            "rawops_menon_decisions_size": ([P, P], S),
+            "rawops_render_record": ([P, P, Rect, P, u32, P, P, P], S),
+            "rawops_render_backward_recorded": ([P, Rect, P, ctypes.c_float, P, P, u32, P, P, P, Rect, P], S),
             "rawops_guided_backward_workspace_size": ([u32, u32, P], S),
             "rawops_guided_backward": ([P, u32, ctypes.c_double, Rect, P, u32, P, P, P, ctypes.c_size_t, P], S),
             "rawops_srgb_encode": ([P, P, ctypes.c_size_t, P], S),

@@ -90,8 +90,18 @@ class RenderChainFn(torch.autograd.Function):
         M = [float(v) for v in matrix.detach().float().reshape(-1).cpu().tolist()]
         plan = Program.from_ops(demosaic, False, [(C.OP_GAINS, g), (C.OP_SCALE, (m,)), (C.OP_MATRIX, M)], [])
         out = torch.empty((viewport.height, viewport.width, 3), dtype=torch.float32, device="cuda")
-        C.check(C.lib().rawops_render(plan.handle, image._handle, viewport.c(), 0, C.QUALITY_FINAL, _ptr(out), None,
-                                      viewport.width, None, _stream()))
+        a = image.metadata.active_area
+        ctx.decisions = None
+        if demosaic == C.DEMOSAIC_MENON and a.width > 1 and a.height > 1:
+            # record the classifier decisions: the adjoint reads them instead of recomputing
+            n = ctypes.c_size_t()
+            C.check(C.lib().rawops_menon_decisions_size(image._handle, ctypes.byref(n)))
+            ctx.decisions = torch.empty(n.value, dtype=torch.uint8, device="cuda")
+            C.check(C.lib().rawops_render_record(plan.handle, image._handle, viewport.c(), _ptr(out), viewport.width,
+                                                 None, _ptr(ctx.decisions), _stream()))
+        else:
+            C.check(C.lib().rawops_render(plan.handle, image._handle, viewport.c(), 0, C.QUALITY_FINAL, _ptr(out),
+                                          None, viewport.width, None, _stream()))
         ctx.image, ctx.demosaic, ctx.viewport = image, demosaic, viewport
         ctx.g, ctx.m, ctx.M = g, m, M
         return out
@@ -102,8 +112,13 @@ class RenderChainFn(torch.autograd.Function):
         gp = torch.zeros(13, dtype=torch.float64, device="cuda")
         g = (ctypes.c_float * 3)(*ctx.g)
         M = (ctypes.c_float * 9)(*ctx.M)
-        C.check(C.lib().rawops_render_backward(ctx.image._handle, ctx.demosaic, ctx.viewport.c(), g, ctx.m, M,
-                                               _ptr(gout), ctx.viewport.width, None, _ptr(gp), _stream()))
+        if ctx.decisions is not None:
+            C.check(C.lib().rawops_render_backward_recorded(ctx.image._handle, ctx.viewport.c(), g, ctx.m, M,
+                                                            _ptr(gout), ctx.viewport.width, None, _ptr(gp),
+                                                            _ptr(ctx.decisions), ctx.viewport.c(), _stream()))
+        else:
+            C.check(C.lib().rawops_render_backward(ctx.image._handle, ctx.demosaic, ctx.viewport.c(), g, ctx.m, M,
+                                                   _ptr(gout), ctx.viewport.width, None, _ptr(gp), _stream()))
         g_gains = gp[0:3].float()
         g_stops = (gp[3] * ctx.m * _LN2).float()
         g_matrix = gp[4:13].float().reshape(3, 3)
@@ -119,9 +134,29 @@ def render_chain(image: RawImage, gains, stops, matrix, demosaic="menon", viewpo
     return RenderChainFn.apply(gains, stops, matrix, image, d, vp)
 
 
+def render_record(image: RawImage, gains, stops, matrix, viewport: Rect = None):
+    """Menon forward of the render chain (FP32 out) that also records the
+    classifier decisions: (out (h, w, 3), decision plane (uint8, one byte per
+    non-green site of the active area, 1 = horizontal))."""
+    vp = viewport or image.metadata.active_area
+    plan = Program.from_ops(C.DEMOSAIC_MENON, False, [(C.OP_GAINS, [float(v) for v in gains]),
+                                                      (C.OP_SCALE, (_multiplier(stops),)),
+                                                      (C.OP_MATRIX, [float(v) for v in np.asarray(matrix, np.float32).reshape(-1)])], [])
+    out = torch.empty((vp.height, vp.width, 3), dtype=torch.float32, device="cuda")
+    n = ctypes.c_size_t()
+    C.check(C.lib().rawops_menon_decisions_size(image._handle, ctypes.byref(n)))
+    dec = torch.zeros(n.value, dtype=torch.uint8, device="cuda")
+    C.check(C.lib().rawops_render_record(plan.handle, image._handle, vp.c(), _ptr(out), vp.width, None, _ptr(dec),
+                                         _stream()))
+    torch.cuda.current_stream().synchronize()
+    return out, dec
+
+
 def render_backward(image: RawImage, gout, gains, stops, matrix, demosaic="menon", viewport: Rect = None,
-                    want_raw=True):
-    """Explicit adjoint: (dL/d normalised raw plane (ah, aw) or None, dL/d params[13])."""
+                    want_raw=True, decisions=None, recorded: Rect = None):
+    """Explicit adjoint: (dL/d normalised raw plane (ah, aw) or None, dL/d params[13]).
+    With ``decisions`` (Menon, from ``render_record`` of the viewport ``recorded``)
+    the classifier is read instead of recomputed where the record covers it."""
     d = C.DEMOSAIC_MENON if demosaic in ("menon", C.DEMOSAIC_MENON) else C.DEMOSAIC_BILINEAR
     vp = viewport or image.metadata.active_area
     a = image.metadata.active_area
@@ -129,8 +164,15 @@ def render_backward(image: RawImage, gout, gains, stops, matrix, demosaic="menon
     gp = torch.zeros(13, dtype=torch.float64, device="cuda")
     g = (ctypes.c_float * 3)(*[float(v) for v in gains])
     M = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(matrix, np.float32).reshape(-1)])
-    C.check(C.lib().rawops_render_backward(image._handle, d, vp.c(), g, _multiplier(stops), M,
-                                           _ptr(gout.contiguous()), vp.width, _ptr(graw), _ptr(gp), _stream()))
+    if decisions is not None:
+        if d != C.DEMOSAIC_MENON:
+            raise ValueError("recorded decisions belong to Menon renders")
+        C.check(C.lib().rawops_render_backward_recorded(image._handle, vp.c(), g, _multiplier(stops), M,
+                                                        _ptr(gout.contiguous()), vp.width, _ptr(graw), _ptr(gp),
+                                                        _ptr(decisions), (recorded or vp).c(), _stream()))
+    else:
+        C.check(C.lib().rawops_render_backward(image._handle, d, vp.c(), g, _multiplier(stops), M,
+                                               _ptr(gout.contiguous()), vp.width, _ptr(graw), _ptr(gp), _stream()))
     return graw, gp
 
 

@@ -40,6 +40,8 @@ struct BackwardArgs {
     int tiles_x, tiles_y;      // tiles of the launch
     int tile0_x, tile0_y;      // first tile (in tiles from the area origin)
     int r0, r1, c0, c1;        // launch-relative tile rows / columns of the interior block
+    const uint8_t* decisions;  // recorded Menon decisions (area plane, dpitch bytes a row) or null
+    uint32_t dpitch;
 };
 
 // dL/dout at a sensor pixel (zero outside the viewport).
@@ -579,6 +581,259 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6))) m
     }
 }
 
+// ------------------------------------------- Menon, recorded decisions --
+//
+// Interior tiles whose adjoint reach (tile +-4) has decisions recorded by
+// the forward (rawops_render_record): the classifier is not recomputed.  The
+// forward values the adjoint needs besides the decisions -- the selected
+// green at non-green sites of tile +-2 and red/blue at green sites of tile
+// +-1, for the demosaic output of the tile's pixels (parameter gradients) --
+// come from the sample codes of tile +-4, with the forward's FP64 operation
+// order.  The gather passes are those of menon_back_interior.
+template <int TW, int TH>
+struct RecLds {
+    static constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
+    static constexpr int GW = (TW + 4) / 2, GH = TH + 4;                   // tile +-2, compacted
+    static constexpr int OFF_RAW = 0;                                       // u16 codes, EW x EH
+    static constexpr int OFF_HZ = (EW * EH * 2 + 15) / 16 * 16;             // decisions, NC
+    static constexpr int OFF_GREEN = (OFF_HZ + NC + 15) / 16 * 16;          // f64 GW x GH
+    static constexpr int OFF_RED = OFF_GREEN + GW * GH * 8;
+    static constexpr int OFF_BLUE = OFF_RED + GW * GH * 8;
+    static constexpr int OFF_PO = OFF_BLUE + GW * GH * 8;                   // f32 POh, POv
+    static constexpr int BYTES = OFF_PO + 2 * NC * 4;
+    // After pass 1: AR/AB over green + red; after pass 2: AGh/AGv over PO;
+    // the parameter reduction over the codes.
+    static_assert(2 * NC * 4 <= 2 * GW * GH * 8, "AR/AB do not fit over green/red");
+    static_assert((kNT / 64) * 9 * 8 <= OFF_HZ, "the parameter reduction does not fit over the codes");
+};
+
+template <int TW, int TH, int N0>
+__device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, const SensorArgs& s, char* lds,
+                                                         int tx, int ty) {
+    using R = RecLds<TW, TH>;
+    constexpr int EW = R::EW, EH = R::EH, HWE = R::HWE, NC = R::NC, GW = R::GW;
+    constexpr int NI = HWE * (EH / 2), J = (NI + kNT - 1) / kNT;   // row-pair items over the extended tile
+    constexpr int NG = GW * ((EH - 4) / 2);                         // row-pair items over tile +-2
+    const int ex = tx - 4, ey = ty - 4;
+    uint16_t* raw = reinterpret_cast<uint16_t*>(lds + R::OFF_RAW);
+    uint8_t* hz = reinterpret_cast<uint8_t*>(lds + R::OFF_HZ);
+    double* green = reinterpret_cast<double*>(lds + R::OFF_GREEN);
+    double* red = reinterpret_cast<double*>(lds + R::OFF_RED);
+    double* blue = reinterpret_cast<double*>(lds + R::OFF_BLUE);
+    auto nx = [](int b) { return (b & 1) ? 1 - N0 : N0; };          // x parity of non-green sites, row parity b
+    SiteLevels lv;
+    lv.init(s, ex, ey);
+    uint32_t red_rows = 0;
+    for (int b = 0; b < 2; ++b) red_rows |= (color_of(s, ex + nx(b), ey + b) == 0 ? 1u : 0u) << b;
+    auto Rn = [&](int u, int v) -> double {    // parity of (u, v) relative to the extended tile
+        return (double)lv.norm(raw[v * EW + u], v & 1, u & 1);
+    };
+    auto g2 = [](int u, int v) { return (v - 2) * GW + ((u - 2) >> 1); };   // tile +-2 compaction
+
+    // codes of the extended tile, its recorded decisions, upstream gradients
+    {
+        const int64_t stride = s.stride;
+        const uint16_t* base = s.samples + ((int64_t)ey * stride + ex);
+        const bool pairs = ((reinterpret_cast<uintptr_t>(base) | (uintptr_t)(stride * 2)) & 3) == 0;
+        for (int i = threadIdx.x; i < EW * EH / 2; i += kNT) {
+            const int v = i / (EW / 2), k = i - v * (EW / 2);
+            const uint16_t* p = base + (int64_t)v * stride + 2 * k;
+            if (pairs) {
+                *reinterpret_cast<uint32_t*>(raw + v * EW + 2 * k) = *reinterpret_cast<const uint32_t*>(p);
+            } else {
+                raw[v * EW + 2 * k] = p[0];
+                raw[v * EW + 2 * k + 1] = p[1];
+            }
+        }
+        for (int i = threadIdx.x; i < NC; i += kNT) {
+            const int v = i / HWE, u = 2 * (i - v * HWE) + nx(v & 1);
+            hz[i] = a.decisions[(size_t)(ey + v - s.ay) * a.dpitch + (size_t)((ex + u - s.ax) >> 1)];
+        }
+    }
+    float goN[J][2][3], goG[J][2][3];
+    uint32_t hasN = 0, hasG = 0;    // bit 2j + b
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it < NI) {
+            const int qr = it / HWE, k = it - qr * HWE;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int v = 2 * qr + b, un = 2 * k + nx(b);
+                if (load_gout(a, ex + un, ey + v, goN[j][b])) hasN |= 1u << (2 * j + b);
+                if (load_gout(a, ex + (un ^ 1), ey + v, goG[j][b])) hasG |= 1u << (2 * j + b);
+            }
+        }
+    }
+    __syncthreads();
+    // selected green at non-green sites of tile +-2 (MenonDemosaic.cpp:107-119, 141-147)
+    for (int it = threadIdx.x; it < NG; it += kNT) {
+        const int qr = it / GW, k = it - qr * GW;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 + 2 * qr + b, u = 2 + 2 * k + nx(b);
+            const double own = Rn(u, v);
+            double gsel;
+            if (hz[v * HWE + (u >> 1)])
+                gsel = 0.5 * (Rn(u - 1, v) + Rn(u + 1, v)) + 0.5 * own - 0.25 * (Rn(u - 2, v) + Rn(u + 2, v));
+            else
+                gsel = 0.5 * (Rn(u, v - 1) + Rn(u, v + 1)) + 0.5 * own - 0.25 * (Rn(u, v - 2) + Rn(u, v + 2));
+            green[g2(u, v)] = gsel;
+        }
+    }
+    __syncthreads();
+    // red and blue at green sites of tile +-1 (MenonDemosaic.cpp:148-161)
+    for (int it = threadIdx.x; it < NG; it += kNT) {
+        const int qr = it / GW, k = it - qr * GW;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 + 2 * qr + b, u = 2 + 2 * k + (nx(b) ^ 1);
+            if (u < 3 || u >= EW - 3 || v < 3 || v >= EH - 3) continue;
+            const double own = Rn(u, v);
+            const double ch = own + 0.5 * (Rn(u - 1, v) + Rn(u + 1, v)) - 0.5 * (green[g2(u - 1, v)] + green[g2(u + 1, v)]);
+            const double cv = own + 0.5 * (Rn(u, v - 1) + Rn(u, v + 1)) - 0.5 * (green[g2(u, v - 1)] + green[g2(u, v + 1)]);
+            const bool red_row = (red_rows >> b) & 1u;
+            red[g2(u, v)] = red_row ? ch : cv;
+            blue[g2(u, v)] = red_row ? cv : ch;
+        }
+    }
+    __syncthreads();
+
+    float* POh = reinterpret_cast<float*>(lds + R::OFF_PO);
+    float* POv = POh + NC;
+    float* AR = reinterpret_cast<float*>(lds + R::OFF_GREEN);
+    float* AB = AR + NC;
+    float* AGh = POh;
+    float* AGv = POv;
+    const bool red0 = (red_rows & 1u) != 0, red1 = (red_rows & 2u) != 0;
+    auto red_row = [&](int b) { return b ? red1 : red0; };
+    auto tile_item = [](int qr, int k) { return qr >= 2 && qr < 2 + TH / 2 && k >= 2 && k < 2 + TW / 2; };
+    float pg[J][2], rn[J][2], gg[J][2][3], rpg[J][2], rpn[J][2];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+    // pass 1
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+        const bool mine = tile_item(qr, k);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
+            const bool rs = red_row(b);
+            const bool hn = (hasN >> (2 * j + b)) & 1u, hg = (hasG >> (2 * j + b)) & 1u;
+            float gd[3], uu[3];
+            if (hn) chain_back(a, goN[j][b], uu, gd);
+            else gd[0] = gd[1] = gd[2] = 0.f;
+            const bool h = hz[c] != 0;
+            const float po = rs ? gd[2] : gd[0];
+            POh[c] = h ? po : 0.f;
+            POv[c] = h ? 0.f : po;
+            pg[j][b] = gd[1];
+            rn[j][b] = gd[0] + gd[2];           // the own sample feeds both r and b
+            if (hg) chain_back(a, goG[j][b], uu, gg[j][b]);
+            else gg[j][b][0] = gg[j][b][1] = gg[j][b][2] = 0.f;
+            if (mine) {
+                if (hn) {
+                    // final chroma (MenonDemosaic.cpp:162-173), as the forward takes it
+                    const int ia = g2(un - (h ? 1 : 0), v - (h ? 0 : 1)), ic = g2(un + (h ? 1 : 0), v + (h ? 0 : 1));
+                    const double mr = 0.5 * (red[ia] + red[ic]);
+                    const double mb = 0.5 * (blue[ia] + blue[ic]);
+                    const float self = lv.norm(raw[v * EW + un], b, nb);
+                    const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
+                    const float d[3] = {rs ? self : other, (float)green[g2(un, v)], rs ? other : self};
+                    param_grads(d, goN[j][b], acc);
+                }
+                if (hg) {
+                    const int g = g2(ug, v);
+                    const float d[3] = {(float)red[g], lv.norm(raw[v * EW + ug], b, nb ^ 1), (float)blue[g]};
+                    param_grads(d, goG[j][b], acc);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // pass 2: green sites with all four neighbours inside the extended tile
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = nx(b), ug = 2 * k + 1 - nb, c = v * HWE + k;
+            if (ug < 1 || ug >= EW - 1 || v < 1 || v >= EH - 1) continue;
+            const float sh = 0.5f * (POh[c - nb] + POh[c + 1 - nb]);
+            const float sv = 0.5f * (POv[c - HWE] + POv[c + HWE]);
+            const float th = red_row(b) ? sh : -sh, tv = red_row(b ^ 1) ? sv : -sv;
+            const float ar = gg[j][b][0] - th - tv;
+            const float ab = gg[j][b][2] + th + tv;
+            AR[c] = ar;
+            AB[c] = ab;
+            rpg[j][b] = ar + ab + gg[j][b][1];
+        }
+    }
+    __syncthreads();
+    // pass 3: non-green sites of tile +-2
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = nx(b), un = 2 * k + nb, c = v * HWE + k;
+            if (un < 2 || un >= EW - 2 || v < 2 || v >= EH - 2) continue;
+            const float* A = red_row(b) ? AR : AB;
+            const float S = (A[c - 1 + nb] + A[c + nb]) + (A[c - HWE] + A[c + HWE]);
+            const float ag = pg[j][b] - 0.5f * S;
+            const bool h = hz[c] != 0;
+            AGh[c] = h ? ag : 0.f;
+            AGv[c] = h ? 0.f : ag;
+            rpn[j][b] = rn[j][b] + 0.5f * S;
+        }
+    }
+    __syncthreads();
+    // pass 4: raw adjoint of the tile through the directional-green stencil
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int it = (int)threadIdx.x + j * kNT;
+        if (it >= NI) break;
+        const int qr = it / HWE, k = it - qr * HWE;
+        if (!tile_item(qr, k)) continue;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, nb = nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
+            const float rgv = rpg[j][b] + 0.5f * (AGh[c - nb] + AGh[c + 1 - nb]) + 0.5f * (AGv[c - HWE] + AGv[c + HWE]);
+            const float rnv = rpn[j][b] + 0.5f * (AGh[c] + AGv[c]) - 0.25f * (AGh[c - 1] + AGh[c + 1]) -
+                              0.25f * (AGv[c - 2 * HWE] + AGv[c + 2 * HWE]);
+            if (a.graw) {
+                float* row = a.graw + (size_t)(ty + v - 4 - s.ay) * s.aw + (size_t)(tx - 4 - s.ax);
+                row[un] = rnv;
+                row[ug] = rgv;
+            }
+        }
+    }
+    flush_params(a, acc, reinterpret_cast<double*>(lds + R::OFF_RAW));
+}
+
+// One workgroup per tile of the recorded block [r0, r1) x [c0, c1).
+template <int TW, int TH>
+__global__ void __launch_bounds__(kNT) menon_backward_recorded(const BackwardArgs args) {
+    __shared__ __attribute__((aligned(16))) char lds[RecLds<TW, TH>::BYTES];
+    (void)args;
+    const BackwardArgs& a = *(const BackwardArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    const int cols = a.c1 - a.c0;
+    const int r = a.r0 + (int)blockIdx.x / cols, c = a.c0 + (int)blockIdx.x % cols;
+    const int tx = __builtin_amdgcn_readfirstlane(a.s.ax + (a.tile0_x + c) * TW);
+    const int ty = __builtin_amdgcn_readfirstlane(a.s.ay + (a.tile0_y + r) * TH);
+    const SensorArgs& s = a.s;
+    const uint32_t n0 = ((s.ng_parity >> ((uint32_t)ty & 1u)) & 1u) ^ ((uint32_t)tx & 1u);
+    if (n0) menon_back_recorded_tile<TW, TH, 1>(a, s, lds, tx, ty);
+    else menon_back_recorded_tile<TW, TH, 0>(a, s, lds, tx, ty);
+}
+
 // ------------------------------------------------------------- bilinear --
 
 template <int TW, int TH>
@@ -682,6 +937,34 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     };
     interior_range(a.tile0_y, tiles_y, TH, a.s.ay, a.s.ah, a.r0, a.r1);
     interior_range(a.tile0_x, a.tiles_x, TW, a.s.ax, a.s.aw, a.c0, a.c1);
+    if (menon && L.decisions) {
+        // Tiles of the interior block whose reach (tile +-4) lies inside the
+        // recorded viewport read the recorded decisions; the rest recompute.
+        auto covered = [](int first, int T, int lo, int rec_lo, int rec_len, int& i0, int& i1) {
+            int n0 = i1, n1 = i0;
+            for (int i = i0; i < i1; ++i) {
+                const int t0 = lo + (first + i) * T;
+                if (t0 - 4 >= rec_lo && t0 + T + 4 <= rec_lo + rec_len) { n0 = std::min(n0, i); n1 = std::max(n1, i + 1); }
+            }
+            if (n0 >= n1) n0 = n1 = i0;
+            i0 = n0; i1 = n1;
+        };
+        covered(a.tile0_y, TH, a.s.ay, (int)L.recorded.y, (int)L.recorded.height, a.r0, a.r1);
+        covered(a.tile0_x, TW, a.s.ax, (int)L.recorded.x, (int)L.recorded.width, a.c0, a.c1);
+        if (a.r0 == a.r1 || a.c0 == a.c1) a.r0 = a.r1 = a.c0 = a.c1 = 0;
+        a.decisions = L.decisions;
+        a.dpitch = decision_pitch((uint32_t)a.s.aw);
+        const unsigned rec = (unsigned)((a.r1 - a.r0) * (a.c1 - a.c0));
+        if (rec) {
+            hipLaunchKernelGGL((menon_backward_recorded<64, 20>), dim3(rec), dim3(kNT), 0, stream, a);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        // the other tiles come first in the border-first order: launch only those
+        if (blocks - rec)
+            hipLaunchKernelGGL((menon_backward<64, 20>), dim3(blocks - rec), dim3(kNT), 0, stream, a);
+        return hipGetLastError();
+    }
     if (a.r0 == a.r1 || a.c0 == a.c1) a.r0 = a.r1 = a.c0 = a.c1 = 0;
     if (menon)
         hipLaunchKernelGGL((menon_backward<64, 20>), dim3(blocks), dim3(kNT), 0, stream, a);

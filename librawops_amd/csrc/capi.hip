@@ -442,6 +442,38 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* src, r
     });
 }
 
+rawops_status rawops_menon_decisions_size(const rawops_source* src, size_t* bytes) {
+    return guarded([&] {
+        if (!src || !src->is_raw || !bytes) fail(RAWOPS_INVALID_ARGUMENT, "decisions need a RAW source");
+        *bytes = (size_t)decision_pitch(src->meta.active_area.width) * src->meta.active_area.height;
+    });
+}
+
+rawops_status rawops_render_record(const rawops_plan* plan, const rawops_source* src, rawops_rect viewport,
+                                   float* dst, uint32_t dst_stride, uint32_t* flags, uint8_t* decisions,
+                                   void* stream) {
+    return guarded([&] {
+        validate_render(plan, src, viewport, 0, RAWOPS_QUALITY_FINAL);
+        if (plan->demosaic != RAWOPS_DEMOSAIC_MENON) fail(RAWOPS_INVALID_ARGUMENT, "decisions are recorded by Menon renders");
+        if (!decisions) fail(RAWOPS_INVALID_ARGUMENT, "decision plane is required");
+        if (!dst_stride) dst_stride = viewport.width;
+        if (dst_stride < viewport.width) fail(RAWOPS_INVALID_ARGUMENT, "output stride is smaller than the viewport");
+        if (!viewport.width || !viewport.height) return;
+        if (src->meta.active_area.width <= 1 || src->meta.active_area.height <= 1)
+            fail(RAWOPS_INVALID_ARGUMENT, "a singleton active-area axis renders bilinear (no decisions)");
+        RenderLaunch L{};
+        L.sensor = sensor_args(*src);
+        L.demosaic = RAWOPS_DEMOSAIC_MENON;
+        L.mip = 0;
+        L.viewport = viewport;
+        L.out = dst;
+        L.ostride = dst_stride;
+        L.flags = flags;
+        L.decisions = decisions;
+        hip_check(launch_render(L, plan->device, plan->device + 1, static_cast<hipStream_t>(stream)), "render launch");
+    });
+}
+
 rawops_status rawops_check_flags(uint32_t* flags, void* stream) {
     return guarded([&] {
         if (!flags) return;
@@ -672,6 +704,35 @@ rawops_status rawops_render_backward(const rawops_source* src, uint32_t demosaic
         L.g_stride = g_stride;
         L.g_raw = g_raw;
         L.g_params = g_params;
+        hip_check(launch_render_backward(L, static_cast<hipStream_t>(stream)), "backward launch");
+    });
+}
+
+rawops_status rawops_render_backward_recorded(const rawops_source* src, rawops_rect viewport, const float gains[3],
+                                              float multiplier, const float matrix[9], const float* g_out,
+                                              uint32_t g_stride, float* g_raw, double* g_params,
+                                              const uint8_t* decisions, rawops_rect recorded, void* stream) {
+    return guarded([&] {
+        if (!src || !src->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "backward needs a RAW source");
+        if (!gains || !matrix || !g_out || !g_params || !decisions) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        inside_or_throw(src->meta.active_area, viewport);
+        inside_or_throw(src->meta.active_area, recorded);
+        if (src->meta.active_area.width <= 1 || src->meta.active_area.height <= 1)
+            fail(RAWOPS_INVALID_ARGUMENT, "a singleton active-area axis renders bilinear (no decisions)");
+        if (!g_stride) g_stride = viewport.width;
+        BackwardLaunch L{};
+        L.sensor = sensor_args(*src);
+        L.demosaic = RAWOPS_DEMOSAIC_MENON;
+        L.viewport = viewport;
+        for (int i = 0; i < 3; ++i) L.gains[i] = gains[i];
+        L.multiplier = multiplier;
+        for (int i = 0; i < 9; ++i) L.matrix[i] = matrix[i];
+        L.g_out = g_out;
+        L.g_stride = g_stride;
+        L.g_raw = g_raw;
+        L.g_params = g_params;
+        L.decisions = decisions;
+        L.recorded = recorded;
         hip_check(launch_render_backward(L, static_cast<hipStream_t>(stream)), "backward launch");
     });
 }

@@ -51,6 +51,36 @@ struct MenonFwdLds {
     static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles must cover whole Bayer cells");
 };
 
+// Per-site normalisation levels of a tile by (row, column) parity relative to
+// its origin (whole-sample reflection keeps parity).  They are uniform over
+// the workgroup; readfirstlane keeps them opaque scalars, so a run-time parity
+// select cannot be folded into an indexed read of the kernel arguments
+// (which would copy them to scratch memory).
+struct SiteLevels {
+    float blk[2][2], rcp[2][2], den[2][2];
+
+    __device__ __forceinline__ void init(const SensorArgs& s, int tx, int ty) {
+        auto uni = [](float v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); };
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const uint32_t site = site_of(s, tx + e, ty + b);
+                blk[b][e] = uni(site_value(s.black, site));
+                rcp[b][e] = uni(site_value(s.rcp, site));
+                den[b][e] = uni(site_value(s.denom, site));
+            }
+    }
+    // Selects rather than an indexed load (see above).
+    __device__ __forceinline__ static float lvl(const float (&v)[2][2], int py, int px) {
+        return py ? (px ? v[1][1] : v[1][0]) : (px ? v[0][1] : v[0][0]);
+    }
+    // Normalised sample (RawEngine.cpp:89-97) of a code at row / column parity (py, px).
+    __device__ __forceinline__ float norm(uint32_t code, int py, int px) const {
+        return normalize_exact((float)code, lvl(blk, py, px), lvl(rcp, py, px), lvl(den, py, px));
+    }
+};
+
 template <int TW, int TH, int NT, int N0, bool BORDER>
 struct MenonFwd {
     using L = MenonFwdLds<TW, TH>;
@@ -62,7 +92,7 @@ struct MenonFwd {
     uint16_t* raw;
     double *gh, *gv, *dh, *dv, *red, *blue;
     uint8_t* hz;
-    float blk[2][2], rcp[2][2], den[2][2];   // levels by (row, column) parity relative to the tile origin
+    SiteLevels lv;          // levels by (row, column) parity relative to the tile origin
 
     __device__ __forceinline__ MenonFwd(const SensorArgs& s_, char* lds, int tx_, int ty_) : s(s_), tx(tx_), ty(ty_) {
         raw = reinterpret_cast<uint16_t*>(lds + L::OFF_RAW);
@@ -75,33 +105,12 @@ struct MenonFwd {
         hz = reinterpret_cast<uint8_t*>(lds + L::OFF_HZ);
         red_rows = 0;
         for (int b = 0; b < 2; ++b) red_rows |= (color_of(s, tx + nx(b), ty + b) == 0 ? 1u : 0u) << b;
-        // The levels are uniform over the workgroup; readfirstlane keeps them
-        // opaque scalars, so a run-time parity select later cannot be folded
-        // into an indexed read of the kernel arguments (which would copy them
-        // to scratch memory).
-        auto uni = [](float v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); };
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const uint32_t site = site_of(s, tx + e, ty + b);
-                blk[b][e] = uni(site_value(s.black, site));
-                rcp[b][e] = uni(site_value(s.rcp, site));
-                den[b][e] = uni(site_value(s.denom, site));
-            }
+        lv.init(s, tx, ty);
     }
 
-    // Normalised sample (RawEngine.cpp:89-97) at LDS index i of a position
-    // with row parity py and column parity px (relative to the tile origin;
-    // whole-sample reflection keeps parity).
-    __device__ __forceinline__ float Rn(int i, int py, int px) const {
-        return normalize_exact((float)raw[i], lvl(blk, py, px), lvl(rcp, py, px), lvl(den, py, px));
-    }
-    // Selects rather than an indexed load, so a run-time parity does not put
-    // the tile state in scratch memory.
-    __device__ __forceinline__ static float lvl(const float (&v)[2][2], int py, int px) {
-        return py ? (px ? v[1][1] : v[1][0]) : (px ? v[0][1] : v[0][0]);
-    }
+    // Normalised sample at LDS index i of a position with row parity py and
+    // column parity px (relative to the tile origin).
+    __device__ __forceinline__ float Rn(int i, int py, int px) const { return lv.norm(raw[i], py, px); }
 
     // x parity (tile-relative) of the non-green sites in rows of parity b.
     __device__ static constexpr int nx(int b) { return (b & 1) ? 1 - N0 : N0; }

@@ -18,9 +18,14 @@ struct RenderLaunch {
     uint8_t* out8;
     uint32_t ostride;
     uint32_t* flags;
+    uint8_t* decisions;      // Menon mip 0: record classifier decisions (may be null)
 };
 
 // native / reduced / post point at device-resident OpLists (owned by a plan).
+
+// Bytes per row of a Menon decision plane: one byte per two columns of the
+// active area (the non-green site of each column pair).
+inline uint32_t decision_pitch(uint32_t active_width) { return (active_width + 1) / 2; }
 hipError_t launch_render(const RenderLaunch& launch, const OpList* native, const OpList* reduced,
                          hipStream_t stream);
 
@@ -34,6 +39,8 @@ struct BackwardLaunch {
     uint32_t g_stride;       // pixels per g_out row
     float* g_raw;            // dL/d(normalised raw), active-area plane (written in full)
     double* g_params;        // 13 accumulated: gains[3], multiplier, matrix[9]
+    const uint8_t* decisions;  // Menon decisions recorded by a forward over `recorded` (may be null)
+    rawops_rect recorded;
 };
 
 hipError_t launch_render_backward(const BackwardLaunch& launch, hipStream_t stream);

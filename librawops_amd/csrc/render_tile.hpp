@@ -24,6 +24,9 @@ struct RenderArgs {
     uint32_t ostride;        // output row stride in pixels
     uint32_t* flags;         // may be null
     uint32_t tap;            // reduced ops applied before the FP32 output; the rest precede the u8 output
+    uint8_t* decisions;      // Menon mip 0: classifier decision per non-green site of the viewport
+                             // (area-relative, index y dpitch + x / 2, 1 = horizontal); may be null
+    uint32_t dpitch;
 };
 
 // Tile-relative x parity of the non-green sites in even tile rows for a
@@ -95,6 +98,16 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         __syncthreads();
         t.classify();
         __syncthreads();
+        if (a.decisions && a.mip == 0) {
+            // record the tile's decisions (inside the viewport) for the adjoint
+            for (int i = threadIdx.x; i < (TW / 2) * TH; i += NT) {
+                const int ry = i / (TW / 2), k = i - ry * (TW / 2);
+                const int rx = 2 * k + MenonFwd<TW, TH, NT, N0, BORDER>::nx(ry & 1);
+                const int sx = tx + rx, sy = ty + ry;
+                if (sx < a.vx + a.vw && sy < a.vy + a.vh && in_area(s, sx, sy))
+                    a.decisions[(size_t)(sy - s.ay) * a.dpitch + (size_t)((sx - s.ax) >> 1)] = t.hz[i];
+            }
+        }
         t.chroma();
         __syncthreads();
         t.finish(px);

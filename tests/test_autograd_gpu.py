@@ -151,6 +151,50 @@ def test_menon_raw_adjoint_large_frame(vp_rel):
     assert rel(graw.cpu().numpy(), ref) <= 1e-4
 
 
+@pytest.mark.parametrize("vp_rel", [None, (37, 29, 201, 87)])
+def test_recorded_decisions(vp_rel):
+    """rawops_render_record: output bits equal rawops_render's and the decision
+    plane equals the FP64 oracle's classifier decisions at every non-green site
+    of the viewport.  The adjoint that reads the record (no classifier
+    recompute on covered interior tiles) equals the recomputing adjoint: the
+    same raw-gradient bits (same gather order) and parameter sums to 1e-12."""
+    w, h, stride, area = 310, 160, 320, (3, 2, 300, 150)
+    m = meta_dict(w, h, stride, 2, 0, 1, area, (500, 510, 505, 515), (9000,) * 4)
+    s = random_samples(h, stride, seed=11, lo=300, hi=9000)
+    img = gpu_image(m, s)
+    om = oracle_meta(m)
+    vp = area if vp_rel is None else (area[0] + vp_rel[0], area[1] + vp_rel[1], vp_rel[2], vp_rel[3])
+    gains, stops = [1.7, 1.0, 1.3], 0.4
+    out_rec, dec = A.render_record(img, gains, stops, CAMW, L.Rect(*vp))
+    plan = A.Program.from_ops(A.C.DEMOSAIC_MENON, False, [(A.C.OP_GAINS, gains), (A.C.OP_SCALE, (A._multiplier(stops),)),
+                                                          (A.C.OP_MATRIX, [float(v) for v in CAMW.reshape(-1)])], [])
+    out = torch.empty_like(out_rec)
+    A.C.check(A.C.lib().rawops_render(plan.handle, img._handle, L.Rect(*vp).c(), 0, A.C.QUALITY_FINAL, A._ptr(out),
+                                      None, vp[2], None, A._stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(out.view(torch.int32), out_rec.view(torch.int32))
+    _, dec_o = O.menon_f64(om, norm_plane(m, s), om.area)
+    d = dec.cpu().numpy()
+    pitch = (area[2] + 1) // 2
+    checked = 0
+    for y in range(vp[1] - area[1], vp[1] - area[1] + vp[3]):
+        for x in range(vp[0] - area[0], vp[0] - area[0] + vp[2]):
+            sx, sy = area[0] + x, area[1] + y
+            site = (((sy & 1) + om.phase_y) & 1) * 2 + (((sx & 1) + om.phase_x) & 1)
+            if [[0, 1, 1, 2], [2, 1, 1, 0], [1, 0, 2, 1], [1, 2, 0, 1]][om.pattern][site] == 1:
+                continue
+            assert d[y * pitch + x // 2] == dec_o[y, x], (x, y)
+            checked += 1
+    assert checked >= vp[2] * vp[3] // 2 - vp[3]
+    rng = np.random.default_rng(5)
+    gout = torch.from_numpy(rng.normal(size=(vp[3], vp[2], 3)).astype(np.float32)).cuda()
+    g0, p0 = A.render_backward(img, gout, gains, stops, CAMW, "menon", L.Rect(*vp))
+    g1, p1 = A.render_backward(img, gout, gains, stops, CAMW, "menon", L.Rect(*vp), decisions=dec, recorded=L.Rect(*vp))
+    torch.cuda.synchronize()
+    assert torch.equal(g0.view(torch.int32), g1.view(torch.int32))
+    assert rel(p1.cpu().numpy(), p0.cpu().numpy()) <= 1e-12
+
+
 @pytest.mark.parametrize("radius", [1, 2, 4])
 def test_guided_filter_gradients_vs_fd(radius):
     rng = np.random.default_rng(radius)
