@@ -49,6 +49,11 @@ struct MenonFwdLds {
     static_assert(GP * GR <= NP * NR && GP * GR <= DHP * DHR, "red/blue do not fit over gv/dh");
     static_assert(TW * TH * 12 <= BYTES, "native staging does not fit");
     static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles must cover whole Bayer cells");
+    // The directional stage reads normalised FP32 samples staged over dh/dv,
+    // which the gradient stage only writes afterwards.
+    static constexpr int OFF_RAWF = OFF_DH;
+    static_assert(OFF_DV == OFF_DH + DHP * DHR * 8 && RW * RH * 4 <= DHP * DHR * 8 + DVP * DVR * 8,
+                  "normalised samples do not fit over dh/dv");
 };
 
 // Per-site normalisation levels of a tile by (row, column) parity relative to
@@ -90,6 +95,7 @@ struct MenonFwd {
     int tx, ty;             // tile origin (sensor)
     uint32_t red_rows;      // bit b: the non-green sites of tile rows with parity b are red
     uint16_t* raw;
+    float* rawf;            // normalised samples, valid from load() until gradients() writes dh/dv
     double *gh, *gv, *dh, *dv, *red, *blue;
     uint8_t* hz;
     SiteLevels lv;          // levels by (row, column) parity relative to the tile origin
@@ -99,6 +105,7 @@ struct MenonFwd {
         gh = reinterpret_cast<double*>(lds + L::OFF_GH);
         gv = reinterpret_cast<double*>(lds + L::OFF_GV);
         dh = reinterpret_cast<double*>(lds + L::OFF_DH);
+        rawf = reinterpret_cast<float*>(lds + L::OFF_RAWF);
         dv = reinterpret_cast<double*>(lds + L::OFF_DV);
         red = reinterpret_cast<double*>(lds + L::OFF_RED);
         blue = reinterpret_cast<double*>(lds + L::OFF_BLUE);
@@ -139,7 +146,11 @@ struct MenonFwd {
         const uint16_t* base = s.samples + ((int64_t)(ty - 6) * stride + (tx - 6));
         const bool pairs = !BORDER && ((reinterpret_cast<uintptr_t>(base) | (uintptr_t)(stride * 2)) & 3) == 0;
         constexpr int J = (N + NT - 1) / NT;
-        auto put = [&](int qr, int qk, int b, int e, uint32_t code) { raw[(2 * qr + b) * RW + 2 * qk + e] = (uint16_t)code; };
+        auto put = [&](int qr, int qk, int b, int e, uint32_t code) {
+            const int i = (2 * qr + b) * RW + 2 * qk + e;
+            raw[i] = (uint16_t)code;
+            rawf[i] = lv.norm(code, b, e);
+        };
         if (pairs) {
             // Interior, 4-byte aligned cells: issue every load first, then convert.
             uint32_t w[J][2];
@@ -159,8 +170,12 @@ struct MenonFwd {
                 const int qr = it / QK, qk = it - qr * QK;
                 if (it < N) {
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)   // both codes of the cell row in one 4-byte store
-                        *reinterpret_cast<uint32_t*>(raw + (2 * qr + b) * RW + 2 * qk) = w[j][b];
+                    for (int b = 0; b < 2; ++b) {   // both codes of the cell row in one 4-byte store
+                        const int i = (2 * qr + b) * RW + 2 * qk;
+                        *reinterpret_cast<uint32_t*>(raw + i) = w[j][b];
+                        const float2 f = {lv.norm(w[j][b] & 0xffffu, b, 0), lv.norm(w[j][b] >> 16, b, 1)};
+                        *reinterpret_cast<float2*>(rawf + i) = f;
+                    }
                 }
             }
             return;
@@ -196,13 +211,12 @@ struct MenonFwd {
                 if (!inside(rx, ry)) continue;
                 const int r0 = rawi(rx, ry);
                 auto R = [&](int ox, int oy) -> double {
-                    const int py = (b + oy) & 1, px = (nx(b) + ox) & 1;
                     if constexpr (BORDER) {
                         int x = rx + ox, y = ry + oy;
                         mirror(x, y);
-                        return (double)Rn(rawi(x, y), py, px);
+                        return (double)rawf[rawi(x, y)];
                     } else {
-                        return (double)Rn(r0 + oy * RW + ox, py, px);
+                        return (double)rawf[r0 + oy * RW + ox];
                     }
                 };
                 const double own = R(0, 0);
