@@ -138,20 +138,14 @@ __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty,
 }
 
 // --------------------------------------------------------------- Menon --
-
-// Interior tiles (forward support of tile +-4 inside the active area, so no
-// read is mirrored): the forward is recomputed over the extended tile
-// (tile +-4) with the row-pair forward stages, then the same transposes as
-// the scatter path below are taken as gathers -- each stage's adjoint at a
-// site is a fixed stencil over its neighbours and their recorded decisions --
-// so no LDS atomics are needed.  Extended-tile coordinates (u, v) = sensor -
-// (ex, ey); the tile is [4, 4+TW) x [4, 4+TH).
 //
-// Work is handed out as row-pair items like the forward stages: item (qr, k)
-// holds, in each row v = 2 qr + b, the non-green site 2k + nx(b) and the
-// green site 2k + 1 - nx(b).  Both have compacted index c = v (EW/2) + k in
-// the adjoint planes, so every neighbour is a compile-time offset from c, and
-// a thread keeps its items' per-site values in registers across passes:
+// The Menon adjoint is the transpose of the branch the classifier takes: with
+// the decisions frozen, the demosaic is linear in the raw samples.  A
+// workgroup owns one tile of dL/draw and works on its extended tile (tile
+// +-4, the adjoint's reach) in coordinates (u, v) = sensor - (ex, ey); the
+// tile is [4, 4+TW) x [4, 4+TH).  Every stage's transpose is a gather -- the
+// adjoint reaching a site is a fixed stencil over its neighbours and their
+// decisions -- so no LDS or global atomics touch the raw gradient:
 //   pass 1: gd = dL/d demosaic output at every site; non-green sites publish
 //           the channel the final chroma step mixes, split by the site's
 //           decision into POh (horizontal) and POv (vertical); parameter
@@ -163,27 +157,30 @@ __device__ __forceinline__ void tile_of(const BackwardArgs& a, int& tx, int& ty,
 //           neighbours' AR (red sites) or AB (blue sites), split into
 //           AGh / AGv by the site's decision
 //   pass 4: tile sites: raw adjoint through the directional-green stencil.
-// LDS reuse: POh/POv over dv (dead after the classifier), AR/AB over gh and
-// AGh/AGv over gv (both dead after pass 1), the parameter reduction over raw.
-template <int TW, int TH, int N0>
-__device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx,
-                                                    int ty) {
-    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
-    constexpr int NI = HWE * (EH / 2), J = (NI + kNT - 1) / kNT;   // row-pair items
-    using F = MenonFwd<EW, EH, kNT, N0, false>;
-    using L = MenonFwdLds<EW, EH>;
-    static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "POh/POv do not fit over dv");
-    static_assert(2 * NC * 4 <= L::NP * L::NR * 8, "AR/AB do not fit over gh");
-    static_assert(2 * NC * 4 <= L::NP * L::NR * 8, "AGh/AGv do not fit over gv");
-    static_assert((kNT / 64) * 9 * 8 <= L::OFF_GH, "the parameter reduction does not fit over raw");
-    static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles must cover whole Bayer cells");
-    const int ex = tx - 4, ey = ty - 4;
-    F t(s, lds, ex, ey);
-    t.load();
-    // Upstream gradients of the thread's sites, fetched while the forward is
-    // recomputed.
+// Adjoint planes are compacted per site class: index v (EW/2) + (u >> 1).
+//
+// The forward values the passes need -- the decisions over tile +-4 and the
+// demosaic output of the tile's pixels -- come either from a recompute of
+// the forward over the extended tile (menon_backward), or from decisions a
+// recording forward wrote (menon_backward_recorded: only the selected green
+// and the green-site chroma are recomputed, from the samples of tile +-4).
+
+template <int N0>
+__device__ constexpr int ng_x(int b) { return (b & 1) ? 1 - N0 : N0; }   // x parity of non-green sites, row parity b
+
+// Upstream gradients of a thread's row-pair items, fetched early.  Item
+// (qr, k) holds, in each row v = 2 qr + b, the non-green site 2k + ng_x(b)
+// and the green site next to it; both have compacted index v (EW/2) + k.
+template <int J>
+struct Upstream {
     float goN[J][2][3], goG[J][2][3];
-    uint32_t hasN = 0, hasG = 0;    // bit 2j + b
+    uint32_t hasN, hasG;    // bit 2j + b
+};
+
+template <int EW, int EH, int N0, int J>
+__device__ __forceinline__ void fetch_upstream(const BackwardArgs& a, int ex, int ey, Upstream<J>& up) {
+    constexpr int HWE = EW / 2, NI = HWE * (EH / 2);
+    up.hasN = up.hasG = 0;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int it = (int)threadIdx.x + j * kNT;
@@ -191,29 +188,30 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
             const int qr = it / HWE, k = it - qr * HWE;
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                const int v = 2 * qr + b, un = 2 * k + F::nx(b);
-                if (load_gout(a, ex + un, ey + v, goN[j][b])) hasN |= 1u << (2 * j + b);
-                if (load_gout(a, ex + (un ^ 1), ey + v, goG[j][b])) hasG |= 1u << (2 * j + b);
+                const int v = 2 * qr + b, un = 2 * k + ng_x<N0>(b);
+                if (load_gout(a, ex + un, ey + v, up.goN[j][b])) up.hasN |= 1u << (2 * j + b);
+                if (load_gout(a, ex + (un ^ 1), ey + v, up.goG[j][b])) up.hasG |= 1u << (2 * j + b);
             }
         }
     }
-    __syncthreads();
-    t.directional();
-    __syncthreads();
-    t.gradients();
-    __syncthreads();
-    t.classify();
-    __syncthreads();
-    t.chroma();
-    __syncthreads();
+}
 
-    float* POh = reinterpret_cast<float*>(lds + L::OFF_DV);
-    float* POv = POh + NC;
-    float* AR = reinterpret_cast<float*>(lds + L::OFF_GH);
-    float* AB = AR + NC;
-    float* AGh = reinterpret_cast<float*>(lds + L::OFF_GV);
-    float* AGv = AGh + NC;
-    const bool red0 = (t.red_rows & 1u) != 0, red1 = (t.red_rows & 2u) != 0;   // non-green sites of row parity b are red
+// The four passes on an extended tile all of whose taps lie inside the
+// active area, as row-pair items (every neighbour a compile-time offset, a
+// thread's per-site values kept in registers across passes).  hz: decisions
+// (index v (EW/2) + (u >> 1)); red_rows bit b: the non-green sites of rows of
+// parity b are red; dn(u, v, b, d) / dg(u, v, b, d): demosaic output of the
+// tile's non-green / green pixel (u, v) of row parity b.  POh/POv are written
+// in pass 1, AR/AB in pass 2, AGh/AGv in pass 3; red is the parameter
+// reduction buffer (written last).
+template <int TW, int TH, int N0, int J, class DN, class DG>
+__device__ __forceinline__ void interior_gather(const BackwardArgs& a, const SensorArgs& s, int tx, int ty,
+                                                const Upstream<J>& up, const uint8_t* hz, uint32_t red_rows,
+                                                float* POh, float* POv, float* AR, float* AB, float* AGh, float* AGv,
+                                                double* red, DN dn, DG dg) {
+    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NI = HWE * (EH / 2);
+    static_assert(J == (NI + kNT - 1) / kNT, "item count");
+    const bool red0 = (red_rows & 1u) != 0, red1 = (red_rows & 2u) != 0;
     auto red_row = [&](int b) { return b ? red1 : red0; };
     auto tile_item = [](int qr, int k) { return qr >= 2 && qr < 2 + TH / 2 && k >= 2 && k < 2 + TW / 2; };
     // per item and row: green-channel gd and gd_r + gd_b of the non-green
@@ -230,37 +228,24 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
         const bool mine = tile_item(qr, k);
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = F::nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
+            const int v = 2 * qr + b, un = 2 * k + ng_x<N0>(b), ug = un ^ 1, c = v * HWE + k;
             const bool rs = red_row(b);
-            const bool hn = (hasN >> (2 * j + b)) & 1u, hg = (hasG >> (2 * j + b)) & 1u;
+            const bool hn = (up.hasN >> (2 * j + b)) & 1u, hg = (up.hasG >> (2 * j + b)) & 1u;
             float gd[3], uu[3];
-            if (hn) chain_back(a, goN[j][b], uu, gd);
+            if (hn) chain_back(a, up.goN[j][b], uu, gd);
             else gd[0] = gd[1] = gd[2] = 0.f;
-            const bool h = t.hz[c] != 0;
+            const bool h = hz[c] != 0;
             const float po = rs ? gd[2] : gd[0];
             POh[c] = h ? po : 0.f;
             POv[c] = h ? 0.f : po;
             pg[j][b] = gd[1];
             rn[j][b] = gd[0] + gd[2];           // the own sample feeds both r and b
-            if (hg) chain_back(a, goG[j][b], uu, gg[j][b]);
+            if (hg) chain_back(a, up.goG[j][b], uu, gg[j][b]);
             else gg[j][b][0] = gg[j][b][1] = gg[j][b][2] = 0.f;
             if (mine) {
-                if (hn) {
-                    // final chroma (MenonDemosaic.cpp:162-173), as the forward takes it
-                    const int ia = F::gi(un - (h ? 1 : 0), v - (h ? 0 : 1));
-                    const int ic = F::gi(un + (h ? 1 : 0), v + (h ? 0 : 1));
-                    const double mr = 0.5 * (t.red[ia] + t.red[ic]);
-                    const double mb = 0.5 * (t.blue[ia] + t.blue[ic]);
-                    const float self = t.Rn(F::rawi(un, v), b, nb);
-                    const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
-                    const float d[3] = {rs ? self : other, (float)t.gh[F::ni(un, v)], rs ? other : self};
-                    param_grads(d, goN[j][b], acc);
-                }
-                if (hg) {
-                    const int g = F::gi(ug, v);
-                    const float d[3] = {(float)t.red[g], t.Rn(F::rawi(ug, v), b, nb ^ 1), (float)t.blue[g]};
-                    param_grads(d, goG[j][b], acc);
-                }
+                float d[3];
+                if (hn) { dn(un, v, b, d); param_grads(d, up.goN[j][b], acc); }
+                if (hg) { dg(ug, v, b, d); param_grads(d, up.goG[j][b], acc); }
             }
         }
     }
@@ -273,7 +258,7 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
         const int qr = it / HWE, k = it - qr * HWE;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = F::nx(b), ug = 2 * k + 1 - nb, c = v * HWE + k;
+            const int v = 2 * qr + b, nb = ng_x<N0>(b), ug = 2 * k + 1 - nb, c = v * HWE + k;
             if (ug < 1 || ug >= EW - 1 || v < 1 || v >= EH - 1) continue;
             // R neighbour: B = r + mean(B) - mean(R); B neighbour: R = b + mean(R) - mean(B)
             const float sh = 0.5f * (POh[c - nb] + POh[c + 1 - nb]);      // row neighbours (row parity b)
@@ -295,12 +280,12 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
         const int qr = it / HWE, k = it - qr * HWE;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = F::nx(b), un = 2 * k + nb, c = v * HWE + k;
+            const int v = 2 * qr + b, nb = ng_x<N0>(b), un = 2 * k + nb, c = v * HWE + k;
             if (un < 2 || un >= EW - 2 || v < 2 || v >= EH - 2) continue;
             const float* A = red_row(b) ? AR : AB;
             const float S = (A[c - 1 + nb] + A[c + nb]) + (A[c - HWE] + A[c + HWE]);
             const float ag = pg[j][b] - 0.5f * S;
-            const bool h = t.hz[c] != 0;
+            const bool h = hz[c] != 0;
             AGh[c] = h ? ag : 0.f;
             AGv[c] = h ? 0.f : ag;
             rpn[j][b] = rn[j][b] + 0.5f * S;
@@ -316,7 +301,7 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
         if (!tile_item(qr, k)) continue;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = F::nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
+            const int v = 2 * qr + b, nb = ng_x<N0>(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
             const float rgv = rpg[j][b] + 0.5f * (AGh[c - nb] + AGh[c + 1 - nb]) + 0.5f * (AGv[c - HWE] + AGv[c + HWE]);
             const float rnv = rpn[j][b] + 0.5f * (AGh[c] + AGv[c]) - 0.25f * (AGh[c - 1] + AGh[c + 1]) -
                               0.25f * (AGv[c - 2 * HWE] + AGv[c + 2 * HWE]);
@@ -327,73 +312,43 @@ __device__ __forceinline__ void menon_back_interior(const BackwardArgs& a, const
             }
         }
     }
-    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_RAW));
+    flush_params(a, acc, red);
 }
 
-// Border tiles: some reads of the forward are mirrored at the active-area
-// edge (whole-sample reflection m, applied per coordinate).  A stage output
-// at p reads its input at m(p + k) for each tap k, so the adjoint reaching a
-// target q gathers over the virtual positions V(q) = {z : m(z) = q} that lie
-// within a tap's reach (2) of the area -- q itself and its reflections
-// across nearby edges -- and applies the interior stencil at each z, counting
-// only sources inside the area.  The forward is recomputed with the border
-// form of the row-pair stages; the passes are those of the interior path,
-// per site, with upstream gradients re-read where a pass needs them.
-// LDS reuse: POh/POv over dv (dead after the classifier), AR/AB over dh,
-// AGh/AGv over gv (blue and red, dead after pass 1), the tile's raw partials
-// over gh, the parameter reduction over raw.
-template <int TW, int TH, int N0>
-__device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx, int ty) {
+// The four passes on an extended tile that reaches the active-area edge.
+// The forward mirrors reads there (whole-sample reflection m, per
+// coordinate): a stage output at p reads its input at m(p + k) for each tap
+// k, so the adjoint reaching a target q gathers over the virtual positions
+// V(q) = {z : m(z) = q} within a tap's reach (2) of the area -- q itself and
+// its reflections across nearby edges -- applying the interior stencil at
+// each z and counting only sources inside the area.  Per site, with
+// upstream gradients re-read where a pass needs them; RP holds the tile's raw
+// partials (written in passes 2 and 3).  Arguments as interior_gather.
+template <int TW, int TH, int N0, class DN, class DG>
+__device__ __forceinline__ void border_gather(const BackwardArgs& a, const SensorArgs& s, int tx, int ty,
+                                              const uint8_t* hz, uint32_t red_rows, float* POh, float* POv,
+                                              float* AR, float* AB, float* AGh, float* AGv, float* RP, double* red,
+                                              DN dn, DG dg) {
     constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
-    using F = MenonFwd<EW, EH, kNT, N0, true>;
-    using L = MenonFwdLds<EW, EH>;
-    static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "POh/POv do not fit over dv");
-    static_assert(2 * NC * 4 <= L::DHP * L::DHR * 8, "AR/AB do not fit over dh");
-    static_assert(2 * NC * 4 <= L::NP * L::NR * 8, "AGh/AGv do not fit over gv");
-    static_assert(TW * TH * 4 <= L::NP * L::NR * 8, "raw partials do not fit over gh");
-    static_assert((kNT / 64) * 9 * 8 <= L::OFF_GH, "the parameter reduction does not fit over raw");
-    const int ex = tx - 4, ey = ty - 4;     // origin of the extended tile (tile +-4)
-    F t(s, lds, ex, ey);
-    t.load();
-    __syncthreads();
-    t.directional();
-    __syncthreads();
-    t.gradients();
-    __syncthreads();
-    t.classify();
-    __syncthreads();
-    t.chroma();
-    __syncthreads();
-
-    float* POh = reinterpret_cast<float*>(lds + L::OFF_DV);    // dv is dead after the classifier
-    float* POv = POh + NC;
-    float* AR = reinterpret_cast<float*>(lds + L::OFF_DH);     // dh (blue) is dead after pass 1
-    float* AB = AR + NC;
-    float* AGh = reinterpret_cast<float*>(lds + L::OFF_GV);
-    float* AGv = AGh + NC;
-    float* RP = reinterpret_cast<float*>(lds + L::OFF_GH);
+    const int ex = tx - 4, ey = ty - 4;
     auto ci = [](int u, int v) { return v * HWE + (u >> 1); };
     auto in_tile = [](int u, int v) { return u >= 4 && u < 4 + TW && v >= 4 && v < 4 + TH; };
     auto in_ext = [](int u, int v) { return u >= 0 && u < EW && v >= 0 && v < EH; };
-    // Area bounds read once (the kernel arguments stay in the argument
-    // segment instead of being copied to scratch memory).
     const int ax = s.ax, ay = s.ay, aw = s.aw, ah = s.ah;
     auto in_a = [&](int u, int v) {
         const int x = ex + u, y = ey + v;
         return x >= ax && y >= ay && x < ax + aw && y < ay + ah;
     };
     auto src = [&](int u, int v) { return in_ext(u, v) && in_a(u, v); };    // a computed stage output
-    auto red_at = [&](int, int v) { return ((t.red_rows >> (v & 1)) & 1u) != 0; };   // non-green sites only
+    auto red_at = [&](int v) { return ((red_rows >> (v & 1)) & 1u) != 0; };  // non-green sites of row v
     auto gd_at = [&](int u, int v, float gd[3], float go[3]) -> bool {
         float uu[3];
         if (!load_gout(a, ex + u, ey + v, go)) { gd[0] = gd[1] = gd[2] = 0.f; return false; }
         chain_back(a, go, uu, gd);
         return true;
     };
-    // Virtual positions of (u, v): per coordinate, the position itself and
-    // the outside positions within 2 of the area that reflect onto it (at
-    // most two; kept in scalars, not an indexed array, so nothing goes to
-    // scratch memory).
+    // Per coordinate: the position itself and the outside positions within 2
+    // of the area that reflect onto it (at most two, kept in scalars).
     auto axis = [](int q, int lo, int n, int& z1, int& z2) {
         int cnt = 1;
         z1 = z2 = q;
@@ -430,48 +385,28 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // pass 1: every site of the extended tile inside the area, as row-pair
-    // items (compile-time CFA parities: the per-site levels are then selected
-    // at compile time)
+    // items (compile-time CFA parities)
     for (int it = threadIdx.x; it < NC / 2; it += kNT) {
         const int qr = it / HWE, k = it - qr * HWE;
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int gsite = 0; gsite < 2; ++gsite) {
-                const int v = 2 * qr + b, pu = F::nx(b) ^ gsite, u = 2 * k + pu;   // pu: parity of u
+                const int v = 2 * qr + b, u = 2 * k + (ng_x<N0>(b) ^ gsite);
                 if (!in_a(u, v)) continue;
                 float gd[3], go[3];
                 const bool has = gd_at(u, v, gd, go);
-                const bool rs = ((t.red_rows >> b) & 1u) != 0;
                 if (!gsite) {
                     const int c = ci(u, v);
-                    const bool h = t.hz[c] != 0;
-                    const float po = rs ? gd[2] : gd[0];
+                    const bool h = hz[c] != 0;
+                    const float po = red_at(v) ? gd[2] : gd[0];
                     POh[c] = h ? po : 0.f;
                     POv[c] = h ? 0.f : po;
                 }
                 if (has && in_tile(u, v)) {
-                    // demosaic output (MenonDemosaic.cpp:162-173), as the forward takes it
                     float d[3];
-                    if (gsite) {
-                        const int g = F::gi(u, v);
-                        d[0] = (float)t.red[g];
-                        d[1] = t.Rn(F::rawi(u, v), b, pu);
-                        d[2] = (float)t.blue[g];
-                    } else {
-                        const bool h = t.hz[ci(u, v)] != 0;
-                        int xa = u - (h ? 1 : 0), ya = v - (h ? 0 : 1), xc = u + (h ? 1 : 0), yc = v + (h ? 0 : 1);
-                        t.mirror(xa, ya);
-                        t.mirror(xc, yc);
-                        const int ia = F::gi(xa, ya), ic = F::gi(xc, yc);
-                        const double mr = 0.5 * (t.red[ia] + t.red[ic]);
-                        const double mb = 0.5 * (t.blue[ia] + t.blue[ic]);
-                        const float self = t.Rn(F::rawi(u, v), b, pu);
-                        const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
-                        d[0] = rs ? self : other;
-                        d[1] = (float)t.gh[F::ni(u, v)];
-                        d[2] = rs ? other : self;
-                    }
+                    if (gsite) dg(u, v, b, d);
+                    else dn(u, v, b, d);
                     param_grads(d, go, acc);
                 }
             }
@@ -479,7 +414,7 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
     __syncthreads();
     // pass 2: green sites inside the area with all four neighbours in the extended tile
     for (int i = threadIdx.x; i < NC; i += kNT) {
-        const int v = i / HWE, u = 2 * (i - v * HWE) + (F::nx(v & 1) ^ 1);
+        const int v = i / HWE, u = 2 * (i - v * HWE) + (ng_x<N0>(v & 1) ^ 1);
         if (u < 1 || u >= EW - 1 || v < 1 || v >= EH - 1 || !in_a(u, v)) continue;
         float gd[3], go[3];
         gd_at(u, v, gd, go);
@@ -488,7 +423,7 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
         auto take = [&](int nu, int nv, const float* P) {
             if (!src(nu, nv)) return;
             const float w = 0.5f * P[ci(nu, nv)];
-            if (red_at(nu, nv)) { ar -= w; ab += w; }
+            if (red_at(nv)) { ar -= w; ab += w; }
             else { ar += w; ab -= w; }
         };
         virtuals(u, v, [&](int zu, int zv) {
@@ -504,9 +439,9 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
     __syncthreads();
     // pass 3: non-green sites of tile +-2 inside the area
     for (int i = threadIdx.x; i < NC; i += kNT) {
-        const int v = i / HWE, u = 2 * (i - v * HWE) + F::nx(v & 1);
+        const int v = i / HWE, u = 2 * (i - v * HWE) + ng_x<N0>(v & 1);
         if (u < 2 || u >= EW - 2 || v < 2 || v >= EH - 2 || !in_a(u, v)) continue;
-        const float* A = red_at(u, v) ? AR : AB;
+        const float* A = red_at(v) ? AR : AB;
         float S = 0.f;
         virtuals(u, v, [&](int zu, int zv) {
             if (src(zu - 1, zv)) S += A[ci(zu - 1, zv)];
@@ -517,7 +452,7 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
         float gd[3], go[3];
         gd_at(u, v, gd, go);
         const float ag = gd[1] - 0.5f * S;
-        const bool h = t.hz[i] != 0;
+        const bool h = hz[i] != 0;
         AGh[i] = h ? ag : 0.f;
         AGv[i] = h ? 0.f : ag;
         if (in_tile(u, v)) RP[(v - 4) * TW + (u - 4)] = (gd[0] + gd[2]) + 0.5f * S;   // the own sample feeds both r and b
@@ -527,7 +462,7 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
     for (int i = threadIdx.x; i < TW * TH; i += kNT) {
         const int v = 4 + i / TW, u = 4 + (i - (i / TW) * TW);
         if (!in_a(u, v)) continue;
-        const bool green = (u & 1) != F::nx(v & 1);
+        const bool green = (u & 1) != ng_x<N0>(v & 1);
         const int step = green ? 1 : 2;            // directional-green taps reaching this sample
         const float w = green ? 0.5f : -0.25f;
         float r = RP[i];
@@ -549,7 +484,81 @@ __device__ __forceinline__ void menon_back_border(const BackwardArgs& a, const S
         }
         if (a.graw) a.graw[(size_t)(ty + v - 4 - ay) * aw + (size_t)(tx + u - 4 - ax)] = r;
     }
-    flush_params(a, acc, reinterpret_cast<double*>(lds + L::OFF_RAW));
+    flush_params(a, acc, red);
+}
+
+// Demosaic output of a non-green pixel (MenonDemosaic.cpp:162-173), as the
+// forward takes it: red/blue means at the two neighbours along the site's
+// decision (indices ia, ic of the minus / plus neighbour), green, own sample.
+__device__ __forceinline__ void finish_nongreen(bool red_site, float self, double g, const double* red,
+                                                const double* blue, int ia, int ic, float d[3]) {
+    const double mr = 0.5 * (red[ia] + red[ic]);
+    const double mb = 0.5 * (blue[ia] + blue[ic]);
+    const float other = (float)((double)self + (red_site ? mb : mr) - (red_site ? mr : mb));
+    d[0] = red_site ? self : other;
+    d[1] = (float)g;
+    d[2] = red_site ? other : self;
+}
+
+// Recomputing adjoint of one tile: the forward's row-pair stages over the
+// extended tile (decisions included), then the gather passes.  LDS reuse
+// (MenonFwdLds of the extended tile): POh/POv over dv (dead after the
+// classifier); interior: AR/AB over gh, AGh/AGv over gv (dead after pass 1);
+// border: AR/AB over dh, AGh/AGv over gv (blue and red, dead after pass 1),
+// raw partials over gh; the parameter reduction over the raw codes.
+template <int TW, int TH, int N0, bool BORDER>
+__device__ __forceinline__ void menon_back_recompute(const BackwardArgs& a, const SensorArgs& s, char* lds, int tx,
+                                                     int ty) {
+    constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
+    constexpr int J = (HWE * (EH / 2) + kNT - 1) / kNT;
+    using F = MenonFwd<EW, EH, kNT, N0, BORDER>;
+    using L = MenonFwdLds<EW, EH>;
+    static_assert(2 * NC * 4 <= L::DVP * L::DVR * 8, "POh/POv do not fit over dv");
+    static_assert(2 * NC * 4 <= L::NP * L::NR * 8 && 2 * NC * 4 <= L::DHP * L::DHR * 8, "AR/AB do not fit");
+    static_assert(2 * NC * 4 <= L::NP * L::NR * 8 && TW * TH * 4 <= L::NP * L::NR * 8, "AG/raw partials do not fit");
+    static_assert((kNT / 64) * 9 * 8 <= L::OFF_GH, "the parameter reduction does not fit over the codes");
+    const int ex = tx - 4, ey = ty - 4;
+    F t(s, lds, ex, ey);
+    t.load();
+    Upstream<J> up;
+    if constexpr (!BORDER) fetch_upstream<EW, EH, N0>(a, ex, ey, up);   // in flight during the recompute
+    __syncthreads();
+    t.directional();
+    __syncthreads();
+    t.gradients();
+    __syncthreads();
+    t.classify();
+    __syncthreads();
+    t.chroma();
+    __syncthreads();
+    auto dn = [&](int u, int v, int b, float d[3]) {
+        const bool h = t.hz[v * HWE + (u >> 1)] != 0;
+        int xa = u - (h ? 1 : 0), ya = v - (h ? 0 : 1), xc = u + (h ? 1 : 0), yc = v + (h ? 0 : 1);
+        t.mirror(xa, ya);
+        t.mirror(xc, yc);
+        finish_nongreen(((t.red_rows >> b) & 1u) != 0, t.Rn(F::rawi(u, v), b, F::nx(b)), t.gh[F::ni(u, v)], t.red,
+                        t.blue, F::gi(xa, ya), F::gi(xc, yc), d);
+    };
+    auto dg = [&](int u, int v, int b, float d[3]) {
+        const int g = F::gi(u, v);
+        d[0] = (float)t.red[g];
+        d[1] = t.Rn(F::rawi(u, v), b, F::nx(b) ^ 1);
+        d[2] = (float)t.blue[g];
+    };
+    float* POh = reinterpret_cast<float*>(lds + L::OFF_DV);
+    double* red = reinterpret_cast<double*>(lds + L::OFF_RAW);
+    if constexpr (!BORDER) {
+        float* AR = reinterpret_cast<float*>(lds + L::OFF_GH);
+        float* AGh = reinterpret_cast<float*>(lds + L::OFF_GV);
+        interior_gather<TW, TH, N0>(a, s, tx, ty, up, t.hz, t.red_rows, POh, POh + NC, AR, AR + NC, AGh, AGh + NC,
+                                    red, dn, dg);
+    } else {
+        float* AR = reinterpret_cast<float*>(lds + L::OFF_DH);
+        float* AGh = reinterpret_cast<float*>(lds + L::OFF_GV);
+        float* RP = reinterpret_cast<float*>(lds + L::OFF_GH);
+        border_gather<TW, TH, N0>(a, s, tx, ty, t.hz, t.red_rows, POh, POh + NC, AR, AR + NC, AGh, AGh + NC, RP,
+                                  red, dn, dg);
+    }
 }
 
 template <int TW, int TH>
@@ -573,23 +582,22 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(6))) m
     // tile of the launch (tile origins step by even sizes)
     const uint32_t n0 = ((s.ng_parity >> ((uint32_t)ty & 1u)) & 1u) ^ ((uint32_t)tx & 1u);
     if (n0) {
-        if (interior) menon_back_interior<TW, TH, 1>(a, s, lds, tx, ty);
-        else menon_back_border<TW, TH, 1>(a, s, lds, tx, ty);
+        if (interior) menon_back_recompute<TW, TH, 1, false>(a, s, lds, tx, ty);
+        else menon_back_recompute<TW, TH, 1, true>(a, s, lds, tx, ty);
     } else {
-        if (interior) menon_back_interior<TW, TH, 0>(a, s, lds, tx, ty);
-        else menon_back_border<TW, TH, 0>(a, s, lds, tx, ty);
+        if (interior) menon_back_recompute<TW, TH, 0, false>(a, s, lds, tx, ty);
+        else menon_back_recompute<TW, TH, 0, true>(a, s, lds, tx, ty);
     }
 }
 
 // ------------------------------------------- Menon, recorded decisions --
 //
-// Interior tiles whose adjoint reach (tile +-4) has decisions recorded by
-// the forward (rawops_render_record): the classifier is not recomputed.  The
-// forward values the adjoint needs besides the decisions -- the selected
-// green at non-green sites of tile +-2 and red/blue at green sites of tile
-// +-1, for the demosaic output of the tile's pixels (parameter gradients) --
-// come from the sample codes of tile +-4, with the forward's FP64 operation
-// order.  The gather passes are those of menon_back_interior.
+// Tiles whose reach (tile +-4, within the active area) has decisions
+// recorded by the forward (rawops_render_record): no classifier recompute.
+// The selected green at the non-green sites of tile +-2 and red/blue at the
+// green sites of tile +-1 (for the demosaic output of the tile's pixels) are
+// recomputed from the samples of tile +-4 in the forward's FP64 operation
+// order, reads mirrored at the area edge as the forward mirrors them.
 template <int TW, int TH>
 struct RecLds {
     static constexpr int EW = TW + 8, EH = TH + 8, HWE = EW / 2, NC = HWE * EH;
@@ -601,18 +609,19 @@ struct RecLds {
     static constexpr int OFF_BLUE = OFF_RED + GW * GH * 8;
     static constexpr int OFF_PO = OFF_BLUE + GW * GH * 8;                   // f32 POh, POv
     static constexpr int BYTES = OFF_PO + 2 * NC * 4;
-    // After pass 1: AR/AB over green + red; after pass 2: AGh/AGv over PO;
-    // the parameter reduction over the codes.
+    // After pass 1: AR/AB over green + red, raw partials over blue; after
+    // pass 2: AGh/AGv over PO; the parameter reduction over the codes.
     static_assert(2 * NC * 4 <= 2 * GW * GH * 8, "AR/AB do not fit over green/red");
+    static_assert(TW * TH * 4 <= GW * GH * 8, "raw partials do not fit over blue");
     static_assert((kNT / 64) * 9 * 8 <= OFF_HZ, "the parameter reduction does not fit over the codes");
 };
 
-template <int TW, int TH, int N0>
+template <int TW, int TH, int N0, bool BORDER>
 __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, const SensorArgs& s, char* lds,
                                                          int tx, int ty) {
     using R = RecLds<TW, TH>;
     constexpr int EW = R::EW, EH = R::EH, HWE = R::HWE, NC = R::NC, GW = R::GW;
-    constexpr int NI = HWE * (EH / 2), J = (NI + kNT - 1) / kNT;   // row-pair items over the extended tile
+    constexpr int J = (HWE * (EH / 2) + kNT - 1) / kNT;
     constexpr int NG = GW * ((EH - 4) / 2);                         // row-pair items over tile +-2
     const int ex = tx - 4, ey = ty - 4;
     uint16_t* raw = reinterpret_cast<uint16_t*>(lds + R::OFF_RAW);
@@ -620,64 +629,67 @@ __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, 
     double* green = reinterpret_cast<double*>(lds + R::OFF_GREEN);
     double* red = reinterpret_cast<double*>(lds + R::OFF_RED);
     double* blue = reinterpret_cast<double*>(lds + R::OFF_BLUE);
-    auto nx = [](int b) { return (b & 1) ? 1 - N0 : N0; };          // x parity of non-green sites, row parity b
     SiteLevels lv;
     lv.init(s, ex, ey);
     uint32_t red_rows = 0;
-    for (int b = 0; b < 2; ++b) red_rows |= (color_of(s, ex + nx(b), ey + b) == 0 ? 1u : 0u) << b;
-    auto Rn = [&](int u, int v) -> double {    // parity of (u, v) relative to the extended tile
-        return (double)lv.norm(raw[v * EW + u], v & 1, u & 1);
+    for (int b = 0; b < 2; ++b) red_rows |= (color_of(s, ex + ng_x<N0>(b), ey + b) == 0 ? 1u : 0u) << b;
+    auto inside = [&](int u, int v) { return !BORDER || in_area(s, ex + u, ey + v); };
+    auto mirror = [&](int& u, int& v) {
+        if (BORDER) {
+            const int sx = ex + u, sy = ey + v;
+            if (!in_area(s, sx, sy)) { u = reflect(sx, s.ax, s.aw) - ex; v = reflect(sy, s.ay, s.ah) - ey; }
+        }
+    };
+    // sample at (u + ox, v + oy), mirrored; parity of the unmirrored position (reflection keeps it)
+    auto Rd = [&](int u, int v, int ox, int oy) -> double {
+        const int py = (v + oy) & 1, px = (u + ox) & 1;
+        int x = u + ox, y = v + oy;
+        mirror(x, y);
+        return (double)lv.norm(raw[y * EW + x], py, px);
     };
     auto g2 = [](int u, int v) { return (v - 2) * GW + ((u - 2) >> 1); };   // tile +-2 compaction
+    auto G = [&](int u, int v, int ox, int oy) -> double {
+        int x = u + ox, y = v + oy;
+        mirror(x, y);
+        return green[g2(x, y)];
+    };
 
     // codes of the extended tile, its recorded decisions, upstream gradients
     {
         const int64_t stride = s.stride;
         const uint16_t* base = s.samples + ((int64_t)ey * stride + ex);
-        const bool pairs = ((reinterpret_cast<uintptr_t>(base) | (uintptr_t)(stride * 2)) & 3) == 0;
+        const bool pairs = !BORDER && ((reinterpret_cast<uintptr_t>(base) | (uintptr_t)(stride * 2)) & 3) == 0;
         for (int i = threadIdx.x; i < EW * EH / 2; i += kNT) {
             const int v = i / (EW / 2), k = i - v * (EW / 2);
             const uint16_t* p = base + (int64_t)v * stride + 2 * k;
             if (pairs) {
                 *reinterpret_cast<uint32_t*>(raw + v * EW + 2 * k) = *reinterpret_cast<const uint32_t*>(p);
             } else {
-                raw[v * EW + 2 * k] = p[0];
-                raw[v * EW + 2 * k + 1] = p[1];
+                if (inside(2 * k, v)) raw[v * EW + 2 * k] = p[0];
+                if (inside(2 * k + 1, v)) raw[v * EW + 2 * k + 1] = p[1];
             }
         }
         for (int i = threadIdx.x; i < NC; i += kNT) {
-            const int v = i / HWE, u = 2 * (i - v * HWE) + nx(v & 1);
-            hz[i] = a.decisions[(size_t)(ey + v - s.ay) * a.dpitch + (size_t)((ex + u - s.ax) >> 1)];
+            const int v = i / HWE, u = 2 * (i - v * HWE) + ng_x<N0>(v & 1);
+            if (inside(u, v)) hz[i] = a.decisions[(size_t)(ey + v - s.ay) * a.dpitch + (size_t)((ex + u - s.ax) >> 1)];
         }
     }
-    float goN[J][2][3], goG[J][2][3];
-    uint32_t hasN = 0, hasG = 0;    // bit 2j + b
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int it = (int)threadIdx.x + j * kNT;
-        if (it < NI) {
-            const int qr = it / HWE, k = it - qr * HWE;
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int v = 2 * qr + b, un = 2 * k + nx(b);
-                if (load_gout(a, ex + un, ey + v, goN[j][b])) hasN |= 1u << (2 * j + b);
-                if (load_gout(a, ex + (un ^ 1), ey + v, goG[j][b])) hasG |= 1u << (2 * j + b);
-            }
-        }
-    }
+    Upstream<J> up;
+    if constexpr (!BORDER) fetch_upstream<EW, EH, N0>(a, ex, ey, up);
     __syncthreads();
     // selected green at non-green sites of tile +-2 (MenonDemosaic.cpp:107-119, 141-147)
     for (int it = threadIdx.x; it < NG; it += kNT) {
         const int qr = it / GW, k = it - qr * GW;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int v = 2 + 2 * qr + b, u = 2 + 2 * k + nx(b);
-            const double own = Rn(u, v);
+            const int v = 2 + 2 * qr + b, u = 2 + 2 * k + ng_x<N0>(b);
+            if (!inside(u, v)) continue;
+            const double own = Rd(u, v, 0, 0);
             double gsel;
             if (hz[v * HWE + (u >> 1)])
-                gsel = 0.5 * (Rn(u - 1, v) + Rn(u + 1, v)) + 0.5 * own - 0.25 * (Rn(u - 2, v) + Rn(u + 2, v));
+                gsel = 0.5 * (Rd(u, v, -1, 0) + Rd(u, v, 1, 0)) + 0.5 * own - 0.25 * (Rd(u, v, -2, 0) + Rd(u, v, 2, 0));
             else
-                gsel = 0.5 * (Rn(u, v - 1) + Rn(u, v + 1)) + 0.5 * own - 0.25 * (Rn(u, v - 2) + Rn(u, v + 2));
+                gsel = 0.5 * (Rd(u, v, 0, -1) + Rd(u, v, 0, 1)) + 0.5 * own - 0.25 * (Rd(u, v, 0, -2) + Rd(u, v, 0, 2));
             green[g2(u, v)] = gsel;
         }
     }
@@ -687,140 +699,45 @@ __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, 
         const int qr = it / GW, k = it - qr * GW;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int v = 2 + 2 * qr + b, u = 2 + 2 * k + (nx(b) ^ 1);
-            if (u < 3 || u >= EW - 3 || v < 3 || v >= EH - 3) continue;
-            const double own = Rn(u, v);
-            const double ch = own + 0.5 * (Rn(u - 1, v) + Rn(u + 1, v)) - 0.5 * (green[g2(u - 1, v)] + green[g2(u + 1, v)]);
-            const double cv = own + 0.5 * (Rn(u, v - 1) + Rn(u, v + 1)) - 0.5 * (green[g2(u, v - 1)] + green[g2(u, v + 1)]);
+            const int v = 2 + 2 * qr + b, u = 2 + 2 * k + (ng_x<N0>(b) ^ 1);
+            if (u < 3 || u >= EW - 3 || v < 3 || v >= EH - 3 || !inside(u, v)) continue;
+            const double own = Rd(u, v, 0, 0);
+            const double ch = own + 0.5 * (Rd(u, v, -1, 0) + Rd(u, v, 1, 0)) - 0.5 * (G(u, v, -1, 0) + G(u, v, 1, 0));
+            const double cv = own + 0.5 * (Rd(u, v, 0, -1) + Rd(u, v, 0, 1)) - 0.5 * (G(u, v, 0, -1) + G(u, v, 0, 1));
             const bool red_row = (red_rows >> b) & 1u;
             red[g2(u, v)] = red_row ? ch : cv;
             blue[g2(u, v)] = red_row ? cv : ch;
         }
     }
     __syncthreads();
-
+    auto dn = [&](int u, int v, int b, float d[3]) {
+        const bool h = hz[v * HWE + (u >> 1)] != 0;
+        int xa = u - (h ? 1 : 0), ya = v - (h ? 0 : 1), xc = u + (h ? 1 : 0), yc = v + (h ? 0 : 1);
+        mirror(xa, ya);
+        mirror(xc, yc);
+        finish_nongreen(((red_rows >> b) & 1u) != 0, lv.norm(raw[v * EW + u], b, ng_x<N0>(b)), green[g2(u, v)], red,
+                        blue, g2(xa, ya), g2(xc, yc), d);
+    };
+    auto dg = [&](int u, int v, int b, float d[3]) {
+        const int g = g2(u, v);
+        d[0] = (float)red[g];
+        d[1] = lv.norm(raw[v * EW + u], b, ng_x<N0>(b) ^ 1);
+        d[2] = (float)blue[g];
+    };
     float* POh = reinterpret_cast<float*>(lds + R::OFF_PO);
-    float* POv = POh + NC;
     float* AR = reinterpret_cast<float*>(lds + R::OFF_GREEN);
-    float* AB = AR + NC;
-    float* AGh = POh;
-    float* AGv = POv;
-    const bool red0 = (red_rows & 1u) != 0, red1 = (red_rows & 2u) != 0;
-    auto red_row = [&](int b) { return b ? red1 : red0; };
-    auto tile_item = [](int qr, int k) { return qr >= 2 && qr < 2 + TH / 2 && k >= 2 && k < 2 + TW / 2; };
-    float pg[J][2], rn[J][2], gg[J][2][3], rpg[J][2], rpn[J][2];
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-
-    // pass 1
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int it = (int)threadIdx.x + j * kNT;
-        if (it >= NI) break;
-        const int qr = it / HWE, k = it - qr * HWE;
-        const bool mine = tile_item(qr, k);
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
-            const bool rs = red_row(b);
-            const bool hn = (hasN >> (2 * j + b)) & 1u, hg = (hasG >> (2 * j + b)) & 1u;
-            float gd[3], uu[3];
-            if (hn) chain_back(a, goN[j][b], uu, gd);
-            else gd[0] = gd[1] = gd[2] = 0.f;
-            const bool h = hz[c] != 0;
-            const float po = rs ? gd[2] : gd[0];
-            POh[c] = h ? po : 0.f;
-            POv[c] = h ? 0.f : po;
-            pg[j][b] = gd[1];
-            rn[j][b] = gd[0] + gd[2];           // the own sample feeds both r and b
-            if (hg) chain_back(a, goG[j][b], uu, gg[j][b]);
-            else gg[j][b][0] = gg[j][b][1] = gg[j][b][2] = 0.f;
-            if (mine) {
-                if (hn) {
-                    // final chroma (MenonDemosaic.cpp:162-173), as the forward takes it
-                    const int ia = g2(un - (h ? 1 : 0), v - (h ? 0 : 1)), ic = g2(un + (h ? 1 : 0), v + (h ? 0 : 1));
-                    const double mr = 0.5 * (red[ia] + red[ic]);
-                    const double mb = 0.5 * (blue[ia] + blue[ic]);
-                    const float self = lv.norm(raw[v * EW + un], b, nb);
-                    const float other = (float)((double)self + (rs ? mb : mr) - (rs ? mr : mb));
-                    const float d[3] = {rs ? self : other, (float)green[g2(un, v)], rs ? other : self};
-                    param_grads(d, goN[j][b], acc);
-                }
-                if (hg) {
-                    const int g = g2(ug, v);
-                    const float d[3] = {(float)red[g], lv.norm(raw[v * EW + ug], b, nb ^ 1), (float)blue[g]};
-                    param_grads(d, goG[j][b], acc);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // pass 2: green sites with all four neighbours inside the extended tile
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int it = (int)threadIdx.x + j * kNT;
-        if (it >= NI) break;
-        const int qr = it / HWE, k = it - qr * HWE;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = nx(b), ug = 2 * k + 1 - nb, c = v * HWE + k;
-            if (ug < 1 || ug >= EW - 1 || v < 1 || v >= EH - 1) continue;
-            const float sh = 0.5f * (POh[c - nb] + POh[c + 1 - nb]);
-            const float sv = 0.5f * (POv[c - HWE] + POv[c + HWE]);
-            const float th = red_row(b) ? sh : -sh, tv = red_row(b ^ 1) ? sv : -sv;
-            const float ar = gg[j][b][0] - th - tv;
-            const float ab = gg[j][b][2] + th + tv;
-            AR[c] = ar;
-            AB[c] = ab;
-            rpg[j][b] = ar + ab + gg[j][b][1];
-        }
-    }
-    __syncthreads();
-    // pass 3: non-green sites of tile +-2
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int it = (int)threadIdx.x + j * kNT;
-        if (it >= NI) break;
-        const int qr = it / HWE, k = it - qr * HWE;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = nx(b), un = 2 * k + nb, c = v * HWE + k;
-            if (un < 2 || un >= EW - 2 || v < 2 || v >= EH - 2) continue;
-            const float* A = red_row(b) ? AR : AB;
-            const float S = (A[c - 1 + nb] + A[c + nb]) + (A[c - HWE] + A[c + HWE]);
-            const float ag = pg[j][b] - 0.5f * S;
-            const bool h = hz[c] != 0;
-            AGh[c] = h ? ag : 0.f;
-            AGv[c] = h ? 0.f : ag;
-            rpn[j][b] = rn[j][b] + 0.5f * S;
-        }
-    }
-    __syncthreads();
-    // pass 4: raw adjoint of the tile through the directional-green stencil
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int it = (int)threadIdx.x + j * kNT;
-        if (it >= NI) break;
-        const int qr = it / HWE, k = it - qr * HWE;
-        if (!tile_item(qr, k)) continue;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int v = 2 * qr + b, nb = nx(b), un = 2 * k + nb, ug = un ^ 1, c = v * HWE + k;
-            const float rgv = rpg[j][b] + 0.5f * (AGh[c - nb] + AGh[c + 1 - nb]) + 0.5f * (AGv[c - HWE] + AGv[c + HWE]);
-            const float rnv = rpn[j][b] + 0.5f * (AGh[c] + AGv[c]) - 0.25f * (AGh[c - 1] + AGh[c + 1]) -
-                              0.25f * (AGv[c - 2 * HWE] + AGv[c + 2 * HWE]);
-            if (a.graw) {
-                float* row = a.graw + (size_t)(ty + v - 4 - s.ay) * s.aw + (size_t)(tx - 4 - s.ax);
-                row[un] = rnv;
-                row[ug] = rgv;
-            }
-        }
-    }
-    flush_params(a, acc, reinterpret_cast<double*>(lds + R::OFF_RAW));
+    double* rbuf = reinterpret_cast<double*>(lds + R::OFF_RAW);
+    if constexpr (!BORDER)
+        interior_gather<TW, TH, N0>(a, s, tx, ty, up, hz, red_rows, POh, POh + NC, AR, AR + NC, POh, POh + NC, rbuf,
+                                    dn, dg);
+    else
+        border_gather<TW, TH, N0>(a, s, tx, ty, hz, red_rows, POh, POh + NC, AR, AR + NC, POh, POh + NC,
+                                  reinterpret_cast<float*>(lds + R::OFF_BLUE), rbuf, dn, dg);
 }
 
 // One workgroup per tile of the recorded block [r0, r1) x [c0, c1).
 template <int TW, int TH>
-__global__ void __launch_bounds__(kNT) menon_backward_recorded(const BackwardArgs args) {
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(8))) menon_backward_recorded(const BackwardArgs args) {
     __shared__ __attribute__((aligned(16))) char lds[RecLds<TW, TH>::BYTES];
     (void)args;
     const BackwardArgs& a = *(const BackwardArgs*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -829,9 +746,16 @@ __global__ void __launch_bounds__(kNT) menon_backward_recorded(const BackwardArg
     const int tx = __builtin_amdgcn_readfirstlane(a.s.ax + (a.tile0_x + c) * TW);
     const int ty = __builtin_amdgcn_readfirstlane(a.s.ay + (a.tile0_y + r) * TH);
     const SensorArgs& s = a.s;
+    // no read of tile +-4 leaves the area: no mirroring
+    const bool interior = tx - 4 >= s.ax && ty - 4 >= s.ay && tx + TW + 4 <= s.ax + s.aw && ty + TH + 4 <= s.ay + s.ah;
     const uint32_t n0 = ((s.ng_parity >> ((uint32_t)ty & 1u)) & 1u) ^ ((uint32_t)tx & 1u);
-    if (n0) menon_back_recorded_tile<TW, TH, 1>(a, s, lds, tx, ty);
-    else menon_back_recorded_tile<TW, TH, 0>(a, s, lds, tx, ty);
+    if (n0) {
+        if (interior) menon_back_recorded_tile<TW, TH, 1, false>(a, s, lds, tx, ty);
+        else menon_back_recorded_tile<TW, TH, 1, true>(a, s, lds, tx, ty);
+    } else {
+        if (interior) menon_back_recorded_tile<TW, TH, 0, false>(a, s, lds, tx, ty);
+        else menon_back_recorded_tile<TW, TH, 0, true>(a, s, lds, tx, ty);
+    }
 }
 
 // ------------------------------------------------------------- bilinear --
@@ -938,19 +862,20 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     interior_range(a.tile0_y, tiles_y, TH, a.s.ay, a.s.ah, a.r0, a.r1);
     interior_range(a.tile0_x, a.tiles_x, TW, a.s.ax, a.s.aw, a.c0, a.c1);
     if (menon && L.decisions) {
-        // Tiles of the interior block whose reach (tile +-4) lies inside the
-        // recorded viewport read the recorded decisions; the rest recompute.
-        auto covered = [](int first, int T, int lo, int rec_lo, int rec_len, int& i0, int& i1) {
-            int n0 = i1, n1 = i0;
-            for (int i = i0; i < i1; ++i) {
+        // Tiles whose reach within the area ((tile +-4) n area) lies inside
+        // the recorded viewport read the recorded decisions: per axis a
+        // contiguous range of the launch's tiles.  The others recompute.
+        auto covered = [](int first, int n, int T, int lo, int len, int rec_lo, int rec_len, int& i0, int& i1) {
+            i0 = n; i1 = 0;
+            for (int i = 0; i < n; ++i) {
                 const int t0 = lo + (first + i) * T;
-                if (t0 - 4 >= rec_lo && t0 + T + 4 <= rec_lo + rec_len) { n0 = std::min(n0, i); n1 = std::max(n1, i + 1); }
+                const int r0 = std::max(t0 - 4, lo), r1 = std::min(t0 + T + 4, lo + len);
+                if (r0 >= rec_lo && r1 <= rec_lo + rec_len) { i0 = std::min(i0, i); i1 = std::max(i1, i + 1); }
             }
-            if (n0 >= n1) n0 = n1 = i0;
-            i0 = n0; i1 = n1;
+            if (i0 >= i1) i0 = i1 = 0;
         };
-        covered(a.tile0_y, TH, a.s.ay, (int)L.recorded.y, (int)L.recorded.height, a.r0, a.r1);
-        covered(a.tile0_x, TW, a.s.ax, (int)L.recorded.x, (int)L.recorded.width, a.c0, a.c1);
+        covered(a.tile0_y, tiles_y, TH, a.s.ay, a.s.ah, (int)L.recorded.y, (int)L.recorded.height, a.r0, a.r1);
+        covered(a.tile0_x, a.tiles_x, TW, a.s.ax, a.s.aw, (int)L.recorded.x, (int)L.recorded.width, a.c0, a.c1);
         if (a.r0 == a.r1 || a.c0 == a.c1) a.r0 = a.r1 = a.c0 = a.c1 = 0;
         a.decisions = L.decisions;
         a.dpitch = decision_pitch((uint32_t)a.s.aw);
@@ -960,7 +885,8 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
-        // the other tiles come first in the border-first order: launch only those
+        // the recomputing kernel's border-first order lists the tiles outside
+        // the block [r0, r1) x [c0, c1) first: launch only those
         if (blocks - rec)
             hipLaunchKernelGGL((menon_backward<64, 20>), dim3(blocks - rec), dim3(kNT), 0, stream, a);
         return hipGetLastError();
