@@ -42,6 +42,7 @@ struct BackwardArgs {
     int r0, r1, c0, c1;        // launch-relative tile rows / columns of the interior block
     const uint8_t* decisions;  // recorded Menon decisions (area plane, dpitch bytes a row) or null
     uint32_t dpitch;
+    int ir0, ir1, ic0, ic1;    // within the recorded block: its tiles that need no mirroring
 };
 
 // dL/dout at a sensor pixel (zero outside the viewport).
@@ -111,23 +112,28 @@ __device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc
 // Border-first tile order: the tiles outside the interior block [r0, r1) x
 // [c0, c1) (slower: mirrored reads, scatter adjoint) take the first block
 // indices so they start with the launch instead of trailing it.
-__device__ __forceinline__ void tile_of_border_first(const BackwardArgs& a, int& tx, int& ty, int TW, int TH) {
-    const int TX = a.tiles_x, TY = a.tiles_y;
-    const int top = a.r0 * TX, bottom = (TY - a.r1) * TX, side = a.c0 + (TX - a.c1), middle = (a.r1 - a.r0) * side;
-    int b = (int)blockIdx.x, r, c;
+// Block b of a TY x TX tile grid, the tiles outside the block [r0, r1) x
+// [c0, c1) first (a bijection; an empty block lists the grid row-major).
+__device__ __forceinline__ void border_first(int b, int TX, int TY, int r0, int r1, int c0, int c1, int& r, int& c) {
+    const int top = r0 * TX, bottom = (TY - r1) * TX, side = c0 + (TX - c1), middle = (r1 - r0) * side;
     if (b < top) {
         r = b / TX; c = b - r * TX;
     } else if ((b -= top) < bottom) {
-        r = a.r1 + b / TX; c = b - (b / TX) * TX;
+        r = r1 + b / TX; c = b - (b / TX) * TX;
     } else if ((b -= bottom) < middle) {
-        r = a.r0 + b / side;
+        r = r0 + b / side;
         const int j = b - (b / side) * side;
-        c = j < a.c0 ? j : a.c1 + (j - a.c0);
+        c = j < c0 ? j : c1 + (j - c0);
     } else {
         b -= middle;
-        const int ci = a.c1 - a.c0;
-        r = a.r0 + b / ci; c = a.c0 + (b - (b / ci) * ci);
+        const int ci = c1 - c0;
+        r = r0 + b / ci; c = c0 + (b - (b / ci) * ci);
     }
+}
+
+__device__ __forceinline__ void tile_of_border_first(const BackwardArgs& a, int& tx, int& ty, int TW, int TH) {
+    int r, c;
+    border_first((int)blockIdx.x, a.tiles_x, a.tiles_y, a.r0, a.r1, a.c0, a.c1, r, c);
     tx = a.s.ax + (a.tile0_x + c) * TW;
     ty = a.s.ay + (a.tile0_y + r) * TH;
 }
@@ -735,14 +741,17 @@ __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, 
                                   reinterpret_cast<float*>(lds + R::OFF_BLUE), rbuf, dn, dg);
 }
 
-// One workgroup per tile of the recorded block [r0, r1) x [c0, c1).
+// One workgroup per tile of the recorded block [r0, r1) x [c0, c1), its
+// mirrored (slower) tiles first so they do not trail the launch.
 template <int TW, int TH>
 __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(8))) menon_backward_recorded(const BackwardArgs args) {
     __shared__ __attribute__((aligned(16))) char lds[RecLds<TW, TH>::BYTES];
     (void)args;
     const BackwardArgs& a = *(const BackwardArgs*)__builtin_amdgcn_kernarg_segment_ptr();
-    const int cols = a.c1 - a.c0;
-    const int r = a.r0 + (int)blockIdx.x / cols, c = a.c0 + (int)blockIdx.x % cols;
+    int r, c;
+    border_first((int)blockIdx.x, a.c1 - a.c0, a.r1 - a.r0, a.ir0, a.ir1, a.ic0, a.ic1, r, c);
+    r += a.r0;
+    c += a.c0;
     const int tx = __builtin_amdgcn_readfirstlane(a.s.ax + (a.tile0_x + c) * TW);
     const int ty = __builtin_amdgcn_readfirstlane(a.s.ay + (a.tile0_y + r) * TH);
     const SensorArgs& s = a.s;
@@ -877,6 +886,19 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
         covered(a.tile0_y, tiles_y, TH, a.s.ay, a.s.ah, (int)L.recorded.y, (int)L.recorded.height, a.r0, a.r1);
         covered(a.tile0_x, a.tiles_x, TW, a.s.ax, a.s.aw, (int)L.recorded.x, (int)L.recorded.width, a.c0, a.c1);
         if (a.r0 == a.r1 || a.c0 == a.c1) a.r0 = a.r1 = a.c0 = a.c1 = 0;
+        // within it, the tiles whose reach stays inside the area (no mirroring)
+        auto unmirrored = [](int first, int from, int to, int T, int lo, int len, int& i0, int& i1) {
+            i0 = to; i1 = from;
+            for (int i = from; i < to; ++i) {
+                const int t0 = lo + (first + i) * T;
+                if (t0 - 4 >= lo && t0 + T + 4 <= lo + len) { i0 = std::min(i0, i); i1 = std::max(i1, i + 1); }
+            }
+            if (i0 >= i1) i0 = i1 = from;
+            i0 -= from; i1 -= from;
+        };
+        unmirrored(a.tile0_y, a.r0, a.r1, TH, a.s.ay, a.s.ah, a.ir0, a.ir1);
+        unmirrored(a.tile0_x, a.c0, a.c1, TW, a.s.ax, a.s.aw, a.ic0, a.ic1);
+        if (a.ir0 == a.ir1 || a.ic0 == a.ic1) a.ir0 = a.ir1 = a.ic0 = a.ic1 = 0;
         a.decisions = L.decisions;
         a.dpitch = decision_pitch((uint32_t)a.s.aw);
         const unsigned rec = (unsigned)((a.r1 - a.r0) * (a.c1 - a.c0));
