@@ -237,6 +237,7 @@ void compile_ops(rawops_plan& plan, const rawops_op_desc* ops, uint32_t n, OpLis
             plan.tables.push_back(dev);
             hip_check(hipMemcpy(dev, host.get(), sizeof(DevCurves), hipMemcpyHostToDevice), "hipMemcpy(curves)");
             o.curves = static_cast<const DevCurves*>(dev);
+            o.knots = host->c[0].n;
             plan.curve_cubic = d.cubic ? 1u : 0u;
             break;
         }
@@ -775,7 +776,8 @@ rawops_status rawops_curve_backward(const rawops_plan* plan, const float* x, con
         if (!plan || plan->reduced.n < 1 || plan->reduced.op[0].kind != OP_CURVES)
             fail(RAWOPS_INVALID_ARGUMENT, "curve backward needs a plan whose first reduced op is a curve");
         const DevCurves* t = plan->reduced.op[0].curves;
-        hip_check(launch_curve_backward(x, gy, n, &t->c[0], plan->curve_cubic, gx, gys, static_cast<hipStream_t>(stream)),
+        hip_check(launch_curve_backward(x, gy, n, &t->c[0], plan->reduced.op[0].knots, plan->curve_cubic, gx, gys,
+                                        static_cast<hipStream_t>(stream)),
                   "curve backward launch");
     });
 }

@@ -35,6 +35,7 @@ struct DevOp {
     uint32_t kind;
     float v[9];
     const DevCurves* curves;
+    uint32_t knots;          // curve ops: knot count of the master curve (host-side launch choices)
 };
 
 struct OpList {

@@ -71,11 +71,18 @@ __global__ void boost_bwd(const float* x, const float* base, const float* gy, fl
 // Bezier B(t) = sum_k b_k(t) P_k with P = (y_i, y_i + h T_i/3, y_i+1 - h T_i+1/3,
 // y_i+1); tangents T are the minmod of the adjacent secant slopes.  dL/dy_j
 // collects the Bernstein weights through P and T; dL/dx uses B'(t)/h.
-__global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevCurve* c, uint32_t cubic, float* gx,
-                          double* gys) {
-    __shared__ double acc[kMaxKnots];
-    __shared__ double tab[5 * kMaxKnots];   // the knot tables, staged: lookups are data-dependent
-    for (int k = threadIdx.x; k < kMaxKnots; k += kPW) acc[k] = 0.0;
+// LANE > 0 (curves of at most LANE knots): each thread accumulates into its
+// own LDS column, so values of one segment in a wavefront (common: the knots
+// are few) do not serialise on shared atomics; the columns are summed at the
+// end.  LANE == 0: shared LDS atomics.
+template <int LANE>
+__global__ void __launch_bounds__(kPW) curve_bwd(const float* xv, const float* gy, size_t n, const DevCurve* c,
+                                                 uint32_t cubic, float* gx, double* gys) {
+    constexpr int KT = LANE > 0 ? LANE : kMaxKnots;          // knots the tables hold
+    __shared__ double acc[LANE > 0 ? LANE * kPW : kMaxKnots];
+    __shared__ double tab[6 * KT];   // the knot tables (+ 1 / segment width), staged: lookups are data-dependent
+    __shared__ double part[LANE > 0 ? (kPW / 64) * LANE : 1];
+    for (int k = threadIdx.x; k < (LANE > 0 ? LANE * kPW : kMaxKnots); k += kPW) acc[k] = 0.0;
     const uint32_t nk = as_const(*c).n;
     for (uint32_t k = threadIdx.x; k < nk; k += kPW) {
         tab[k] = c->x[k];
@@ -83,10 +90,16 @@ __global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevC
         tab[2 * nk + k] = c->slope[k];
         tab[3 * nk + k] = c->c0[k];
         tab[4 * nk + k] = c->c1[k];
+        tab[5 * nk + k] = k + 1 < nk ? 1.0 / (c->x[k + 1] - c->x[k]) : 0.0;
     }
     __syncthreads();
+    auto add = [&](uint32_t k, double w) {
+        if constexpr (LANE > 0) acc[k * kPW + threadIdx.x] += w;
+        else atomicAdd(&acc[k], w);
+    };
     const LdsCurve l{nk, 0u, tab, tab + nk, tab + 2 * nk, tab + 3 * nk, tab + 4 * nk};
     const LdsCurve* cl = &l;
+    const double* inv_h = tab + 5 * nk;   // 1 / (x[k+1] - x[k]): gradients need no quotient rounding
     for (size_t e = blockIdx.x * (size_t)kPW + threadIdx.x; e < n; e += (size_t)gridDim.x * kPW) {
         const double v = xv[e], g = gy[e];
         uint32_t lo = 0, hi = nk;
@@ -96,9 +109,9 @@ __global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevC
         }
         const uint32_t left = lo == 0 ? 0u : lo - 1u;
         const uint32_t i = left < nk - 2u ? left : nk - 2u;
-        const double h = cl->x[i + 1] - cl->x[i];
+        const double h = cl->x[i + 1] - cl->x[i], ih = inv_h[i];
         if (v == cl->x[left]) {            // exact knot: value y_left
-            atomicAdd(&acc[left], g);
+            add(left, g);
             double slope = cl->slope[left < nk - 1 ? left : nk - 2];   // linear: right-hand secant
             if (cubic) {                                                // cubic: the knot tangent
                 slope = left < nk - 1 ? 3.0 * (cl->c0[left] - cl->y[left]) / (cl->x[left + 1] - cl->x[left])
@@ -110,37 +123,49 @@ __global__ void curve_bwd(const float* xv, const float* gy, size_t n, const DevC
         if (!cubic || v < cl->x[0] || v > cl->x[nk - 1]) {
             // y = y_left + (v - x_left) * s_i,  s_i = (y_i+1 - y_i) / h
             const double dv = v - cl->x[left];
-            atomicAdd(&acc[left], g);
-            atomicAdd(&acc[i + 1], g * dv / h);
-            atomicAdd(&acc[i], -g * dv / h);
+            add(left, g);
+            add(i + 1, g * dv * ih);
+            add(i, -g * dv * ih);
             gx[e] = (float)(g * cl->slope[i]);
             continue;
         }
-        const double t = (v - cl->x[i]) / h, u = 1.0 - t;
+        const double t = (v - cl->x[i]) * ih, u = 1.0 - t;
         const double b0 = u * u * u, b1 = 3 * u * u * t, b2 = 3 * u * t * t, b3 = t * t * t;
         const double P0 = cl->y[i], P1 = cl->c0[i], P2 = cl->c1[i], P3 = cl->y[i + 1];
-        gx[e] = (float)(g * 3.0 * (u * u * (P1 - P0) + 2 * u * t * (P2 - P1) + t * t * (P3 - P2)) / h);
-        atomicAdd(&acc[i], g * (b0 + b1));
-        atomicAdd(&acc[i + 1], g * (b2 + b3));
+        gx[e] = (float)(g * 3.0 * (u * u * (P1 - P0) + 2 * u * t * (P2 - P1) + t * t * (P3 - P2)) * ih);
+        add(i, g * (b0 + b1));
+        add(i + 1, g * (b2 + b3));
         // Tangent T_j -> secant slopes (minmod picks one of them, or none).
         auto tangent = [&](uint32_t j, double w) {   // w = dL/dT_j
-            if (j == 0) { atomicAdd(&acc[1], w / (cl->x[1] - cl->x[0])); atomicAdd(&acc[0], -w / (cl->x[1] - cl->x[0])); return; }
-            if (j == nk - 1) {
-                const double hh = cl->x[nk - 1] - cl->x[nk - 2];
-                atomicAdd(&acc[nk - 1], w / hh); atomicAdd(&acc[nk - 2], -w / hh); return;
-            }
+            if (j == 0) { add(1, w * inv_h[0]); add(0, -w * inv_h[0]); return; }
+            if (j == nk - 1) { add(nk - 1, w * inv_h[nk - 2]); add(nk - 2, -w * inv_h[nk - 2]); return; }
             const double a = cl->slope[j - 1], b = cl->slope[j];
             if (a == 0 || b == 0 || signbit(a) != signbit(b)) return;
             const uint32_t k = fabs(a) <= fabs(b) ? j - 1 : j;
-            const double hh = cl->x[k + 1] - cl->x[k];
-            atomicAdd(&acc[k + 1], w / hh);
-            atomicAdd(&acc[k], -w / hh);
+            add(k + 1, w * inv_h[k]);
+            add(k, -w * inv_h[k]);
         };
-        tangent(i, g * b1 * h / 3.0);
-        tangent(i + 1, -g * b2 * h / 3.0);
+        tangent(i, g * b1 * h * (1.0 / 3.0));
+        tangent(i + 1, -g * b2 * h * (1.0 / 3.0));
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < (int)nk; k += kPW) atomicAdd(gys + k, acc[k]);
+    if constexpr (LANE > 0) {
+        // sum the columns: per knot a wavefront reduction, then the waves
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (uint32_t k = 0; k < nk; ++k) {
+            double v = acc[k * kPW + threadIdx.x];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) part[wave * LANE + k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < nk) {
+            double t = 0;
+            for (int w = 0; w < kPW / 64; ++w) t += part[w * LANE + threadIdx.x];
+            atomicAdd(gys + threadIdx.x, t);
+        }
+    } else {
+        __syncthreads();
+        for (int k = threadIdx.x; k < (int)nk; k += kPW) atomicAdd(gys + k, acc[k]);
+    }
 }
 
 unsigned blocks_for(size_t n) { return (unsigned)std::min<size_t>((n + kPW - 1) / kPW, 8192); }
@@ -180,10 +205,14 @@ hipError_t launch_boost_backward(const float* x, const float* base, const float*
     hipLaunchKernelGGL(boost_bwd, dim3(reduce_blocks_for(n)), dim3(kPW), 0, s, x, base, gy, strength, gx, gbase, gs, n);
     return hipGetLastError();
 }
-hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t cubic,
+hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t knots,
+                                 uint32_t cubic,
                                  float* gx, double* gys, hipStream_t s) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(curve_bwd, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
+    if (knots <= 16)   // per-thread accumulator columns
+        hipLaunchKernelGGL(curve_bwd<16>, dim3(reduce_blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
+    else
+        hipLaunchKernelGGL(curve_bwd<0>, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
     return hipGetLastError();
 }
 

@@ -89,7 +89,8 @@ hipError_t launch_exposure_backward(const float* x, const float* gy, float m, fl
 hipError_t launch_boost(const float* x, const float* base, float strength, float* y, size_t n, hipStream_t s);
 hipError_t launch_boost_backward(const float* x, const float* base, const float* gy, float strength, float* gx,
                                  float* gbase, double* gs, size_t n, hipStream_t s);
-hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t cubic,
+hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t knots,
+                                 uint32_t cubic,
                                  float* gx, double* gys, hipStream_t s);
 
 }  // namespace rawops
