@@ -148,7 +148,6 @@ def render_record(image: RawImage, gains, stops, matrix, viewport: Rect = None):
     dec = torch.zeros(n.value, dtype=torch.uint8, device="cuda")
     C.check(C.lib().rawops_render_record(plan.handle, image._handle, vp.c(), _ptr(out), vp.width, None, _ptr(dec),
                                          _stream()))
-    torch.cuda.current_stream().synchronize()
     return out, dec
 
 
