@@ -16,7 +16,9 @@
 //                same float; 2 bytes a sample keep three workgroups per CU)
 //   gh     f64   tile +-4, non-green sites; overwritten in place by the
 //                selected green (classifier output)
-//   gv     f64   tile +-4, non-green sites; red at green sites afterwards
+//   gv     f64   tile +-4, non-green sites; red at green sites afterwards,
+//                or (SAMPLES) the normalised samples of the non-green sites
+//                of tile +-2, stored by the classifier, with red over dv
 //   dh     f64   x [-4, TW+2), y [-4, TH+4); blue at green sites afterwards
 //   dv     f64   x [-4, TW+4), y [-4, TH+2)
 //   hz     u8    classifier decision at the tile's non-green sites
@@ -29,7 +31,7 @@
 
 namespace rawops {
 
-template <int TW, int TH>
+template <int TW, int TH, bool SAMPLES = false>
 struct MenonFwdLds {
     static constexpr int RW = TW + 12, RH = TH + 12;     // raw, origin (-6, -6)
     static constexpr int NP = (TW + 8) / 2, NR = TH + 8; // gh / gv, origin (-4, -4)
@@ -43,15 +45,17 @@ struct MenonFwdLds {
     static constexpr int OFF_DV = OFF_DH + DHP * DHR * 8;
     static constexpr int OFF_HZ = OFF_DV + DVP * DVR * 8;
     static constexpr int BYTES = OFF_HZ + (TW / 2) * TH;
-    static constexpr int OFF_RED = OFF_GV, OFF_BLUE = OFF_DH;
+    static constexpr int OFF_RED = SAMPLES ? OFF_DV : OFF_GV, OFF_BLUE = OFF_DH;
     // Native-tile staging for mip reduction: every plane is dead by then.
     static constexpr int OFF_NATIVE = 0;
-    static_assert(GP * GR <= NP * NR && GP * GR <= DHP * DHR, "red/blue do not fit over gv/dh");
+    static_assert(GP * GR <= NP * NR && GP * GR <= DHP * DHR && GP * GR <= DVP * DVR,
+                  "red/blue do not fit over gv or dv / dh");
     static_assert(TW * TH * 12 <= BYTES, "native staging does not fit");
     static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles must cover whole Bayer cells");
     // The directional stage reads normalised FP32 samples staged over dh/dv,
-    // which the gradient stage only writes afterwards.
-    static constexpr int OFF_RAWF = OFF_DH;
+    // which the gradient stage only writes afterwards: two planes by column
+    // parity (RFP per row), so that the site-per-lane reads are contiguous.
+    static constexpr int OFF_RAWF = OFF_DH, RFP = RW / 2, RF_PLANE = RFP * RH;
     static_assert(OFF_DV == OFF_DH + DHP * DHR * 8 && RW * RH * 4 <= DHP * DHR * 8 + DVP * DVR * 8,
                   "normalised samples do not fit over dh/dv");
 };
@@ -86,16 +90,16 @@ struct SiteLevels {
     }
 };
 
-template <int TW, int TH, int NT, int N0, bool BORDER>
+template <int TW, int TH, int NT, int N0, bool BORDER, bool SAMPLES = false>
 struct MenonFwd {
-    using L = MenonFwdLds<TW, TH>;
+    using L = MenonFwdLds<TW, TH, SAMPLES>;
     static constexpr int RW = L::RW, NP = L::NP, DHP = L::DHP, DVP = L::DVP, GP = L::GP;
 
     const SensorArgs& s;
     int tx, ty;             // tile origin (sensor)
     uint32_t red_rows;      // bit b: the non-green sites of tile rows with parity b are red
     uint16_t* raw;
-    float* rawf;            // normalised samples, valid from load() until gradients() writes dh/dv
+    float* rawf;            // normalised samples by column parity, valid from load() until gradients() writes dh/dv
     double *gh, *gv, *dh, *dv, *red, *blue;
     uint8_t* hz;
     SiteLevels lv;          // levels by (row, column) parity relative to the tile origin
@@ -133,6 +137,9 @@ struct MenonFwd {
         }
     }
     __device__ __forceinline__ static int rawi(int rx, int ry) { return (ry + 6) * RW + rx + 6; }
+    __device__ __forceinline__ static int rfi(int rx, int ry) {
+        return ((rx + 6) & 1) * L::RF_PLANE + (ry + 6) * L::RFP + ((rx + 6) >> 1);
+    }
     __device__ __forceinline__ static int ni(int rx, int ry) { return (ry + 4) * NP + ((rx + 4) >> 1); }
     __device__ __forceinline__ static int gi(int rx, int ry) { return (ry + 2) * GP + ((rx + 2) >> 1); }
 
@@ -149,7 +156,7 @@ struct MenonFwd {
         auto put = [&](int qr, int qk, int b, int e, uint32_t code) {
             const int i = (2 * qr + b) * RW + 2 * qk + e;
             raw[i] = (uint16_t)code;
-            rawf[i] = lv.norm(code, b, e);
+            rawf[e * L::RF_PLANE + (2 * qr + b) * L::RFP + qk] = lv.norm(code, b, e);
         };
         if (pairs) {
             // Interior, 4-byte aligned cells: issue every load first, then convert.
@@ -173,8 +180,9 @@ struct MenonFwd {
                     for (int b = 0; b < 2; ++b) {   // both codes of the cell row in one 4-byte store
                         const int i = (2 * qr + b) * RW + 2 * qk;
                         *reinterpret_cast<uint32_t*>(raw + i) = w[j][b];
-                        const float2 f = {lv.norm(w[j][b] & 0xffffu, b, 0), lv.norm(w[j][b] >> 16, b, 1)};
-                        *reinterpret_cast<float2*>(rawf + i) = f;
+                        const int f = (2 * qr + b) * L::RFP + qk;
+                        rawf[f] = lv.norm(w[j][b] & 0xffffu, b, 0);
+                        rawf[L::RF_PLANE + f] = lv.norm(w[j][b] >> 16, b, 1);
                     }
                 }
             }
@@ -198,10 +206,17 @@ struct MenonFwd {
 
     // ----------------------------------------------------- directional --
     // Directional greens at non-green sites, tile +-4 (MenonDemosaic.cpp:107-119).
-    __device__ __forceinline__ void directional() {
+    // The samples the gradient stage reads (own, x + 2, y + 2) can be kept
+    // in registers across the barrier (Carry): the staged floats lie over
+    // dh / dv, which that stage writes.  Both stages walk the same items.
+    static constexpr int JD = (NP * ((TH + 8) / 2) + NT - 1) / NT;
+    struct Carry { float own[JD][2], h2[JD][2], v2[JD][2]; };
+
+    __device__ __forceinline__ void directional() { Carry c; directional(c); }
+    __device__ __forceinline__ void directional(Carry& c) {
         constexpr int QK = NP, QR = (TH + 8) / 2, N = QK * QR;
 #pragma unroll
-        for (int j = 0; j < (N + NT - 1) / NT; ++j) {
+        for (int j = 0; j < JD; ++j) {
             const int it = (int)threadIdx.x + j * NT;
             if (it >= N) break;
             const int qr = it / QK, k = it - qr * QK;
@@ -209,19 +224,25 @@ struct MenonFwd {
             for (int b = 0; b < 2; ++b) {
                 const int ry = -4 + 2 * qr + b, rx = -4 + 2 * k + nx(b);
                 if (!inside(rx, ry)) continue;
-                const int r0 = rawi(rx, ry);
-                auto R = [&](int ox, int oy) -> double {
+                const int f0 = (ry + 6) * L::RFP + k;   // rx + 6 = 2 k + 2 + nx(b)
+                auto F = [&](int ox, int oy) -> float {
                     if constexpr (BORDER) {
                         int x = rx + ox, y = ry + oy;
                         mirror(x, y);
-                        return (double)rawf[rawi(x, y)];
+                        return rawf[rfi(x, y)];
                     } else {
-                        return (double)rawf[r0 + oy * RW + ox];
+                        const int nb = nx(b);
+                        return rawf[((nb + ox) & 1) * L::RF_PLANE + f0 + oy * L::RFP + ((nb + 2 + ox) >> 1)];
                     }
                 };
-                const double own = R(0, 0);
-                const double h = 0.5 * (R(-1, 0) + R(1, 0)) + 0.5 * own - 0.25 * (R(-2, 0) + R(2, 0));
-                const double v = 0.5 * (R(0, -1) + R(0, 1)) + 0.5 * own - 0.25 * (R(0, -2) + R(0, 2));
+                auto R = [&](int ox, int oy) -> double { return (double)F(ox, oy); };
+                const float fo = F(0, 0), fh = F(2, 0), fv = F(0, 2);
+                c.own[j][b] = fo;
+                c.h2[j][b] = fh;
+                c.v2[j][b] = fv;
+                const double own = fo;
+                const double h = 0.5 * (R(-1, 0) + R(1, 0)) + 0.5 * own - 0.25 * (R(-2, 0) + (double)fh);
+                const double v = 0.5 * (R(0, -1) + R(0, 1)) + 0.5 * own - 0.25 * (R(0, -2) + (double)fv);
                 const int n0 = (2 * qr + b) * NP + k;
                 gh[n0] = h;
                 gv[n0] = v;
@@ -231,11 +252,16 @@ struct MenonFwd {
 
     // ------------------------------------------------------- gradients --
     // dh = |ch - ch(x+2)|, dv = |cv - cv(y+2)| with c = own - directional
-    // green (MenonDemosaic.cpp:117-125).
-    __device__ __forceinline__ void gradients() {
+    // green (MenonDemosaic.cpp:117-125); samples from the directional
+    // stage's Carry, or (without one) normalised again from the codes.
+    __device__ __forceinline__ void gradients() { gradients_impl<false>(nullptr); }
+    __device__ __forceinline__ void gradients(const Carry& c) { gradients_impl<true>(&c); }
+
+    template <bool CARRIED>
+    __device__ __forceinline__ void gradients_impl(const Carry* c) {
         constexpr int QK = NP, QR = (TH + 8) / 2, N = QK * QR;
 #pragma unroll
-        for (int j = 0; j < (N + NT - 1) / NT; ++j) {
+        for (int j = 0; j < JD; ++j) {
             const int it = (int)threadIdx.x + j * NT;
             if (it >= N) break;
             const int qr = it / QK, k = it - qr * QK;
@@ -245,15 +271,15 @@ struct MenonFwd {
                 const bool do_h = k < DHP, do_v = 2 * qr + b < TH + 6;
                 if (!inside(rx, ry)) continue;
                 const int r0 = rawi(rx, ry), n0 = (2 * qr + b) * NP + k, nb = nx(b);
-                const double own = (double)Rn(r0, b, nb);
+                const double own = CARRIED ? (double)c->own[j][b] : (double)Rn(r0, b, nb);
                 if (do_h) {
                     double there;
                     if constexpr (BORDER) {
                         int x = rx + 2, y = ry;
                         mirror(x, y);
-                        there = (double)Rn(rawi(x, y), b, nb) - gh[ni(x, y)];
+                        there = (CARRIED ? (double)c->h2[j][b] : (double)Rn(rawi(x, y), b, nb)) - gh[ni(x, y)];
                     } else {
-                        there = (double)Rn(r0 + 2, b, nb) - gh[n0 + 1];
+                        there = (CARRIED ? (double)c->h2[j][b] : (double)Rn(r0 + 2, b, nb)) - gh[n0 + 1];
                     }
                     dh[(2 * qr + b) * DHP + k] = fabs((own - gh[n0]) - there);
                 }
@@ -262,9 +288,9 @@ struct MenonFwd {
                     if constexpr (BORDER) {
                         int x = rx, y = ry + 2;
                         mirror(x, y);
-                        there = (double)Rn(rawi(x, y), b, nb) - gv[ni(x, y)];
+                        there = (CARRIED ? (double)c->v2[j][b] : (double)Rn(rawi(x, y), b, nb)) - gv[ni(x, y)];
                     } else {
-                        there = (double)Rn(r0 + 2 * RW, b, nb) - gv[n0 + 2 * NP];
+                        there = (CARRIED ? (double)c->v2[j][b] : (double)Rn(r0 + 2 * RW, b, nb)) - gv[n0 + 2 * NP];
                     }
                     dv[(2 * qr + b) * DVP + k] = fabs((own - gv[n0]) - there);
                 }
@@ -284,10 +310,14 @@ struct MenonFwd {
             const int it = (int)threadIdx.x + j * NT;
             if (it >= N) break;
             const int qr = it / QK, k = it - qr * QK;
+            // Both sites' sums before any store: the two windows share four
+            // of their eight dh and dv reads.
+            bool hor[2];
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
                 const int nb = nx(b);
                 const int ry = -2 + 2 * qr + b, rx = -2 + 2 * k + nb;
+                hor[b] = false;
                 if (!inside(rx, ry)) continue;
                 const int h0 = (2 * qr + 2 + b) * DHP + k, v0 = (2 * qr + 2 + b) * DVP + k;
                 auto DH = [&](int ox, int oy) -> double {
@@ -314,9 +344,18 @@ struct MenonFwd {
                 h += DH(-1, -1);       v += DV(-1, -1);
                 h += DH(0, -2);        v += DV(-2, 0);
                 h += DH(-2, -2);       v += DV(-2, -2);
-                const bool horizontal = v >= h;
+                hor[b] = v >= h;
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int nb = nx(b);
+                const int ry = -2 + 2 * qr + b, rx = -2 + 2 * k + nb;
+                if (!inside(rx, ry)) continue;
+                const bool horizontal = hor[b];
                 const int n0 = (2 * qr + 2 + b) * NP + k + 1;
                 gh[n0] = horizontal ? gh[n0] : gv[n0];
+                // Only this site reads gv here: it takes the site's sample for the chroma stage.
+                if constexpr (SAMPLES) gv[n0] = (double)Rn(rawi(rx, ry), b, nb);
                 if (rx >= 0 && rx < TW && ry >= 0 && ry < TH) hz[ry * (TW / 2) + (rx >> 1)] = horizontal ? 1 : 0;
             }
         }
@@ -325,7 +364,9 @@ struct MenonFwd {
     // ---------------------------------------------------------- chroma --
     // Red and blue at green sites, tile +-1 (MenonDemosaic.cpp:148-161):
     // c = green + mean(raw) - mean(green) along the row (cH) and the column
-    // (cV); red takes the direction of the red neighbours.
+    // (cV); red takes the direction of the red neighbours.  The four
+    // neighbours are non-green sites (of tile +-2): with SAMPLES their
+    // samples come from gv.
     __device__ __forceinline__ void chroma() {
         constexpr int QK = (TW + 2) / 2, QR = (TH + 2) / 2, N = QK * QR;
 #pragma unroll
@@ -343,6 +384,15 @@ struct MenonFwd {
                 const int g0 = (2 * qr + 3 + b) * NP + k;   // row of this site in the gh/green plane
                 auto R = [&](int ox, int oy) -> double {
                     const int py = (p + oy) & 1, px = (1 - np + ox) & 1;   // rx = -1 + 2k + np
+                    if (SAMPLES && (ox != 0 || oy != 0)) {
+                        if constexpr (BORDER) {
+                            int x = rx + ox, y = ry + oy;
+                            mirror(x, y);
+                            return gv[ni(x, y)];
+                        } else {
+                            return gv[g0 + oy * NP + ((3 + np + ox) >> 1)];
+                        }
+                    }
                     if constexpr (BORDER) {
                         int x = rx + ox, y = ry + oy;
                         mirror(x, y);
@@ -412,7 +462,7 @@ struct MenonFwd {
                     }
                     const double mr = 0.5 * (red[ia] + red[ic]);
                     const double mb = 0.5 * (blue[ia] + blue[ic]);
-                    const float self = Rn(rawi(xn, ry), b, nb);
+                    const float self = SAMPLES ? (float)gv[(2 * qy + 4 + b) * NP + qx + 2] : Rn(rawi(xn, ry), b, nb);
                     const double own = (double)self;
                     const bool red_site = (red_rows >> b) & 1u;
                     // R site: bb = rr + mean(blue) - mean(red); B site: rr = bb + mean(red) - mean(blue).

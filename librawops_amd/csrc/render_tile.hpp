@@ -61,7 +61,7 @@ struct BilinearLds {
 
 template <int DEMOSAIC, int TW, int TH>
 struct Lds;
-template <int TW, int TH> struct Lds<RAWOPS_DEMOSAIC_MENON, TW, TH> { static constexpr int BYTES = MenonFwdLds<TW, TH>::BYTES; static constexpr int OFF_NATIVE = MenonFwdLds<TW, TH>::OFF_NATIVE; };
+template <int TW, int TH> struct Lds<RAWOPS_DEMOSAIC_MENON, TW, TH> { static constexpr int BYTES = MenonFwdLds<TW, TH, true>::BYTES; static constexpr int OFF_NATIVE = MenonFwdLds<TW, TH, true>::OFF_NATIVE; };
 template <int TW, int TH> struct Lds<RAWOPS_DEMOSAIC_BILINEAR, TW, TH> { static constexpr int BYTES = BilinearLds<TW, TH>::BYTES; static constexpr int OFF_NATIVE = BilinearLds<TW, TH>::OFF_NATIVE; };
 
 // Level pixels per tile side and native tile origin.
@@ -89,12 +89,14 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
     for (int p = 0; p < NPX; ++p) err[p] = 0;
 
     if constexpr (DEMOSAIC == RAWOPS_DEMOSAIC_MENON) {
-        MenonFwd<TW, TH, NT, N0, BORDER> t(s, lds, tx, ty);
+        using F = MenonFwd<TW, TH, NT, N0, BORDER, true>;
+        F t(s, lds, tx, ty);
         t.load();
         __syncthreads();
-        t.directional();
+        typename F::Carry carry;
+        t.directional(carry);
         __syncthreads();
-        t.gradients();
+        t.gradients(carry);
         __syncthreads();
         t.classify();
         __syncthreads();
@@ -102,7 +104,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
             // record the tile's decisions (inside the viewport) for the adjoint
             for (int i = threadIdx.x; i < (TW / 2) * TH; i += NT) {
                 const int ry = i / (TW / 2), k = i - ry * (TW / 2);
-                const int rx = 2 * k + MenonFwd<TW, TH, NT, N0, BORDER>::nx(ry & 1);
+                const int rx = 2 * k + F::nx(ry & 1);
                 const int sx = tx + rx, sy = ty + ry;
                 if (sx < a.vx + a.vw && sy < a.vy + a.vh && in_area(s, sx, sy))
                     a.decisions[(size_t)(sy - s.ay) * a.dpitch + (size_t)((sx - s.ax) >> 1)] = t.hz[i];
