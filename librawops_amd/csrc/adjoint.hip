@@ -78,12 +78,35 @@ __device__ __forceinline__ void param_grads(const float d[3], const float go[3],
         for (int c = 0; c < 3; ++c) acc[r * 3 + c] += (double)go[r] * (double)d[c];
 }
 
+// x plus the value DPP control CTRL moves into this lane (rows outside
+// ROW_MASK, and lanes without a source, add zero).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    return x + __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the wave's 64 lanes, complete in lane 63: pairs and quads
+// (quad permutes), eight and sixteen lanes (row half mirror, row mirror),
+// then rows 1 and 3 take row 0 and 2 (row broadcast 15) and rows 2 and 3
+// take rows 0 + 1 (row broadcast 31).  Register moves only.
+__device__ __forceinline__ double wave_sum63(double x) {
+    x = dpp_add<0xb1, 0xf>(x);    // quad_perm [1, 0, 3, 2]
+    x = dpp_add<0x4e, 0xf>(x);    // quad_perm [2, 3, 0, 1]
+    x = dpp_add<0x141, 0xf>(x);   // row_half_mirror
+    x = dpp_add<0x140, 0xf>(x);   // row_mirror
+    x = dpp_add<0x142, 0xa>(x);   // row_bcast:15
+    x = dpp_add<0x143, 0xc>(x);   // row_bcast:31
+    return x;
+}
+
 __device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc)[9], double* red /* [waves][9] LDS */) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i)
-        for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc[i] = wave_sum63(acc[i]);
+    if (lane == 63)
         for (int i = 0; i < 9; ++i) red[wave * 9 + i] = acc[i];
     __syncthreads();
     if (threadIdx.x < 13) {
@@ -183,6 +206,9 @@ struct Upstream {
     uint32_t hasN, hasG;    // bit 2j + b
 };
 
+// Branch-free: every load is issued, from a position clamped into the
+// viewport (and the last item), so that the loads stay in flight together;
+// the has-bits say which are real.
 template <int EW, int EH, int N0, int J>
 __device__ __forceinline__ void fetch_upstream(const BackwardArgs& a, int ex, int ey, Upstream<J>& up) {
     constexpr int HWE = EW / 2, NI = HWE * (EH / 2);
@@ -190,14 +216,22 @@ __device__ __forceinline__ void fetch_upstream(const BackwardArgs& a, int ex, in
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int it = (int)threadIdx.x + j * kNT;
-        if (it < NI) {
-            const int qr = it / HWE, k = it - qr * HWE;
+        const int ic = min(it, NI - 1);
+        const int qr = ic / HWE, k = ic - qr * HWE;
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int v = 2 * qr + b, un = 2 * k + ng_x<N0>(b);
-                if (load_gout(a, ex + un, ey + v, up.goN[j][b])) up.hasN |= 1u << (2 * j + b);
-                if (load_gout(a, ex + (un ^ 1), ey + v, up.goG[j][b])) up.hasG |= 1u << (2 * j + b);
-            }
+        for (int b = 0; b < 2; ++b) {
+            const int v = 2 * qr + b, un = 2 * k + ng_x<N0>(b);
+            const int ly = ey + v - a.vy, lyc = min(max(ly, 0), a.vh - 1);
+            const bool row = it < NI && ly == lyc;
+            const float* prow = a.gout + (size_t)lyc * a.gstride * 3;
+            auto fetch = [&](int u, float go[3]) -> bool {
+                const int lx = ex + u - a.vx, lxc = min(max(lx, 0), a.vw - 1);
+                const float* p = prow + (size_t)lxc * 3;
+                go[0] = p[0]; go[1] = p[1]; go[2] = p[2];
+                return row && lx == lxc;
+            };
+            if (fetch(un, up.goN[j][b])) up.hasN |= 1u << (2 * j + b);
+            if (fetch(un ^ 1, up.goG[j][b])) up.hasG |= 1u << (2 * j + b);
         }
     }
 }
@@ -660,27 +694,56 @@ __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, 
         return green[g2(x, y)];
     };
 
-    // codes of the extended tile, its recorded decisions, upstream gradients
+    // Codes of the extended tile and its recorded decisions: every load
+    // issued (unconditionally, at clamped in-bounds addresses) before any is
+    // consumed.  The upstream gradients are fetched after the LDS stores, so
+    // they stay in flight through the recompute below.
     {
+        constexpr int NW = EW * EH / 2, JW = (NW + kNT - 1) / kNT, JD = (NC + kNT - 1) / kNT;
         const int64_t stride = s.stride;
         const uint16_t* base = s.samples + ((int64_t)ey * stride + ex);
         const bool pairs = !BORDER && ((reinterpret_cast<uintptr_t>(base) | (uintptr_t)(stride * 2)) & 3) == 0;
-        for (int i = threadIdx.x; i < EW * EH / 2; i += kNT) {
-            const int v = i / (EW / 2), k = i - v * (EW / 2);
-            const uint16_t* p = base + (int64_t)v * stride + 2 * k;
-            if (pairs) {
-                *reinterpret_cast<uint32_t*>(raw + v * EW + 2 * k) = *reinterpret_cast<const uint32_t*>(p);
-            } else {
+        uint32_t w[JW], dec[JD];
+        if (pairs) {
+#pragma unroll
+            for (int j = 0; j < JW; ++j) {
+                const int i = min((int)threadIdx.x + j * kNT, NW - 1);
+                const int v = i / (EW / 2), k = i - v * (EW / 2);
+                // 32-bit offsets (24-bit products) from the uniform base
+                w[j] = *reinterpret_cast<const uint32_t*>(base + (__umul24((uint32_t)v, (uint32_t)stride) + 2u * k));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < JD; ++j) {
+            const int i = min((int)threadIdx.x + j * kNT, NC - 1);
+            const int v = i / HWE, u = 2 * (i - v * HWE) + ng_x<N0>(v & 1);
+            const int sx = min(max(ex + u, s.ax), s.ax + s.aw - 1), sy = min(max(ey + v, s.ay), s.ay + s.ah - 1);
+            dec[j] = a.decisions[__umul24((uint32_t)(sy - s.ay), a.dpitch) + (uint32_t)((sx - s.ax) >> 1)];
+        }
+        if (pairs) {
+#pragma unroll
+            for (int j = 0; j < JW; ++j) {
+                const int i = (int)threadIdx.x + j * kNT;
+                const int v = i / (EW / 2), k = i - v * (EW / 2);
+                if (i < NW) *reinterpret_cast<uint32_t*>(raw + v * EW + 2 * k) = w[j];
+            }
+        } else {
+            for (int i = threadIdx.x; i < NW; i += kNT) {
+                const int v = i / (EW / 2), k = i - v * (EW / 2);
+                const uint16_t* p = base + (int64_t)v * stride + 2 * k;
                 if (inside(2 * k, v)) raw[v * EW + 2 * k] = p[0];
                 if (inside(2 * k + 1, v)) raw[v * EW + 2 * k + 1] = p[1];
             }
         }
-        for (int i = threadIdx.x; i < NC; i += kNT) {
+#pragma unroll
+        for (int j = 0; j < JD; ++j) {
+            const int i = (int)threadIdx.x + j * kNT;
             const int v = i / HWE, u = 2 * (i - v * HWE) + ng_x<N0>(v & 1);
-            if (inside(u, v)) hz[i] = a.decisions[(size_t)(ey + v - s.ay) * a.dpitch + (size_t)((ex + u - s.ax) >> 1)];
+            if (i < NC && inside(u, v)) hz[i] = (uint8_t)dec[j];
         }
     }
     Upstream<J> up;
+    __builtin_amdgcn_sched_barrier(0);
     if constexpr (!BORDER) fetch_upstream<EW, EH, N0>(a, ex, ey, up);
     __syncthreads();
     // selected green at non-green sites of tile +-2 (MenonDemosaic.cpp:107-119, 141-147)
