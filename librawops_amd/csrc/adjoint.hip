@@ -102,6 +102,20 @@ __device__ __forceinline__ double wave_sum63(double x) {
     return x;
 }
 
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// The workgroup's nine sums P[r][c] = sum gout_r d_c -> the 13 parameter
+// gradients (added to a.gparams):
+//   k < 3:   m sum_r M[r][k] P[r][k]             (dL/d gain k)
+//   k == 3:  sum_c g_c sum_r M[r][c] P[r][c]     (dL/d exposure multiplier)
+//   k >= 4:  m g_c P[k - 4], c = (k - 4) mod 3   (dL/d M)
+// Per wave by DPP, then across waves in wave 0 (its lanes 0..8 sum one P
+// each over the waves, in wave order; the P are then broadcast).
 __device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc)[9], double* red /* [waves][9] LDS */) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -109,27 +123,38 @@ __device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc
     if (lane == 63)
         for (int i = 0; i < 9; ++i) red[wave * 9 + i] = acc[i];
     __syncthreads();
-    if (threadIdx.x < 13) {
-        double P[9];
-        for (int i = 0; i < 9; ++i) {
-            double t = 0;
-            for (int w = 0; w < kNT / 64; ++w) t += red[w * 9 + i];
-            P[i] = t;
-        }
-        const int k = threadIdx.x;
-        double v;
-        if (k < 3) {
-            v = (double)a.m * ((double)a.M[k] * P[k] + (double)a.M[3 + k] * P[3 + k] + (double)a.M[6 + k] * P[6 + k]);
-        } else if (k == 3) {
-            v = 0;
-            for (int c = 0; c < 3; ++c)
-                v += (double)a.g[c] * ((double)a.M[c] * P[c] + (double)a.M[3 + c] * P[3 + c] + (double)a.M[6 + c] * P[6 + c]);
-        } else {
-            const int c = (k - 4) % 3;
-            v = (double)a.m * (double)a.g[c] * P[k - 4];
-        }
-        atomicAdd(a.gparams + k, v);
+    if (wave != 0) return;
+    double t = 0;
+    if (lane < 9) {
+#pragma unroll
+        for (int w = 0; w < kNT / 64; ++w) t += red[w * 9 + lane];
     }
+    double P[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) P[i] = readlane_f64(t, i);
+    const double m = (double)a.m;
+    double S[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        S[c] = (double)a.M[c] * P[c] + (double)a.M[3 + c] * P[3 + c] + (double)a.M[6 + c] * P[6 + c];
+    if (lane >= 13) return;
+    const int k = lane;
+    double v;
+    if (k < 3) {
+        v = m * (k == 0 ? S[0] : k == 1 ? S[1] : S[2]);
+    } else if (k == 3) {
+        v = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v += (double)a.g[c] * S[c];
+    } else {
+        const int c = (k - 4) % 3;
+        const double g = c == 0 ? (double)a.g[0] : c == 1 ? (double)a.g[1] : (double)a.g[2];
+        double pk = P[0];
+#pragma unroll
+        for (int i = 1; i < 9; ++i) pk = (k - 4 == i) ? P[i] : pk;
+        v = m * g * pk;
+    }
+    atomicAdd(a.gparams + k, v);
 }
 
 // Border-first tile order: the tiles outside the interior block [r0, r1) x
