@@ -364,8 +364,8 @@ def main():
     mm = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(compiled_matrix(node), np.float32)])
     sp = ctypes.c_void_p(stream.cuda_stream)
 
-    # The training forward records the classifier decisions (one byte per
-    # non-green site, 12 MB here) so the adjoint reads them instead of
+    # The training forward records the classifier decisions (one bit per
+    # non-green site, 1.5 MB here) so the adjoint reads them instead of
     # recomputing the FP64 classifier.
     nbytes = ctypes.c_size_t()
     C.check(C.lib().rawops_menon_decisions_size(img._handle, ctypes.byref(nbytes)))

@@ -225,13 +225,15 @@ rawops_status rawops_render_backward(const rawops_source* source, uint32_t demos
                                      double* g_params, void* stream);
 
 /* Menon classifier decisions recorded by the forward for the adjoint (no
- * reference counterpart).  A decision plane holds one byte per non-green
- * photosite of the active area, at plane[y * ((active_width + 1) / 2) + x / 2]
- * for area-relative (x, y); 1 = horizontal.  Its size in bytes: */
+ * reference counterpart).  A decision record of a viewport (vx, vy, vw, vh)
+ * holds one bit per non-green photosite, 1 = horizontal: the site at sensor
+ * (x, y) is bit ((x - vx) / 2) % 32 of the 32-bit word
+ * (y - vy) * ((vw + 63) / 64) + (x - vx) / 64 (device memory, 4-byte
+ * aligned).  The size in bytes that holds the record of any viewport: */
 rawops_status rawops_menon_decisions_size(const rawops_source* source, size_t* bytes);
 /* rawops_render of a Menon plan at mip 0 (FP32 output only) that also writes
- * the decision of every non-green photosite of `viewport` into `decisions`
- * (device).  Output bits equal rawops_render's. */
+ * the decision record of `viewport` into `decisions` (device, every word of
+ * the viewport's record is written).  Output bits equal rawops_render's. */
 rawops_status rawops_render_record(const rawops_plan* plan, const rawops_source* source, rawops_rect viewport,
                                    float* dst_rgb, uint32_t dst_stride_pixels, uint32_t* device_flags,
                                    uint8_t* decisions, void* stream);

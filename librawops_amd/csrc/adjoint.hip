@@ -40,8 +40,9 @@ struct BackwardArgs {
     int tiles_x, tiles_y;      // tiles of the launch
     int tile0_x, tile0_y;      // first tile (in tiles from the area origin)
     int r0, r1, c0, c1;        // launch-relative tile rows / columns of the interior block
-    const uint8_t* decisions;  // recorded Menon decisions (area plane, dpitch bytes a row) or null
-    uint32_t dpitch;
+    const uint32_t* decisions; // recorded Menon decisions (bits, relative to the recorded viewport) or null
+    uint32_t dwords;           // decision words per recorded row
+    int rx, ry, rw, rh;        // the recorded viewport
     int ir0, ir1, ic0, ic1;    // within the recorded block: its tiles that need no mirroring
 };
 
@@ -740,10 +741,13 @@ __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, 
         }
 #pragma unroll
         for (int j = 0; j < JD; ++j) {
+            // the word holding the site's bit (clamped into the record: sites
+            // outside the area are not read)
             const int i = min((int)threadIdx.x + j * kNT, NC - 1);
             const int v = i / HWE, u = 2 * (i - v * HWE) + ng_x<N0>(v & 1);
-            const int sx = min(max(ex + u, s.ax), s.ax + s.aw - 1), sy = min(max(ey + v, s.ay), s.ay + s.ah - 1);
-            dec[j] = a.decisions[__umul24((uint32_t)(sy - s.ay), a.dpitch) + (uint32_t)((sx - s.ax) >> 1)];
+            const int sx = min(max(ex + u, a.rx), a.rx + a.rw - 1), sy = min(max(ey + v, a.ry), a.ry + a.rh - 1);
+            dec[j] = a.decisions[__umul24((uint32_t)(sy - a.ry), a.dwords) + (uint32_t)((sx - a.rx) >> 6)] >>
+                     (((sx - a.rx) >> 1) & 31);
         }
         if (pairs) {
 #pragma unroll
@@ -764,7 +768,7 @@ __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, 
         for (int j = 0; j < JD; ++j) {
             const int i = (int)threadIdx.x + j * kNT;
             const int v = i / HWE, u = 2 * (i - v * HWE) + ng_x<N0>(v & 1);
-            if (i < NC && inside(u, v)) hz[i] = (uint8_t)dec[j];
+            if (i < NC && inside(u, v)) hz[i] = (uint8_t)(dec[j] & 1u);
         }
     }
     Upstream<J> up;
@@ -988,7 +992,9 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
         unmirrored(a.tile0_x, a.c0, a.c1, TW, a.s.ax, a.s.aw, a.ic0, a.ic1);
         if (a.ir0 == a.ir1 || a.ic0 == a.ic1) a.ir0 = a.ir1 = a.ic0 = a.ic1 = 0;
         a.decisions = L.decisions;
-        a.dpitch = decision_pitch((uint32_t)a.s.aw);
+        a.dwords = decision_words(L.recorded.width);
+        a.rx = (int)L.recorded.x; a.ry = (int)L.recorded.y;
+        a.rw = (int)L.recorded.width; a.rh = (int)L.recorded.height;
         const unsigned rec = (unsigned)((a.r1 - a.r0) * (a.c1 - a.c0));
         if (rec) {
             hipLaunchKernelGGL((menon_backward_recorded<64, 20>), dim3(rec), dim3(kNT), 0, stream, a);

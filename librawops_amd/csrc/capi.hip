@@ -446,7 +446,7 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* src, r
 rawops_status rawops_menon_decisions_size(const rawops_source* src, size_t* bytes) {
     return guarded([&] {
         if (!src || !src->is_raw || !bytes) fail(RAWOPS_INVALID_ARGUMENT, "decisions need a RAW source");
-        *bytes = (size_t)decision_pitch(src->meta.active_area.width) * src->meta.active_area.height;
+        *bytes = (size_t)decision_words(src->meta.active_area.width) * 4 * src->meta.active_area.height;
     });
 }
 
@@ -457,6 +457,7 @@ rawops_status rawops_render_record(const rawops_plan* plan, const rawops_source*
         validate_render(plan, src, viewport, 0, RAWOPS_QUALITY_FINAL);
         if (plan->demosaic != RAWOPS_DEMOSAIC_MENON) fail(RAWOPS_INVALID_ARGUMENT, "decisions are recorded by Menon renders");
         if (!decisions) fail(RAWOPS_INVALID_ARGUMENT, "decision plane is required");
+        if (reinterpret_cast<uintptr_t>(decisions) & 3) fail(RAWOPS_INVALID_ARGUMENT, "decision plane must be 4-byte aligned");
         if (!dst_stride) dst_stride = viewport.width;
         if (dst_stride < viewport.width) fail(RAWOPS_INVALID_ARGUMENT, "output stride is smaller than the viewport");
         if (!viewport.width || !viewport.height) return;
@@ -470,7 +471,7 @@ rawops_status rawops_render_record(const rawops_plan* plan, const rawops_source*
         L.out = dst;
         L.ostride = dst_stride;
         L.flags = flags;
-        L.decisions = decisions;
+        L.decisions = reinterpret_cast<uint32_t*>(decisions);
         hip_check(launch_render(L, plan->device, plan->device + 1, static_cast<hipStream_t>(stream)), "render launch");
     });
 }
@@ -716,6 +717,7 @@ rawops_status rawops_render_backward_recorded(const rawops_source* src, rawops_r
     return guarded([&] {
         if (!src || !src->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "backward needs a RAW source");
         if (!gains || !matrix || !g_out || !g_params || !decisions) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        if (reinterpret_cast<uintptr_t>(decisions) & 3) fail(RAWOPS_INVALID_ARGUMENT, "decision plane must be 4-byte aligned");
         inside_or_throw(src->meta.active_area, viewport);
         inside_or_throw(src->meta.active_area, recorded);
         if (src->meta.active_area.width <= 1 || src->meta.active_area.height <= 1)
@@ -732,7 +734,7 @@ rawops_status rawops_render_backward_recorded(const rawops_source* src, rawops_r
         L.g_stride = g_stride;
         L.g_raw = g_raw;
         L.g_params = g_params;
-        L.decisions = decisions;
+        L.decisions = reinterpret_cast<const uint32_t*>(decisions);
         L.recorded = recorded;
         hip_check(launch_render_backward(L, static_cast<hipStream_t>(stream)), "backward launch");
     });

@@ -18,14 +18,16 @@ struct RenderLaunch {
     uint8_t* out8;
     uint32_t ostride;
     uint32_t* flags;
-    uint8_t* decisions;      // Menon mip 0: record classifier decisions (may be null)
+    uint32_t* decisions;     // Menon mip 0: record classifier decisions (may be null)
 };
 
 // native / reduced / post point at device-resident OpLists (owned by a plan).
 
-// Bytes per row of a Menon decision plane: one byte per two columns of the
-// active area (the non-green site of each column pair).
-inline uint32_t decision_pitch(uint32_t active_width) { return (active_width + 1) / 2; }
+// Menon decision record of a viewport (vx, vy, vw, vh): one bit per non-green
+// photosite, 1 = horizontal; the site at (x, y) is bit ((x - vx) >> 1) & 31
+// of 32-bit word (y - vy) decision_words(vw) + ((x - vx) >> 6) (a 64-pixel
+// column span of a row per word).
+inline uint32_t decision_words(uint32_t width) { return (width + 63) / 64; }
 hipError_t launch_render(const RenderLaunch& launch, const OpList* native, const OpList* reduced,
                          hipStream_t stream);
 
@@ -39,7 +41,7 @@ struct BackwardLaunch {
     uint32_t g_stride;       // pixels per g_out row
     float* g_raw;            // dL/d(normalised raw), active-area plane (written in full)
     double* g_params;        // 13 accumulated: gains[3], multiplier, matrix[9]
-    const uint8_t* decisions;  // Menon decisions recorded by a forward over `recorded` (may be null)
+    const uint32_t* decisions;  // Menon decisions recorded by a forward over `recorded` (may be null)
     rawops_rect recorded;
 };
 

@@ -24,9 +24,10 @@ struct RenderArgs {
     uint32_t ostride;        // output row stride in pixels
     uint32_t* flags;         // may be null
     uint32_t tap;            // reduced ops applied before the FP32 output; the rest precede the u8 output
-    uint8_t* decisions;      // Menon mip 0: classifier decision per non-green site of the viewport
-                             // (area-relative, index y dpitch + x / 2, 1 = horizontal); may be null
-    uint32_t dpitch;
+    uint32_t* decisions;     // Menon mip 0: classifier decisions of the viewport, one bit per
+                             // non-green site (viewport-relative: word (y - vy) dwords + (x - vx) / 64,
+                             // bit ((x - vx) / 2) mod 32, 1 = horizontal); may be null
+    uint32_t dwords;         // decision words per viewport row
 };
 
 // Tile-relative x parity of the non-green sites in even tile rows for a
@@ -101,13 +102,20 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         t.classify();
         __syncthreads();
         if (a.decisions && a.mip == 0) {
-            // record the tile's decisions (inside the viewport) for the adjoint
-            for (int i = threadIdx.x; i < (TW / 2) * TH; i += NT) {
+            // Record the tile's decisions for the adjoint: a tile row's 32
+            // non-green sites are one word (half a wave's ballot).
+            static_assert(TW == 64 && ((TW / 2) * TH) % NT == 0, "one decision word per tile row");
+#pragma unroll
+            for (int j = 0; j < (TW / 2) * TH / NT; ++j) {
+                const int i = (int)threadIdx.x + j * NT;
                 const int ry = i / (TW / 2), k = i - ry * (TW / 2);
-                const int rx = 2 * k + F::nx(ry & 1);
-                const int sx = tx + rx, sy = ty + ry;
-                if (sx < a.vx + a.vw && sy < a.vy + a.vh && in_area(s, sx, sy))
-                    a.decisions[(size_t)(sy - s.ay) * a.dpitch + (size_t)((sx - s.ax) >> 1)] = t.hz[i];
+                const int sx = tx + 2 * k + F::nx(ry & 1), sy = ty + ry;
+                const bool h = t.hz[i] != 0 && sx < a.vx + a.vw && in_area(s, sx, sy);
+                const uint64_t m = __ballot(h);
+                const int lane = (int)threadIdx.x & 63;
+                if ((lane & 31) == 0 && sy < a.vy + a.vh)
+                    a.decisions[(size_t)(sy - a.vy) * a.dwords + (size_t)((tx - a.vx) >> 6)] =
+                        lane ? (uint32_t)(m >> 32) : (uint32_t)m;
             }
         }
         t.chroma();
