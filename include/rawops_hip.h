@@ -226,10 +226,11 @@ rawops_status rawops_render_backward(const rawops_source* source, uint32_t demos
 
 /* Menon classifier decisions recorded by the forward for the adjoint (no
  * reference counterpart).  A decision record of a viewport (vx, vy, vw, vh)
- * holds one bit per non-green photosite, 1 = horizontal: the site at sensor
- * (x, y) is bit ((x - vx) / 2) % 32 of the 32-bit word
- * (y - vy) * ((vw + 63) / 64) + (x - vx) / 64 (device memory, 4-byte
- * aligned).  The size in bytes that holds the record of any viewport: */
+ * holds one bit per non-green photosite, 1 = horizontal, 32 words per
+ * 64 x 32 tile of the viewport: with X = x - vx, Y = y - vy the site at
+ * sensor (x, y) is bit (X / 2) % 32 of the 32-bit word
+ * ((Y / 32) * ((vw + 63) / 64) + X / 64) * 32 + Y % 32 (device memory,
+ * 4-byte aligned).  The size in bytes that holds the record of any viewport: */
 rawops_status rawops_menon_decisions_size(const rawops_source* source, size_t* bytes);
 /* rawops_render of a Menon plan at mip 0 (FP32 output only) that also writes
  * the decision record of `viewport` into `decisions` (device, every word of

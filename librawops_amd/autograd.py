@@ -137,8 +137,9 @@ def render_chain(image: RawImage, gains, stops, matrix, demosaic="menon", viewpo
 def render_record(image: RawImage, gains, stops, matrix, viewport: Rect = None):
     """Menon forward of the render chain (FP32 out) that also records the
     classifier decisions: (out (h, w, 3), decision record (uint8 bytes of
-    little-endian 32-bit words: the non-green site at sensor (x, y) is bit
-    ((x - vx) // 2) % 32 of word (y - vy) * ((vw + 63) // 64) + (x - vx) // 64,
+    little-endian 32-bit words, 32 per 64 x 32 tile of the viewport: with
+    X = x - vx, Y = y - vy the non-green site at sensor (x, y) is bit
+    (X // 2) % 32 of word ((Y // 32) * ((vw + 63) // 64) + X // 64) * 32 + Y % 32,
     1 = horizontal))."""
     vp = viewport or image.metadata.active_area
     plan = Program.from_ops(C.DEMOSAIC_MENON, False, [(C.OP_GAINS, [float(v) for v in gains]),

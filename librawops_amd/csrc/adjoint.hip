@@ -41,7 +41,7 @@ struct BackwardArgs {
     int tile0_x, tile0_y;      // first tile (in tiles from the area origin)
     int r0, r1, c0, c1;        // launch-relative tile rows / columns of the interior block
     const uint32_t* decisions; // recorded Menon decisions (bits, relative to the recorded viewport) or null
-    uint32_t dwords;           // decision words per recorded row
+    uint32_t dtiles;           // 64 x 32 tiles per recorded row (32 decision words each)
     int rx, ry, rw, rh;        // the recorded viewport
     int ir0, ir1, ic0, ic1;    // within the recorded block: its tiles that need no mirroring
 };
@@ -746,8 +746,8 @@ __device__ __forceinline__ void menon_back_recorded_tile(const BackwardArgs& a, 
             const int i = min((int)threadIdx.x + j * kNT, NC - 1);
             const int v = i / HWE, u = 2 * (i - v * HWE) + ng_x<N0>(v & 1);
             const int sx = min(max(ex + u, a.rx), a.rx + a.rw - 1), sy = min(max(ey + v, a.ry), a.ry + a.rh - 1);
-            dec[j] = a.decisions[__umul24((uint32_t)(sy - a.ry), a.dwords) + (uint32_t)((sx - a.rx) >> 6)] >>
-                     (((sx - a.rx) >> 1) & 31);
+            const uint32_t X = (uint32_t)(sx - a.rx), Y = (uint32_t)(sy - a.ry);
+            dec[j] = a.decisions[((__umul24(Y >> 5, a.dtiles) + (X >> 6)) << 5) + (Y & 31u)] >> ((X >> 1) & 31u);
         }
         if (pairs) {
 #pragma unroll
@@ -992,7 +992,7 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
         unmirrored(a.tile0_x, a.c0, a.c1, TW, a.s.ax, a.s.aw, a.ic0, a.ic1);
         if (a.ir0 == a.ir1 || a.ic0 == a.ic1) a.ir0 = a.ir1 = a.ic0 = a.ic1 = 0;
         a.decisions = L.decisions;
-        a.dwords = decision_words(L.recorded.width);
+        a.dtiles = decision_tiles(L.recorded.width);
         a.rx = (int)L.recorded.x; a.ry = (int)L.recorded.y;
         a.rw = (int)L.recorded.width; a.rh = (int)L.recorded.height;
         const unsigned rec = (unsigned)((a.r1 - a.r0) * (a.c1 - a.c0));

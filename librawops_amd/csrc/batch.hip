@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kThreads) batch_kernel(const BatchItem* __rest
         const BatchItem& q = items[lo];
         RenderArgs a;
         a.decisions = nullptr;
-        a.dwords = 0;
+        a.dtiles = 0;
         a.s = q.sensor;
         a.native = q.native;
         a.reduced = q.reduced;

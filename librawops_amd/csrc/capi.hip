@@ -446,7 +446,7 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* src, r
 rawops_status rawops_menon_decisions_size(const rawops_source* src, size_t* bytes) {
     return guarded([&] {
         if (!src || !src->is_raw || !bytes) fail(RAWOPS_INVALID_ARGUMENT, "decisions need a RAW source");
-        *bytes = (size_t)decision_words(src->meta.active_area.width) * 4 * src->meta.active_area.height;
+        *bytes = decision_bytes(src->meta.active_area.width, src->meta.active_area.height);
     });
 }
 

@@ -55,7 +55,7 @@ hipError_t launch(const RenderLaunch& L, const OpList* native, const OpList* red
     a.flags = L.flags;
     a.tap = 0xffffffffu;
     a.decisions = L.decisions;
-    a.dwords = decision_words((uint32_t)a.vw);
+    a.dtiles = decision_tiles((uint32_t)a.vw);
     const unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL((render_kernel<DEMOSAIC, TW, TH>), dim3(blocks), dim3(kRenderThreads), 0, stream, a);

@@ -24,10 +24,14 @@ struct RenderLaunch {
 // native / reduced / post point at device-resident OpLists (owned by a plan).
 
 // Menon decision record of a viewport (vx, vy, vw, vh): one bit per non-green
-// photosite, 1 = horizontal; the site at (x, y) is bit ((x - vx) >> 1) & 31
-// of 32-bit word (y - vy) decision_words(vw) + ((x - vx) >> 6) (a 64-pixel
-// column span of a row per word).
-inline uint32_t decision_words(uint32_t width) { return (width + 63) / 64; }
+// photosite, 1 = horizontal, in blocks of 32 words per 64 x 32 tile of the
+// viewport (a tile's block is one 128-byte line, written whole).  With
+// X = x - vx, Y = y - vy the site at (x, y) is bit (X >> 1) & 31 of word
+// ((Y >> 5) decision_tiles(vw) + (X >> 6)) 32 + (Y & 31).
+inline uint32_t decision_tiles(uint32_t width) { return (width + 63) / 64; }
+inline size_t decision_bytes(uint32_t width, uint32_t height) {
+    return (size_t)decision_tiles(width) * ((height + 31) / 32) * 128;
+}
 hipError_t launch_render(const RenderLaunch& launch, const OpList* native, const OpList* reduced,
                          hipStream_t stream);
 

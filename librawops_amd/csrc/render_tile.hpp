@@ -25,9 +25,9 @@ struct RenderArgs {
     uint32_t* flags;         // may be null
     uint32_t tap;            // reduced ops applied before the FP32 output; the rest precede the u8 output
     uint32_t* decisions;     // Menon mip 0: classifier decisions of the viewport, one bit per
-                             // non-green site (viewport-relative: word (y - vy) dwords + (x - vx) / 64,
-                             // bit ((x - vx) / 2) mod 32, 1 = horizontal); may be null
-    uint32_t dwords;         // decision words per viewport row
+                             // non-green site, 32 words per tile (word tile 32 + row, bit x / 2
+                             // within the tile, 1 = horizontal); may be null
+    uint32_t dtiles;         // tiles per viewport row
 };
 
 // Tile-relative x parity of the non-green sites in even tile rows for a
@@ -104,7 +104,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         if (a.decisions && a.mip == 0) {
             // Record the tile's decisions for the adjoint: a tile row's 32
             // non-green sites are one word (half a wave's ballot).
-            static_assert(TW == 64 && ((TW / 2) * TH) % NT == 0, "one decision word per tile row");
+            static_assert(TW == 64 && TH == 32 && ((TW / 2) * TH) % NT == 0, "one decision word per tile row");
 #pragma unroll
             for (int j = 0; j < (TW / 2) * TH / NT; ++j) {
                 const int i = (int)threadIdx.x + j * NT;
@@ -114,7 +114,7 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
                 const uint64_t m = __ballot(h);
                 const int lane = (int)threadIdx.x & 63;
                 if ((lane & 31) == 0 && sy < a.vy + a.vh)
-                    a.decisions[(size_t)(sy - a.vy) * a.dwords + (size_t)((tx - a.vx) >> 6)] =
+                    a.decisions[((size_t)((ty - a.vy) >> 5) * a.dtiles + (size_t)((tx - a.vx) >> 6)) * 32 + ry] =
                         lane ? (uint32_t)(m >> 32) : (uint32_t)m;
             }
         }

@@ -175,7 +175,7 @@ def test_recorded_decisions(vp_rel):
     assert torch.equal(out.view(torch.int32), out_rec.view(torch.int32))
     _, dec_o = O.menon_f64(om, norm_plane(m, s), om.area)
     d = dec.cpu().numpy().view(np.uint32)
-    words = (vp[2] + 63) // 64
+    tiles = (vp[2] + 63) // 64
     checked = 0
     for y in range(vp[1] - area[1], vp[1] - area[1] + vp[3]):
         for x in range(vp[0] - area[0], vp[0] - area[0] + vp[2]):
@@ -184,7 +184,7 @@ def test_recorded_decisions(vp_rel):
             if [[0, 1, 1, 2], [2, 1, 1, 0], [1, 0, 2, 1], [1, 2, 0, 1]][om.pattern][site] == 1:
                 continue
             rx, ry = sx - vp[0], sy - vp[1]
-            bit = (int(d[ry * words + rx // 64]) >> ((rx // 2) % 32)) & 1
+            bit = (int(d[((ry // 32) * tiles + rx // 64) * 32 + ry % 32]) >> ((rx // 2) % 32)) & 1
             assert bit == dec_o[y, x], (x, y)
             checked += 1
     assert checked >= vp[2] * vp[3] // 2 - vp[3]
