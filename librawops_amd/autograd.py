@@ -319,10 +319,11 @@ def tone_curve(x, knots_y, knots_x, cubic=True):
 
 class FullChainCropsFn(torch.autograd.Function):
     """``full_chain`` over several equal-size viewports of one image as one
-    autograd node: the same kernels in the same order per crop, with the
-    parameters read to the host and the two op plans built once per call
-    (instead of once per crop and op), and the parameter gradients of all
-    crops accumulated on the device."""
+    autograd node: the render, the guided filters and the render adjoint per
+    crop, the pointwise stages (detail boost, sRGB, their adjoints, the
+    exposure adjoint) once over the stacked crops; the parameters read to the
+    host and the two op plans built once per call, and the parameter gradients
+    of all crops accumulated on the device."""
 
     @staticmethod
     def forward(ctx, gains, stops, matrix, strength, image: RawImage, viewports, demosaic: int, radius: int,
@@ -341,27 +342,23 @@ class FullChainCropsFn(torch.autograd.Function):
         if any((vp.width, vp.height) != (w, h) for vp in viewports):
             raise ValueError("full_chain_crops needs viewports of one size")
         for i, vp in enumerate(viewports):
-            FullChainCropsFn._forward_one(image, vp, render, expose, radius, eps, sb, lin[i], lin2[i], base[i],
-                                          boosted[i], out[i])
+            C.check(C.lib().rawops_render(render.handle, image._handle, vp.c(), 0, C.QUALITY_FINAL, _ptr(lin[i]), None,
+                                          w, None, _stream()))
+        # the exposure (a radius-0 pass: pointwise) over the crops stacked as one raster
+        stack = _Raster(lin.view(n * h, w, 3), 0)
+        C.check(C.lib().rawops_guided_filter(stack.h, C.Rect(0, 0, w, n * h), 0, 0, 1.0, expose.handle, _ptr(lin2), w,
+                                             None, 0, None, _stream()))
+        for i in range(n):
+            src2 = _Raster(lin2[i], 0)
+            C.check(C.lib().rawops_guided_filter(src2.h, C.Rect(0, 0, w, h), 0, radius, eps, _empty_plan().handle,
+                                                 _ptr(base[i]), w, None, 0, None, _stream()))
+        C.check(C.lib().rawops_detail_boost(_ptr(lin2), _ptr(base), sb, _ptr(boosted), lin2.numel(), _stream()))
+        C.check(C.lib().rawops_srgb_encode(_ptr(boosted), _ptr(out), boosted.numel(), _stream()))
         ctx.save_for_backward(lin, lin2, base, boosted)
         ctx.image, ctx.viewports, ctx.demosaic, ctx.radius, ctx.eps = image, viewports, demosaic, radius, eps
         ctx.g, ctx.m, ctx.M, ctx.sb = g, m, M, sb
         ctx.plans = (render, expose)
         return out
-
-    @staticmethod
-    def _forward_one(image, vp, render, expose, radius, eps, sb, lin, lin2, base, boosted, out):
-        h, w = lin.shape[:2]
-        C.check(C.lib().rawops_render(render.handle, image._handle, vp.c(), 0, C.QUALITY_FINAL, _ptr(lin), None, w,
-                                      None, _stream()))
-        src = _Raster(lin, 0)
-        C.check(C.lib().rawops_guided_filter(src.h, C.Rect(0, 0, w, h), 0, 0, 1.0, expose.handle, _ptr(lin2), w, None,
-                                             0, None, _stream()))
-        src2 = _Raster(lin2, 0)
-        C.check(C.lib().rawops_guided_filter(src2.h, C.Rect(0, 0, w, h), 0, radius, eps, _empty_plan().handle,
-                                             _ptr(base), w, None, 0, None, _stream()))
-        C.check(C.lib().rawops_detail_boost(_ptr(lin2), _ptr(base), sb, _ptr(boosted), lin2.numel(), _stream()))
-        C.check(C.lib().rawops_srgb_encode(_ptr(boosted), _ptr(out), boosted.numel(), _stream()))
 
     @staticmethod
     def backward(ctx, gy):
@@ -374,21 +371,22 @@ class FullChainCropsFn(torch.autograd.Function):
         geps = torch.zeros(1, dtype=torch.float64, device="cuda")
         g3 = (ctypes.c_float * 3)(*ctx.g)
         M9 = (ctypes.c_float * 9)(*ctx.M)
-        t1, t2, t3 = torch.empty_like(lin[0]), torch.empty_like(lin[0]), torch.empty_like(lin[0])
-        for i, vp in enumerate(ctx.viewports):
-            N = lin[i].numel()
-            C.check(C.lib().rawops_srgb_encode_backward(_ptr(boosted[i]), _ptr(gy[i]), _ptr(t1), N, _stream()))
-            C.check(C.lib().rawops_detail_boost_backward(_ptr(lin2[i]), _ptr(base[i]), _ptr(t1), ctx.sb, _ptr(t2),
-                                                         _ptr(t3), _ptr(gsb), N, _stream()))
-            if ctx.radius:
+        t1, t2, t3 = torch.empty_like(lin), torch.empty_like(lin), torch.empty_like(lin)
+        N = lin.numel()
+        C.check(C.lib().rawops_srgb_encode_backward(_ptr(boosted), _ptr(gy), _ptr(t1), N, _stream()))
+        C.check(C.lib().rawops_detail_boost_backward(_ptr(lin2), _ptr(base), _ptr(t1), ctx.sb, _ptr(t2), _ptr(t3),
+                                                     _ptr(gsb), N, _stream()))
+        if ctx.radius:
+            for i in range(n):
                 src = _Raster(lin2[i], 0)
-                C.check(C.lib().rawops_guided_backward(src.h, ctx.radius, ctx.eps, C.Rect(0, 0, w, h), _ptr(t3), w,
-                                                       _ptr(t1), _ptr(geps), None, 0, _stream()))
-                t2.add_(t1)
-            else:
-                t2.add_(t3)
-            C.check(C.lib().rawops_exposure_backward(_ptr(lin[i]), _ptr(t2), ctx.m, _ptr(t1), _ptr(gst), N, _stream()))
-            C.check(C.lib().rawops_render_backward(ctx.image._handle, ctx.demosaic, vp.c(), g3, 1.0, M9, _ptr(t1), w,
+                C.check(C.lib().rawops_guided_backward(src.h, ctx.radius, ctx.eps, C.Rect(0, 0, w, h), _ptr(t3[i]), w,
+                                                       _ptr(t1[i]), _ptr(geps), None, 0, _stream()))
+            t2.add_(t1)
+        else:
+            t2.add_(t3)
+        C.check(C.lib().rawops_exposure_backward(_ptr(lin), _ptr(t2), ctx.m, _ptr(t1), _ptr(gst), N, _stream()))
+        for i, vp in enumerate(ctx.viewports):
+            C.check(C.lib().rawops_render_backward(ctx.image._handle, ctx.demosaic, vp.c(), g3, 1.0, M9, _ptr(t1[i]), w,
                                                    None, _ptr(gp), _stream()))
         return (gp[0:3].float(), gst[0].float(), gp[4:13].float().reshape(3, 3), gsb[0].float(),
                 None, None, None, None, None)
