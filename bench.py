@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200,
                     help="untimed steps first (the GPU clocks ramp up over the first ~20 ms of work)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-record", action="store_true",
+                    help="fwd+bwd leg with the plain forward and the recomputing adjoint (no decision record)")
     ap.add_argument("--main-only", action="store_true", help="skip the C1/C3/C4/C5 legs")
     ap.add_argument("--c5-steps", type=int, default=200, help="Adam steps of the differentiable-chain leg")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -373,11 +375,20 @@ def main():
     dptr = ctypes.c_void_p(decisions.data_ptr())
 
     def forward_record():
+        if args.no_record:
+            DeviceRender.run(compiled, area, level, out=out, flags=flags, stream=stream)
+            return
         C.check(C.lib().rawops_render_record(compiled.program.handle, img._handle, area.c(),
                                              ctypes.c_void_p(out.data_ptr()), area.width,
                                              ctypes.c_void_p(flags.data_ptr()), dptr, sp))
 
     def backward():
+        if args.no_record:
+            C.check(C.lib().rawops_render_backward(img._handle, C.DEMOSAIC_MENON, area.c(), g3, 1.0, mm,
+                                                   ctypes.c_void_p(gout.data_ptr()), area.width,
+                                                   ctypes.c_void_p(graw.data_ptr()),
+                                                   ctypes.c_void_p(gpar.data_ptr()), sp))
+            return
         C.check(C.lib().rawops_render_backward_recorded(img._handle, area.c(), g3, 1.0, mm,
                                                         ctypes.c_void_p(gout.data_ptr()), area.width,
                                                         ctypes.c_void_p(graw.data_ptr()),
@@ -445,7 +456,9 @@ def main():
     line["fwd_bwd"] = {"value": round(px * bsteps / fb_wall / 1e6, 2), "unit": "Mpix/s",
                        "ms_per_step": round(fb_wall * 1e3 / bsteps, 4), "backward_kernel_ms": round(bwd_ms, 4),
                        "forward_record_ms": round(fwd_rec_ms, 4),
-                       "backward": "menon_backward_recorded<64,20> (classifier decisions recorded by the forward) + menon_backward<64,20> on border tiles: dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
+                       "backward": ("menon_backward<64,20> (classifier recomputed)" if args.no_record else
+                                    "menon_backward_recorded<64,20> (classifier decisions recorded by the forward) + menon_backward<64,20> on border tiles")
+                                   + ": dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["fwd_bwd"]["gpu_fwd_bwd_over_cpu_fwd"] = round(line["fwd_bwd"]["value"] / cpu["value"], 1)
