@@ -33,6 +33,7 @@ struct BackwardArgs {
     SensorArgs s;
     int vx, vy, vw, vh;        // viewport (native sensor coordinates)
     float g[3], m, M[9];
+    float K[9];                // K[c][r] = g_c m M[r][c] (chain back to the demosaic output)
     const float* gout;         // viewport RGB
     uint32_t gstride;          // pixels per gout row
     float* graw;               // active-area plane, aw x ah
@@ -55,13 +56,12 @@ __device__ __forceinline__ bool load_gout(const BackwardArgs& a, int sx, int sy,
     return true;
 }
 
-// gd_c = g_c * (m * (M^T gout)_c): gradient at the demosaic output.
-__device__ __forceinline__ void chain_back(const BackwardArgs& a, const float go[3], float u[3], float gd[3]) {
+// gd_c = g_c m (M^T gout)_c: gradient at the demosaic output, through the
+// combined matrix K[c][r] = g_c m M[r][c] (formed in FP64 on the host,
+// rounded once).
+__device__ __forceinline__ void chain_back(const BackwardArgs& a, const float go[3], float gd[3]) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        u[c] = a.M[c] * go[0] + a.M[3 + c] * go[1] + a.M[6 + c] * go[2];
-        gd[c] = a.g[c] * (a.m * u[c]);
-    }
+    for (int c = 0; c < 3; ++c) gd[c] = a.K[3 * c] * go[0] + a.K[3 * c + 1] * go[1] + a.K[3 * c + 2] * go[2];
 }
 
 // Parameter gradients.  With u = M^T gout and out = M (m g (.) d), every
@@ -70,22 +70,31 @@ __device__ __forceinline__ void chain_back(const BackwardArgs& a, const float go
 //     dL/dg_c    = m sum_r M[r][c] P[r][c]
 //     dL/dm      = sum_c g_c sum_r M[r][c] P[r][c]
 //     dL/dM[r][c] = m g_c P[r][c]
-// so a pixel adds nine exact FP64 products and the combinations are taken
-// once per workgroup.
-__device__ __forceinline__ void param_grads(const float d[3], const float go[3], double (&acc)[9]) {
+// so a pixel adds nine products and the combinations are taken once per
+// workgroup.  A thread's few products (at most eight pixels) are summed by
+// FP32 fused multiply-adds (relative error ~1e-7 of the partial sum); the
+// partials are summed in FP64 from there on.
+__device__ __forceinline__ void param_grads(const float d[3], const float go[3], float (&acc)[9]) {
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[r * 3 + c] += (double)go[r] * (double)d[c];
+        for (int c = 0; c < 3; ++c) acc[r * 3 + c] = __builtin_fmaf(go[r], d[c], acc[r * 3 + c]);
 }
 
 // x plus the value DPP control CTRL moves into this lane (rows outside
-// ROW_MASK, and lanes without a source, add zero).
+// ROW_MASK, and lanes without a source, add zero).  A control that gives
+// every lane a source (ROW_MASK 0xf) needs no zeroed old value.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_add(double x) {
     const uint64_t u = __builtin_bit_cast(uint64_t, x);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    uint32_t lo, hi;
+    if constexpr (ROW_MASK == 0xf) {
+        lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+        hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+    } else {
+        lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+        hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+    }
     return x + __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
@@ -117,10 +126,11 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
 //   k >= 4:  m g_c P[k - 4], c = (k - 4) mod 3   (dL/d M)
 // Per wave by DPP, then across waves in wave 0 (its lanes 0..8 sum one P
 // each over the waves, in wave order; the P are then broadcast).
-__device__ __forceinline__ void flush_params(const BackwardArgs& a, double (&acc)[9], double* red /* [waves][9] LDS */) {
+__device__ __forceinline__ void flush_params(const BackwardArgs& a, const float (&part)[9], double* red /* [waves][9] LDS */) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) acc[i] = wave_sum63(acc[i]);
+    for (int i = 0; i < 9; ++i) acc[i] = wave_sum63((double)part[i]);
     if (lane == 63)
         for (int i = 0; i < 9; ++i) red[wave * 9 + i] = acc[i];
     __syncthreads();
@@ -283,7 +293,7 @@ __device__ __forceinline__ void interior_gather(const BackwardArgs& a, const Sen
     // per item and row: green-channel gd and gd_r + gd_b of the non-green
     // site, gd of the green site, raw partials of both
     float pg[J][2], rn[J][2], gg[J][2][3], rpg[J][2], rpn[J][2];
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // pass 1
 #pragma unroll
@@ -297,8 +307,8 @@ __device__ __forceinline__ void interior_gather(const BackwardArgs& a, const Sen
             const int v = 2 * qr + b, un = 2 * k + ng_x<N0>(b), ug = un ^ 1, c = v * HWE + k;
             const bool rs = red_row(b);
             const bool hn = (up.hasN >> (2 * j + b)) & 1u, hg = (up.hasG >> (2 * j + b)) & 1u;
-            float gd[3], uu[3];
-            if (hn) chain_back(a, up.goN[j][b], uu, gd);
+            float gd[3];
+            if (hn) chain_back(a, up.goN[j][b], gd);
             else gd[0] = gd[1] = gd[2] = 0.f;
             const bool h = hz[c] != 0;
             const float po = rs ? gd[2] : gd[0];
@@ -306,7 +316,7 @@ __device__ __forceinline__ void interior_gather(const BackwardArgs& a, const Sen
             POv[c] = h ? 0.f : po;
             pg[j][b] = gd[1];
             rn[j][b] = gd[0] + gd[2];           // the own sample feeds both r and b
-            if (hg) chain_back(a, up.goG[j][b], uu, gg[j][b]);
+            if (hg) chain_back(a, up.goG[j][b], gg[j][b]);
             else gg[j][b][0] = gg[j][b][1] = gg[j][b][2] = 0.f;
             if (mine) {
                 float d[3];
@@ -408,9 +418,8 @@ __device__ __forceinline__ void border_gather(const BackwardArgs& a, const Senso
     auto src = [&](int u, int v) { return in_ext(u, v) && in_a(u, v); };    // a computed stage output
     auto red_at = [&](int v) { return ((red_rows >> (v & 1)) & 1u) != 0; };  // non-green sites of row v
     auto gd_at = [&](int u, int v, float gd[3], float go[3]) -> bool {
-        float uu[3];
         if (!load_gout(a, ex + u, ey + v, go)) { gd[0] = gd[1] = gd[2] = 0.f; return false; }
-        chain_back(a, go, uu, gd);
+        chain_back(a, go, gd);
         return true;
     };
     // Per coordinate: the position itself and the outside positions within 2
@@ -448,7 +457,7 @@ __device__ __forceinline__ void border_gather(const BackwardArgs& a, const Senso
             }
         }
     };
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // pass 1: every site of the extended tile inside the area, as row-pair
     // items (compile-time CFA parities)
@@ -872,7 +881,7 @@ __global__ void __launch_bounds__(kNT) bilinear_backward(const BackwardArgs a) {
     load_raw<TW, TH, 1, true, kNT>(s, tx, ty, raw);
     for (int i = threadIdx.x; i < TW * TH; i += kNT) adj[i] = 0.0f;
     __syncthreads();
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     constexpr int EW = TW + 2, EH = TH + 2;
     for (int i = threadIdx.x; i < EW * EH; i += kNT) {
         const int v = i / EW, u = i - v * EW;       // tile coordinates + 1
@@ -880,8 +889,8 @@ __global__ void __launch_bounds__(kNT) bilinear_backward(const BackwardArgs a) {
         if (!in_area(s, sx, sy)) continue;
         float go[3];
         if (!load_gout(a, sx, sy, go)) continue;
-        float uu[3], gd[3];
-        chain_back(a, go, uu, gd);
+        float gd[3];
+        chain_back(a, go, gd);
         const bool mine = u >= 1 && u <= TW && v >= 1 && v <= TH;
         if (mine) {
             float d[3];
@@ -930,6 +939,9 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     for (int i = 0; i < 3; ++i) a.g[i] = L.gains[i];
     a.m = L.multiplier;
     for (int i = 0; i < 9; ++i) a.M[i] = L.matrix[i];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            a.K[3 * c + r] = (float)((double)L.gains[c] * (double)L.multiplier * (double)L.matrix[3 * r + c]);
     a.gout = L.g_out;
     a.gstride = L.g_stride;
     a.graw = L.g_raw;
