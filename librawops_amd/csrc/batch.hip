@@ -92,6 +92,49 @@ __global__ void __launch_bounds__(kThreads) batch_kernel(const BatchItem* __rest
     }
 }
 
+// One workgroup per (request, tile) item, the render kernel's 512-thread
+// tile: no claim loop.  The workgroup that finishes a request's last tile
+// stamps its completion time, as in the persistent kernel.
+__global__ void __launch_bounds__(kRenderThreads) __attribute__((amdgpu_waves_per_eu(6)))
+batch_tiles(const BatchItem* __restrict__ items, int n, uint32_t* remaining, unsigned long long* done) {
+    __shared__ __attribute__((aligned(16))) char lds[cmax(Lds<RAWOPS_DEMOSAIC_MENON, 64, 32>::BYTES,
+                                                          Lds<RAWOPS_DEMOSAIC_BILINEAR, 64, 32>::BYTES)];
+    const int item = (int)blockIdx.x;
+    int lo = 0, hi = n - 1;                         // request owning this item
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].first <= item) lo = mid; else hi = mid - 1;
+    }
+    const BatchItem& q = items[lo];
+    RenderArgs a;
+    a.decisions = nullptr;
+    a.dtiles = 0;
+    a.s = q.sensor;
+    a.native = q.native;
+    a.reduced = q.reduced;
+    a.mip = q.mip;
+    a.vx = q.vx; a.vy = q.vy; a.vw = q.vw; a.vh = q.vh;
+    a.ox0 = q.ox0; a.oy0 = q.oy0;
+    a.tiles_x = q.tiles_x;
+    a.out = q.out;
+    a.out8 = q.out8;
+    a.ostride = q.ostride;
+    a.flags = q.flags;
+    a.tap = q.tap;
+    int tx, ty, lx0, ly0;
+    tile_origin<64, 32>(a, item - q.first, tx, ty, lx0, ly0);
+    const SensorArgs s = a.s;
+    if (q.demosaic == RAWOPS_DEMOSAIC_MENON)
+        render_any_tile<RAWOPS_DEMOSAIC_MENON, 64, 32, kRenderThreads>(a, s, lds, tx, ty, lx0, ly0);
+    else
+        render_any_tile<RAWOPS_DEMOSAIC_BILINEAR, 64, 32, kRenderThreads>(a, s, lds, tx, ty, lx0, ly0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicSub(&remaining[lo], 1u) == 1u) done[lo] = wall_clock64();
+    }
+}
+
 template <int TW, int TH>
 hipError_t launch_variant(const BatchItem* items, int n, int total, void* state, uint32_t* remaining,
                           unsigned long long* done, int grid, hipStream_t stream) {
@@ -115,8 +158,9 @@ int occupancy() {
 
 // Launch configurations the autotuner chooses between: tile shape and the
 // number of resident workgroups per CU the persistent grid is sized for
-// (0 = as many as fit).  Only the 64x32 tile is compiled in.
-const BatchShape kBatchShapes[kBatchShapeCount] = {{64, 32, 0}, {64, 32, 1}};
+// (0 = as many as fit), or one workgroup per item (-1).  Only the 64x32 tile
+// is compiled in.
+const BatchShape kBatchShapes[kBatchShapeCount] = {{64, 32, 0}, {64, 32, 1}, {64, 32, -1}};
 
 size_t batch_state_bytes() { return sizeof(BatchState); }
 
@@ -124,6 +168,7 @@ int batch_grid(int shape) {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (kBatchShapes[shape].per_cu < 0) return 0;   // grid = item count, set at launch
     const int fit = occupancy<64, 32>();
     const int per = kBatchShapes[shape].per_cu > 0 && kBatchShapes[shape].per_cu < fit ? kBatchShapes[shape].per_cu : fit;
     return cus * per;
@@ -131,7 +176,14 @@ int batch_grid(int shape) {
 
 hipError_t launch_batch(int shape, const BatchItem* items, int n, int total, void* state, uint32_t* remaining,
                         unsigned long long* done, int grid, hipStream_t stream) {
-    (void)shape;
+    if (kBatchShapes[shape].per_cu < 0) {
+        hipLaunchKernelGGL(batch_reset, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0, stream,
+                           static_cast<BatchState*>(state), items, n, remaining);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || total <= 0) return e;
+        hipLaunchKernelGGL(batch_tiles, dim3(total), dim3(kRenderThreads), 0, stream, items, n, remaining, done);
+        return hipGetLastError();
+    }
     return launch_variant<64, 32>(items, n, total, state, remaining, done, grid, stream);
 }
 

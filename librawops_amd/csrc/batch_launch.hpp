@@ -21,8 +21,11 @@ struct BatchItem {
     uint32_t* flags;
 };
 
+// per_cu: resident workgroups per CU the persistent grid is sized for
+// (0 = as many as fit); -1: no persistent loop, one 512-thread workgroup per
+// (request, tile) item running the one-request render kernel's tile.
 struct BatchShape { int tw, th, per_cu; };
-constexpr int kBatchShapeCount = 2;
+constexpr int kBatchShapeCount = 3;
 extern const BatchShape kBatchShapes[kBatchShapeCount];
 
 size_t batch_state_bytes();

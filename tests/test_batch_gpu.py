@@ -52,7 +52,7 @@ def requests(imgs):
     return reqs, singles
 
 
-@pytest.mark.parametrize("shape", [0, 1, None])
+@pytest.mark.parametrize("shape", [0, 1, 2, None])
 def test_batch_matches_single_renders(shape):
     imgs = sources()
     reqs, singles = requests(imgs)
