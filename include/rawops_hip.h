@@ -297,6 +297,23 @@ rawops_status rawops_guided_filter(const rawops_source* raster, rawops_rect view
                                    size_t workspace_bytes, uint32_t* device_flags,
                                    void* stream);
 
+/* A stack of `images` equal-size images (no reference counterpart; for
+ * callers that filter many crops of one size, such as the fit over crops):
+ * the raster's height splits into `images` images of height H each,
+ * consecutive in memory, and each is filtered on its own (level 0, its own
+ * borders) in one launch, with the same results as rawops_guided_filter /
+ * rawops_guided_backward on each image.  Outputs are consecutive too: image i
+ * of dst_rgb / g_out starts i * viewport.height * stride pixels in, image i of
+ * g_in i * width * H pixels in.  Workspaces are allocated stream-ordered. */
+rawops_status rawops_guided_filter_stack(const rawops_source* raster, uint32_t images, rawops_rect viewport,
+                                         uint32_t radius, double epsilon, const rawops_plan* post,
+                                         float* dst_rgb, uint32_t dst_stride_pixels, uint32_t* device_flags,
+                                         void* stream);
+rawops_status rawops_guided_backward_stack(const rawops_source* raster, uint32_t images, uint32_t radius,
+                                           double epsilon, rawops_rect viewport, const float* g_out,
+                                           uint32_t g_out_stride_pixels, float* g_in, double* g_eps,
+                                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif

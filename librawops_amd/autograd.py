@@ -319,9 +319,10 @@ def tone_curve(x, knots_y, knots_x, cubic=True):
 
 class FullChainCropsFn(torch.autograd.Function):
     """``full_chain`` over several equal-size viewports of one image as one
-    autograd node: the render, the guided filters and the render adjoint per
-    crop, the pointwise stages (detail boost, sRGB, their adjoints, the
-    exposure adjoint) once over the stacked crops; the parameters read to the
+    autograd node: the render and the render adjoint per crop, the guided
+    filter and its adjoint as one launch over the stacked crops (each crop its
+    own image), the pointwise stages (exposure, detail boost, sRGB, their
+    adjoints) once over the stack; the parameters read to the
     host and the two op plans built once per call, and the parameter gradients
     of all crops accumulated on the device."""
 
@@ -348,10 +349,10 @@ class FullChainCropsFn(torch.autograd.Function):
         stack = _Raster(lin.view(n * h, w, 3), 0)
         C.check(C.lib().rawops_guided_filter(stack.h, C.Rect(0, 0, w, n * h), 0, 0, 1.0, expose.handle, _ptr(lin2), w,
                                              None, 0, None, _stream()))
-        for i in range(n):
-            src2 = _Raster(lin2[i], 0)
-            C.check(C.lib().rawops_guided_filter(src2.h, C.Rect(0, 0, w, h), 0, radius, eps, _empty_plan().handle,
-                                                 _ptr(base[i]), w, None, 0, None, _stream()))
+        # the guided filter of every crop in one launch (each crop its own image)
+        stack2 = _Raster(lin2.view(n * h, w, 3), 0)
+        C.check(C.lib().rawops_guided_filter_stack(stack2.h, n, C.Rect(0, 0, w, h), radius, eps,
+                                                   _empty_plan().handle, _ptr(base), w, None, _stream()))
         C.check(C.lib().rawops_detail_boost(_ptr(lin2), _ptr(base), sb, _ptr(boosted), lin2.numel(), _stream()))
         C.check(C.lib().rawops_srgb_encode(_ptr(boosted), _ptr(out), boosted.numel(), _stream()))
         ctx.save_for_backward(lin, lin2, base, boosted)
@@ -377,10 +378,9 @@ class FullChainCropsFn(torch.autograd.Function):
         C.check(C.lib().rawops_detail_boost_backward(_ptr(lin2), _ptr(base), _ptr(t1), ctx.sb, _ptr(t2), _ptr(t3),
                                                      _ptr(gsb), N, _stream()))
         if ctx.radius:
-            for i in range(n):
-                src = _Raster(lin2[i], 0)
-                C.check(C.lib().rawops_guided_backward(src.h, ctx.radius, ctx.eps, C.Rect(0, 0, w, h), _ptr(t3[i]), w,
-                                                       _ptr(t1[i]), _ptr(geps), None, 0, _stream()))
+            stack2 = _Raster(lin2.view(n * h, w, 3), 0)
+            C.check(C.lib().rawops_guided_backward_stack(stack2.h, n, ctx.radius, ctx.eps, C.Rect(0, 0, w, h),
+                                                         _ptr(t3), w, _ptr(t1), _ptr(geps), _stream()))
             t2.add_(t1)
         else:
             t2.add_(t3)

@@ -832,6 +832,95 @@ rawops_status rawops_guided_backward(const rawops_source* raster, uint32_t radiu
     });
 }
 
+rawops_status rawops_guided_filter_stack(const rawops_source* raster, uint32_t images, rawops_rect viewport,
+                                         uint32_t radius, double epsilon, const rawops_plan* post, float* dst,
+                                         uint32_t dst_stride, uint32_t* flags, void* stream) {
+    return guarded([&] {
+        if (!raster || raster->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires a scene-linear raster");
+        if (!images || raster->height % images) fail(RAWOPS_INVALID_ARGUMENT, "the raster height must split into the images");
+        if (radius > 32 || !std::isfinite(epsilon) || epsilon < 0x1p-24 || epsilon > 65536.0)
+            fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires radius 0..32 and finite epsilon [2^-24,65536]");
+        if (!post || !dst) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        const uint32_t H = raster->height / images;
+        inside_or_throw({0, 0, raster->width, H}, viewport);
+        if (!dst_stride) dst_stride = viewport.width;
+        if (dst_stride < viewport.width) fail(RAWOPS_INVALID_ARGUMENT, "output stride is smaller than the viewport");
+        if (!viewport.width || !viewport.height) return;
+        GuidedLaunch L{};
+        L.src = raster->pixels;
+        L.src_stride = raster->stride;
+        L.width = raster->width;
+        L.height = H;
+        L.mip = 0;
+        L.viewport = viewport;
+        L.radius = radius;
+        L.epsilon = epsilon;
+        const auto w = working_y_weights(raster->space);
+        L.wr = w[0];
+        L.wb = w[1];
+        L.out = dst;
+        L.ostride = dst_stride;
+        L.flags = flags;
+        L.slices = images;
+        L.src_zstride = (size_t)H * raster->stride * 3;
+        L.out_zstride = (size_t)viewport.height * dst_stride * 3;
+        L.post_ops = L.fused_ops = post->reduced.n;
+        for (uint32_t i = 0; i < post->reduced.n; ++i)
+            if (post->reduced.op[i].kind == OP_CURVES) { L.fused_ops = i; break; }
+        auto st = static_cast<hipStream_t>(stream);
+        const size_t need = radius ? guided_workspace_bytes(L) : 0;
+        void* owned = nullptr;
+        if (need) {
+            hip_check(hipMallocAsync(&owned, need, st), "hipMallocAsync(guided workspace)");
+            L.workspace = owned;
+        }
+        const hipError_t e = launch_guided(L, post->device + 1, st);
+        if (owned) (void)hipFreeAsync(owned, st);
+        hip_check(e, "guided launch");
+    });
+}
+
+rawops_status rawops_guided_backward_stack(const rawops_source* raster, uint32_t images, uint32_t radius,
+                                           double epsilon, rawops_rect viewport, const float* g_out,
+                                           uint32_t g_stride, float* g_in, double* g_eps, void* stream) {
+    return guarded([&] {
+        if (!raster || raster->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires a scene-linear raster");
+        if (!images || raster->height % images) fail(RAWOPS_INVALID_ARGUMENT, "the raster height must split into the images");
+        if (radius < 1 || radius > 32 || !std::isfinite(epsilon) || epsilon < 0x1p-24 || epsilon > 65536.0)
+            fail(RAWOPS_INVALID_ARGUMENT, "guided backward requires radius 1..32 and finite epsilon [2^-24,65536]");
+        const uint32_t H = raster->height / images;
+        inside_or_throw({0, 0, raster->width, H}, viewport);
+        if (!g_out || !g_in || !g_eps) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        if (!g_stride) g_stride = viewport.width;
+        GuidedBackwardLaunch L{};
+        L.src = raster->pixels;
+        L.src_stride = raster->stride;
+        L.width = raster->width;
+        L.height = H;
+        L.radius = radius;
+        L.epsilon = epsilon;
+        const auto w = working_y_weights(raster->space);
+        L.wr = w[0];
+        L.wb = w[1];
+        L.g_out = g_out;
+        L.g_stride = g_stride;
+        L.viewport = viewport;
+        L.g_in = g_in;
+        L.g_eps = g_eps;
+        L.slices = images;
+        L.src_zstride = (size_t)H * raster->stride * 3;
+        L.g_out_zstride = (size_t)viewport.height * g_stride * 3;
+        auto st = static_cast<hipStream_t>(stream);
+        void* owned = nullptr;
+        hip_check(hipMallocAsync(&owned, guided_backward_workspace_bytes(raster->width, H) * images, st),
+                  "hipMallocAsync(guided backward workspace)");
+        L.workspace = owned;
+        const hipError_t e = launch_guided_backward(L, st);
+        (void)hipFreeAsync(owned, st);
+        hip_check(e, "guided backward launch");
+    });
+}
+
 rawops_status rawops_guided_workspace_size(uint32_t width, uint32_t height, rawops_rect viewport, uint32_t mip,
                                            uint32_t radius, size_t* bytes) {
     return guarded([&] {

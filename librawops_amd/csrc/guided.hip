@@ -56,7 +56,18 @@ struct GuidedArgs {
     uint32_t ostride;
     uint32_t* flags;
     const OpList* __restrict__ post;
+    size_t zsrc, zcoef, zout;   // floats between consecutive images of a stack (blockIdx.z)
 };
+
+// Image blockIdx.z of a stack of equal-size images, consecutive in memory.
+__device__ __forceinline__ GuidedArgs slice_of(const GuidedArgs& g0) {
+    GuidedArgs g = g0;
+    const size_t z = blockIdx.z;
+    g.src += z * g0.zsrc;
+    g.out += z * g0.zout;
+    if (g0.coef) g.coef += z * g0.zcoef;
+    return g;
+}
 
 // Raster pixel at level coordinates; mip 1/2 average clipped native
 // footprints in FP64 (RasterSourceNode::render_level RawEngine.cpp:363-406).
@@ -203,7 +214,8 @@ __device__ __forceinline__ void moments(const GuidedArgs& g, const float p[3], c
     m[4] = y * y; m[5] = y * d0; m[6] = y * d1; m[7] = y * d2;
 }
 
-__global__ void __launch_bounds__(kStripThreads) gf_coefficients(const GuidedArgs g) {
+__global__ void __launch_bounds__(kStripThreads) gf_coefficients(const GuidedArgs g0) {
+    const GuidedArgs g = slice_of(g0);
     __shared__ double prefix[kStripCols * 8];
     __shared__ double totals[4 * 8];
     const int span = kStripCols - 2 * g.r;
@@ -248,7 +260,8 @@ __global__ void __launch_bounds__(kStripThreads) gf_coefficients(const GuidedArg
     if (err && g.flags) atomicOr(g.flags, err);
 }
 
-__global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g) {
+__global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g0) {
+    const GuidedArgs g = slice_of(g0);
     __shared__ double prefix[kStripCols * 6];
     __shared__ double totals[4 * 6];
     const int span = kStripCols - 2 * g.r;
@@ -293,7 +306,8 @@ __global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g) {
 // every curve has at most kTailKnots knots: its lookups are data-dependent,
 // and from LDS they do not wait on divergent global loads.
 constexpr int kTailKnots = 64;
-__global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g) {
+__global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
+    const GuidedArgs g = slice_of(g0);
     __shared__ double tables[4 * 5 * kTailKnots];
     const size_t n = (size_t)g.vw * g.vh;
     Const<OpList>& L = as_const(*g.post);
@@ -325,7 +339,8 @@ __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g) {
 
 // Radius 0: the filter is the identity (SpatialOps.cpp:180); only the
 // post-filter program runs.
-__global__ void gf_identity(const GuidedArgs g) {
+__global__ void gf_identity(const GuidedArgs g0) {
+    const GuidedArgs g = slice_of(g0);
     const size_t n = (size_t)g.vw * g.vh;
     uint32_t err = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -364,13 +379,25 @@ struct GuidedBackArgs {
     float* ws;               // 11 planes [H][W] of per-centre adjoint terms
     float* gp;               // dL/dp, W x H x 3 (written in full)
     double* geps;            // accumulated dL/deps
+    size_t zp, zgq, zws, zgp;   // floats between consecutive images of a stack (blockIdx.z)
 };
+
+__device__ __forceinline__ GuidedBackArgs slice_of(const GuidedBackArgs& g0) {
+    GuidedBackArgs g = g0;
+    const size_t z = blockIdx.z;
+    g.p += z * g0.zp;
+    g.gq += z * g0.zgq;
+    g.ws += z * g0.zws;
+    g.gp += z * g0.zgp;
+    return g;
+}
 
 __device__ __forceinline__ double window_count(int x, int y, int W, int H, int r) {
     return (double)((min(W, x + r + 1) - max(0, x - r)) * (min(H, y + r + 1) - max(0, y - r)));
 }
 
-__global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBackArgs g) {
+__global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBackArgs g0) {
+    const GuidedBackArgs g = slice_of(g0);
     __shared__ double prefix[kStripCols * 14];
     __shared__ double totals[4 * 14];
     __shared__ double eps_part[kStripThreads / 64];
@@ -443,7 +470,8 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_centres(const GuidedBac
     }
 }
 
-__global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBackArgs g) {
+__global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBackArgs g0) {
+    const GuidedBackArgs g = slice_of(g0);
     __shared__ double prefix[kStripCols * 11];
     __shared__ double totals[4 * 11];
     const int span = kStripCols - 2 * g.r;
@@ -491,7 +519,7 @@ size_t guided_workspace_bytes(const GuidedLaunch& L) {
     const int cx0 = std::max(0, (int)L.viewport.x - r), cy0 = std::max(0, (int)L.viewport.y - r);
     const int cx1 = std::min(LW, (int)(L.viewport.x + L.viewport.width) + r);
     const int cy1 = std::min(LH, (int)(L.viewport.y + L.viewport.height) + r);
-    return (size_t)(cx1 - cx0) * (size_t)(cy1 - cy0) * 6 * sizeof(float);
+    return (size_t)(cx1 - cx0) * (size_t)(cy1 - cy0) * 6 * sizeof(float) * std::max(1u, L.slices);
 }
 
 size_t guided_backward_workspace_bytes(uint32_t width, uint32_t height) {
@@ -515,10 +543,15 @@ hipError_t launch_guided_backward(const GuidedBackwardLaunch& L, hipStream_t str
     g.ws = static_cast<float*>(L.workspace);
     g.gp = L.g_in;
     g.geps = L.g_eps;
+    const unsigned nz = std::max(1u, L.slices);
+    g.zp = L.src_zstride;
+    g.zgq = L.g_out_zstride;
+    g.zws = (size_t)11 * g.W * g.H;
+    g.zgp = (size_t)3 * g.W * g.H;
     const int span = kStripCols - 2 * g.r;
     const int strips = (g.W + span - 1) / span;
-    g.band = band_rows(strips, g.H);
-    dim3 grid((unsigned)strips, (unsigned)((g.H + g.band - 1) / g.band));
+    g.band = band_rows(strips, g.H);   // as for one image: the same sums, bit for bit
+    dim3 grid((unsigned)strips, (unsigned)((g.H + g.band - 1) / g.band), nz);
     hipLaunchKernelGGL(gf_back_centres, grid, dim3(kStripThreads), 0, stream, g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -549,24 +582,28 @@ hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t 
     g.ostride = L.ostride;
     g.flags = L.flags;
     g.post = post;
+    const unsigned nz = std::max(1u, L.slices);
+    g.zsrc = L.src_zstride;
+    g.zout = L.out_zstride;
     if (!g.vw || !g.vh) return hipSuccess;
     if (g.r == 0) {
         const size_t n = (size_t)g.vw * g.vh;
         const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
-        hipLaunchKernelGGL(gf_identity, dim3(blocks), dim3(256), 0, stream, g);
+        hipLaunchKernelGGL(gf_identity, dim3(blocks, 1, nz), dim3(256), 0, stream, g);
         return hipGetLastError();
     }
     g.coef = static_cast<float*>(L.workspace);
+    g.zcoef = (size_t)6 * g.cw * g.ch;
     const int span = kStripCols - 2 * g.r;
     const int strips_c = (g.cw + span - 1) / span;
-    g.band = band_rows(strips_c, g.ch);
-    dim3 grid_c((unsigned)strips_c, (unsigned)((g.ch + g.band - 1) / g.band));
+    g.band = band_rows(strips_c, g.ch);   // as for one image: the same sums, bit for bit
+    dim3 grid_c((unsigned)strips_c, (unsigned)((g.ch + g.band - 1) / g.band), nz);
     hipLaunchKernelGGL(gf_coefficients, grid_c, dim3(kStripThreads), 0, stream, g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int strips_a = (g.vw + span - 1) / span;
     g.band = band_rows(strips_a, g.vh);
-    dim3 grid_a((unsigned)strips_a, (unsigned)((g.vh + g.band - 1) / g.band));
+    dim3 grid_a((unsigned)strips_a, (unsigned)((g.vh + g.band - 1) / g.band), nz);
     g.fused = L.fused_ops;
     hipLaunchKernelGGL(gf_apply, grid_a, dim3(kStripThreads), 0, stream, g);
     e = hipGetLastError();
@@ -574,7 +611,7 @@ hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t 
     if (L.fused_ops < L.post_ops) {
         const size_t n = (size_t)g.vw * g.vh;
         const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
-        hipLaunchKernelGGL(gf_ops_tail, dim3(blocks), dim3(256), 0, stream, g);
+        hipLaunchKernelGGL(gf_ops_tail, dim3(blocks, 1, nz), dim3(256), 0, stream, g);
         e = hipGetLastError();
     }
     return e;

@@ -66,6 +66,9 @@ struct GuidedLaunch {
     void* workspace;         // guided_workspace_bytes() of device memory (r > 0)
     uint32_t fused_ops;      // leading post-filter ops run inside the filter pass; the rest
     uint32_t post_ops;       // (from the first curve op on, up to post_ops) run in a pass of their own
+    uint32_t slices;         // a stack of this many images, each filtered on its own (0 or 1: one)
+    size_t src_zstride;      // floats between consecutive images of the source stack
+    size_t out_zstride;      // floats between consecutive output images
 };
 
 size_t guided_workspace_bytes(const GuidedLaunch& launch);
@@ -82,7 +85,10 @@ struct GuidedBackwardLaunch {
     rawops_rect viewport;
     float* g_in;             // dL/dp, width x height x 3
     double* g_eps;           // accumulated
-    void* workspace;         // guided_backward_workspace_bytes()
+    void* workspace;         // guided_backward_workspace_bytes() per image of the stack
+    uint32_t slices;         // a stack of this many images (0 or 1: one); dL/dp images consecutive
+    size_t src_zstride;      // floats between consecutive source images
+    size_t g_out_zstride;    // floats between consecutive dL/dq images
 };
 size_t guided_backward_workspace_bytes(uint32_t width, uint32_t height);
 hipError_t launch_guided_backward(const GuidedBackwardLaunch& launch, hipStream_t stream);

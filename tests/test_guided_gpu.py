@@ -95,3 +95,46 @@ def test_non_finite_input_rejected():
     n = L.WorkingYGuidedFilterNode(L.RasterSourceNode(raster), L.Rect(0, 0, 4, 4), L.WorkingYGuidedFilterSettings(1, 1e-3))
     with pytest.raises(L.RawOpsError):
         L.Renderer().render_image(n, L.Rect(0, 0, 4, 4), L.RenderRequest(L.Rect(0, 0, 4, 4)))
+
+
+@pytest.mark.parametrize("radius", [0, 3, 8])
+def test_stack_matches_single_images(radius):
+    """rawops_guided_filter_stack / rawops_guided_backward_stack over a stack
+    of equal-size images equal the single-image calls on each image (output
+    and dL/dp bits; dL/deps to FP64 accumulation order)."""
+    from librawops_amd import _capi as C
+    from librawops_amd import autograd as A
+    n, h, w = 5, 37, 53
+    stack = torch.from_numpy(np.stack([synth.lognormal_raster(w, h, seed=70 + i) for i in range(n)])).cuda()
+    vp = C.Rect(3, 2, w - 7, h - 5)
+    plan = A._empty_plan()
+    expose = A.Program.from_ops(C.DEMOSAIC_BILINEAR, False, [], [(C.OP_SCALE, (1.5,))])
+    for post in (plan, expose):
+        want = torch.empty((n, vp.height, vp.width, 3), dtype=torch.float32, device="cuda")
+        for i in range(n):
+            src = A._Raster(stack[i].contiguous(), 0)
+            C.check(C.lib().rawops_guided_filter(src.h, vp, 0, radius, 1e-3, post.handle, A._ptr(want[i]), vp.width,
+                                                 None, 0, None, A._stream()))
+        got = torch.empty_like(want)
+        whole = A._Raster(stack.view(n * h, w, 3), 0)
+        C.check(C.lib().rawops_guided_filter_stack(whole.h, n, vp, radius, 1e-3, post.handle, A._ptr(got), vp.width,
+                                                   None, A._stream()))
+        torch.cuda.synchronize()
+        assert torch.equal(got.view(torch.int32), want.view(torch.int32)), (radius, post is plan)
+    if radius == 0:
+        return
+    rng = np.random.default_rng(radius)
+    gq = torch.from_numpy(rng.normal(size=(n, vp.height, vp.width, 3)).astype(np.float32)).cuda()
+    want, got = torch.empty_like(stack), torch.empty_like(stack)
+    ge_want = torch.zeros(1, dtype=torch.float64, device="cuda")
+    ge_got = torch.zeros(1, dtype=torch.float64, device="cuda")
+    for i in range(n):
+        src = A._Raster(stack[i].contiguous(), 0)
+        C.check(C.lib().rawops_guided_backward(src.h, radius, 1e-3, vp, A._ptr(gq[i]), vp.width, A._ptr(want[i]),
+                                               A._ptr(ge_want), None, 0, A._stream()))
+    whole = A._Raster(stack.view(n * h, w, 3), 0)
+    C.check(C.lib().rawops_guided_backward_stack(whole.h, n, radius, 1e-3, vp, A._ptr(gq), vp.width, A._ptr(got),
+                                                 A._ptr(ge_got), A._stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(got.view(torch.int32), want.view(torch.int32))
+    assert abs(ge_got.item() - ge_want.item()) <= 1e-9 * max(1.0, abs(ge_want.item()))
