@@ -311,7 +311,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+                                   "(row-band sharding across GPUs is designed, not run)"}))
         sys.exit(2)
 
     import numpy as np

@@ -80,6 +80,6 @@ def test_batch_matches_single_renders(shape):
 
 
 @pytest.mark.skip(reason="32x32 and 128x16 batch tiles are compiled out: the 32x32 variant hung on the GPU "
-                         "and its cause is not yet shown (see DESIGN.md)")
+                         "and its cause is not yet shown")
 def test_batch_small_and_wide_tiles():
     pass
