@@ -310,7 +310,7 @@ def main():
         print(json.dumps({"calibration": "device copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus above 1 is not supported "
                                    "(row-band sharding across GPUs is designed, not run)"}))
         sys.exit(2)
 

@@ -640,7 +640,7 @@ ImageGraph::ImageGraph(std::shared_ptr<const Node> source, Rect bounds, GraphRec
 
 void ImageGraph::finish(std::shared_ptr<const Node> linear, bool linear_output) {
     if (recipe_.output_mode == OutputMode::IccDisplay)
-        throw std::invalid_argument("ICC display output is not available in this build");
+        throw std::invalid_argument("ICC display output is not supported by this library");
     if (recipe_.output_mode != OutputMode::LegacyBounded && recipe_.output_mode != OutputMode::SrgbPreview)
         throw std::invalid_argument("unknown output mode");
     if (linear_output) {

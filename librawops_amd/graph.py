@@ -884,7 +884,7 @@ class ImageGraph:
         recipe = recipe or GraphRecipe()
         self.recipe = recipe
         if display_transform is not None or recipe.output_mode == OutputMode.IccDisplay:
-            raise InvalidArgument("ICC display output is not available in this build")
+            raise InvalidArgument("ICC display output is not supported by this library")
         if recipe.output_mode not in (OutputMode.LegacyBounded, OutputMode.SrgbPreview):
             raise InvalidArgument("unknown output mode")
         if isinstance(image, RawImage):
