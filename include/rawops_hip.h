@@ -297,6 +297,23 @@ rawops_status rawops_guided_filter(const rawops_source* raster, rawops_rect view
                                    size_t workspace_bytes, uint32_t* device_flags,
                                    void* stream);
 
+/* ---- analysis (ImageAnalysis.cpp:10-70) ---- */
+/* Per-channel histogram of a device FP32 RGB image (width x height, row
+ * stride stride_pixels, 0 = width), accumulated into device counters so that
+ * an image can be analysed in bands: counts[c * bins + k] (uint64), tallies[3c
+ * + 0..2] = underflow, overflow, nonfinite (uint64), extrema[2c], [2c + 1] =
+ * order keys of the finite minimum and maximum (uint32; initialise them to
+ * 0xFFFFFFFF and 0; key(v) = bits(v) ^ 0x80000000 for v >= +0 and ~bits(v)
+ * for v <= -0, so keys sort as the floats do).  Bins: `bins` (1..65536) equal
+ * bins of [lower, upper], half-open except the last, which includes upper;
+ * bin = min(bins - 1, (uint32)((double(v) - lower) / (upper - lower) * bins)).
+ * Values below lower / above upper are tallied, not clipped; NaN and
+ * infinities count as nonfinite and do not enter the extrema.  Replaces the
+ * per-tile loop of histogram_rgb (ImageAnalysis.cpp:47-63). */
+rawops_status rawops_histogram_rgb(const float* rgb, uint32_t width, uint32_t height, uint32_t stride_pixels,
+                                   uint32_t bins, double lower, double upper, uint64_t* counts,
+                                   uint64_t* tallies, uint32_t* extrema, void* stream);
+
 /* A stack of `images` equal-size images (no reference counterpart; for
  * callers that filter many crops of one size, such as the fit over crops):
  * the raster's height splits into `images` images of height H each,

@@ -16,5 +16,7 @@ from .graph import (BayerPattern, CameraColorTransform, CancellationToken, Curve
                     WorkingYGuidedFilterSettings, CameraToWorkingNode, compile_chain, request_bounds,
                     validate_raw_demosaic, validate_working_y_guided_filter_settings)
 
+from .analysis import ChannelHistogram, HistogramOptions, RgbHistogram, histogram_rgb  # noqa: F401
+
 MENON = RawDemosaicIdentity("rawengine.menon_base", 1)
 BILINEAR = RawDemosaicIdentity("rawengine.bilinear", 1)

@@ -832,6 +832,35 @@ rawops_status rawops_guided_backward(const rawops_source* raster, uint32_t radiu
     });
 }
 
+rawops_status rawops_histogram_rgb(const float* rgb, uint32_t width, uint32_t height, uint32_t stride_pixels,
+                                   uint32_t bins, double lower, double upper, uint64_t* counts, uint64_t* tallies,
+                                   uint32_t* extrema, void* stream) {
+    return guarded([&] {
+        const double span = upper - lower;
+        if (bins == 0 || bins > 65536 || !std::isfinite(lower) || !std::isfinite(upper) || !(span > 0.0) ||
+            !std::isfinite(span))
+            fail(RAWOPS_INVALID_ARGUMENT, "histogram requires 1..65536 bins and a finite positive range");
+        if (!counts || !tallies || !extrema) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        if (!stride_pixels) stride_pixels = width;
+        if (stride_pixels < width) fail(RAWOPS_INVALID_ARGUMENT, "image stride is smaller than its width");
+        if (!width || !height) return;
+        if (!rgb) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        HistogramLaunch h{};
+        h.rgb = rgb;
+        h.width = width;
+        h.height = height;
+        h.stride = stride_pixels;
+        h.bins = bins;
+        h.lower = lower;
+        h.upper = upper;
+        h.span = span;
+        h.counts = reinterpret_cast<unsigned long long*>(counts);
+        h.tallies = reinterpret_cast<unsigned long long*>(tallies);
+        h.extrema = extrema;
+        hip_check(launch_histogram(h, static_cast<hipStream_t>(stream)), "histogram launch");
+    });
+}
+
 rawops_status rawops_guided_filter_stack(const rawops_source* raster, uint32_t images, rawops_rect viewport,
                                          uint32_t radius, double epsilon, const rawops_plan* post, float* dst,
                                          uint32_t dst_stride, uint32_t* flags, void* stream) {

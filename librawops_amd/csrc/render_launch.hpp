@@ -94,6 +94,20 @@ size_t guided_backward_workspace_bytes(uint32_t width, uint32_t height);
 hipError_t launch_guided_backward(const GuidedBackwardLaunch& launch, hipStream_t stream);
 hipError_t launch_guided(const GuidedLaunch& launch, const OpList* post, hipStream_t stream);
 
+// Per-channel histogram of an FP32 RGB image, accumulated into device
+// counters: counts [3][bins], tallies [3][3] (underflow, overflow,
+// nonfinite), extrema [3][2] (order keys of the finite minimum and maximum).
+struct HistogramLaunch {
+    const float* rgb;
+    uint32_t width, height, stride;   // stride in pixels
+    uint32_t bins;
+    double lower, upper, span;        // span = upper - lower
+    unsigned long long* counts;
+    unsigned long long* tallies;
+    uint32_t* extrema;
+};
+hipError_t launch_histogram(const HistogramLaunch& launch, hipStream_t stream);
+
 hipError_t launch_srgb(const float* x, float* y, size_t n, hipStream_t s);
 hipError_t launch_srgb_backward(const float* x, const float* gy, float* gx, size_t n, hipStream_t s);
 hipError_t launch_exposure_backward(const float* x, const float* gy, float m, float* gx, double* g_stops, size_t n,

@@ -844,7 +844,8 @@ class Renderer:
                 w, h = min(T, v.x + v.width - x), min(T, v.y + v.height - y)
                 callback(Tile(Rect(x, y, w, h), image[y - v.y:y - v.y + h, x - v.x:x - v.x + w].copy(), desc))
 
-    def _render_viewport(self, node, viewport, level):
+    def _render_viewport_device(self, node, viewport, level):
+        """The viewport rendered into a device tensor (h, w, 3), flags checked."""
         t = _torch()
         compiled = compile_chain(node)
         out = t.empty((viewport.height, viewport.width, 3), dtype=t.float32, device="cuda")
@@ -853,7 +854,10 @@ class Renderer:
             DeviceRender.run(compiled, viewport, level, out=out, flags=flags)
             C.check(C.lib().rawops_check_flags(ctypes.c_void_p(flags.data_ptr()),
                                                ctypes.c_void_p(t.cuda.current_stream().cuda_stream)))
-        return out.cpu().numpy()
+        return out
+
+    def _render_viewport(self, node, viewport, level):
+        return self._render_viewport_device(node, viewport, level).cpu().numpy()
 
     def render_image(self, target, *args, cancellation: CancellationToken = None) -> Tile:
         if isinstance(target, ImageGraph):

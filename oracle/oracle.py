@@ -466,3 +466,29 @@ def curve_map_f64(knots, values, cubic=True):
     out = np.zeros_like(v)
     L.ro_curve_map_f64(_p(xs), _p(ys), len(xs), int(cubic), _p(v), _p(out), v.size)
     return out
+
+
+def histogram_rgb(rgb, bins, lower, upper):
+    """Per-channel histogram of a float32 (h, w, 3) image, restating the
+    per-sample loop of histogram_rgb (ImageAnalysis.cpp:52-63): nonfinite
+    samples are tallied and skipped; the finite extrema include outliers;
+    double(sample) < lower / > upper are tallied as underflow / overflow; else
+    bin = min(bins - 1, uint32((double(sample) - lower) / span * bins)).
+    Returns [(counts uint64[bins], underflow, overflow, nonfinite, minimum,
+    maximum)] for R, G, B (extrema None when no sample is finite)."""
+    span = upper - lower
+    out = []
+    flat = np.asarray(rgb, np.float32).reshape(-1, 3)
+    for c in range(3):
+        s = flat[:, c]
+        fin = np.isfinite(s)
+        f = s[fin]
+        mn = float(f.min()) if f.size else None
+        mx = float(f.max()) if f.size else None
+        d = f.astype(np.float64)
+        under, over = d < lower, d > upper
+        inside = d[~under & ~over]
+        b = np.minimum(bins - 1, ((inside - lower) / span * bins).astype(np.uint32))
+        counts = np.bincount(b, minlength=bins).astype(np.uint64)
+        out.append((counts, int(under.sum()), int(over.sum()), int((~fin).sum()), mn, mx))
+    return out
