@@ -784,3 +784,123 @@ void Renderer::render_device(const Node& output, Rect source_bounds, RenderReque
 }
 
 }  // namespace rawengine
+
+// ---------------------------------------------------------------- analysis --
+
+#include "../../../include/rawengine/ImageAnalysis.hpp"
+
+namespace rawengine {
+
+namespace {
+
+float from_order_key(uint32_t key) {
+    const uint32_t bits = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+    float v;
+    std::memcpy(&v, &bits, sizeof v);
+    return v;
+}
+
+}  // namespace
+
+RgbHistogram histogram_rgb(const Node& output, Rect source_bounds, const RenderRequest& request,
+                           const HistogramOptions& options, const CancellationToken* cancellation) {
+    const double span = options.upper - options.lower;
+    if (options.bins == 0 || options.bins > 65536 || !std::isfinite(options.lower) ||
+        !std::isfinite(options.upper) || !(span > 0.0) || !std::isfinite(span))
+        throw std::invalid_argument("histogram requires 1..65536 bins and a finite positive range");
+    auto check_cancelled = [&] {
+        if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
+    };
+    check_cancelled();
+    RgbHistogram result;
+    result.request = request;
+    result.options = options;
+    result.descriptor = output.output_descriptor();
+    if (result.descriptor.format != PixelFormat::RGBFloat32)
+        throw std::invalid_argument("histogram requires RGBFloat32 output");
+    const Rect v = request.viewport;
+    const uint64_t right = uint64_t(v.x) + v.width, bottom = uint64_t(v.y) + v.height;
+    if (!v.width || !v.height || right > (uint64_t{1} << 32) || bottom > (uint64_t{1} << 32))
+        throw std::invalid_argument("histogram requires a nonempty addressable viewport");
+    const uint32_t bins = options.bins;
+    detail::DeviceBuffer counts(size_t(3) * bins * sizeof(uint64_t)), tallies(9 * sizeof(uint64_t)),
+        extrema(6 * sizeof(uint32_t));
+    detail::hip(hipMemset(counts.p, 0, size_t(3) * bins * sizeof(uint64_t)), "hipMemset");
+    detail::hip(hipMemset(tallies.p, 0, 9 * sizeof(uint64_t)), "hipMemset");
+    const uint32_t init[6] = {~0u, 0u, ~0u, 0u, ~0u, 0u};
+    detail::hip(hipMemcpy(extrema.p, init, sizeof init, hipMemcpyHostToDevice), "hipMemcpy");
+    auto analyse = [&](const float* dev, uint32_t w, uint32_t h) {
+        check(rawops_histogram_rgb(dev, w, h, w, bins, options.lower, options.upper,
+                                   static_cast<uint64_t*>(counts.p), static_cast<uint64_t*>(tallies.p),
+                                   static_cast<uint32_t*>(extrema.p), nullptr));
+    };
+    if (detail::compilable(output)) {
+        detail::validate_rect(detail::request_bounds(source_bounds, request.level), v);
+        if (!output.supports_level(request.level)) throw std::invalid_argument("node does not support this render level");
+        if (!request.tile_size) throw std::invalid_argument("callback and tile size are required");
+        // Bands of whole rows (at least a tile high, up to ~8 M pixels): the
+        // histogram does not depend on how the viewport is partitioned.
+        const auto chain = detail::compile(output);
+        const uint32_t band_h = std::min<uint32_t>(v.height, std::max<uint32_t>(request.tile_size, (8u << 20) / v.width + 1));
+        detail::DeviceBuffer dev(size_t(v.width) * band_h * 3 * sizeof(float));
+        detail::DeviceBuffer flags(sizeof(uint32_t));
+        detail::hip(hipMemset(flags.p, 0, sizeof(uint32_t)), "hipMemset");
+        for (uint64_t y = v.y; y < bottom; y += band_h) {
+            check_cancelled();
+            const Rect band{v.x, uint32_t(y), v.width, uint32_t(std::min<uint64_t>(band_h, bottom - y))};
+            detail::run(*chain, band, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p),
+                        nullptr);
+            check(rawops_check_flags(static_cast<uint32_t*>(flags.p), nullptr));
+            analyse(static_cast<const float*>(dev.p), band.width, band.height);
+        }
+    } else {
+        // A foreign node renders through its own code; each tile must be the
+        // one the renderer asked for before its storage is read.
+        uint64_t next_x = v.x, next_y = v.y;
+        detail::DeviceBuffer dev(size_t(std::max<uint32_t>(1, request.tile_size)) *
+                                 std::max<uint32_t>(1, request.tile_size) * 3 * sizeof(float));
+        Renderer{}.render_tiles(output, source_bounds, request, [&](const Tile& tile) {
+            check_cancelled();
+            const Rect expected{uint32_t(next_x), uint32_t(next_y),
+                                uint32_t(std::min<uint64_t>(request.tile_size, right - next_x)),
+                                uint32_t(std::min<uint64_t>(request.tile_size, bottom - next_y))};
+            const uint64_t pixels = uint64_t(expected.width) * expected.height;
+            if (tile.bounds.x != expected.x || tile.bounds.y != expected.y || tile.bounds.width != expected.width ||
+                tile.bounds.height != expected.height || tile.rgb.size() % 3 != 0 || tile.rgb.size() / 3 != pixels ||
+                tile.descriptor != result.descriptor)
+                throw std::invalid_argument("histogram received inconsistent tile bounds, storage or descriptor");
+            detail::hip(hipMemcpy(dev.p, tile.rgb.data(), tile.rgb.size() * sizeof(float), hipMemcpyHostToDevice),
+                        "hipMemcpy");
+            analyse(static_cast<const float*>(dev.p), expected.width, expected.height);
+            next_x += expected.width;
+            if (next_x == right) { next_x = v.x; next_y += expected.height; }
+            check_cancelled();
+        }, cancellation);
+    }
+    std::vector<uint64_t> c(size_t(3) * bins), t(9);
+    uint32_t e[6];
+    detail::hip(hipMemcpy(c.data(), counts.p, c.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+    detail::hip(hipMemcpy(t.data(), tallies.p, t.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+    detail::hip(hipMemcpy(e, extrema.p, sizeof e, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (int k = 0; k < 3; ++k) {
+        auto& ch = result.channels[k];
+        ch.counts.assign(c.begin() + size_t(k) * bins, c.begin() + size_t(k + 1) * bins);
+        ch.underflow = t[3 * k];
+        ch.overflow = t[3 * k + 1];
+        ch.nonfinite = t[3 * k + 2];
+        if (e[2 * k] != ~0u) {
+            ch.minimum = from_order_key(e[2 * k]);
+            ch.maximum = from_order_key(e[2 * k + 1]);
+        }
+    }
+    result.pixel_count = uint64_t(v.width) * v.height;
+    check_cancelled();
+    return result;
+}
+
+RgbHistogram histogram_rgb(const ImageGraph& graph, const RenderRequest& request, const HistogramOptions& options,
+                           const CancellationToken* cancellation) {
+    return histogram_rgb(graph.output(), graph.source_bounds(), request, options, cancellation);
+}
+
+}  // namespace rawengine

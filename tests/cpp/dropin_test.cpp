@@ -4,6 +4,7 @@
 #include <RawEngine.hpp>
 #include <SpatialOps.hpp>
 #include <ToneOps.hpp>
+#include <ImageAnalysis.hpp>
 
 #include <cmath>
 #include <cstdio>
@@ -62,6 +63,101 @@ public:
     }
     ImageDescriptor output_descriptor() const noexcept override { return ImageDescriptor::camera_linear(); }
 };
+
+// A foreign node for histogram_rgb: R = (x mod 9) / 4 - 1/2, G = (y mod 9) / 4 - 1/2, B = NaN;
+// optionally a malformed tile or a cancellation from inside render().
+class ResidueProbe final : public Node {
+public:
+    int fault = 0;                         // 1 bounds, 2 storage, 3 descriptor
+    CancellationToken* cancel_on_render = nullptr;
+    mutable unsigned calls = 0;
+    ImageDescriptor output_descriptor() const noexcept override { return ImageDescriptor::camera_linear(); }
+    Tile render(Rect r) const override {
+        ++calls;
+        Tile t{r, {}, output_descriptor()};
+        for (uint32_t y = r.y; y < r.y + r.height; ++y)
+            for (uint32_t x = r.x; x < r.x + r.width; ++x) {
+                t.rgb.push_back(float(x % 9) * 0.25f - 0.5f);
+                t.rgb.push_back(float(y % 9) * 0.25f - 0.5f);
+                t.rgb.push_back(std::nanf(""));
+            }
+        if (fault == 1) ++t.bounds.x;
+        if (fault == 2) t.rgb.pop_back();
+        if (fault == 3) t.descriptor = ImageDescriptor::srgb_output();
+        if (cancel_on_render) cancel_on_render->cancel();
+        return t;
+    }
+};
+
+static void histogram_checks() {
+    ResidueProbe probe;
+    const Rect bounds{7, 5, 31, 23};
+    const RenderRequest request{Rect{9, 7, 19, 13}, 4};
+    const auto h = histogram_rgb(probe, bounds, request, HistogramOptions{4, 0.0, 1.0});
+    bool edges = h.pixel_count == 247;
+    for (int c = 0; c < 2; ++c) {
+        uint64_t want[4] = {0, 0, 0, 0}, below = 0, above = 0;
+        for (uint32_t y = 7; y < 20; ++y)
+            for (uint32_t x = 9; x < 28; ++x) {
+                const uint32_t r = (c == 0 ? x : y) % 9;   // value r / 4 - 1/2
+                if (r < 2) ++below;
+                else if (r > 6) ++above;
+                else ++want[r == 6 ? 3 : r - 2];           // 1.0 joins the closed last bin
+            }
+        edges = edges && h.channels[c].counts == std::vector<uint64_t>(want, want + 4) &&
+                h.channels[c].underflow == below && h.channels[c].overflow == above &&
+                h.channels[c].nonfinite == 0 && h.channels[c].minimum == -0.5f && h.channels[c].maximum == 1.5f;
+    }
+    edges = edges && h.channels[2].nonfinite == 247 && !h.channels[2].minimum && !h.channels[2].maximum;
+    CHECK(edges, "histogram_rgb: bin edges, outliers, nonfinite, extrema over a foreign node's tiles");
+    bool partition = true;
+    for (uint32_t ts : {1u, 3u, 7u, 256u}) {
+        const auto o = histogram_rgb(probe, bounds, RenderRequest{request.viewport, ts}, HistogramOptions{4, 0.0, 1.0});
+        for (int c = 0; c < 3; ++c)
+            partition = partition && o.channels[c].counts == h.channels[c].counts &&
+                        o.channels[c].underflow == h.channels[c].underflow &&
+                        o.channels[c].overflow == h.channels[c].overflow &&
+                        o.channels[c].nonfinite == h.channels[c].nonfinite &&
+                        o.channels[c].minimum == h.channels[c].minimum && o.channels[c].maximum == h.channels[c].maximum;
+    }
+    CHECK(partition, "histogram_rgb: independent of the tile partition");
+    const double nan = std::nan(""), inf = std::numeric_limits<double>::infinity();
+    bool rejected = true;
+    const unsigned calls = probe.calls;
+    for (const HistogramOptions& o : {HistogramOptions{0, 0, 1}, HistogramOptions{65537, 0, 1}, HistogramOptions{4, 1, 1},
+                                      HistogramOptions{4, 2, 1}, HistogramOptions{4, nan, 1}, HistogramOptions{4, 0, inf},
+                                      HistogramOptions{4, -1e308, 1e308}})
+        rejected = rejected && throws_as([&] { histogram_rgb(probe, bounds, request, o); }, 2);
+    rejected = rejected && probe.calls == calls;
+    rejected = rejected && throws_as([&] { histogram_rgb(probe, bounds, RenderRequest{Rect{9, 7, 0, 1}}); }, 2);
+    rejected = rejected && throws_as([&] { histogram_rgb(probe, bounds, RenderRequest{Rect{36, 26, 4, 4}}); }, 1);
+    for (int f = 1; f <= 3; ++f) {
+        probe.fault = f;
+        rejected = rejected && throws_as([&] { histogram_rgb(probe, bounds, request); }, 2);
+    }
+    probe.fault = 0;
+    CHECK(rejected, "histogram_rgb: invalid options (before rendering), viewports and malformed tiles rejected");
+    CancellationToken pre;
+    pre.cancel();
+    const unsigned before = probe.calls;
+    bool cancelled = throws_as([&] { histogram_rgb(probe, bounds, request, {}, &pre); }, 4) && probe.calls == before;
+    CancellationToken during;
+    probe.cancel_on_render = &during;
+    cancelled = cancelled && throws_as([&] { histogram_rgb(probe, bounds, RenderRequest{bounds}, {}, &during); }, 4);
+    probe.cancel_on_render = nullptr;
+    CHECK(cancelled, "histogram_rgb: cancellation before and during the render throws");
+    RawMetadata m;
+    m.width = 5; m.height = 3; m.black_levels.fill(0); m.white_levels.fill(65535);
+    ImageGraph g(RawImage(m, std::vector<uint16_t>(15, 32768)), GraphRecipe{});
+    const auto gh = histogram_rgb(g, RenderRequest{Rect{0, 0, 5, 3}, 2});
+    bool covered = gh.pixel_count == 15 && gh.descriptor == g.output().output_descriptor();
+    for (const auto& c : gh.channels) {
+        uint64_t n = c.underflow + c.overflow + c.nonfinite;
+        for (auto k : c.counts) n += k;
+        covered = covered && n == 15;
+    }
+    CHECK(covered, "histogram_rgb: ImageGraph overload covers every pixel on the device path");
+}
 
 int main() {
     const std::array<double, 9> identity{1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -170,6 +266,7 @@ int main() {
         CHECK(worst <= 1e-5f, "GuidedFilter(r=4)+Exposure == oracle within 1e-5");
         CHECK(throws_as([&] { WorkingYGuidedFilterNode(raster, Rect{0, 0, W, H}, {9, 1e-3}); }, 2), "radius 9 rejected");
     }
+    histogram_checks();
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }
