@@ -460,6 +460,30 @@ def main():
                        "backward": ("menon_backward<64,20> (classifier recomputed)" if args.no_record else
                                     "menon_backward_recorded<64,20> (classifier decisions recorded by the forward) + menon_backward<64,20> on border tiles")
                                    + ": dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
+    # Analysis of the rendered frame: histogram_rgb (256 bins of [0, 1]) over the
+    # 24 MP FP32 output, reading 12 B/px once.
+    hbins = 256
+    hcounts = torch.zeros(3 * hbins, dtype=torch.int64, device="cuda")
+    htallies = torch.zeros(9, dtype=torch.int64, device="cuda")
+    hext = torch.tensor([-1, 0] * 3, dtype=torch.int32, device="cuda")
+
+    def histogram():
+        C.check(C.lib().rawops_histogram_rgb(ctypes.c_void_p(out.data_ptr()), area.width, area.height, area.width,
+                                             hbins, 0.0, 1.0, ctypes.c_void_p(hcounts.data_ptr()),
+                                             ctypes.c_void_p(htallies.data_ptr()), ctypes.c_void_p(hext.data_ptr()), sp))
+    for _ in range(5):
+        histogram()
+    h0, h1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    h0.record(stream)
+    for _ in range(20):
+        histogram()
+    h1.record(stream)
+    torch.cuda.synchronize()
+    hist_ms = h0.elapsed_time(h1) / 20
+    hist_gbs = 12 * px / (hist_ms * 1e-3) / 1e9
+    line["histogram_rgb"] = {"kernel_ms": round(hist_ms, 4), "bins": hbins, "achieved_gbs": round(hist_gbs, 1),
+                             "frac": round(hist_gbs / PEAK_HBM_GBS, 4),
+                             "workload": "histogram_rgb of the 24 MP FP32 render (12 B/px read once)"}
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["fwd_bwd"]["gpu_fwd_bwd_over_cpu_fwd"] = round(line["fwd_bwd"]["value"] / cpu["value"], 1)

@@ -4,7 +4,7 @@
 // and the finite extrema, accumulated into device counters so that an image
 // can be analysed band by band.
 //
-// One pass over the pixels (grid-stride, several pixels per thread).  With up
+// One pass over the pixels (a band of rows per workgroup).  With up
 // to kLdsBins bins a workgroup counts into LDS and adds its nonzero bins to
 // the global counters once; wider histograms count with global atomics.  The
 // tallies and extrema are reduced per wave with shuffles and added by one
@@ -35,11 +35,13 @@ __global__ void __launch_bounds__(kHistThreads) histogram_rgb(const HistogramLau
     }
     uint32_t under[3] = {0, 0, 0}, over[3] = {0, 0, 0}, nonfinite[3] = {0, 0, 0};
     uint32_t kmin[3] = {~0u, ~0u, ~0u}, kmax[3] = {0, 0, 0};
-    const size_t n = (size_t)h.width * h.height;
     const double bins = (double)h.bins;
-    for (size_t p = (size_t)blockIdx.x * kHistThreads + threadIdx.x; p < n; p += (size_t)gridDim.x * kHistThreads) {
-        const size_t y = p / h.width, x = p - y * h.width;
-        const float* q = h.rgb + (y * h.stride + x) * 3;
+    // a band of whole rows per workgroup, threads across each row
+    const uint32_t rows = (h.height + gridDim.x - 1) / gridDim.x;
+    const uint32_t y0 = blockIdx.x * rows, y1 = min(h.height, y0 + rows);
+    for (uint32_t y = y0; y < y1; ++y)
+    for (uint32_t x = threadIdx.x; x < h.width; x += kHistThreads) {
+        const float* q = h.rgb + ((size_t)y * h.stride + x) * 3;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float s = q[c];
@@ -87,9 +89,11 @@ __global__ void __launch_bounds__(kHistThreads) histogram_rgb(const HistogramLau
 hipError_t launch_histogram(const HistogramLaunch& h, hipStream_t stream) {
     const size_t n = (size_t)h.width * h.height;
     if (!n) return hipSuccess;
-    // at least eight pixels per thread: the LDS clear and flush are per workgroup
+    // At least eight pixels per thread, and at most 512 workgroups: each adds
+    // its bins to the same global counters (1024 or 2048 workgroups, or
+    // 512- and 1024-thread ones, measured slower on a 24 MP frame).
     const size_t want = (n + (size_t)kHistThreads * 8 - 1) / ((size_t)kHistThreads * 8);
-    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(want, 4096));
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>({want, 512, (size_t)h.height}));
     hipLaunchKernelGGL(histogram_rgb, dim3(blocks), dim3(kHistThreads), 0, stream, h);
     return hipGetLastError();
 }
