@@ -263,7 +263,9 @@ __device__ __forceinline__ void fetch_upstream(const BackwardArgs& a, int ex, in
             auto fetch = [&](int u, float go[3]) -> bool {
                 const int lx = ex + u - a.vx, lxc = min(max(lx, 0), a.vw - 1);
                 const float* p = prow + (size_t)lxc * 3;
-                go[0] = p[0]; go[1] = p[1]; go[2] = p[2];
+                // read once: non-temporal, so the stream does not evict the samples
+                go[0] = __builtin_nontemporal_load(p); go[1] = __builtin_nontemporal_load(p + 1);
+                go[2] = __builtin_nontemporal_load(p + 2);
                 return row && lx == lxc;
             };
             if (fetch(un, up.goN[j][b])) up.hasN |= 1u << (2 * j + b);
@@ -383,8 +385,8 @@ __device__ __forceinline__ void interior_gather(const BackwardArgs& a, const Sen
                               0.25f * (AGv[c - 2 * HWE] + AGv[c + 2 * HWE]);
             if (a.graw) {
                 float* row = a.graw + (size_t)(ty + v - 4 - s.ay) * s.aw + (size_t)(tx - 4 - s.ax);
-                row[un] = rnv;
-                row[ug] = rgv;
+                __builtin_nontemporal_store(rnv, row + un);
+                __builtin_nontemporal_store(rgv, row + ug);
             }
         }
     }

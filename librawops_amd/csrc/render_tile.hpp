@@ -159,11 +159,15 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
                     float* o = a.out + ((size_t)ly * a.ostride + (size_t)lx) * 3;
                     const float* p0 = px[(2 * j + b) * 2];
                     const float* p1 = px[(2 * j + b) * 2 + 1];
+                    // Streaming stores (non-temporal): the frame's output should not
+                    // push its input samples out of the caches.
                     if (lx + 1 < a.vw) {
-                        o[0] = p0[0]; o[1] = p0[1]; o[2] = p0[2];
-                        o[3] = p1[0]; o[4] = p1[1]; o[5] = p1[2];
+                        __builtin_nontemporal_store(p0[0], o); __builtin_nontemporal_store(p0[1], o + 1);
+                        __builtin_nontemporal_store(p0[2], o + 2); __builtin_nontemporal_store(p1[0], o + 3);
+                        __builtin_nontemporal_store(p1[1], o + 4); __builtin_nontemporal_store(p1[2], o + 5);
                     } else {
-                        o[0] = p0[0]; o[1] = p0[1]; o[2] = p0[2];
+                        __builtin_nontemporal_store(p0[0], o); __builtin_nontemporal_store(p0[1], o + 1);
+                        __builtin_nontemporal_store(p0[2], o + 2);
                     }
                 }
             }
