@@ -19,6 +19,7 @@ namespace {
 
 constexpr int kHistThreads = 256;
 constexpr uint32_t kLdsBins = 2048;
+constexpr int kPix = 4;           // pixels whose loads each thread has in flight together
 
 __device__ __forceinline__ uint32_t order_key(float v) {
     const uint32_t b = __builtin_bit_cast(uint32_t, v);
@@ -36,25 +37,58 @@ __global__ void __launch_bounds__(kHistThreads) histogram_rgb(const HistogramLau
     uint32_t under[3] = {0, 0, 0}, over[3] = {0, 0, 0}, nonfinite[3] = {0, 0, 0};
     uint32_t kmin[3] = {~0u, ~0u, ~0u}, kmax[3] = {0, 0, 0};
     const double bins = (double)h.bins;
-    // a band of whole rows per workgroup, threads across each row
+    // The bin is trunc((sample - lower) / span * bins) with a correctly
+    // rounded division.  A product by the reciprocal differs from that
+    // quotient times bins by at most 5 units in the last place (two more
+    // roundings on each side), so where it lies farther than a relative 1e-13
+    // from every bin edge both truncate alike; samples nearer an edge take the
+    // division.  Reciprocals outside [1e-300, 1e300] always divide.
+    const double inv = 1.0 / h.span;
+    const bool by_inverse = inv >= 1e-300 && inv <= 1e300;
+    // a band of whole rows per workgroup, threads across each row; every
+    // thread issues the loads of kPix pixels before it counts any of them
     const uint32_t rows = (h.height + gridDim.x - 1) / gridDim.x;
     const uint32_t y0 = blockIdx.x * rows, y1 = min(h.height, y0 + rows);
-    for (uint32_t y = y0; y < y1; ++y)
-    for (uint32_t x = threadIdx.x; x < h.width; x += kHistThreads) {
-        const float* q = h.rgb + ((size_t)y * h.stride + x) * 3;
+    for (uint32_t y = y0; y < y1; ++y) {
+        const float* row = h.rgb + (size_t)y * h.stride * 3;
+        for (uint32_t x0 = 0; x0 < h.width; x0 += kHistThreads * kPix) {
+            float v[kPix][3];
+            bool live[kPix];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float s = q[c];
-            if (!__builtin_isfinite(s)) { ++nonfinite[c]; continue; }
-            const uint32_t k = order_key(s);
-            kmin[c] = min(kmin[c], k);
-            kmax[c] = max(kmax[c], k);
-            if ((double)s < h.lower) { ++under[c]; continue; }
-            if ((double)s > h.upper) { ++over[c]; continue; }
-            // Divide before multiplying (no overflow for extreme ranges).
-            const uint32_t bin = min(h.bins - 1, (uint32_t)(((double)s - h.lower) / h.span * bins));
-            if (in_lds) atomicAdd(&lh[c * h.bins + bin], 1u);
-            else atomicAdd(h.counts + (size_t)c * h.bins + bin, 1ull);
+            for (int p = 0; p < kPix; ++p) {
+                const uint32_t x = x0 + threadIdx.x + p * kHistThreads;
+                live[p] = x < h.width;
+                const float* q = row + (size_t)min(x, h.width - 1) * 3;
+                v[p][0] = q[0]; v[p][1] = q[1]; v[p][2] = q[2];
+            }
+#pragma unroll
+            for (int p = 0; p < kPix; ++p)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float s = v[p][c];
+                const bool fin = live[p] && __builtin_isfinite(s);
+                nonfinite[c] += live[p] && !fin;
+                const uint32_t k = order_key(s);
+                kmin[c] = fin ? min(kmin[c], k) : kmin[c];
+                kmax[c] = fin ? max(kmax[c], k) : kmax[c];
+                const double d = (double)s;
+                const bool lo = fin && d < h.lower, hi = fin && d > h.upper;
+                under[c] += lo;
+                over[c] += hi;
+                if (fin && !lo && !hi) {
+                    const double a = d - h.lower;
+                    const double t1 = a * inv * bins;
+                    const double e = rint(t1);
+                    uint32_t bin;
+                    if (!by_inverse || (e >= 1.0 && e < bins && fabs(t1 - e) <= 1e-13 * t1))
+                        bin = (uint32_t)(a / h.span * bins);      // Divide before multiplying (no overflow for extreme ranges).
+                    else
+                        bin = (uint32_t)t1;
+                    bin = min(h.bins - 1, bin);
+                    if (in_lds) atomicAdd(&lh[c * h.bins + bin], 1u);
+                    else atomicAdd(h.counts + (size_t)c * h.bins + bin, 1ull);
+                }
+            }
         }
     }
 #pragma unroll

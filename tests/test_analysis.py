@@ -120,6 +120,27 @@ def test_device_histogram_matches_oracle(bins, lower, upper):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bins,lower,upper", [(256, 0.0, 1.0), (100, -0.3, 1.7), (1000, 0.1, 0.7), (7, -1.0, 2.5),
+                                              (65536, 0.0, 1.0)])
+def test_device_histogram_bin_edges(bins, lower, upper):
+    """Samples on and one float32 step either side of every bin edge: the
+    device bins them by a reciprocal product and divides only near edges, so
+    these are the samples where the two could part."""
+    k = np.arange(bins + 1, dtype=np.float64)
+    edge = (lower + k * ((upper - lower) / bins)).astype(np.float32)
+    vals = np.concatenate([edge, np.nextafter(edge, np.float32(-np.inf)), np.nextafter(edge, np.float32(np.inf))])
+    w = 256
+    n = -(-vals.size // w) * w
+    flat = np.resize(vals, n)
+    img = np.stack([flat, np.roll(flat, 1), np.roll(flat, 2)], axis=-1).reshape(n // w, w, 3)
+    want = O.histogram_rgb(img, bins, lower, upper)
+    got = device_histogram(img, bins, lower, upper)
+    for c in range(3):
+        assert np.array_equal(got[c][0], want[c][0]), c
+        assert got[c][1:] == want[c][1:], c
+
+
+@pytest.mark.gpu
 def test_histogram_rgb_api():
     import librawops_amd as L
     from tests.helpers import random_samples
