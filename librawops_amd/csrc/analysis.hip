@@ -26,9 +26,10 @@ __device__ __forceinline__ uint32_t order_key(float v) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+template <bool IN_LDS>
 __global__ void __launch_bounds__(kHistThreads) histogram_rgb(const HistogramLaunch h) {
-    __shared__ uint32_t lh[3 * kLdsBins];
-    const bool in_lds = h.bins <= kLdsBins;
+    __shared__ uint32_t lh[IN_LDS ? 3 * kLdsBins : 1];
+    constexpr bool in_lds = IN_LDS;
     const uint32_t nb = 3 * h.bins;
     if (in_lds) {
         for (uint32_t i = threadIdx.x; i < nb; i += kHistThreads) lh[i] = 0;
@@ -61,6 +62,10 @@ __global__ void __launch_bounds__(kHistThreads) histogram_rgb(const HistogramLau
                 const float* q = row + (size_t)min(x, h.width - 1) * 3;
                 v[p][0] = q[0]; v[p][1] = q[1]; v[p][2] = q[2];
             }
+            // Every sample's bin from the reciprocal product, without branches
+            // (so that the samples' FP64 chains overlap); the division only
+            // for samples near an edge, after the batch, when a lane has one.
+            uint32_t bin[kPix][3], inc[kPix][3], near = 0;
 #pragma unroll
             for (int p = 0; p < kPix; ++p)
 #pragma unroll
@@ -75,19 +80,28 @@ __global__ void __launch_bounds__(kHistThreads) histogram_rgb(const HistogramLau
                 const bool lo = fin && d < h.lower, hi = fin && d > h.upper;
                 under[c] += lo;
                 over[c] += hi;
-                if (fin && !lo && !hi) {
-                    const double a = d - h.lower;
-                    const double t1 = a * inv * bins;
-                    const double e = rint(t1);
-                    uint32_t bin;
-                    if (!by_inverse || (e >= 1.0 && e < bins && fabs(t1 - e) <= 1e-13 * t1))
-                        bin = (uint32_t)(a / h.span * bins);      // Divide before multiplying (no overflow for extreme ranges).
-                    else
-                        bin = (uint32_t)t1;
-                    bin = min(h.bins - 1, bin);
-                    if (in_lds) atomicAdd(&lh[c * h.bins + bin], 1u);
-                    else atomicAdd(h.counts + (size_t)c * h.bins + bin, 1ull);
-                }
+                const bool counted = fin && !lo && !hi;
+                const double t1 = (d - h.lower) * inv * bins;
+                const double e = rint(t1);
+                const bool edge = !by_inverse || (e >= 1.0 && e < bins && fabs(t1 - e) <= 1e-13 * t1);
+                near |= (uint32_t)(counted && edge) << (p * 3 + c);
+                bin[p][c] = min(h.bins - 1, counted ? (uint32_t)t1 : 0u);
+                inc[p][c] = counted;
+            }
+            if (near) {
+#pragma unroll
+                for (int p = 0; p < kPix; ++p)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (near >> (p * 3 + c) & 1u)     // Divide before multiplying (no overflow for extreme ranges).
+                        bin[p][c] = min(h.bins - 1, (uint32_t)(((double)v[p][c] - h.lower) / h.span * bins));
+            }
+#pragma unroll
+            for (int p = 0; p < kPix; ++p)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (in_lds) atomicAdd(&lh[c * h.bins + bin[p][c]], inc[p][c]);
+                else if (inc[p][c]) atomicAdd(h.counts + (size_t)c * h.bins + bin[p][c], 1ull);
             }
         }
     }
@@ -130,7 +144,8 @@ hipError_t launch_histogram(const HistogramLaunch& h, hipStream_t stream) {
     // threads 0.35 ms; per-wave bin copies gained nothing).
     const size_t want = (n + (size_t)kHistThreads * 8 - 1) / ((size_t)kHistThreads * 8);
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>({want, 512, (size_t)h.height}));
-    hipLaunchKernelGGL(histogram_rgb, dim3(blocks), dim3(kHistThreads), 0, stream, h);
+    if (h.bins <= kLdsBins) hipLaunchKernelGGL(histogram_rgb<true>, dim3(blocks), dim3(kHistThreads), 0, stream, h);
+    else hipLaunchKernelGGL(histogram_rgb<false>, dim3(blocks), dim3(kHistThreads), 0, stream, h);
     return hipGetLastError();
 }
 
