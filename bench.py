@@ -172,41 +172,35 @@ def leg_c3_guided(steps, radius=8, eps=1e-3):
 
 def leg_c4_batch(replays, seed=4):
     """64 heterogeneous requests over 8 Bayer sources (2000x1500 .. 6048x4024,
-    RGGB/GRBG): viewports 256^2 .. full frame, (mip 0, Final) / (mip 1 or 2,
-    Preview), bilinear and Menon, FP32 linear + uint8 sRGB preview, one
-    captured persistent launch (autotuned launch shape)."""
+    RGGB/GRBG; synth.c4_workload): viewports 256^2 .. full frame, (mip 0,
+    Final) / (mip 1 or 2, Preview), bilinear and Menon, FP32 linear + uint8
+    sRGB preview, one hipGraph-captured launch (device tile shape chosen by
+    the autotuner).  Checked request by request against the oracle in
+    tests/test_c4_batch_gpu.py."""
     import numpy as np
     import torch
     import librawops_amd as L
     from librawops_amd import synth
     from librawops_amd.batch import Batch, BatchRequest
-    rng = np.random.default_rng(seed)
-    sizes = [(int(w) // 2 * 2, int(h) // 2 * 2) for w, h in zip(np.linspace(2000, 6048, 8), np.linspace(1500, 4024, 8))]
-    cam = tuple(synth.camera_to_xyz_for(synth.camera_to_prophoto_seeded(seed)))
+    sources, specs, cam_w = synth.c4_workload(seed)
+    cam = tuple(synth.camera_to_xyz_for(cam_w))
+    recipe = L.GraphRecipe(red_gain=synth.C4_GAINS[0], green_gain=synth.C4_GAINS[1], blue_gain=synth.C4_GAINS[2],
+                           exposure_stops=synth.C4_STOPS,
+                           camera_color=L.CameraColorTransform(cam, L.WorkingSpace.LinearProPhotoD50),
+                           output_mode=L.OutputMode.SrgbPreview)
     graphs = []
-    for i, (w, h) in enumerate(sizes):
-        pat = 0 if i % 2 == 0 else 3
-        s = synth.gradient_bayer(w, h, pat, 512, 16383, 20.0, seed=seed + i)
-        md = L.RawMetadata(w, h, w, L.BayerPattern(pat), 0, 0, L.Rect(0, 0, 0, 0), (512,) * 4, (16383,) * 4)
+    for m, s in sources:
+        md = L.RawMetadata(m["width"], m["height"], m["row_stride_samples"], L.BayerPattern(m["pattern"]), 0, 0,
+                           L.Rect(*m["active_area"]), m["black_levels"], m["white_levels"])
         img = L.RawImage(md, torch.from_numpy(s.view(np.int16)).cuda())
-        rec = L.GraphRecipe(red_gain=1.9, blue_gain=1.4, exposure_stops=0.3,
-                            camera_color=L.CameraColorTransform(cam, L.WorkingSpace.LinearProPhotoD50),
-                            output_mode=L.OutputMode.SrgbPreview)
-        graphs.append([L.ImageGraph(img, rec, demosaic=d) for d in (L.BILINEAR, L.MENON)])
+        graphs.append({d: L.ImageGraph(img, recipe, demosaic=L.MENON if d == "menon" else L.BILINEAR)
+                       for d in ("bilinear", "menon")})
     reqs, native_px, level_px = [], 0, 0
-    for k in range(64):
-        i = int(rng.integers(0, 8))
-        g = graphs[i][k % 2]
-        mip = int(rng.integers(0, 3))
+    for r in specs:
+        x, y, w, h = r["viewport"]
+        mip = r["mip"]
         level = L.RenderLevel(mip, L.RenderQuality.Final if mip == 0 else L.RenderQuality.Preview)
-        b = L.request_bounds(g.source_bounds(), level)
-        lo = max(1, 256 >> mip)
-        area = float(np.exp(rng.uniform(np.log(lo * lo), np.log(b.width * b.height))))
-        aspect = b.width / b.height
-        w = int(min(b.width, max(lo, round((area * aspect) ** 0.5))))
-        h = int(min(b.height, max(lo, round(area / w))))
-        vp = L.Rect(b.x + int(rng.integers(0, b.width - w + 1)), b.y + int(rng.integers(0, b.height - h + 1)), w, h)
-        reqs.append(BatchRequest(g.output(), vp, level, 0,
+        reqs.append(BatchRequest(graphs[r["source"]][r["demosaic"]].output(), L.Rect(x, y, w, h), level, 0,
                                  torch.empty((h, w, 3), dtype=torch.float32, device="cuda"),
                                  torch.empty((h, w, 3), dtype=torch.uint8, device="cuda")))
         level_px += w * h
@@ -223,12 +217,12 @@ def leg_c4_batch(replays, seed=4):
     tw, th, tuned = batch.shape()
     ms = total / replays
     return {"workload": "64 requests, 8 sources 2000x1500..6048x4024 RGGB/GRBG, viewports 256^2..full, mip 0/1/2, "
-                        "bilinear+Menon, FP32 linear + uint8 sRGB preview, one hipGraph launch (autotuned: persistent "
-                        "claim loop or one workgroup per tile)",
+                        "bilinear+Menon, FP32 linear + uint8 sRGB preview, one hipGraph launch",
             "batch_ms": round(ms, 3), "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
             "aggregate_output_mpix_s": round(level_px / ms / 1e3, 1),
             "aggregate_native_mpix_s": round(native_px / ms / 1e3, 1),
+            "native_mpix": round(native_px / 1e6, 2),
             "tile": f"{tw}x{th}", "autotune_ms": round(tuned, 3)}
 
 

@@ -84,15 +84,26 @@ def histogram_rgb(target, *args, options: HistogramOptions = None, cancellation=
     if not request.tile_size:
         raise InvalidArgument("callback and tile size are required")
     import torch
-    image = Renderer()._render_viewport_device(node, v, request.level)
     counts = torch.zeros(3 * bins, dtype=torch.int64, device="cuda")
     tallies = torch.zeros(9, dtype=torch.int64, device="cuda")
     extrema = torch.tensor([-1, 0] * 3, dtype=torch.int32, device="cuda")    # keys 0xFFFFFFFF, 0
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    C.check(C.lib().rawops_histogram_rgb(ctypes.c_void_p(image.data_ptr()), v.width, v.height, v.width, bins,
-                                         lower, upper, ctypes.c_void_p(counts.data_ptr()),
-                                         ctypes.c_void_p(tallies.data_ptr()), ctypes.c_void_p(extrema.data_ptr()),
-                                         stream))
+    # Bands of whole rows (at least a tile high, up to ~8 M pixels), each
+    # rendered on the device and added into the same counters: the histogram
+    # does not depend on how the viewport is partitioned, and the device
+    # staging stays one band, not the whole viewport.
+    band_h = min(v.height, max(int(request.tile_size), (8 << 20) // v.width + 1))
+    y = v.y
+    while y < v.y + v.height:
+        if cancellation is not None and cancellation.is_cancelled():
+            raise RenderCancelled("render cancelled")
+        band = type(v)(v.x, y, v.width, min(band_h, v.y + v.height - y))
+        image = Renderer()._render_viewport_device(node, band, request.level)
+        C.check(C.lib().rawops_histogram_rgb(ctypes.c_void_p(image.data_ptr()), band.width, band.height, band.width,
+                                             bins, lower, upper, ctypes.c_void_p(counts.data_ptr()),
+                                             ctypes.c_void_p(tallies.data_ptr()),
+                                             ctypes.c_void_p(extrema.data_ptr()), stream))
+        y += band.height
     if cancellation is not None and cancellation.is_cancelled():
         raise RenderCancelled("render cancelled")
     c = counts.cpu().numpy().view(np.uint64).reshape(3, bins)

@@ -857,8 +857,11 @@ RgbHistogram histogram_rgb(const Node& output, Rect source_bounds, const RenderR
         // A foreign node renders through its own code; each tile must be the
         // one the renderer asked for before its storage is read.
         uint64_t next_x = v.x, next_y = v.y;
-        detail::DeviceBuffer dev(size_t(std::max<uint32_t>(1, request.tile_size)) *
-                                 std::max<uint32_t>(1, request.tile_size) * 3 * sizeof(float));
+        // The largest tile the loop can hand out is min(tile_size, viewport)
+        // on each side (64-bit product: no wrap for any tile_size).
+        const uint64_t tile_w = std::min<uint64_t>(std::max<uint32_t>(1, request.tile_size), v.width);
+        const uint64_t tile_h = std::min<uint64_t>(std::max<uint32_t>(1, request.tile_size), v.height);
+        detail::DeviceBuffer dev(size_t(tile_w * tile_h * 3 * sizeof(float)));
         Renderer{}.render_tiles(output, source_bounds, request, [&](const Tile& tile) {
             check_cancelled();
             const Rect expected{uint32_t(next_x), uint32_t(next_y),

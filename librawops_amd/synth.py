@@ -143,3 +143,50 @@ def monotone_knots(n=16, seed=3):
     ys = np.cumsum(np.concatenate([[0.0], rng.uniform(0.05, 1.0, n - 1)]))
     ys = ys / ys[-1] * 1.2
     return list(zip(xs.tolist(), ys.tolist()))
+
+
+# Recipe of every C4 request: WB (1.9, 1, 1.4), +0.3 EV, camera->ProPhoto,
+# SrgbPreview output (tone shoulder 0.25, gamma 1).
+C4_GAINS = (1.9, 1.0, 1.4)
+C4_STOPS = 0.3
+
+
+def c4_workload(seed=4, count=64):
+    """The heterogeneous batch: ``count`` render requests over 8 sources.
+
+    Sources: 14-bit Bayer mosaics of the config-2 scene generator, sized
+    2000x1500 .. 6048x4024 (evenly spaced), RGGB for even and GRBG for odd
+    source numbers, tight rows, whole-sensor active area, black 512, white
+    16383.  Requests: a random source, bilinear (even k) or Menon (odd k),
+    level (mip 0, Final) / (mip 1 or 2, Preview), a viewport of log-uniform
+    area between 256^2 (at the level) and the whole level bounds at the
+    source's aspect ratio and a random offset, and a render_tiles tile size of
+    128 / 256 / 512.  Returns (sources [(meta dict, samples)], requests
+    [dict(source, demosaic, mip, viewport (x, y, w, h), tile_size)],
+    camera->ProPhoto matrix)."""
+    rng = np.random.default_rng(seed)
+    sizes = [(int(w) // 2 * 2, int(h) // 2 * 2)
+             for w, h in zip(np.linspace(2000, 6048, 8), np.linspace(1500, 4024, 8))]
+    sources = []
+    for i, (w, h) in enumerate(sizes):
+        pat = 0 if i % 2 == 0 else 2
+        meta = dict(width=w, height=h, row_stride_samples=w, pattern=pat, cfa_phase_x=0, cfa_phase_y=0,
+                    active_area=(0, 0, w, h), black_levels=(512,) * 4, white_levels=(16383,) * 4)
+        sources.append((meta, scene_bayer(w, h, pat, 0, 0, w, (0, 0, w, h), (512,) * 4, 16383,
+                                          seed=seed * 100 + i)))
+    requests = []
+    for k in range(count):
+        i = int(rng.integers(0, 8))
+        mip = int(rng.integers(0, 3))
+        aw, ah = sizes[i]
+        s = 1 << mip
+        bw, bh = -(-aw // s), -(-ah // s)          # request_bounds, RawEngine.cpp:31-43
+        lo = max(1, 256 >> mip)
+        area = float(np.exp(rng.uniform(np.log(lo * lo), np.log(bw * bh))))
+        w = int(min(bw, max(lo, round((area * bw / bh) ** 0.5))))
+        h = int(min(bh, max(lo, round(area / w))))
+        x = int(rng.integers(0, bw - w + 1))
+        y = int(rng.integers(0, bh - h + 1))
+        requests.append(dict(source=i, demosaic="menon" if k % 2 else "bilinear", mip=mip, viewport=(x, y, w, h),
+                             tile_size=(128, 256, 512)[k % 3]))
+    return sources, requests, camera_to_prophoto_seeded(seed)

@@ -350,8 +350,10 @@ def request_bounds(area, mip):
     return (0, 0, -(-area[2] // s), -(-area[3] // s))
 
 
-def render_raw(meta: Bayer, samples, chain: Chain, viewport, mip=0):
-    """Whole-viewport render of ``chain`` (what Renderer::render_image returns)."""
+def render_raw(meta: Bayer, samples, chain: Chain, viewport, mip=0, threads=1):
+    """Whole-viewport render of ``chain`` (what Renderer::render_image returns).
+    ``threads`` > 1 runs Menon in row bands (the same pixels: each band
+    reads its full support, MenonDemosaic.cpp:177-218 blocks are independent)."""
     area = meta.area
     b = request_bounds(area, mip)
     if (viewport[0] < b[0] or viewport[1] < b[1] or viewport[0] + viewport[2] > b[0] + b[2]
@@ -359,7 +361,10 @@ def render_raw(meta: Bayer, samples, chain: Chain, viewport, mip=0):
         raise IndexError("viewport is outside the active RAW area")   # RawEngine.cpp:22-29
 
     def native(rect):
-        t = demosaic(meta, samples, rect, chain.demosaic)
+        if chain.demosaic == "menon" and threads > 1 and meta.area[2] > 1 and meta.area[3] > 1:
+            t = menon_bands(meta, samples, rect, threads)
+        else:
+            t = demosaic(meta, samples, rect, chain.demosaic)
         scale3(t, chain.wb)
         scale(t, exp2f(chain.exposure_stops))
         if chain.camera_to_xyz is not None:

@@ -111,7 +111,8 @@ static void histogram_checks() {
     edges = edges && h.channels[2].nonfinite == 247 && !h.channels[2].minimum && !h.channels[2].maximum;
     CHECK(edges, "histogram_rgb: bin edges, outliers, nonfinite, extrema over a foreign node's tiles");
     bool partition = true;
-    for (uint32_t ts : {1u, 3u, 7u, 256u}) {
+    // tile sizes far beyond the viewport stage min(tile_size, viewport) per side
+    for (uint32_t ts : {1u, 3u, 7u, 256u, 65536u, 0x80000000u}) {
         const auto o = histogram_rgb(probe, bounds, RenderRequest{request.viewport, ts}, HistogramOptions{4, 0.0, 1.0});
         for (int c = 0; c < 3; ++c)
             partition = partition && o.channels[c].counts == h.channels[c].counts &&

@@ -141,6 +141,33 @@ def test_device_histogram_bin_edges(bins, lower, upper):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("width", [2049, 4100, 6001])
+@pytest.mark.parametrize("bins,lower,upper", [(256, 0.0, 1.0), (1000, -0.3, 1.7), (4096, 0.0, 1.0)])
+def test_device_histogram_wide_rows(width, bins, lower, upper):
+    """Rows wider than one pass of the workgroup (256 threads x 8 pixel
+    slots): every pixel slot p >= 1, column batches starting at 2048 and
+    beyond, and a ragged last batch.  Bin-edge samples (on, and one float32
+    step either side of, every edge) sit at columns 256 p + j, so each slot
+    meets them."""
+    img = random_image(7, width, seed=width + bins)
+    k = np.arange(bins + 1, dtype=np.float64)
+    edge = (lower + k * ((upper - lower) / bins)).astype(np.float32)
+    vals = np.concatenate([edge, np.nextafter(edge, np.float32(-np.inf)), np.nextafter(edge, np.float32(np.inf))])
+    cols = (256 * np.arange(width // 256 + 1)[:, None] + np.arange(8)[None, :]).reshape(-1)
+    cols = cols[cols < width]
+    rng = np.random.default_rng(bins)
+    for r in range(img.shape[0]):
+        for c in range(3):
+            img[r, cols, c] = rng.choice(vals, cols.size)
+    want = O.histogram_rgb(img, bins, lower, upper)
+    for bands in (1, 2):
+        got = device_histogram(img, bins, lower, upper, bands)
+        for c in range(3):
+            assert np.array_equal(got[c][0], want[c][0]), (c, bands)
+            assert got[c][1:] == want[c][1:], (c, bands)
+
+
+@pytest.mark.gpu
 def test_histogram_rgb_api():
     import librawops_amd as L
     from tests.helpers import random_samples

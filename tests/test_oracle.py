@@ -212,6 +212,69 @@ def test_guided_filter_rational_kernel():
             assert np.max(np.abs(out - exp)) <= 1e-6 * max(1.0, np.max(np.abs(exp)))
 
 
+def _box_gf(rgb, radius, eps, wr, wb):
+    """Guided filter by its textbook closed form (He et al.; the expanded
+    kernel of python_guided_filter_tests.py:104-125 summed over centres):
+    per centre k, a_k = cov_k(g, p) / (var_k(g) + eps), b_k = mean_k(p) -
+    a_k mean_k(g) over the window clipped to the image; output = mean over
+    the centres whose window holds the pixel of a_k g + b_k.  Box sums by
+    2-D prefix sums in float64, independent of the oracle's ordered
+    difference sums."""
+    p = np.asarray(rgb, np.float64)
+    H, W = p.shape[:2]
+    g = wr * p[..., 0] + (1.0 - wr - wb) * p[..., 1] + wb * p[..., 2]
+
+    def box(a):
+        S = np.zeros((H + 1, W + 1) + a.shape[2:])
+        S[1:, 1:] = a.cumsum(0).cumsum(1)
+        y0 = np.clip(np.arange(H) - radius, 0, H)[:, None]
+        y1 = np.clip(np.arange(H) + radius + 1, 0, H)[:, None]
+        x0 = np.clip(np.arange(W) - radius, 0, W)[None, :]
+        x1 = np.clip(np.arange(W) + radius + 1, 0, W)[None, :]
+        return S[y1, x1] - S[y0, x1] - S[y1, x0] + S[y0, x0]
+
+    n = box(np.ones((H, W)))
+    mg = box(g) / n
+    mp = box(p) / n[..., None]
+    var = box(g * g) / n - mg * mg
+    cov = box(g[..., None] * p) / n[..., None] - mg[..., None] * mp
+    a = cov / (var + eps)[..., None]
+    b = mp - a * mg[..., None]
+    return (box(a) * g[..., None] + box(b)) / n[..., None]
+
+
+@pytest.mark.parametrize("radius", [12, 16])
+def test_guided_filter_large_radius_closed_form(radius):
+    """Radii the reference does not admit (it stops at 8): the oracle's FP64
+    and FP32 filters against the closed form on images wider and taller than
+    the 33-pixel window, so windows clip on one side, both sides and neither;
+    a planted radius off by one must fail."""
+    from librawops_amd import synth as S
+    img = S.lognormal_raster(41, 37, seed=radius)
+    for space in (0, 1):
+        wr, wb = O.WEIGHTS[space]
+        for eps in (1e-4, 1e-2):
+            want = _box_gf(img, radius, eps, wr, wb)
+            got64 = O.guided_filter_f64(img.astype(np.float64), radius, eps, space)
+            assert np.max(np.abs(got64 - want)) <= 1e-9 * max(1.0, np.max(np.abs(want)))
+            got32 = O.guided_chain(img, space, radius, eps, (0, 0, 41, 37))
+            assert np.max(np.abs(got32 - want)) <= 2e-6 * max(1.0, np.max(np.abs(want)))
+            wrong = _box_gf(img, radius - 1, eps, wr, wb)
+            assert np.max(np.abs(got64 - wrong)) > 1e-4
+
+
+def test_menon_bands_equal_whole_render():
+    """The oracle's banded Menon (the CPU baseline's and the batch test's
+    threaded form) renders the same pixels as the one-call form."""
+    rng = np.random.default_rng(4)
+    s = rng.integers(0, 16383, (131, 214)).astype(np.uint16)
+    m = O.Bayer(210, 131, 214, 1, 1, 0, (3, 2, 203, 125), (510, 512, 511, 513), (16383,) * 4)
+    for rect in [(3, 2, 203, 125), (20, 9, 97, 61)]:
+        whole = O.demosaic(m, s, rect, "menon")
+        for threads in (2, 3, 7):
+            assert _bits(O.menon_bands(m, s, rect, threads), whole), (rect, threads)
+
+
 @needs_ref
 @pytest.mark.parametrize("demosaic", ["bilinear", "menon"])
 def test_oracle_matches_compiled_reference_chains(demosaic):

@@ -9,7 +9,7 @@ import pytest
 import librawops_amd as L
 from oracle import oracle as O
 from tests.golden import menon_fixture
-from tests.helpers import (bits_equal, first_mismatch, gpu_image, gpu_render, meta_dict, oracle_meta,
+from tests.helpers import (bits_equal, first_mismatch, gpu_chain, gpu_image, gpu_render, meta_dict, oracle_meta,
                            random_samples)
 
 pytestmark = pytest.mark.gpu
@@ -185,3 +185,28 @@ def test_validation_errors():
     with pytest.raises(L.RenderCancelled):
         L.Renderer().render_tiles(node, img.metadata.active_area, L.RenderRequest(L.Rect(0, 0, 4, 4)),
                                   lambda t: None, cancellation=tok)
+
+
+@pytest.mark.parametrize("mip", [0, 1, 2])
+@pytest.mark.parametrize("source,target", [(0, 1), (1, 0)])
+def test_working_space_convert_chain(mip, source, target):
+    """CameraToWorking -> WorkingSpaceConvertNode (RawEngine.cpp:603-672) on the
+    device: the FP32 working->working matrix after the reduction, bit for bit
+    (FP32 products, no contraction) against the oracle's chain followed by
+    the same matrix."""
+    m = meta_dict(146, 102, 150, 2, 1, 0, (3, 2, 139, 97), (510, 512, 511, 513), (16383,) * 4)
+    s = random_samples(102, 150, seed=21 + mip, hi=16383)
+    img = gpu_image(m, s)
+    om = oracle_meta(m)
+    chain = O.Chain(demosaic="menon", wb=(1.8, 1.0, 1.3), exposure_stops=0.4, camera_to_xyz=CAM,
+                    working_space=source)
+    node = L.WorkingSpaceConvertNode(gpu_chain(img, chain), L.WorkingSpace(target))
+    q = L.RenderQuality.Final if mip == 0 else L.RenderQuality.Preview
+    b = O.request_bounds(om.area, mip)
+    vp = (b[0] + 3, b[1] + 1, b[2] - 5, b[3] - 2)
+    got = L.Renderer().render_image(node, img.metadata.active_area,
+                                    L.RenderRequest(L.Rect(*vp), 64, L.RenderLevel(mip, q))).rgb
+    exp = O.render_raw(om, s, chain, vp, mip)
+    O.mat3(exp, O.working_to_working_f32(source, target))
+    assert bits_equal(got, exp), first_mismatch(got, exp)
+    assert node.output_descriptor() == L.ImageDescriptor.scene_linear(L.WorkingSpace(target))
