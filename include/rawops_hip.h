@@ -297,6 +297,25 @@ rawops_status rawops_guided_filter(const rawops_source* raster, rawops_rect view
                                    size_t workspace_bytes, uint32_t* device_flags,
                                    void* stream);
 
+/* rawops_guided_filter for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The per-centre coefficient
+ * workspace lives in a static device array of
+ * RAWOPS_GUIDED_STATIC_SCRATCH_BYTES per device (held until the process
+ * ends) instead of caller or stream-ordered memory; the call itself
+ * allocates nothing.  A request whose workspace
+ * (rawops_guided_workspace_size) exceeds that size returns
+ * RAWOPS_LENGTH_ERROR before any launch.  Every call on a device shares the
+ * one array: calls that overlap on a device (concurrent streams or threads)
+ * race and must be serialised by the caller.  Same outputs as
+ * rawops_guided_filter, bit for bit. */
+#define RAWOPS_GUIDED_STATIC_SCRATCH_BYTES (64u << 20)
+rawops_status rawops_guided_filter_static(const rawops_source* raster, rawops_rect viewport,
+                                          uint32_t mip, uint32_t radius, double epsilon,
+                                          const rawops_plan* post, float* dst_rgb,
+                                          uint32_t dst_stride_pixels, uint32_t* device_flags,
+                                          void* stream);
+
 /* ---- analysis (ImageAnalysis.cpp:10-70) ---- */
 /* Per-channel histogram of a device FP32 RGB image (width x height, row
  * stride stride_pixels, 0 = width), accumulated into device counters so that

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "librawops_hip.so")
 _LIB = None
 
 OK, INVALID_ARGUMENT, OUT_OF_RANGE, DOMAIN_ERROR, LENGTH_ERROR, OVERFLOW_ERROR, CANCELLED, LOGIC_ERROR, DEVICE_ERROR = range(9)
+# RAWOPS_GUIDED_STATIC_SCRATCH_BYTES: the static workspace of rawops_guided_filter_static
+GUIDED_STATIC_SCRATCH_BYTES = 64 << 20
 
 OP_GAINS, OP_SCALE, OP_MATRIX, OP_TONE, OP_CLIP, OP_SRGB_ENCODE, OP_CURVES = range(1, 8)
 DEMOSAIC_BILINEAR, DEMOSAIC_MENON = 0, 1
@@ -151,6 +153,7 @@ def lib():
             "rawops_guided_workspace_size": ([u32, u32, Rect, u32, u32, P], S),
             "rawops_guided_filter": ([P, Rect, u32, u32, ctypes.c_double, P, P, u32, P,
                                       ctypes.c_size_t, P, P], S),
+            "rawops_guided_filter_static": ([P, Rect, u32, u32, ctypes.c_double, P, P, u32, P, P], S),
             "rawops_histogram_rgb": ([P, u32, u32, u32, u32, ctypes.c_double, ctypes.c_double, P, P, P, P], S),
             "rawops_guided_filter_stack": ([P, u32, Rect, u32, ctypes.c_double, P, P, u32, P, P], S),
             "rawops_guided_backward_stack": ([P, u32, u32, ctypes.c_double, Rect, P, u32, P, P, P], S),

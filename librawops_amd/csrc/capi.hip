@@ -7,6 +7,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -960,59 +961,119 @@ rawops_status rawops_guided_workspace_size(uint32_t width, uint32_t height, rawo
     });
 }
 
+}  // extern "C"
+
+// Scratch of rawops_guided_filter_static: the per-centre coefficient planes
+// (6 floats per centre) of one call.  One copy per device, held until the
+// process ends; calls that overlap on one device share it and race.
+static constexpr size_t kGuidedScratchBytes = size_t(RAWOPS_GUIDED_STATIC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kGuidedScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress with the device current).  The array's name in
+// host code is not a device address; only this pointer is passed on.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* guided_scratch() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) fail(RAWOPS_DEVICE_ERROR, "device ordinal beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) fail(RAWOPS_DEVICE_ERROR, "static scratch has no device address");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Shared body of rawops_guided_filter and rawops_guided_filter_static.
+// workspace: caller memory of workspace_bytes, the static scratch (static_ws),
+// or NULL (allocated stream-ordered).
+void guided_filter_body(const rawops_source* raster, rawops_rect viewport, uint32_t mip, uint32_t radius,
+                        double epsilon, const rawops_plan* post, float* dst, uint32_t dst_stride, void* workspace,
+                        size_t workspace_bytes, bool static_ws, uint32_t* flags, void* stream) {
+    if (!raster || raster->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires a scene-linear raster");
+    if (radius > 32 || !std::isfinite(epsilon) || epsilon < 0x1p-24 || epsilon > 65536.0)
+        fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires radius 0..32 and finite epsilon [2^-24,65536]");
+    const rawops_rect image = level_bounds({0, 0, raster->width, raster->height}, mip,
+                                           mip ? RAWOPS_QUALITY_PREVIEW : RAWOPS_QUALITY_FINAL);
+    if (viewport.x < image.x || viewport.y < image.y ||
+        uint64_t(viewport.x) + viewport.width > uint64_t(image.x) + image.width ||
+        uint64_t(viewport.y) + viewport.height > uint64_t(image.y) + image.height)
+        fail(RAWOPS_OUT_OF_RANGE, "spatial viewport is outside image bounds");
+    if (!dst_stride) dst_stride = viewport.width;
+    if (!viewport.width || !viewport.height) return;
+    GuidedLaunch L{};
+    L.src = raster->pixels;
+    L.src_stride = raster->stride;
+    L.width = raster->width;
+    L.height = raster->height;
+    L.mip = mip;
+    L.viewport = viewport;
+    L.radius = radius;
+    L.epsilon = epsilon;
+    const auto w = working_y_weights(raster->space);
+    L.wr = w[0];
+    L.wb = w[1];
+    L.out = dst;
+    L.ostride = dst_stride;
+    L.flags = flags;
+    if (!post) fail(RAWOPS_INVALID_ARGUMENT, "guided filter needs a post-filter plan (may be empty)");
+    auto st = static_cast<hipStream_t>(stream);
+    const size_t need = radius ? guided_workspace_bytes(L) : 0;
+    void* owned = nullptr;
+    if (static_ws) {
+        if (need > kGuidedScratchBytes)
+            fail(RAWOPS_LENGTH_ERROR, "guided filter workspace of " + std::to_string(need) +
+                                          " bytes exceeds the static scratch of " +
+                                          std::to_string(kGuidedScratchBytes) + " bytes");
+        if (need) L.workspace = guided_scratch();
+    } else if (need) {
+        if (workspace) {
+            if (workspace_bytes < need) fail(RAWOPS_INVALID_ARGUMENT, "guided filter workspace is too small");
+            L.workspace = workspace;
+        } else {
+            hip_check(hipMallocAsync(&owned, need, st), "hipMallocAsync(guided workspace)");
+            L.workspace = owned;
+        }
+    }
+    // Curve lookups are long-latency (data-dependent table reads, an FP64
+    // division): inside the strip pass they stall its per-row loop, so
+    // they run in a separate pointwise pass over the output.
+    L.post_ops = L.fused_ops = post->reduced.n;
+    for (uint32_t i = 0; i < post->reduced.n; ++i)
+        if (post->reduced.op[i].kind == OP_CURVES) { L.fused_ops = i; break; }
+    const hipError_t e = launch_guided(L, post->device + 1, st);
+    if (owned) (void)hipFreeAsync(owned, st);
+    hip_check(e, "guided launch");
+}
+
+}  // namespace
+
+extern "C" {
+
 rawops_status rawops_guided_filter(const rawops_source* raster, rawops_rect viewport, uint32_t mip,
                                    uint32_t radius, double epsilon, const rawops_plan* post, float* dst,
                                    uint32_t dst_stride, void* workspace, size_t workspace_bytes,
                                    uint32_t* flags, void* stream) {
     return guarded([&] {
-        if (!raster || raster->is_raw) fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires a scene-linear raster");
-        if (radius > 32 || !std::isfinite(epsilon) || epsilon < 0x1p-24 || epsilon > 65536.0)
-            fail(RAWOPS_INVALID_ARGUMENT, "guided filter requires radius 0..32 and finite epsilon [2^-24,65536]");
-        const rawops_rect image = level_bounds({0, 0, raster->width, raster->height}, mip,
-                                               mip ? RAWOPS_QUALITY_PREVIEW : RAWOPS_QUALITY_FINAL);
-        if (viewport.x < image.x || viewport.y < image.y ||
-            uint64_t(viewport.x) + viewport.width > uint64_t(image.x) + image.width ||
-            uint64_t(viewport.y) + viewport.height > uint64_t(image.y) + image.height)
-            fail(RAWOPS_OUT_OF_RANGE, "spatial viewport is outside image bounds");
-        if (!dst_stride) dst_stride = viewport.width;
-        if (!viewport.width || !viewport.height) return;
-        GuidedLaunch L{};
-        L.src = raster->pixels;
-        L.src_stride = raster->stride;
-        L.width = raster->width;
-        L.height = raster->height;
-        L.mip = mip;
-        L.viewport = viewport;
-        L.radius = radius;
-        L.epsilon = epsilon;
-        const auto w = working_y_weights(raster->space);
-        L.wr = w[0];
-        L.wb = w[1];
-        L.out = dst;
-        L.ostride = dst_stride;
-        L.flags = flags;
-        if (!post) fail(RAWOPS_INVALID_ARGUMENT, "guided filter needs a post-filter plan (may be empty)");
-        auto st = static_cast<hipStream_t>(stream);
-        const size_t need = radius ? guided_workspace_bytes(L) : 0;
-        void* owned = nullptr;
-        if (need) {
-            if (workspace) {
-                if (workspace_bytes < need) fail(RAWOPS_INVALID_ARGUMENT, "guided filter workspace is too small");
-                L.workspace = workspace;
-            } else {
-                hip_check(hipMallocAsync(&owned, need, st), "hipMallocAsync(guided workspace)");
-                L.workspace = owned;
-            }
-        }
-        // Curve lookups are long-latency (data-dependent table reads, an FP64
-        // division): inside the strip pass they stall its per-row loop, so
-        // they run in a separate pointwise pass over the output.
-        L.post_ops = L.fused_ops = post->reduced.n;
-        for (uint32_t i = 0; i < post->reduced.n; ++i)
-            if (post->reduced.op[i].kind == OP_CURVES) { L.fused_ops = i; break; }
-        const hipError_t e = launch_guided(L, post->device + 1, st);
-        if (owned) (void)hipFreeAsync(owned, st);
-        hip_check(e, "guided launch");
+        guided_filter_body(raster, viewport, mip, radius, epsilon, post, dst, dst_stride, workspace,
+                           workspace_bytes, false, flags, stream);
+    });
+}
+
+rawops_status rawops_guided_filter_static(const rawops_source* raster, rawops_rect viewport, uint32_t mip,
+                                          uint32_t radius, double epsilon, const rawops_plan* post, float* dst,
+                                          uint32_t dst_stride, uint32_t* flags, void* stream) {
+    return guarded([&] {
+        guided_filter_body(raster, viewport, mip, radius, epsilon, post, dst, dst_stride, nullptr, 0, true, flags,
+                           stream);
     });
 }
 
