@@ -440,6 +440,8 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* src, r
         L.out8 = dst8;
         L.ostride = dst_stride;
         L.flags = flags;
+        L.native_host = &plan->native;
+        L.reduced_host = &plan->reduced;
         hip_check(launch_render(L, plan->device, plan->device + 1, static_cast<hipStream_t>(stream)), "render launch");
     });
 }

@@ -1,0 +1,378 @@
+// Streaming Menon (DDFAPD) render of the interior of a viewport: register
+// windows and wavefront lane shifts instead of LDS tiles.
+//
+// A wavefront owns a strip of 128 sensor columns, two per lane (lane L holds
+// columns xA = X0 + 112 w - 8 + 2 L and xA + 1), and walks down a band of
+// rows.  Every Menon stage (MenonDemosaic.cpp:89-174) runs once per row on
+// the lane's one non-green and one green site, in the reference's FP64
+// operation order (this file is compiled with -ffp-contract=off), and keeps
+// the rows later stages read in a sliding register window:
+//
+//   step s (final row s):  raw row s+6 arrives (loaded PF rows ahead)
+//                          directional greens, ch and dh at row s+4
+//                          cv and dv at row s+2, classifier at row s+2
+//                          red / blue at the green site of row s+1
+//                          final RGB of row s, the op program, stores
+//
+// Horizontal neighbours at x +- 1 and x +- 2 are the lane's own other column
+// or a column of lane L -+ 1, read with DPP wave_shr:1 / wave_shl:1 (lane L
+// receives lane L - 1's / L + 1's value); vertical neighbours are rows of
+// the lane's own window.  The support of an output pixel is +-6 columns, so
+// lanes 4..59 (112 columns) hold valid results; lanes 0..3 and 60..63 are the
+// strip's halo.  The kernel renders only a rectangle whose 6-pixel support
+// lies inside the active area (no reflection, no clipping): the caller
+// renders the tiles around it with the tile kernel.
+#include "render_launch.hpp"
+#include "stream_launch.hpp"
+
+namespace rawops {
+namespace {
+
+constexpr int kStreamThreads = 256;   // four wavefronts, side by side
+constexpr int kPrefetch = 4;          // raw rows in flight ahead of use
+
+// bound_ctrl: lane 0 (wave_shr) / lane 63 (wave_shl) read 0, so no "old"
+// value has to be set up in the destination first.
+__device__ __forceinline__ int lane_prev(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int lane_next(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ float prev_f(float v) { return __builtin_bit_cast(float, lane_prev(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ float next_f(float v) { return __builtin_bit_cast(float, lane_next(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ double prev_d(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)lane_prev((int)(uint32_t)b), hi = (uint32_t)lane_prev((int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double next_d(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)lane_next((int)(uint32_t)b), hi = (uint32_t)lane_next((int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// N0: lane column (0 = xA, 1 = xA + 1) of the non-green site in rows of even
+// offset from Y0; RED0: the non-green sites of those rows are red.  Rows of
+// odd offset have the other column and the other colour.
+template <int N0, int RED0, int OPS>
+struct Strip {
+    static constexpr int ncol(int q) { return N0 ^ q; }
+    static constexpr bool red_row(int q) { return (RED0 ^ q) != 0; }
+
+    const StreamArgs& a;
+    const int lane, xA;
+    const int xa, xb;                 // load columns (clamped into the area)
+    const int row_lo, row_hi;         // load rows are clamped into [row_lo, row_hi]
+    uint32_t e_all = 0;
+
+    // Windows, index k = row s + k (red / blue: row s + k - 1).
+    float rn[7] = {}, rg[7] = {};     // normalised samples at the non-green / green site
+    double gh[3] = {}, gv[3] = {};    // directional greens, rows s+2..s+4 (index k - 2)
+    double dh[5] = {}, dhp[5] = {};   // horizontal gradients (own, lane L-1), rows s..s+4
+    double dv[3] = {}, dvp[3] = {}, dvn[3] = {};   // vertical gradients (own, L-1, L+1), rows s..s+2
+    double gr[3] = {};                // selected green at the non-green site, rows s..s+2
+    bool hz[3] = {};                  // classifier decision (horizontal), rows s..s+2
+    double red[3] = {}, blue[3] = {}; // red / blue at the green site, rows s-1..s+1
+    uint32_t ring[kPrefetch] = {};    // codes (xA | xA+1 << 16) of rows s+6 .. s+5+kPrefetch
+    float pend[2][3] = {};            // op-list form: row s - 1's pixels, waiting for its pair
+    bool pend_store = false;          // ... and whether row s - 1 is stored
+
+    __device__ Strip(const StreamArgs& a_, int lane_, int xA_)
+        : a(a_), lane(lane_), xA(xA_),
+          xa(min(max(xA_, a_.ax), a_.ax + a_.aw - 1)), xb(min(max(xA_ + 1, a_.ax), a_.ax + a_.aw - 1)),
+          row_lo(a_.ay), row_hi(a_.ay + a_.ah - 1) {}
+
+    __device__ __forceinline__ uint32_t load_row(int y) const {
+        y = min(max(y, row_lo), row_hi);
+        const uint16_t* p = a.samples + (size_t)y * a.stride;
+        return (uint32_t)p[xa] | ((uint32_t)p[xb] << 16);
+    }
+
+    // Normalised sample (RawEngine.cpp:89-97) of a code in a row of offset
+    // parity q, lane column c.
+    __device__ __forceinline__ float norm(uint32_t code, int q, int c) const {
+        return normalize_exact((float)code, a.blk[q][c], a.rcp[q][c], a.den[q][c]);
+    }
+
+    // One row step; Q = offset parity of the final row s.  `final`: form
+    // row s's pixels (the rows of a band's row pairs); `store`: row s is a
+    // row of the rectangle.
+    template <int Q>
+    __device__ __forceinline__ void step(int s, bool final, bool store) {
+        // ---- raw row s + 6 (offset parity Q) ------------------------------
+        {
+            const uint32_t c = ring[0];
+#pragma unroll
+            for (int k = 0; k + 1 < kPrefetch; ++k) ring[k] = ring[k + 1];
+            ring[kPrefetch - 1] = load_row(s + 6 + kPrefetch);
+            const float fa = norm(c & 0xffffu, Q, 0), fb = norm(c >> 16, Q, 1);
+            rn[6] = ncol(Q) ? fb : fa;
+            rg[6] = ncol(Q) ? fa : fb;
+        }
+        // ---- directional greens, ch, dh at row s + 4 (MenonDemosaic.cpp:107-119) ----
+        double gh4, gv4, dh4, dhp4;
+        {
+            const double own = rn[4];
+            const float np = prev_f(rn[4]), nn = next_f(rn[4]);
+            float gl, gright;
+            if (ncol(Q) == 0) { gl = prev_f(rg[4]); gright = rg[4]; }
+            else { gl = rg[4]; gright = next_f(rg[4]); }
+            gh4 = 0.5 * ((double)gl + (double)gright) + 0.5 * own - 0.25 * ((double)np + (double)nn);
+            gv4 = 0.5 * ((double)rg[3] + (double)rg[5]) + 0.5 * own - 0.25 * ((double)rn[2] + (double)rn[6]);
+            const double ch = own - gh4;
+            dh4 = fabs(ch - next_d(ch));          // dh = |ch(x) - ch(x + 2)|
+            dhp4 = prev_d(dh4);
+        }
+        gh[2] = gh4;
+        gv[2] = gv4;
+        dh[4] = dh4;
+        dhp[4] = dhp4;
+        // ---- cv, dv at row s + 2: dv = |cv(y) - cv(y + 2)| ------------------
+        {
+            const double cv2 = (double)rn[2] - gv[0];
+            const double cv4 = (double)rn[4] - gv[2];
+            const double d = fabs(cv2 - cv4);
+            dv[2] = d;
+            dvp[2] = prev_d(d);
+            dvn[2] = next_d(d);
+        }
+        // ---- classifier at row s + 2 (MenonDemosaic.cpp:126-147) ---------
+        // Terms in the reference's order, weights {1,1,1,3,3,1,1,1}; the
+        // gradients are >= +0, so the first term equals 0 + term.
+        {
+            constexpr int n = ncol(Q);
+            double h = dh[4];                      // ( 0,  2)
+            h += dhp[4];                           // (-2,  2)
+            h += n == 0 ? dhp[3] : dh[3];          // (-1,  1)
+            h += 3.0 * dh[2];                      // ( 0,  0)
+            h += 3.0 * dhp[2];                     // (-2,  0)
+            h += n == 0 ? dhp[1] : dh[1];          // (-1, -1)
+            h += dh[0];                            // ( 0, -2)
+            h += dhp[0];                           // (-2, -2)
+            double v = dvn[2];                     // ( 2,  0)
+            v += dvn[0];                           // ( 2, -2)
+            v += n == 0 ? dv[1] : dvn[1];          // ( 1, -1)
+            v += 3.0 * dv[2];                      // ( 0,  0)
+            v += 3.0 * dv[0];                      // ( 0, -2)
+            v += n == 0 ? dvp[1] : dv[1];          // (-1, -1)
+            v += dvp[2];                           // (-2,  0)
+            v += dvp[0];                           // (-2, -2)
+            const bool horizontal = v >= h;
+            hz[2] = horizontal;
+            gr[2] = horizontal ? gh[0] : gv[0];
+        }
+        // ---- red / blue at the green site of row s + 1 (MenonDemosaic.cpp:148-161) ----
+        {
+            constexpr int q1 = Q ^ 1;
+            const double own = rg[1];
+            float rl, rr;
+            double gl, grt;
+            if (ncol(q1) == 0) { rl = rn[1]; gl = gr[1]; rr = next_f(rn[1]); grt = next_d(gr[1]); }
+            else { rl = prev_f(rn[1]); gl = prev_d(gr[1]); rr = rn[1]; grt = gr[1]; }
+            const double ch = own + 0.5 * ((double)rl + (double)rr) - 0.5 * (gl + grt);
+            const double cv = own + 0.5 * ((double)rn[0] + (double)rn[2]) - 0.5 * (gr[0] + gr[2]);
+            red[2] = red_row(q1) ? ch : cv;
+            blue[2] = red_row(q1) ? cv : ch;
+        }
+        // ---- final RGB of row s (MenonDemosaic.cpp:162-173) ----------------
+        if (final) finish<Q>(s, store);
+        // ---- slide the windows ----------------------------------------------
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { rn[k] = rn[k + 1]; rg[k] = rg[k + 1]; }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            gh[k] = gh[k + 1]; gv[k] = gv[k + 1];
+            dv[k] = dv[k + 1]; dvp[k] = dvp[k + 1]; dvn[k] = dvn[k + 1];
+            gr[k] = gr[k + 1]; hz[k] = hz[k + 1];
+            red[k] = red[k + 1]; blue[k] = blue[k + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { dh[k] = dh[k + 1]; dhp[k] = dhp[k + 1]; }
+    }
+
+    template <int Q>
+    __device__ __forceinline__ void finish(int s, bool store) {
+        constexpr int n = ncol(Q);
+        // Red / blue along the decision at the non-green site; the two
+        // horizontal neighbours are green sites of row s, the two vertical
+        // ones green sites of rows s -+ 1 in this column.
+        double hl_r, hl_b, hr_r, hr_b;
+        if (n == 0) { hl_r = prev_d(red[1]); hl_b = prev_d(blue[1]); hr_r = red[1]; hr_b = blue[1]; }
+        else { hl_r = red[1]; hl_b = blue[1]; hr_r = next_d(red[1]); hr_b = next_d(blue[1]); }
+        const bool horizontal = hz[0];
+        const double mr = 0.5 * (horizontal ? hl_r + hr_r : red[0] + red[2]);
+        const double mb = 0.5 * (horizontal ? hl_b + hr_b : blue[0] + blue[2]);
+        const double own = rn[0];
+        float px[2][3];
+        float* P = px[n];
+        float* G = px[1 - n];
+        if (red_row(Q)) {
+            P[0] = rn[0];
+            P[2] = (float)(own + mb - mr);     // bb = rr + mean(blue) - mean(red)
+        } else {
+            P[2] = rn[0];
+            P[0] = (float)(own + mr - mb);     // rr = bb + mean(red) - mean(blue)
+        }
+        P[1] = (float)gr[0];
+        G[0] = (float)red[1];
+        G[1] = rg[0];
+        G[2] = (float)blue[1];
+
+        if constexpr ((OPS & STREAM_OPS_LINEAR) != 0) {
+            // WhiteBalanceNode, ExposureNode, CameraToWorkingNode
+            // (RawEngine.cpp:408-548): each product and sum rounds to float.
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                float r = px[p][0], g = px[p][1], b = px[p][2];
+                if constexpr ((OPS & STREAM_LIN_GAINS) != 0) {
+                    r *= a.gains[0]; g *= a.gains[1]; b *= a.gains[2];
+                    r *= a.mult; g *= a.mult; b *= a.mult;
+                }
+                if constexpr ((OPS & STREAM_LIN_MATRIX) != 0) {
+                    const float R = a.m[0] * r + a.m[1] * g + a.m[2] * b;
+                    const float Gc = a.m[3] * r + a.m[4] * g + a.m[5] * b;
+                    const float B = a.m[6] * r + a.m[7] * g + a.m[8] * b;
+                    r = R; g = Gc; b = B;
+                }
+                px[p][0] = r; px[p][1] = g; px[p][2] = b;
+            }
+            if (store) store_row(s, px);
+        } else {
+            // The device op lists, once per row pair: row s - 1 waits in
+            // `pend` for row s.
+            if constexpr (Q == 0) {
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) pend[p][c] = px[p][c];
+                pend_store = store;
+            } else {
+                float q[4][3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    q[0][c] = pend[0][c]; q[1][c] = pend[1][c]; q[2][c] = px[0][c]; q[3][c] = px[1][c];
+                }
+                uint32_t err[4] = {0u, 0u, 0u, 0u};
+                apply_ops_px(*a.native, 0, as_const(*a.native).n, q, err);
+                const OpList& R = *a.reduced;
+                const uint32_t nr = as_const(R).n;
+                const uint32_t tap = a.tap < nr ? a.tap : nr;
+                apply_ops_px(R, 0, tap, q, err);
+                const bool ok = lane >= 4 && lane < 60;
+                const bool okA = ok && xA < a.X1, okB = ok && xA + 1 < a.X1;
+                if (a.out) {
+                    if (pend_store) store_f32(s - 1, q[0], q[1], okA, okB);
+                    if (store) store_f32(s, q[2], q[3], okA, okB);
+                }
+                if (a.out8) {
+                    apply_ops_px(R, tap, nr, q, err);
+                    if (pend_store) store_u8(s - 1, q[0], q[1], okA, okB);
+                    if (store) store_u8(s, q[2], q[3], okA, okB);
+                }
+                if (pend_store) e_all |= (okA ? err[0] : 0u) | (okB ? err[1] : 0u);
+                if (store) e_all |= (okA ? err[2] : 0u) | (okB ? err[3] : 0u);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store_row(int s, const float (&px)[2][3]) {
+        const bool ok = lane >= 4 && lane < 60;
+        store_f32(s, px[0], px[1], ok && xA < a.X1, ok && xA + 1 < a.X1);
+    }
+
+    __device__ __forceinline__ void store_f32(int s, const float* pa, const float* pb, bool okA, bool okB) const {
+        if (!a.out) return;
+        float* d = a.out + ((size_t)(s - a.vy) * a.ostride + (size_t)(xA - a.vx)) * 3;
+        if (okB) {
+            d[0] = pa[0]; d[1] = pa[1]; d[2] = pa[2];
+            d[3] = pb[0]; d[4] = pb[1]; d[5] = pb[2];
+        } else if (okA) {
+            d[0] = pa[0]; d[1] = pa[1]; d[2] = pa[2];
+        }
+    }
+
+    __device__ __forceinline__ void store_u8(int s, const float* pa, const float* pb, bool okA, bool okB) const {
+        uint8_t* d = a.out8 + ((size_t)(s - a.vy) * a.ostride + (size_t)(xA - a.vx)) * 3;
+        if (okA) { d[0] = quantize_u8(pa[0]); d[1] = quantize_u8(pa[1]); d[2] = quantize_u8(pa[2]); }
+        if (okB) { d[3] = quantize_u8(pb[0]); d[4] = quantize_u8(pb[1]); d[5] = quantize_u8(pb[2]); }
+    }
+};
+
+template <int N0, int RED0, int OPS>
+__global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const StreamArgs a) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kStreamThreads / 64) + (threadIdx.x >> 6)));
+    if (wave >= a.wave_cols * a.bands) return;   // whole wavefronts leave together
+    const int band = wave / a.wave_cols, wcol = wave - band * a.wave_cols;
+    const int lane = (int)(threadIdx.x & 63);
+    const int xA = a.X0 + 112 * wcol - 8 + 2 * lane;
+    const int yb = a.Y0 + band * a.band_rows;                 // even offset from Y0
+    const int ye = min(yb + a.band_rows, a.Y1);
+    const int ye4 = yb + ((ye - yb + 3) & ~3);                // whole groups of four rows
+    Strip<N0, RED0, OPS> t(a, lane, xA);
+    // Rows yb-6 .. yb-3 of the window before the first step (s = yb - 8
+    // reads rows s+2 .. s+6), then the rows in flight.
+#pragma unroll
+    for (int k = 2; k < 6; ++k) {
+        const uint32_t c = t.load_row(yb - 8 + k);
+        const int q = k & 1;
+        const float fa = t.norm(c & 0xffffu, q, 0), fb = t.norm(c >> 16, q, 1);
+        t.rn[k] = (N0 ^ q) ? fb : fa;
+        t.rg[k] = (N0 ^ q) ? fa : fb;
+    }
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) t.ring[k] = t.load_row(yb - 2 + k);
+    // Eight warm-up rows (their final rows lie above the band), then the
+    // band, four rows per iteration: the windows' registers rename within
+    // an iteration and only the few values older than four rows move.
+    for (int s = yb - 8; s < yb; s += 4) {
+        t.template step<0>(s, false, false);
+        t.template step<1>(s + 1, false, false);
+        t.template step<0>(s + 2, false, false);
+        t.template step<1>(s + 3, false, false);
+    }
+    for (int s = yb; s < ye4; s += 4) {
+        t.template step<0>(s, true, true);
+        t.template step<1>(s + 1, true, s + 1 < ye);
+        t.template step<0>(s + 2, true, s + 2 < ye);
+        t.template step<1>(s + 3, true, s + 3 < ye);
+    }
+    if (t.e_all && a.flags) atomicOr(a.flags, t.e_all);
+}
+
+}  // namespace
+
+hipError_t launch_menon_stream(const StreamArgs& args, int n0, int red0, int ops, hipStream_t stream) {
+    StreamArgs a = args;
+    if (a.X1 <= a.X0 || a.Y1 <= a.Y0) return hipSuccess;
+    a.wave_cols = (a.X1 - a.X0 + 111) / 112;
+    const int H = a.Y1 - a.Y0;
+    // 16-row bands: measured on the 24 MP frame (stream kernel alone, bands
+    // of 8 / 12 / 16 / 20 / 24 / 32 / 48 / 72 rows: 122.5 / 110.9 / 106.5 /
+    // 113.3 / 116.5 / 123.5 / 130.7 / 128.5 us).  Longer bands repeat fewer
+    // warm-up rows but keep every wavefront resident for the whole launch in
+    // the same phase of its row step, and issue fewer instructions per cycle.
+    if (a.band_rows <= 0) a.band_rows = 16;
+    a.bands = (H + a.band_rows - 1) / a.band_rows;
+    const int waves = a.wave_cols * a.bands;
+    const unsigned blocks = (unsigned)((waves + kStreamThreads / 64 - 1) / (kStreamThreads / 64));
+    const dim3 grid(blocks), block(kStreamThreads);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, stream, a); };
+#define RAWOPS_STREAM_FORMS(N, R)                                                              \
+    switch (ops) {                                                                             \
+    case STREAM_OPS_LINEAR: go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR>); break;           \
+    case STREAM_OPS_LINEAR | STREAM_LIN_GAINS:                                                 \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS>); break;            \
+    case STREAM_OPS_LINEAR | STREAM_LIN_MATRIX:                                                \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_MATRIX>); break;           \
+    case STREAM_OPS_LINEAR | STREAM_LIN_GAINS | STREAM_LIN_MATRIX:                             \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS | STREAM_LIN_MATRIX>); \
+        break;                                                                                 \
+    default: go(menon_stream_kernel<N, R, STREAM_OPS_LISTS>); break;                           \
+    }
+    if (n0) {
+        if (red0) { RAWOPS_STREAM_FORMS(1, 1) } else { RAWOPS_STREAM_FORMS(1, 0) }
+    } else {
+        if (red0) { RAWOPS_STREAM_FORMS(0, 1) } else { RAWOPS_STREAM_FORMS(0, 0) }
+    }
+#undef RAWOPS_STREAM_FORMS
+    return hipGetLastError();
+}
+
+}  // namespace rawops

@@ -19,6 +19,8 @@ struct RenderLaunch {
     uint32_t ostride;
     uint32_t* flags;
     uint32_t* decisions;     // Menon mip 0: record classifier decisions (may be null)
+    const OpList* native_host;   // host copies of the plan's op lists (may be null)
+    const OpList* reduced_host;
 };
 
 // native / reduced / post point at device-resident OpLists (owned by a plan).
