@@ -70,19 +70,52 @@ struct Strip {
     double gr[3] = {};                // selected green at the non-green site, rows s..s+2
     bool hz[3] = {};                  // classifier decision (horizontal), rows s..s+2
     double red[3] = {}, blue[3] = {}; // red / blue at the green site, rows s-1..s+1
-    uint32_t ring[kPrefetch] = {};    // codes (xA | xA+1 << 16) of rows s+6 .. s+5+kPrefetch
+    // Codes at xA / xA + 1 of rows s+6 .. s+5+kPrefetch, each straight from
+    // its 16-bit load (combining the two halves would make the loop wait for
+    // the loads it has just issued).
+    uint32_t ring_a[kPrefetch] = {}, ring_b[kPrefetch] = {};
     float pend[2][3] = {};            // op-list form: row s - 1's pixels, waiting for its pair
     bool pend_store = false;          // ... and whether row s - 1 is stored
 
-    __device__ Strip(const StreamArgs& a_, int lane_, int xA_)
+    // FP32 output of the band's rows [yb, ye) through a buffer resource:
+    // a store at an offset beyond the resource's size is discarded by the
+    // hardware, so halo lanes and rows past the band store at kDrop and no
+    // branch skips a store (a skipped store on one path makes every later
+    // load wait for all earlier stores on the others).
+    static constexpr uint32_t kDrop = 0x80000000u;
+    int yb = 0, ye = 0;
+    __amdgpu_buffer_rsrc_t rgb;
+
+    __device__ Strip(const StreamArgs& a_, int lane_, int xA_, int yb_, int ye_)
         : a(a_), lane(lane_), xA(xA_),
           xa(min(max(xA_, a_.ax), a_.ax + a_.aw - 1)), xb(min(max(xA_ + 1, a_.ax), a_.ax + a_.aw - 1)),
-          row_lo(a_.ay), row_hi(a_.ay + a_.ah - 1) {}
+          row_lo(a_.ay), row_hi(a_.ay + a_.ah - 1), yb(yb_), ye(ye_) {
+        float* base = a_.out ? a_.out + (size_t)(yb_ - a_.vy) * a_.ostride * 3 : nullptr;
+        const int bytes = a_.out ? (int)((size_t)(ye_ - yb_) * a_.ostride * 12) : 0;
+        rgb = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, bytes, 0x00020000);
+    }
 
-    __device__ __forceinline__ uint32_t load_row(int y) const {
+    // Byte offsets of the two pixels of row s in the band's output, kDrop
+    // where the pixel is not one of the rectangle's.
+    __device__ __forceinline__ void offsets(int s, bool row_ok, uint32_t& oa, uint32_t& ob) const {
+        const bool ok = row_ok && lane >= 4 && lane < 60;
+        const uint32_t o = (uint32_t)(((s - yb) * (int)a.ostride + (xA - a.vx)) * 12);
+        oa = ok && xA < a.X1 ? o : kDrop;
+        ob = ok && xA + 1 < a.X1 ? o + 12u : kDrop;
+    }
+
+    __device__ __forceinline__ void store3(const float* v, uint32_t off) const {
+        typedef __attribute__((ext_vector_type(3))) unsigned u3;
+        const u3 d = {__builtin_bit_cast(unsigned, v[0]), __builtin_bit_cast(unsigned, v[1]),
+                      __builtin_bit_cast(unsigned, v[2])};
+        __builtin_amdgcn_raw_buffer_store_b96(d, rgb, off, 0, 0);
+    }
+
+    __device__ __forceinline__ void load_row(int y, uint32_t& ca, uint32_t& cb) const {
         y = min(max(y, row_lo), row_hi);
         const uint16_t* p = a.samples + (size_t)y * a.stride;
-        return (uint32_t)p[xa] | ((uint32_t)p[xb] << 16);
+        ca = p[xa];
+        cb = p[xb];
     }
 
     // Normalised sample (RawEngine.cpp:89-97) of a code in a row of offset
@@ -91,18 +124,22 @@ struct Strip {
         return normalize_exact((float)code, a.blk[q][c], a.rcp[q][c], a.den[q][c]);
     }
 
+    // Stages of a row step; a band's first warm-up steps skip those whose
+    // rows no later stage of the band reads.
+    enum : int { ST_DV = 1, ST_CLS = 2, ST_CHR = 4, ST_ALL = 7 };
+
     // One row step; Q = offset parity of the final row s.  `final`: form
-    // row s's pixels (the rows of a band's row pairs); `store`: row s is a
+    // row s's pixels (the rows of a band's row groups); `store`: row s is a
     // row of the rectangle.
-    template <int Q>
+    template <int Q, int ST = ST_ALL>
     __device__ __forceinline__ void step(int s, bool final, bool store) {
         // ---- raw row s + 6 (offset parity Q) ------------------------------
         {
-            const uint32_t c = ring[0];
+            const uint32_t ca = ring_a[0], cb = ring_b[0];
 #pragma unroll
-            for (int k = 0; k + 1 < kPrefetch; ++k) ring[k] = ring[k + 1];
-            ring[kPrefetch - 1] = load_row(s + 6 + kPrefetch);
-            const float fa = norm(c & 0xffffu, Q, 0), fb = norm(c >> 16, Q, 1);
+            for (int k = 0; k + 1 < kPrefetch; ++k) { ring_a[k] = ring_a[k + 1]; ring_b[k] = ring_b[k + 1]; }
+            load_row(s + 6 + kPrefetch, ring_a[kPrefetch - 1], ring_b[kPrefetch - 1]);
+            const float fa = norm(ca, Q, 0), fb = norm(cb, Q, 1);
             rn[6] = ncol(Q) ? fb : fa;
             rg[6] = ncol(Q) ? fa : fb;
         }
@@ -125,7 +162,7 @@ struct Strip {
         dh[4] = dh4;
         dhp[4] = dhp4;
         // ---- cv, dv at row s + 2: dv = |cv(y) - cv(y + 2)| ------------------
-        {
+        if constexpr ((ST & ST_DV) != 0) {
             const double cv2 = (double)rn[2] - gv[0];
             const double cv4 = (double)rn[4] - gv[2];
             const double d = fabs(cv2 - cv4);
@@ -136,7 +173,7 @@ struct Strip {
         // ---- classifier at row s + 2 (MenonDemosaic.cpp:126-147) ---------
         // Terms in the reference's order, weights {1,1,1,3,3,1,1,1}; the
         // gradients are >= +0, so the first term equals 0 + term.
-        {
+        if constexpr ((ST & ST_CLS) != 0) {
             constexpr int n = ncol(Q);
             double h = dh[4];                      // ( 0,  2)
             h += dhp[4];                           // (-2,  2)
@@ -159,7 +196,7 @@ struct Strip {
             gr[2] = horizontal ? gh[0] : gv[0];
         }
         // ---- red / blue at the green site of row s + 1 (MenonDemosaic.cpp:148-161) ----
-        {
+        if constexpr ((ST & ST_CHR) != 0) {
             constexpr int q1 = Q ^ 1;
             const double own = rg[1];
             float rl, rr;
@@ -233,7 +270,9 @@ struct Strip {
                 }
                 px[p][0] = r; px[p][1] = g; px[p][2] = b;
             }
-            if (store) store_row(s, px);
+            uint32_t oa, ob;
+            offsets(s, store, oa, ob);
+            if (a.out) { store3(px[0], oa); store3(px[1], ob); }
         } else {
             // The device op lists, once per row pair: row s - 1 waits in
             // `pend` for row s.
@@ -258,8 +297,11 @@ struct Strip {
                 const bool ok = lane >= 4 && lane < 60;
                 const bool okA = ok && xA < a.X1, okB = ok && xA + 1 < a.X1;
                 if (a.out) {
-                    if (pend_store) store_f32(s - 1, q[0], q[1], okA, okB);
-                    if (store) store_f32(s, q[2], q[3], okA, okB);
+                    uint32_t oa, ob;
+                    offsets(s - 1, pend_store, oa, ob);
+                    store3(q[0], oa); store3(q[1], ob);
+                    offsets(s, store, oa, ob);
+                    store3(q[2], oa); store3(q[3], ob);
                 }
                 if (a.out8) {
                     apply_ops_px(R, tap, nr, q, err);
@@ -269,22 +311,6 @@ struct Strip {
                 if (pend_store) e_all |= (okA ? err[0] : 0u) | (okB ? err[1] : 0u);
                 if (store) e_all |= (okA ? err[2] : 0u) | (okB ? err[3] : 0u);
             }
-        }
-    }
-
-    __device__ __forceinline__ void store_row(int s, const float (&px)[2][3]) {
-        const bool ok = lane >= 4 && lane < 60;
-        store_f32(s, px[0], px[1], ok && xA < a.X1, ok && xA + 1 < a.X1);
-    }
-
-    __device__ __forceinline__ void store_f32(int s, const float* pa, const float* pb, bool okA, bool okB) const {
-        if (!a.out) return;
-        float* d = a.out + ((size_t)(s - a.vy) * a.ostride + (size_t)(xA - a.vx)) * 3;
-        if (okB) {
-            d[0] = pa[0]; d[1] = pa[1]; d[2] = pa[2];
-            d[3] = pb[0]; d[4] = pb[1]; d[5] = pb[2];
-        } else if (okA) {
-            d[0] = pa[0]; d[1] = pa[1]; d[2] = pa[2];
         }
     }
 
@@ -305,28 +331,34 @@ __global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const Stre
     const int yb = a.Y0 + band * a.band_rows;                 // even offset from Y0
     const int ye = min(yb + a.band_rows, a.Y1);
     const int ye4 = yb + ((ye - yb + 3) & ~3);                // whole groups of four rows
-    Strip<N0, RED0, OPS> t(a, lane, xA);
+    Strip<N0, RED0, OPS> t(a, lane, xA, yb, ye);
     // Rows yb-6 .. yb-3 of the window before the first step (s = yb - 8
     // reads rows s+2 .. s+6), then the rows in flight.
 #pragma unroll
     for (int k = 2; k < 6; ++k) {
-        const uint32_t c = t.load_row(yb - 8 + k);
+        uint32_t ca, cb;
+        t.load_row(yb - 8 + k, ca, cb);
         const int q = k & 1;
-        const float fa = t.norm(c & 0xffffu, q, 0), fb = t.norm(c >> 16, q, 1);
+        const float fa = t.norm(ca, q, 0), fb = t.norm(cb, q, 1);
         t.rn[k] = (N0 ^ q) ? fb : fa;
         t.rg[k] = (N0 ^ q) ? fa : fb;
     }
 #pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) t.ring[k] = t.load_row(yb - 2 + k);
-    // Eight warm-up rows (their final rows lie above the band), then the
-    // band, four rows per iteration: the windows' registers rename within
-    // an iteration and only the few values older than four rows move.
-    for (int s = yb - 8; s < yb; s += 4) {
-        t.template step<0>(s, false, false);
-        t.template step<1>(s + 1, false, false);
-        t.template step<0>(s + 2, false, false);
-        t.template step<1>(s + 3, false, false);
-    }
+    for (int k = 0; k < kPrefetch; ++k) t.load_row(yb - 2 + k, t.ring_a[k], t.ring_b[k]);
+    // Eight warm-up rows (their final rows lie above the band; each runs
+    // only the stages whose rows the band reads: dv from row yb - 4, the
+    // classifier from yb - 2, red / blue from yb - 1), then the band, four
+    // rows per iteration: the windows' registers rename within an iteration
+    // and only the few values older than four rows move.
+    using S = Strip<N0, RED0, OPS>;
+    t.template step<0, 0>(yb - 8, false, false);
+    t.template step<1, 0>(yb - 7, false, false);
+    t.template step<0, S::ST_DV>(yb - 6, false, false);
+    t.template step<1, S::ST_DV>(yb - 5, false, false);
+    t.template step<0, S::ST_DV | S::ST_CLS>(yb - 4, false, false);
+    t.template step<1, S::ST_DV | S::ST_CLS>(yb - 3, false, false);
+    t.template step<0, S::ST_ALL>(yb - 2, false, false);
+    t.template step<1, S::ST_ALL>(yb - 1, false, false);
     for (int s = yb; s < ye4; s += 4) {
         t.template step<0>(s, true, true);
         t.template step<1>(s + 1, true, s + 1 < ye);
