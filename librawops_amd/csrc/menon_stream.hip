@@ -19,9 +19,9 @@
 // receives lane L - 1's / L + 1's value); vertical neighbours are rows of
 // the lane's own window.  The support of an output pixel is +-6 columns, so
 // lanes 4..59 (112 columns) hold valid results; lanes 0..3 and 60..63 are the
-// strip's halo.  The kernel renders only a rectangle whose 6-pixel support
-// lies inside the active area (no reflection, no clipping): the caller
-// renders the tiles around it with the tile kernel.
+// strip's halo.  Strips that reach the edge of the active area run the edge
+// form (reflected loads, zero gradients outside the area, mirrored rows and
+// columns; see Strip), the others a form with no edge logic.
 #include "render_launch.hpp"
 #include "stream_launch.hpp"
 
@@ -48,19 +48,51 @@ __device__ __forceinline__ double next_d(double v) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// Whole-sample reflection into [lo, hi] (MenonDemosaic.cpp:60-66) for a
+// coordinate at most one area size outside, then clamped so that far-out
+// positions (whose values no output reads) still load in bounds.
+__device__ __forceinline__ int reflect_in(int v, int lo, int hi) {
+    v = v < lo ? 2 * lo - v : (v > hi ? 2 * hi - v : v);
+    return min(max(v, lo), hi);
+}
+
+// Value of lane `src` (any lane of the wavefront).
+__device__ __forceinline__ double lane_read(double v, int src) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // N0: lane column (0 = xA, 1 = xA + 1) of the non-green site in rows of even
 // offset from Y0; RED0: the non-green sites of those rows are red.  Rows of
 // odd offset have the other column and the other colour.
-template <int N0, int RED0, int OPS>
+//
+// E: the strip's support reaches the edge of the active area.  The reference
+// reads outside the area by whole-sample reflection, except the classifier,
+// which reads gradients outside it as zero (MenonDemosaic.cpp:126-147).  The
+// edge form loads reflected samples, so every symmetric stencil evaluated at
+// a position outside the area (directional greens, ch / cv, red / blue)
+// yields the value at its mirror image, which is what the reference reads
+// there; the gradients of positions outside the area are set to zero; the
+// selected green of a column outside the area is taken from its mirror
+// column's lane, and the windows' rows outside the area from their mirror
+// rows where the next stage reads them.  (Needs an area of at least 16 x 16,
+// so that one reflection reaches every position a result depends on.)
+template <int N0, int RED0, int OPS, bool E>
 struct Strip {
     static constexpr int ncol(int q) { return N0 ^ q; }
     static constexpr bool red_row(int q) { return (RED0 ^ q) != 0; }
 
     const StreamArgs& a;
     const int lane, xA;
-    const int xa, xb;                 // load columns (clamped into the area)
-    const int row_lo, row_hi;         // load rows are clamped into [row_lo, row_hi]
+    const int xa, xb;                 // load columns (clamped, or reflected, into the area)
+    const int row_lo, row_hi;         // first and last area rows
     uint32_t e_all = 0;
+    // Edge form: the lane's non-green column by row parity lies outside the
+    // area, and the lane holding its mirror column.
+    bool out_n[2] = {false, false};
+    int mirror_lane[2] = {0, 0};
 
     // Windows, index k = row s + k (red / blue: row s + k - 1).
     float rn[7] = {}, rg[7] = {};     // normalised samples at the non-green / green site
@@ -88,12 +120,25 @@ struct Strip {
 
     __device__ Strip(const StreamArgs& a_, int lane_, int xA_, int yb_, int ye_)
         : a(a_), lane(lane_), xA(xA_),
-          xa(min(max(xA_, a_.ax), a_.ax + a_.aw - 1)), xb(min(max(xA_ + 1, a_.ax), a_.ax + a_.aw - 1)),
+          xa(E ? reflect_in(xA_, a_.ax, a_.ax + a_.aw - 1) : min(max(xA_, a_.ax), a_.ax + a_.aw - 1)),
+          xb(E ? reflect_in(xA_ + 1, a_.ax, a_.ax + a_.aw - 1) : min(max(xA_ + 1, a_.ax), a_.ax + a_.aw - 1)),
           row_lo(a_.ay), row_hi(a_.ay + a_.ah - 1), yb(yb_), ye(ye_) {
         float* base = a_.out ? a_.out + (size_t)(yb_ - a_.vy) * a_.ostride * 3 : nullptr;
         const int bytes = a_.out ? (int)((size_t)(ye_ - yb_) * a_.ostride * 12) : 0;
         rgb = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, bytes, 0x00020000);
+        if constexpr (E) {
+            const int x0 = xA_ - 2 * lane_;   // lane 0's first column
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int x = xA_ + ncol(q);
+                out_n[q] = x < a_.ax || x >= a_.ax + a_.aw;
+                const int m = out_n[q] ? reflect_in(x, a_.ax, a_.ax + a_.aw - 1) : x;
+                mirror_lane[q] = min(max((m - x0) >> 1, 0), 63);
+            }
+        }
     }
+
+    __device__ __forceinline__ bool row_out(int y) const { return y < row_lo || y > row_hi; }
 
     // Byte offsets of the two pixels of row s in the band's output, kDrop
     // where the pixel is not one of the rectangle's.
@@ -112,7 +157,7 @@ struct Strip {
     }
 
     __device__ __forceinline__ void load_row(int y, uint32_t& ca, uint32_t& cb) const {
-        y = min(max(y, row_lo), row_hi);
+        y = E ? reflect_in(y, row_lo, row_hi) : min(max(y, row_lo), row_hi);
         const uint16_t* p = a.samples + (size_t)y * a.stride;
         ca = p[xa];
         cb = p[xb];
@@ -155,6 +200,9 @@ struct Strip {
             gv4 = 0.5 * ((double)rg[3] + (double)rg[5]) + 0.5 * own - 0.25 * ((double)rn[2] + (double)rn[6]);
             const double ch = own - gh4;
             dh4 = fabs(ch - next_d(ch));          // dh = |ch(x) - ch(x + 2)|
+            if constexpr (E) {
+                if (out_n[Q] || row_out(s + 4)) dh4 = 0.0;   // read as zero outside the area
+            }
             dhp4 = prev_d(dh4);
         }
         gh[2] = gh4;
@@ -165,7 +213,10 @@ struct Strip {
         if constexpr ((ST & ST_DV) != 0) {
             const double cv2 = (double)rn[2] - gv[0];
             const double cv4 = (double)rn[4] - gv[2];
-            const double d = fabs(cv2 - cv4);
+            double d = fabs(cv2 - cv4);
+            if constexpr (E) {
+                if (out_n[Q] || row_out(s + 2)) d = 0.0;
+            }
             dv[2] = d;
             dvp[2] = prev_d(d);
             dvn[2] = next_d(d);
@@ -194,10 +245,16 @@ struct Strip {
             const bool horizontal = v >= h;
             hz[2] = horizontal;
             gr[2] = horizontal ? gh[0] : gv[0];
+            if constexpr (E) gr[2] = lane_read(gr[2], out_n[Q] ? mirror_lane[Q] : lane);
         }
         // ---- red / blue at the green site of row s + 1 (MenonDemosaic.cpp:148-161) ----
         if constexpr ((ST & ST_CHR) != 0) {
             constexpr int q1 = Q ^ 1;
+            if constexpr (E) {
+                // The green of a row outside the area is its mirror row's.
+                if (s + 1 == row_lo) gr[0] = gr[2];
+                if (s + 1 == row_hi) gr[2] = gr[0];
+            }
             const double own = rg[1];
             float rl, rr;
             double gl, grt;
@@ -227,6 +284,11 @@ struct Strip {
     template <int Q>
     __device__ __forceinline__ void finish(int s, bool store) {
         constexpr int n = ncol(Q);
+        if constexpr (E) {
+            // Red / blue of a row outside the area are its mirror row's.
+            if (s == row_lo) { red[0] = red[2]; blue[0] = blue[2]; }
+            if (s == row_hi) { red[2] = red[0]; blue[2] = blue[0]; }
+        }
         // Red / blue along the decision at the non-green site; the two
         // horizontal neighbours are green sites of row s, the two vertical
         // ones green sites of rows s -+ 1 in this column.
@@ -321,17 +383,10 @@ struct Strip {
     }
 };
 
-template <int N0, int RED0, int OPS>
-__global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const StreamArgs a) {
-    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kStreamThreads / 64) + (threadIdx.x >> 6)));
-    if (wave >= a.wave_cols * a.bands) return;   // whole wavefronts leave together
-    const int band = wave / a.wave_cols, wcol = wave - band * a.wave_cols;
-    const int lane = (int)(threadIdx.x & 63);
-    const int xA = a.X0 + 112 * wcol - 8 + 2 * lane;
-    const int yb = a.Y0 + band * a.band_rows;                 // even offset from Y0
-    const int ye = min(yb + a.band_rows, a.Y1);
+template <int N0, int RED0, int OPS, bool E>
+__device__ __forceinline__ void run_strip(const StreamArgs& a, int lane, int xA, int yb, int ye) {
     const int ye4 = yb + ((ye - yb + 3) & ~3);                // whole groups of four rows
-    Strip<N0, RED0, OPS> t(a, lane, xA, yb, ye);
+    Strip<N0, RED0, OPS, E> t(a, lane, xA, yb, ye);
     // Rows yb-6 .. yb-3 of the window before the first step (s = yb - 8
     // reads rows s+2 .. s+6), then the rows in flight.
 #pragma unroll
@@ -350,7 +405,7 @@ __global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const Stre
     // classifier from yb - 2, red / blue from yb - 1), then the band, four
     // rows per iteration: the windows' registers rename within an iteration
     // and only the few values older than four rows move.
-    using S = Strip<N0, RED0, OPS>;
+    using S = Strip<N0, RED0, OPS, E>;
     t.template step<0, 0>(yb - 8, false, false);
     t.template step<1, 0>(yb - 7, false, false);
     t.template step<0, S::ST_DV>(yb - 6, false, false);
@@ -366,6 +421,24 @@ __global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const Stre
         t.template step<1>(s + 3, true, s + 3 < ye);
     }
     if (t.e_all && a.flags) atomicOr(a.flags, t.e_all);
+}
+
+template <int N0, int RED0, int OPS>
+__global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const StreamArgs a) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kStreamThreads / 64) + (threadIdx.x >> 6)));
+    if (wave >= a.wave_cols * a.bands) return;   // whole wavefronts leave together
+    const int band = wave / a.wave_cols, wcol = wave - band * a.wave_cols;
+    const int lane = (int)(threadIdx.x & 63);
+    const int x0 = a.X0 + 112 * wcol - 8;                     // lane 0's first column
+    const int xA = x0 + 2 * lane;
+    const int yb = a.Y0 + band * a.band_rows;                 // even offset from Y0
+    const int ye = min(yb + a.band_rows, a.Y1);
+    // A strip whose columns or rows (with the 6-sample support of the band's
+    // rows and the warm-up rows above them) come within reach of the area's
+    // edge takes the edge form.
+    const bool edge = x0 < a.ax + 8 || x0 + 128 > a.ax + a.aw - 8 || yb - 16 < a.ay || ye + 16 > a.ay + a.ah;
+    if (edge) run_strip<N0, RED0, OPS, true>(a, lane, xA, yb, ye);
+    else run_strip<N0, RED0, OPS, false>(a, lane, xA, yb, ye);
 }
 
 }  // namespace

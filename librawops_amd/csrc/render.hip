@@ -19,7 +19,6 @@
 // BORDER path, which reflects reads at the active-area edge and clips
 // computation to the area exactly as the reference's clipped planes do.
 #include <cstdlib>
-#include <mutex>
 
 #include "render_tile.hpp"
 #include "render_launch.hpp"
@@ -33,13 +32,10 @@ template <int DEMOSAIC, int TW, int TH>
 __global__ void __launch_bounds__(kRenderThreads) render_kernel(const RenderArgs a) {
     __shared__ __attribute__((aligned(16))) char lds[Lds<DEMOSAIC, TW, TH>::BYTES];
     int tx, ty, lx0, ly0;
-    tile_origin<TW, TH>(a, border_tile(a, (int)blockIdx.x), tx, ty, lx0, ly0);
+    tile_origin<TW, TH>(a, (int)blockIdx.x, tx, ty, lx0, ly0);
     const SensorArgs s = a.s;
     render_any_tile<DEMOSAIC, TW, TH, kRenderThreads>(a, s, lds, tx, ty, lx0, ly0);
 }
-
-int ceil_div(int a, int b) { return a >= 0 ? (a + b - 1) / b : -((-a) / b); }
-int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
 uint32_t site_of_host(const SensorArgs& s, int x, int y) {
     return ((((uint32_t)y & 1u) + s.phase_y) & 1u) * 2u + ((((uint32_t)x & 1u) + s.phase_x) & 1u);
@@ -62,35 +58,6 @@ int stream_band_rows() {
         return r >= 4 && r <= 4096 ? r & ~3 : 0;
     }();
     return rows;
-}
-
-// The border tiles of a split render run on a library-owned stream beside
-// the caller's (default priority, one per device): forked after the work
-// already on the caller's stream and joined back before anything after the
-// render, so the caller sees one stream-ordered operation.  Fork and join
-// reuse two events per device; the mutex keeps concurrent callers' record /
-// wait pairs from interleaving.
-struct SideStream {
-    hipStream_t stream = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    bool ready = false;
-};
-constexpr int kMaxSideDevices = 64;
-std::mutex g_side_mutex;
-SideStream g_side[kMaxSideDevices];
-
-SideStream* side_stream() {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxSideDevices) return nullptr;
-    SideStream& s = g_side[dev];
-    if (!s.ready) {
-        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
-        if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess)
-            return nullptr;
-        s.ready = true;
-    }
-    return &s;
 }
 
 // The FP32 output's op program (native ops, then the reduced ops before the
@@ -148,70 +115,39 @@ hipError_t launch(const RenderLaunch& L, const OpList* native, const OpList* red
     a.tap = 0xffffffffu;
     a.decisions = L.decisions;
     a.dtiles = decision_tiles((uint32_t)a.vw);
-    a.tiles_y = tiles_y;
-    a.bx0 = a.bx1 = a.by0 = a.by1 = 0;
-    unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
+    const unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
     if (!blocks) return hipSuccess;
-    if (DEMOSAIC == RAWOPS_DEMOSAIC_MENON && a.mip == 0 && !a.decisions && stream_enabled()) {
-        // Tiles whose 6-pixel support lies inside the area form a rectangle
-        // of the tile grid: the streaming kernel renders it, the tile kernel
-        // the ring of tiles around it.
-        const SensorArgs& S = a.s;
-        const int i0 = std::max(0, ceil_div(S.ax + 6 - a.vx, TW));
-        const int i1 = std::min(a.tiles_x, floor_div(S.ax + S.aw - 6 - TW - a.vx, TW) + 1);
-        const int j0 = std::max(0, ceil_div(S.ay + 6 - a.vy, TH));
-        const int j1 = std::min(tiles_y, floor_div(S.ay + S.ah - 6 - TH - a.vy, TH) + 1);
-        if (i1 - i0 >= 2 && j1 - j0 >= 2) {
-            StreamArgs t{};
-            t.samples = S.samples;
-            t.stride = S.stride;
-            t.ax = S.ax; t.ay = S.ay; t.aw = S.aw; t.ah = S.ah;
-            t.X0 = a.vx + i0 * TW;
-            t.X1 = std::min(a.vx + i1 * TW, a.vx + a.vw);
-            t.Y0 = a.vy + j0 * TH;
-            t.Y1 = std::min(a.vy + j1 * TH, a.vy + a.vh);
-            t.vx = a.vx; t.vy = a.vy;
-            t.band_rows = stream_band_rows();
-            for (int q = 0; q < 2; ++q)
-                for (int c = 0; c < 2; ++c) {
-                    const uint32_t site = site_of_host(S, t.X0 + c, t.Y0 + q);
-                    t.blk[q][c] = S.black[site];
-                    t.rcp[q][c] = S.rcp[site];
-                    t.den[q][c] = S.denom[site];
-                }
-            t.native = native;
-            t.reduced = reduced;
-            t.tap = a.tap;
-            t.out = a.out;
-            t.out8 = a.out8;
-            t.ostride = a.ostride;
-            t.flags = a.flags;
-            const int n0 = (int)(((S.ng_parity >> ((uint32_t)t.Y0 & 1u)) & 1u) ^ ((uint32_t)t.X0 & 1u));
-            const int xng = t.X0 + n0;
-            const uint32_t site_ng = site_of_host(S, xng, t.Y0);
-            const int red0 = ((S.colors >> (2u * site_ng)) & 3u) == 0 ? 1 : 0;
-            const int ops = linear_chain(L, a.tap, t);
-            a.bx0 = i0; a.bx1 = i1; a.by0 = j0; a.by1 = j1;
-            blocks -= (unsigned)((i1 - i0) * (j1 - j0));
-            if (!blocks) return launch_menon_stream(t, n0, red0, ops, stream);
-            std::lock_guard<std::mutex> lock(g_side_mutex);
-            SideStream* side = side_stream();
-            if (!side) {
-                hipError_t e = launch_menon_stream(t, n0, red0, ops, stream);
-                if (e != hipSuccess) return e;
-            } else {
-                hipError_t e = hipEventRecord(side->fork, stream);
-                if (e == hipSuccess) e = hipStreamWaitEvent(side->stream, side->fork, 0);
-                if (e != hipSuccess) return e;
-                hipLaunchKernelGGL((render_kernel<DEMOSAIC, TW, TH>), dim3(blocks), dim3(kRenderThreads), 0,
-                                   side->stream, a);
-                e = hipGetLastError();
-                if (e == hipSuccess) e = hipEventRecord(side->join, side->stream);
-                if (e == hipSuccess) e = launch_menon_stream(t, n0, red0, ops, stream);
-                const hipError_t j = hipStreamWaitEvent(stream, side->join, 0);
-                return e != hipSuccess ? e : j;
+    const SensorArgs& S = a.s;
+    if (DEMOSAIC == RAWOPS_DEMOSAIC_MENON && a.mip == 0 && !a.decisions && S.aw >= 16 && S.ah >= 16 &&
+        a.ostride <= (1u << 23) && stream_enabled()) {
+        // The streaming kernel renders the whole viewport (its edge form
+        // handles the strips that reach the edge of the active area).
+        StreamArgs t{};
+        t.samples = S.samples;
+        t.stride = S.stride;
+        t.ax = S.ax; t.ay = S.ay; t.aw = S.aw; t.ah = S.ah;
+        t.X0 = a.vx; t.X1 = a.vx + a.vw;
+        t.Y0 = a.vy; t.Y1 = a.vy + a.vh;
+        t.vx = a.vx; t.vy = a.vy;
+        t.band_rows = stream_band_rows();
+        for (int q = 0; q < 2; ++q)
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t site = site_of_host(S, t.X0 + c, t.Y0 + q);
+                t.blk[q][c] = S.black[site];
+                t.rcp[q][c] = S.rcp[site];
+                t.den[q][c] = S.denom[site];
             }
-        }
+        t.native = native;
+        t.reduced = reduced;
+        t.tap = a.tap;
+        t.out = a.out;
+        t.out8 = a.out8;
+        t.ostride = a.ostride;
+        t.flags = a.flags;
+        const int n0 = (int)(((S.ng_parity >> ((uint32_t)t.Y0 & 1u)) & 1u) ^ ((uint32_t)t.X0 & 1u));
+        const uint32_t site_ng = site_of_host(S, t.X0 + n0, t.Y0);
+        const int red0 = ((S.colors >> (2u * site_ng)) & 3u) == 0 ? 1 : 0;
+        return launch_menon_stream(t, n0, red0, linear_chain(L, a.tap, t), stream);
     }
     hipLaunchKernelGGL((render_kernel<DEMOSAIC, TW, TH>), dim3(blocks), dim3(kRenderThreads), 0, stream, a);
     return hipGetLastError();

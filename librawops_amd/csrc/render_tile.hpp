@@ -28,26 +28,7 @@ struct RenderArgs {
                              // non-green site, 32 words per tile (word tile 32 + row, bit x / 2
                              // within the tile, 1 = horizontal); may be null
     uint32_t dtiles;         // tiles per viewport row
-    // Border-only launch: the tiles of [bx0, bx1) x [by0, by1) are rendered
-    // elsewhere and the grid covers the others (bx0 == bx1: every tile).
-    int tiles_y, bx0, bx1, by0, by1;
 };
-
-// Tile number of block b of a border-only launch: the rows above the
-// skipped rectangle, the rows below it, then the left and right parts of
-// the rows beside it.
-__device__ __forceinline__ int border_tile(const RenderArgs& a, int b) {
-    if (a.bx0 == a.bx1) return b;
-    const int above = a.by0 * a.tiles_x;
-    if (b < above) return b;
-    b -= above;
-    const int below = (a.tiles_y - a.by1) * a.tiles_x;
-    if (b < below) return a.by1 * a.tiles_x + b;
-    b -= below;
-    const int side = a.bx0 + (a.tiles_x - a.bx1);
-    const int row = b / side, k = b - row * side;
-    return (a.by0 + row) * a.tiles_x + (k < a.bx0 ? k : a.bx1 + (k - a.bx0));
-}
 
 // Tile-relative x parity of the non-green sites in even tile rows for a
 // render whose tiles start at native (ox0 + i TW, oy0 + j TH), TW, TH even.

@@ -66,3 +66,26 @@ def test_stream_interior_matches_oracle(pattern, chain):
             assert np.max(np.abs(got - exp)) <= 1e-5, (vp, chain)
         else:
             assert bits_equal(got, exp), (vp, chain, first_mismatch(got, exp))
+
+
+@pytest.mark.parametrize("pattern,phase", [(0, (0, 0)), (1, (1, 0)), (2, (0, 1)), (3, (1, 1))])
+@pytest.mark.parametrize("area", [(0, 0, 16, 16), (3, 2, 17, 23), (1, 4, 19, 16), (2, 1, 131, 47), (0, 3, 230, 121)])
+def test_stream_edges_match_oracle(pattern, phase, area):
+    """The edge form (reflected samples, zero gradients outside the area,
+    mirrored rows and columns): whole areas from the smallest the streaming
+    kernel takes (16 x 16) up, and viewports touching one edge at a time."""
+    ax, ay, aw, ah = area
+    w, h = ax + aw + 5, ay + ah + 4
+    stride = w + 3
+    m = meta_dict(w, h, stride, pattern, phase[0], phase[1], area, (510, 512, 511, 513), (16383,) * 4)
+    s = random_samples(h, stride, seed=7 * pattern + aw, hi=16383)
+    img = gpu_image(m, s)
+    om = oracle_meta(m)
+    c = CHAINS["wb_exposure_matrix"]
+    node = chain_node(img, c)
+    vps = [(ax, ay, aw, ah), (ax, ay + 1, aw // 2 + 1, ah - 2), (ax + aw // 2, ay, aw - aw // 2, ah // 2 + 1),
+           (ax + 1, ay + ah // 2, aw - 1, ah - ah // 2), (ax + 2, ay + 3, aw - 4, ah - 6)]
+    for vp in vps:
+        got = L.Renderer().render_image(node, img.metadata.active_area, L.RenderRequest(L.Rect(*vp), 64)).rgb
+        exp = O.render_raw(om, s, c, vp)
+        assert bits_equal(got, exp), (vp, first_mismatch(got, exp))
