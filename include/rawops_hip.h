@@ -153,6 +153,15 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* source
                             rawops_rect viewport, uint32_t mip, uint32_t quality,
                             float* dst_rgb, uint8_t* dst_u8, uint32_t dst_stride_pixels,
                             uint32_t* device_flags, void* stream);
+/* Render path of Menon mip-0 renders (no reference counterpart: a tuning
+ * switch, process-wide).  path 0: automatic (the streaming kernel for
+ * viewports large enough to fill the device, about 2.75 MP and up, the tile
+ * kernel below); 1: the streaming kernel wherever the active area is at
+ * least 16 x 16; 2: the tile kernel only.  band_rows: rows each streaming
+ * wavefront renders (0 = 16, else a multiple of 4 in 4..4096).  Every path
+ * renders the same bits. */
+rawops_status rawops_set_menon_path(uint32_t path, uint32_t band_rows);
+
 /* Reads a device_flags word (synchronising `stream`) and resets it to zero. */
 rawops_status rawops_check_flags(uint32_t* device_flags, void* stream);
 

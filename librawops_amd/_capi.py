@@ -153,6 +153,7 @@ def lib():
             "rawops_guided_workspace_size": ([u32, u32, Rect, u32, u32, P], S),
             "rawops_guided_filter": ([P, Rect, u32, u32, ctypes.c_double, P, P, u32, P,
                                       ctypes.c_size_t, P, P], S),
+            "rawops_set_menon_path": ([u32, u32], S),
             "rawops_guided_filter_static": ([P, Rect, u32, u32, ctypes.c_double, P, P, u32, P, P], S),
             "rawops_histogram_rgb": ([P, u32, u32, u32, u32, ctypes.c_double, ctypes.c_double, P, P, P, P], S),
             "rawops_guided_filter_stack": ([P, u32, Rect, u32, ctypes.c_double, P, P, u32, P, P], S),

@@ -446,6 +446,16 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* src, r
     });
 }
 
+rawops_status rawops_set_menon_path(uint32_t path, uint32_t band_rows) {
+    return guarded([&] {
+        if (path > 2) fail(RAWOPS_INVALID_ARGUMENT, "Menon render path must be 0 (automatic), 1 (streaming) or 2 (tiles)");
+        if (band_rows && (band_rows < 4 || band_rows > 4096 || band_rows % 4))
+            fail(RAWOPS_INVALID_ARGUMENT, "streaming band rows must be 0 or a multiple of 4 in 4..4096");
+        g_menon_path.store(path, std::memory_order_relaxed);
+        g_stream_band.store(band_rows, std::memory_order_relaxed);
+    });
+}
+
 rawops_status rawops_menon_decisions_size(const rawops_source* src, size_t* bytes) {
     return guarded([&] {
         if (!src || !src->is_raw || !bytes) fail(RAWOPS_INVALID_ARGUMENT, "decisions need a RAW source");
@@ -475,6 +485,8 @@ rawops_status rawops_render_record(const rawops_plan* plan, const rawops_source*
         L.ostride = dst_stride;
         L.flags = flags;
         L.decisions = reinterpret_cast<uint32_t*>(decisions);
+        L.native_host = &plan->native;
+        L.reduced_host = &plan->reduced;
         hip_check(launch_render(L, plan->device, plan->device + 1, static_cast<hipStream_t>(stream)), "render launch");
     });
 }

@@ -22,6 +22,8 @@
 // strip's halo.  Strips that reach the edge of the active area run the edge
 // form (reflected loads, zero gradients outside the area, mirrored rows and
 // columns; see Strip), the others a form with no edge logic.
+#include <algorithm>
+
 #include "render_launch.hpp"
 #include "stream_launch.hpp"
 
@@ -79,7 +81,7 @@ __device__ __forceinline__ double lane_read(double v, int src) {
 // column's lane, and the windows' rows outside the area from their mirror
 // rows where the next stage reads them.  (Needs an area of at least 16 x 16,
 // so that one reflection reaches every position a result depends on.)
-template <int N0, int RED0, int OPS, bool E>
+template <int N0, int RED0, int OPS, bool E, bool REC>
 struct Strip {
     static constexpr int ncol(int q) { return N0 ^ q; }
     static constexpr bool red_row(int q) { return (RED0 ^ q) != 0; }
@@ -245,6 +247,7 @@ struct Strip {
             const bool horizontal = v >= h;
             hz[2] = horizontal;
             gr[2] = horizontal ? gh[0] : gv[0];
+            if constexpr (REC) record<Q>(s + 2, horizontal);
             if constexpr (E) gr[2] = lane_read(gr[2], out_n[Q] ? mirror_lane[Q] : lane);
         }
         // ---- red / blue at the green site of row s + 1 (MenonDemosaic.cpp:148-161) ----
@@ -279,6 +282,28 @@ struct Strip {
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) { dh[k] = dh[k + 1]; dhp[k] = dhp[k + 1]; }
+    }
+
+    // The decision of row c (a band row) into the record: the strip's valid
+    // lanes hold 56 consecutive non-green sites of the row, which touch at
+    // most three 32-site words; lanes 0..2 OR in one word each (a word's
+    // other sites belong to the neighbouring strip).
+    template <int Q>
+    __device__ __forceinline__ void record(int c, bool horizontal) {
+        if (c < yb || c >= ye) return;
+        const bool valid = lane >= 4 && lane < 60 && xA + ncol(Q) < a.X1;
+        const uint64_t m = __ballot(horizontal && valid);
+        const int k0 = (xA - 2 * lane - a.vx) >> 1;        // site index of lane 0
+        const int first = (k0 + 4) >> 5, last = (k0 + 59) >> 5;
+        const int word = first + lane;
+        if (lane < 3 && word <= last) {
+            const int shift = word * 32 - k0;
+            const uint32_t bits = (uint32_t)(shift >= 0 ? m >> shift : m << -shift);
+            if (bits) {
+                const int Y = c - a.vy;
+                atomicOr(a.decisions + ((size_t)(Y >> 5) * a.dtiles + (size_t)word) * 32 + (Y & 31), bits);
+            }
+        }
     }
 
     template <int Q>
@@ -383,10 +408,10 @@ struct Strip {
     }
 };
 
-template <int N0, int RED0, int OPS, bool E>
+template <int N0, int RED0, int OPS, bool E, bool REC>
 __device__ __forceinline__ void run_strip(const StreamArgs& a, int lane, int xA, int yb, int ye) {
     const int ye4 = yb + ((ye - yb + 3) & ~3);                // whole groups of four rows
-    Strip<N0, RED0, OPS, E> t(a, lane, xA, yb, ye);
+    Strip<N0, RED0, OPS, E, REC> t(a, lane, xA, yb, ye);
     // Rows yb-6 .. yb-3 of the window before the first step (s = yb - 8
     // reads rows s+2 .. s+6), then the rows in flight.
 #pragma unroll
@@ -405,7 +430,7 @@ __device__ __forceinline__ void run_strip(const StreamArgs& a, int lane, int xA,
     // classifier from yb - 2, red / blue from yb - 1), then the band, four
     // rows per iteration: the windows' registers rename within an iteration
     // and only the few values older than four rows move.
-    using S = Strip<N0, RED0, OPS, E>;
+    using S = Strip<N0, RED0, OPS, E, REC>;
     t.template step<0, 0>(yb - 8, false, false);
     t.template step<1, 0>(yb - 7, false, false);
     t.template step<0, S::ST_DV>(yb - 6, false, false);
@@ -423,7 +448,7 @@ __device__ __forceinline__ void run_strip(const StreamArgs& a, int lane, int xA,
     if (t.e_all && a.flags) atomicOr(a.flags, t.e_all);
 }
 
-template <int N0, int RED0, int OPS>
+template <int N0, int RED0, int OPS, bool REC = false>
 __global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const StreamArgs a) {
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kStreamThreads / 64) + (threadIdx.x >> 6)));
     if (wave >= a.wave_cols * a.bands) return;   // whole wavefronts leave together
@@ -437,8 +462,12 @@ __global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const Stre
     // rows and the warm-up rows above them) come within reach of the area's
     // edge takes the edge form.
     const bool edge = x0 < a.ax + 8 || x0 + 128 > a.ax + a.aw - 8 || yb - 16 < a.ay || ye + 16 > a.ay + a.ah;
-    if (edge) run_strip<N0, RED0, OPS, true>(a, lane, xA, yb, ye);
-    else run_strip<N0, RED0, OPS, false>(a, lane, xA, yb, ye);
+    if (edge) run_strip<N0, RED0, OPS, true, REC>(a, lane, xA, yb, ye);
+    else run_strip<N0, RED0, OPS, false, REC>(a, lane, xA, yb, ye);
+}
+
+__global__ void zero_words(uint32_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
 }  // namespace
@@ -459,6 +488,31 @@ hipError_t launch_menon_stream(const StreamArgs& args, int n0, int red0, int ops
     const unsigned blocks = (unsigned)((waves + kStreamThreads / 64 - 1) / (kStreamThreads / 64));
     const dim3 grid(blocks), block(kStreamThreads);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, stream, a); };
+    if (a.decisions) {
+        // The record's words are OR-ed into: clear them first.
+        const size_t words = (size_t)a.dtiles * (size_t)((a.Y1 - a.Y0 + 31) / 32) * 32;
+        hipLaunchKernelGGL(zero_words, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, stream,
+                           a.decisions, words);
+        if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+#define RAWOPS_STREAM_REC(N, R)                                                                \
+    switch (ops) {                                                                             \
+    case STREAM_OPS_LINEAR: go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR, true>); break;     \
+    case STREAM_OPS_LINEAR | STREAM_LIN_GAINS:                                                 \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS, true>); break;      \
+    case STREAM_OPS_LINEAR | STREAM_LIN_MATRIX:                                                \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_MATRIX, true>); break;     \
+    default:                                                                                   \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS | STREAM_LIN_MATRIX, true>); \
+        break;                                                                                 \
+    }
+        if (n0) {
+            if (red0) { RAWOPS_STREAM_REC(1, 1) } else { RAWOPS_STREAM_REC(1, 0) }
+        } else {
+            if (red0) { RAWOPS_STREAM_REC(0, 1) } else { RAWOPS_STREAM_REC(0, 0) }
+        }
+#undef RAWOPS_STREAM_REC
+        return hipGetLastError();
+    }
 #define RAWOPS_STREAM_FORMS(N, R)                                                              \
     switch (ops) {                                                                             \
     case STREAM_OPS_LINEAR: go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR>); break;           \

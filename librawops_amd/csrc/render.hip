@@ -18,7 +18,7 @@
 // INTERIOR path (no mirroring, no zero extension); tiles near the edge take the
 // BORDER path, which reflects reads at the active-area edge and clips
 // computation to the area exactly as the reference's clipped planes do.
-#include <cstdlib>
+#include <atomic>
 
 #include "render_tile.hpp"
 #include "render_launch.hpp"
@@ -41,24 +41,31 @@ uint32_t site_of_host(const SensorArgs& s, int x, int y) {
     return ((((uint32_t)y & 1u) + s.phase_y) & 1u) * 2u + ((((uint32_t)x & 1u) + s.phase_x) & 1u);
 }
 
-// The streaming kernel renders the interior of Menon mip-0 viewports unless
-// RAWOPS_MENON_STREAM=0 (tile kernel only); RAWOPS_STREAM_BAND sets its rows
-// per wavefront (a multiple of four; default 16).
-bool stream_enabled() {
-    static const bool on = [] {
-        const char* v = std::getenv("RAWOPS_MENON_STREAM");
-        return !(v && v[0] == '0');
-    }();
-    return on;
+} // namespace
+
+// Menon mip-0 render path (rawops_set_menon_path): 0 automatic, 1 streaming
+// wherever the area allows it, 2 tile kernel only; rows per streaming
+// wavefront (0: 16).
+std::atomic<uint32_t> g_menon_path{0}, g_stream_band{0};
+
+namespace {
+
+// The streaming kernel wants enough wavefronts to fill the chip: viewports
+// smaller than about 1536 strips of 112 x 16 (2.75 MP) render faster as
+// tiles, whose 512-thread workgroups split them finer.
+int stream_band_rows();
+
+bool use_stream(const RenderArgs& a) {
+    const SensorArgs& S = a.s;
+    if (S.aw < 16 || S.ah < 16 || a.ostride > (1u << 23)) return false;
+    const uint32_t path = g_menon_path.load(std::memory_order_relaxed);
+    if (path == 2) return false;
+    if (path == 1) return true;
+    const int band = stream_band_rows() ? stream_band_rows() : 16;
+    return (long)((a.vw + 111) / 112) * ((a.vh + band - 1) / band) >= 1536;
 }
-int stream_band_rows() {
-    static const int rows = [] {
-        const char* v = std::getenv("RAWOPS_STREAM_BAND");
-        const int r = v ? std::atoi(v) : 0;
-        return r >= 4 && r <= 4096 ? r & ~3 : 0;
-    }();
-    return rows;
-}
+
+int stream_band_rows() { return (int)g_stream_band.load(std::memory_order_relaxed); }
 
 // The FP32 output's op program (native ops, then the reduced ops before the
 // tap) as a linear chain [gains] [multiplier] [3x3] in this order, each at
@@ -118,11 +125,14 @@ hipError_t launch(const RenderLaunch& L, const OpList* native, const OpList* red
     const unsigned blocks = (unsigned)a.tiles_x * (unsigned)tiles_y;
     if (!blocks) return hipSuccess;
     const SensorArgs& S = a.s;
-    if (DEMOSAIC == RAWOPS_DEMOSAIC_MENON && a.mip == 0 && !a.decisions && S.aw >= 16 && S.ah >= 16 &&
-        a.ostride <= (1u << 23) && stream_enabled()) {
+    StreamArgs t{};
+    const int ops = DEMOSAIC == RAWOPS_DEMOSAIC_MENON && a.mip == 0 ? linear_chain(L, a.tap, t) : STREAM_OPS_LISTS;
+    // Recording forwards stream only with linear chains (the recording
+    // forms are compiled for those).
+    if (DEMOSAIC == RAWOPS_DEMOSAIC_MENON && a.mip == 0 && (!a.decisions || ops != STREAM_OPS_LISTS) &&
+        use_stream(a)) {
         // The streaming kernel renders the whole viewport (its edge form
         // handles the strips that reach the edge of the active area).
-        StreamArgs t{};
         t.samples = S.samples;
         t.stride = S.stride;
         t.ax = S.ax; t.ay = S.ay; t.aw = S.aw; t.ah = S.ah;
@@ -144,10 +154,12 @@ hipError_t launch(const RenderLaunch& L, const OpList* native, const OpList* red
         t.out8 = a.out8;
         t.ostride = a.ostride;
         t.flags = a.flags;
+        t.decisions = a.decisions;
+        t.dtiles = a.dtiles;
         const int n0 = (int)(((S.ng_parity >> ((uint32_t)t.Y0 & 1u)) & 1u) ^ ((uint32_t)t.X0 & 1u));
         const uint32_t site_ng = site_of_host(S, t.X0 + n0, t.Y0);
         const int red0 = ((S.colors >> (2u * site_ng)) & 3u) == 0 ? 1 : 0;
-        return launch_menon_stream(t, n0, red0, linear_chain(L, a.tap, t), stream);
+        return launch_menon_stream(t, n0, red0, ops, stream);
     }
     hipLaunchKernelGGL((render_kernel<DEMOSAIC, TW, TH>), dim3(blocks), dim3(kRenderThreads), 0, stream, a);
     return hipGetLastError();

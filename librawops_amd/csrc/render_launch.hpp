@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "../../include/rawops_hip.h"
 #include "device_common.hpp"
 
@@ -36,6 +38,8 @@ inline size_t decision_bytes(uint32_t width, uint32_t height) {
 }
 hipError_t launch_render(const RenderLaunch& launch, const OpList* native, const OpList* reduced,
                          hipStream_t stream);
+// Menon mip-0 render path and streaming band rows (rawops_set_menon_path).
+extern std::atomic<uint32_t> g_menon_path, g_stream_band;
 
 // Adjoint of out = M * (m * (g (.) demosaic(raw))) over a native viewport.
 struct BackwardLaunch {

@@ -30,6 +30,10 @@ struct StreamArgs {
     // CameraToWorkingNode).  A chain with only one of the first two runs the
     // other as exact factors of 1.0f (x * 1.0f == x for every float).
     float gains[3], mult, m[9];
+    // Menon decisions of the viewport (the 64 x 32-tile record of
+    // render_launch.hpp, zeroed by the launcher), or null.
+    uint32_t* decisions;
+    uint32_t dtiles;             // record tiles per viewport row
 };
 
 // Op program forms of the streaming kernel: the device op lists run by the

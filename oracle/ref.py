@@ -40,6 +40,8 @@ def lib():
                                         P, P, P, ctypes.c_int, Rect, ctypes.c_uint32,
                                         ctypes.c_uint32, ctypes.c_uint32, P]
         L.refh_matrices.argtypes = [P, ctypes.c_int, P, P, P]
+        L.refh_time_menon_bands.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_float, P, Rect, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
         L.refh_exposure_multiplier.argtypes = [ctypes.c_float]
         L.refh_exposure_multiplier.restype = ctypes.c_float
         _LIB = L
@@ -96,6 +98,20 @@ def guided_chain(raster, space, radius, eps, viewport, stops=0.0, knots=None, cu
     return out
 
 
+def time_menon_bands(meta: Bayer, samples, wb, stops, camera_to_xyz, rect, threads, reps):
+    """Seconds per round of `threads` concurrent band requests over `rect`
+    (Menon [-> WB -> exposure] -> camera-to-ProPhoto; wb None: no WB or
+    exposure node), one OpenMP thread each."""
+    L = lib()
+    s = np.ascontiguousarray(samples, np.uint16)
+    w = np.asarray(wb, np.float32) if wb is not None else None
+    cam = np.asarray(camera_to_xyz, np.float64)
+    out = ctypes.c_double()
+    _check(L.refh_time_menon_bands(ctypes.byref(meta.struct()), _p(s), s.size, _p(w), float(stops), _p(cam),
+                                   Rect(*rect), threads, reps, ctypes.byref(out)))
+    return out.value
+
+
 def matrices(camera_to_xyz, space):
     L = lib()
     cam = np.asarray(camera_to_xyz, np.float64)
@@ -110,5 +126,5 @@ def exposure_multiplier(stops):
     return np.float32(lib().refh_exposure_multiplier(float(np.float32(stops))))
 
 
-__all__ = ["available", "render_raw", "guided_chain", "matrices", "exposure_multiplier",
+__all__ = ["available", "render_raw", "guided_chain", "matrices", "exposure_multiplier", "time_menon_bands",
            "Bayer", "Chain", "request_bounds"]

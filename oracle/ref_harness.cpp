@@ -7,13 +7,19 @@
 #include "SpatialOps.hpp"
 #include "ToneOps.hpp"
 
+#include <chrono>
 #include <cstring>
+#include <thread>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "rawops_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 using namespace rawengine;
 
@@ -167,6 +173,52 @@ float refh_exposure_multiplier(float stops) {
         v = ExposureNode(src, stops).render({0, 0, 1, 1}).rgb[0];
     });
     return v;
+}
+
+// Wall seconds per round of the reference's fastest legal Menon
+// configuration on the host: `threads` concurrent band requests
+// (render_image of one band of `rect` each, tile = band, one OpenMP thread
+// per request) on one const Menon [-> WB -> exposure] -> camera-to-working
+// chain built once (wb = NULL: no WB or exposure node), `reps` rounds after
+// one untimed round.
+int refh_time_menon_bands(const ro_bayer* b, const uint16_t* samples, size_t count, const float wb[3], float stops,
+                          const double cam[9], ro_rect rect, uint32_t threads, uint32_t reps, double* seconds) {
+    return guarded([&] {
+        const RawImage image = make_image(b, samples, count);
+        const Rect area = image.metadata().active_area;
+        std::shared_ptr<const Node> n =
+            std::make_shared<RawUnpackNode>(image, RawDemosaicIdentity{"rawengine.menon_base", 1});
+        if (wb) {
+            n = std::make_shared<WhiteBalanceNode>(n, wb[0], wb[1], wb[2]);
+            n = std::make_shared<ExposureNode>(n, stops);
+        }
+        CameraColorTransform t;
+        for (int i = 0; i < 9; ++i) t.camera_to_xyz_d50[i] = cam[i];
+        t.target = WorkingSpace::LinearProPhotoD50;
+        n = std::make_shared<CameraToWorkingNode>(n, t, area);
+        const uint32_t band = (rect.height + threads - 1) / threads;
+        auto round = [&] {
+            std::vector<std::thread> pool;
+            for (uint32_t k = 0; k < threads; ++k) {
+                const uint32_t y0 = k * band;
+                if (y0 >= rect.height) break;
+                const Rect r{rect.x, rect.y + y0, rect.width, std::min(band, rect.height - y0)};
+                pool.emplace_back([&, r] {
+#ifdef _OPENMP
+                    omp_set_num_threads(1);
+#endif
+                    const RenderRequest req{r, std::max(r.width, r.height), RenderLevel{}};
+                    const Tile out = Renderer{}.render_image(*n, area, req);
+                    if (out.rgb.empty()) throw std::runtime_error("empty band");
+                });
+            }
+            for (auto& th : pool) th.join();
+        };
+        round();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < reps; ++i) round();
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+    });
 }
 
 } // extern "C"

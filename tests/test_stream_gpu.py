@@ -1,12 +1,13 @@
-"""The streaming Menon kernel (the interior of mip-0 Menon renders: a
-wavefront per 112-column strip walking down 16-row bands, register windows
-and lane shifts) together with the tile kernel on the ring of border tiles,
-against the CPU oracle bit for bit (MenonDemosaic.cpp:89-174 and the chain
+"""The streaming Menon kernel (mip-0 Menon renders of viewports that fill
+the device: a wavefront per 112-column strip walking down 16-row bands,
+register windows and lane shifts, an edge form for the strips that reach the
+active area's edge), forced on here for every size, against the CPU oracle
+bit for bit (MenonDemosaic.cpp:89-174 and the chain
 nodes after it).  Viewports at every row and column parity over all four
 CFA patterns reach the four compiled (non-green column, red/blue) forms;
 chains reach every op form: none, WB / exposure only, 3x3 only, all three,
 and the device op lists (tone + clip, sRGB encode), including viewports
-whose interior ends mid-tile and bands that end mid-group."""
+whose bands end mid-group."""
 import numpy as np
 import pytest
 
@@ -15,6 +16,38 @@ from oracle import oracle as O
 from tests.helpers import bits_equal, first_mismatch, gpu_image, meta_dict, oracle_meta, random_samples
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True, scope="module")
+def streaming_path():
+    """Every Menon mip-0 render of this module on the streaming kernel (the
+    automatic choice keeps viewports this small on the tile kernel)."""
+    from librawops_amd import _capi as C
+    C.check(C.lib().rawops_set_menon_path(1, 0))
+    yield
+    C.check(C.lib().rawops_set_menon_path(0, 0))
+
+
+def test_paths_render_the_same_bits():
+    """Streaming (at several band heights) and tile renders of one viewport."""
+    from librawops_amd import _capi as C
+    w, h, stride, area = 520, 300, 528, (4, 3, 511, 291)
+    m = meta_dict(w, h, stride, 1, 1, 0, area, (510, 512, 511, 513), (16383,) * 4)
+    s = random_samples(h, stride, seed=77, hi=16383)
+    img = gpu_image(m, s)
+    node = chain_node(img, CHAINS["exposure_matrix"])
+    req = L.RenderRequest(L.Rect(5, 4, 500, 280), 64)
+    outs = []
+    try:
+        for path, band in ((2, 0), (1, 0), (1, 4), (1, 12), (1, 64)):
+            C.check(C.lib().rawops_set_menon_path(path, band))
+            outs.append(L.Renderer().render_image(node, img.metadata.active_area, req).rgb)
+    finally:
+        C.check(C.lib().rawops_set_menon_path(1, 0))
+    for o in outs[1:]:
+        assert bits_equal(o, outs[0])
+    assert C.lib().rawops_set_menon_path(3, 0) == C.INVALID_ARGUMENT
+    assert C.lib().rawops_set_menon_path(1, 6) == C.INVALID_ARGUMENT
 
 CAM = (0.7, 0.2, 0.1, 0.25, 0.7, 0.05, 0.02, 0.1, 0.88)
 
@@ -89,3 +122,32 @@ def test_stream_edges_match_oracle(pattern, phase, area):
         got = L.Renderer().render_image(node, img.metadata.active_area, L.RenderRequest(L.Rect(*vp), 64)).rgb
         exp = O.render_raw(om, s, c, vp)
         assert bits_equal(got, exp), (vp, first_mismatch(got, exp))
+
+
+@pytest.mark.parametrize("vp_rel", [(0, 0, None, None), (3, 1, 300, 97), (66, 33, 257, 140)])
+def test_stream_recording_matches_tile_recording(vp_rel):
+    """rawops_render_record on the streaming path (decision bits OR-ed into a
+    cleared record by each strip) writes the same record words and output
+    bits as the tile kernel (one record word per tile row)."""
+    import torch
+    from librawops_amd import autograd as A
+    w, h, stride, area = 452, 214, 460, (3, 2, 443, 207)
+    m = meta_dict(w, h, stride, 2, 0, 1, area, (500, 510, 505, 515), (9000,) * 4)
+    s = random_samples(h, stride, seed=13, lo=300, hi=9000)
+    img = gpu_image(m, s)
+    ax, ay, aw, ah = area
+    dx, dy, vw, vh = vp_rel
+    vp = L.Rect(ax + dx, ay + dy, vw or aw - dx, vh or ah - dy)
+    cam = np.array([[1.05, -0.03, -0.02], [-0.06, 1.02, 0.04], [0.03, 0.09, 0.88]], np.float32)
+    from librawops_amd import _capi as C
+    try:
+        C.check(C.lib().rawops_set_menon_path(2, 0))
+        out_t, dec_t = A.render_record(img, [1.7, 1.0, 1.3], 0.4, cam, vp)
+        C.check(C.lib().rawops_set_menon_path(1, 0))
+        out_s, dec_s = A.render_record(img, [1.7, 1.0, 1.3], 0.4, cam, vp)
+        torch.cuda.synchronize()
+    finally:
+        C.check(C.lib().rawops_set_menon_path(1, 0))
+    assert torch.equal(out_s.view(torch.int32), out_t.view(torch.int32))
+    n = ((vp.width + 63) // 64) * ((vp.height + 31) // 32) * 32
+    assert torch.equal(dec_s.view(torch.int32)[:n], dec_t.view(torch.int32)[:n])
