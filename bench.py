@@ -160,7 +160,7 @@ def leg_c1_bilinear(steps):
             "gpu_over_cpu": round(px / ms / 1e3 / cpu["value"], 1)}
 
 
-def leg_c3_guided(steps, radius=8, eps=1e-3):
+def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
     """Working-Y guided filter + 0.7 EV + 16-point monotone curve on a
     6000x4000 FP32 lognormal raster (Rec.2020 guide): fused forward, and
     the backward chain (curve, exposure, guided filter) for dL/dp, dL/deps,
@@ -229,10 +229,18 @@ def leg_c3_guided(steps, radius=8, eps=1e-3):
         C.check(C.lib().rawops_guided_backward(img._handle, radius, eps, vp.c(), P(g2), W, P(gp), P(ge),
                                                P(wsb), wsb.numel(), sp))
     gfb_ms = _timed(gf_only, max(1, steps // 2), st)
-    px = W * H
     # CPU forward: the reference's guided filter chain (serial per request,
     # SpatialOps.cpp:168-287) as concurrent row-band requests over a 1024^2
     # crop of the raster, or the restatement's OpenMP bands without it.
+    px = W * H
+    res = {"workload": f"6000x4000 FP32 lognormal raster, Rec.2020 guide, r={radius}, eps={eps}, +0.7 EV, "
+                       "16-point cubic curve; fwd fused, bwd = curve + exposure + guided filter",
+           "forward_ms": round(fwd_ms, 4), "forward_gbs": round(24 * px / fwd_ms / 1e6, 1),
+           "backward_chain_ms": round(bwd_ms, 4), "guided_backward_ms": round(gfb_ms, 4),
+           "guided_backward_gbs": round(36 * px / gfb_ms / 1e6, 1),
+           "fwd_bwd_mpix_s": round(px / (fwd_ms + bwd_ms) / 1e3, 1)}
+    if not cpu_leg:
+        return res
     import threading
     import numpy as np
     from oracle import oracle as O
@@ -268,13 +276,9 @@ def leg_c3_guided(steps, radius=8, eps=1e-3):
     cpu = {"value": round(1024 * 1024 / cpu_ms / 1e3, 2), "unit": "Mpix/s",
            "kind": "reference" if have_ref else "port", "cores": threads,
            "sample": f"{reps} forward renders of a 1024^2 crop, {threads} concurrent row-band requests"}
-    return {"workload": f"6000x4000 FP32 lognormal raster, Rec.2020 guide, r={radius}, eps={eps}, +0.7 EV, "
-                        "16-point cubic curve; fwd fused, bwd = curve + exposure + guided filter",
-            "forward_ms": round(fwd_ms, 4), "forward_gbs": round(24 * px / fwd_ms / 1e6, 1),
-            "backward_chain_ms": round(bwd_ms, 4), "guided_backward_ms": round(gfb_ms, 4),
-            "guided_backward_gbs": round(36 * px / gfb_ms / 1e6, 1),
-            "fwd_bwd_mpix_s": round(px / (fwd_ms + bwd_ms) / 1e3, 1), "cpu_baseline_forward": cpu,
-            "gpu_forward_over_cpu": round(px / fwd_ms / 1e3 / cpu["value"], 1)}
+    res["cpu_baseline_forward"] = cpu
+    res["gpu_forward_over_cpu"] = round(px / fwd_ms / 1e3 / cpu["value"], 1)
+    return res
 
 
 def leg_c4_batch(replays, seed=4):
@@ -400,6 +404,8 @@ def main():
     ap.add_argument("--menon-path", type=int, default=0,
                     help="Menon render path (rawops_set_menon_path): 0 automatic, 1 streaming, 2 tiles")
     ap.add_argument("--stream-band", type=int, default=0, help="rows per streaming wavefront (0: 16)")
+    ap.add_argument("--leg", choices=("c1", "c3", "c4", "c5"), default=None,
+                    help="run only this configuration's leg and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -428,6 +434,11 @@ def main():
     torch.cuda.set_device(0)
     from librawops_amd import _capi as C
     C.check(C.lib().rawops_set_menon_path(args.menon_path, args.stream_band))
+    if args.leg:
+        fn = {"c1": lambda: leg_c1_bilinear(args.steps), "c3": lambda: leg_c3_guided(args.steps, cpu_leg=not args.no_cpu),
+              "c4": lambda: leg_c4_batch(args.steps), "c5": lambda: leg_c5_chain(args.c5_steps)}[args.leg]
+        print(json.dumps({"leg": args.leg, **fn()}))
+        return
     meta, samples, cam = synth.config2_menon(seed=2)
     md = L.RawMetadata(meta["width"], meta["height"], meta["row_stride_samples"], L.BayerPattern(meta["pattern"]),
                        meta["cfa_phase_x"], meta["cfa_phase_y"], L.Rect(*meta["active_area"]),

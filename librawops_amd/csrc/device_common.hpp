@@ -143,23 +143,6 @@ struct LdsCurve {
     const double *x, *y, *slope, *c0, *c1;
 };
 
-// Stage curve c (n <= cap knots) into LDS at base (5 n doubles); the whole
-// workgroup copies, the caller synchronises.
-__device__ __forceinline__ LdsCurve stage_curve(const DevCurve* c, double* base) {
-    Const<DevCurve>& s = as_const(*c);
-    const uint32_t n = s.n;
-    LdsCurve l{n, s.identity, base, base + n, base + 2 * n, base + 3 * n, base + 4 * n};
-    if (!l.identity)
-        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-            base[k] = s.x[k];
-            base[n + k] = s.y[k];
-            base[2 * n + k] = s.slope[k];
-            base[3 * n + k] = s.c0[k];
-            base[4 * n + k] = s.c1[k];
-        }
-    return l;
-}
-
 // ExtendedCurvesNode::map, ToneOps.cpp:820-831, over a DevCurve in the
 // constant address space or an LdsCurve.
 template <class C>
@@ -222,17 +205,105 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
     return (float)mapped;
 }
 
-// Master then channel curve with LDS-staged tables (curves[0] master,
-// curves[1 + channel]); same checks and rounding as curves_channel.
-__device__ inline float curves_channel_lds(const LdsCurve* curves, uint32_t cubic, int channel, float value,
-                                           uint32_t& err) {
-    if (curves[0].identity && curves[channel + 1].identity) return value;
-    if (!finite(value)) { err |= FLAG_INVALID; return value; }
-    const double master = curve_map_in(curves[0], cubic, (double)value);
-    if (!__builtin_isfinite(master)) { err |= FLAG_INVALID; return value; }
-    const double mapped = curve_map_in(curves[channel + 1], cubic, master);
-    if (!__builtin_isfinite(mapped) || fabs(mapped) > 3.4028234663852886e38) { err |= FLAG_INVALID; return value; }
-    return (float)mapped;
+// A curve staged in LDS for the filter passes: the knot abscissae (for the
+// knot search) and, per knot k, {x_k, y_k, slope_i, 1 / (x_k+1 - x_k),
+// q0..q3} with i = min(k, n - 2) and q the power-basis coefficients of the
+// segment's cubic (control points y_k, c0_k, c1_k, y_k+1: q0 = y_k,
+// q1 = 3 (c0 - y_k), q2 = 3 (y_k - 2 c0 + c1), q3 = y_k+1 - y_k + 3 (c0 - c1)),
+// so a cubic value is three fused multiply-adds (Horner) instead of the six
+// blends of de Casteljau's construction (ToneOps.cpp:788-792).  Same knot
+// selection, exact-knot and linear-extension rules as curve_map_in; cubic
+// values agree with it to FP64 rounding (the stored results are float32).
+struct LdsPoly {
+    uint32_t n, identity;
+    const double* x;
+    const double* rec;   // 8 doubles per knot, 16-byte aligned
+};
+
+constexpr int kPolyRec = 8;
+
+// Stage curve c (n <= cap knots) at base (9 n doubles, 16-byte aligned);
+// the whole workgroup works, the caller synchronises.
+__device__ __forceinline__ LdsPoly stage_poly(const DevCurve* c, double* base) {
+    Const<DevCurve>& s = as_const(*c);
+    const uint32_t n = s.n;
+    LdsPoly l{n, s.identity, base + kPolyRec * n, base};
+    if (!l.identity)
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            const uint32_t i = k < n - 2u ? k : n - 2u;
+            double* r = base + kPolyRec * k;
+            r[0] = s.x[k];
+            r[1] = s.y[k];
+            r[2] = s.slope[i];
+            if (k + 1 < n) {
+                const double y0 = s.y[k], y1 = s.y[k + 1], c0 = s.c0[k], c1 = s.c1[k];
+                r[3] = 1.0 / (s.x[k + 1] - s.x[k]);
+                r[4] = y0;
+                r[5] = 3.0 * (c0 - y0);
+                r[6] = 3.0 * ((y0 - 2.0 * c0) + c1);
+                r[7] = (y1 - y0) + 3.0 * (c0 - c1);
+            } else {
+                r[3] = r[4] = r[5] = r[6] = r[7] = 0.0;
+            }
+            base[kPolyRec * n + k] = s.x[k];
+        }
+    return l;
+}
+
+// The curve at v given cnt = the number of knots <= v (no branches).
+__device__ __forceinline__ double poly_eval(const LdsPoly& c, uint32_t cubic, double v, uint32_t cnt) {
+    const uint32_t left = cnt == 0 ? 0u : cnt - 1u;
+    const double2* r = reinterpret_cast<const double2*>(c.rec + kPolyRec * left);
+    const double2 a = r[0], b = r[1], q01 = r[2], q23 = r[3];
+    const double xl = a.x, yl = a.y, sl = b.x, inv = b.y;
+    const double lin = sl == 0 ? yl : yl + (v - xl) * sl;
+    const double t = (v - xl) * inv;
+    const double cub = __builtin_fma(__builtin_fma(__builtin_fma(q23.y, t, q23.x), t, q01.y), t, q01.x);
+    const bool linear = !cubic || v < c.x[0] || v > c.x[c.n - 1];
+    return v == xl ? yl : (linear ? lin : cub);
+}
+
+// Number of knots <= v for three values at once (one pass over the knots).
+__device__ __forceinline__ void knot_counts3(const LdsPoly& c, const double v[3], uint32_t cnt[3]) {
+    cnt[0] = cnt[1] = cnt[2] = 0;
+    for (uint32_t k = 0; k < c.n; ++k) {
+        const double xk = c.x[k];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) cnt[ch] += v[ch] >= xk ? 1u : 0u;
+    }
+}
+
+// ExtendedCurvesNode::map (ToneOps.cpp:820-831) for the three channels of a
+// pixel over LDS-staged curves (master curves[0], channel curves[1..3]):
+// the checks and flags of curves_channel.
+__device__ inline void curves3_lds(const LdsPoly* curves, uint32_t cubic, float o[3], uint32_t& err) {
+    double v[3], m[3];
+    uint32_t cnt[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) v[ch] = (double)o[ch];
+    if (!curves[0].identity) {
+        knot_counts3(curves[0], v, cnt);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) m[ch] = poly_eval(curves[0], cubic, v[ch], cnt[ch]);
+    } else {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) m[ch] = v[ch];
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const LdsPoly& cc = curves[ch + 1];
+        if (curves[0].identity && cc.identity) continue;
+        double mapped = m[ch];
+        if (!cc.identity) {
+            uint32_t k = 0;
+            for (uint32_t j = 0; j < cc.n; ++j) k += m[ch] >= cc.x[j] ? 1u : 0u;
+            mapped = poly_eval(cc, cubic, m[ch], k);
+        }
+        const bool bad = !finite(o[ch]) || !__builtin_isfinite(m[ch]) || !__builtin_isfinite(mapped) ||
+                         fabs(mapped) > 3.4028234663852886e38;
+        if (bad) err |= FLAG_INVALID;
+        o[ch] = bad ? o[ch] : (float)mapped;
+    }
 }
 
 // The fused op program: each op rounds exactly like its reference node.

@@ -308,17 +308,17 @@ __global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g0) {
 constexpr int kTailKnots = 64;
 __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
     const GuidedArgs g = slice_of(g0);
-    __shared__ double tables[4 * 5 * kTailKnots];
+    __shared__ __attribute__((aligned(16))) double tables[4 * (kPolyRec + 1) * kTailKnots];
     const size_t n = (size_t)g.vw * g.vh;
     Const<OpList>& L = as_const(*g.post);
     const uint32_t end = L.n;
     const DevCurves* cv = L.op[g.fused].curves;
     bool staged = true;
     for (int c = 0; c < 4; ++c) staged = staged && (as_const(*cv).c[c].identity || as_const(*cv).c[c].n <= kTailKnots);
-    LdsCurve curves[4];
+    LdsPoly curves[4];
     const uint32_t cubic = as_const(*cv).cubic;
     if (staged) {
-        for (int c = 0; c < 4; ++c) curves[c] = stage_curve(&cv->c[c], tables + c * 5 * kTailKnots);
+        for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * (kPolyRec + 1) * kTailKnots);
         __syncthreads();
     }
     uint32_t err = 0;
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
         float* d = g.out + ((size_t)y * g.ostride + (size_t)x) * 3;
         float p[3] = {d[0], d[1], d[2]};
         if (staged) {
-            for (int c = 0; c < 3; ++c) p[c] = curves_channel_lds(curves, cubic, c, p[c], err);
+            curves3_lds(curves, cubic, p, err);
             apply_ops(*g.post, g.fused + 1, end, p[0], p[1], p[2], err);
         } else {
             apply_ops(*g.post, g.fused, end, p[0], p[1], p[2], err);
@@ -336,6 +336,323 @@ __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
     }
     if (err && g.flags) atomicOr(g.flags, err);
 }
+
+// ---------------------------------------------------------------- fused --
+//
+// The whole filter and its post-filter program in one pass at level 0, with
+// the per-centre coefficients kept on chip instead of a workspace round trip.
+// A workgroup (8 wavefronts) owns a strip of kFusedCols pixel columns
+// j = 0..255 at x = px0 + j, px0 = ox - 2r, and a band of output rows; the
+// centres of the strip are j in [r, 256 - r), its outputs j in [2r, 256 - 2r)
+// (Wo = 256 - 4r columns from x = ox).  Step s, with c = c_lo + s:
+//
+//   wave 0: the vertical window sums of Y and Y^2 of centre row c, per pixel
+//     column (a lane holds 4 adjacent columns), slid down one row per step;
+//     their sums along x as sliding sums over the lane's run of 4 centres,
+//     read back from a wave-private LDS row zero-padded by r at both ends;
+//     each centre's mean Y and 1 / (var + eps), written to a double-buffered
+//     row.
+//   waves 1..3 (channel ch): the same for p_ch and Y p_ch of row c (kept to
+//     the next step), and, from last step's sums and wave 0's row, a_ch and
+//     b_ch of centre row c - 1, summed along x over each output column's
+//     r-window the same way and written (FP32) into a ring of 2r + 3 centre
+//     rows in LDS.
+//   waves 4..7: a thread per output column adds the ring row c - 2 to the
+//     sums of its window and subtracts row c - 2r - 3, forms output row
+//     c - 2 - r and runs the post-filter program (curves included), and
+//     stores the pixel.
+//
+// One barrier per step; the ring slot written in a step is the one read as
+// the leaving row a step later.  One workgroup per CU (ring of 2r + 3 rows x
+// 6 planes; 98 KB at r = 8).
+constexpr int kFusedCols = 256;
+constexpr int kFusedThreads = 512;
+constexpr int kFusedMaxR = 11;
+constexpr int kFusedRun = 4;                                        // columns per coefficient lane
+constexpr int kFusedLogical = kFusedCols + 2 * kFusedMaxR;          // zero-padded run row
+constexpr int kFusedPlane = kFusedLogical + (kFusedLogical >> 2);   // + one pad double per 4 (banks)
+constexpr int kFusedMinBand = 64;                                   // rows per band at least
+
+// Ring geometry at radius R: a row of each plane holds the outputs from
+// column RB = 2R rounded down to a multiple of 4 (so a coefficient lane's 4
+// outputs are one aligned 16-byte group), WR floats (a multiple of 4).
+template <int R> struct FusedRing {
+    static constexpr int D = 2 * R + 3;
+    static constexpr int RB = (2 * R) & ~3;
+    static constexpr int WR = ((kFusedCols - 2 * R - RB) + 3) & ~3;
+    static constexpr int floats = D * 6 * WR;
+};
+
+__device__ __forceinline__ int run_index(int i) { return i + (i >> 2); }
+
+__device__ __forceinline__ int clipped_count(int v, int r, int n) {
+    return min(n, v + r + 1) - max(0, v - r);
+}
+
+// Four adjacent pixels of row yy at columns x0..x0+3; columns outside
+// [0, W) load a clamped in-bounds pixel (its moments are discarded).
+__device__ __forceinline__ void load4(const float* __restrict__ src, uint32_t stride, int yy, int x0, int W,
+                                      float p[12]) {
+    const float* row = src + (size_t)yy * stride * 3;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int x = min(max(x0 + t, 0), W - 1);
+        const float* q = row + (size_t)x * 3;
+        p[3 * t] = q[0]; p[3 * t + 1] = q[1]; p[3 * t + 2] = q[2];
+    }
+}
+
+// Sliding sums over a zero-padded run row: s[t] = sum_{i=0}^{2R} b[i0 + t + i]
+// (two partial sums for the first window, so the adds overlap).
+template <int R>
+__device__ __forceinline__ void run_sums(const double* __restrict__ b, int i0, double s[4]) {
+    double e = 0.0, o = 0.0;
+#pragma unroll
+    for (int i = 0; i <= 2 * R; i += 2) e += b[run_index(i0 + i)];
+#pragma unroll
+    for (int i = 1; i <= 2 * R; i += 2) o += b[run_index(i0 + i)];
+    double acc = e + o;
+    s[0] = acc;
+#pragma unroll
+    for (int t = 1; t < 4; ++t) {
+        acc = acc + b[run_index(i0 + 2 * R + t)] - b[run_index(i0 + t - 1)];
+        s[t] = acc;
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kFusedThreads) gf_fused(const GuidedArgs g0) {
+    const GuidedArgs g = slice_of(g0);
+    using RG = FusedRing<R>;
+    constexpr int D = RG::D, RB = RG::RB, WR = RG::WR;
+    __shared__ __attribute__((aligned(16))) float ring[RG::floats];
+    __shared__ double runs[4][2][kFusedPlane];
+    __shared__ double my_d[2][2][4][64];        // [buffer][mean Y, 1/(var+eps)][column of the lane][lane]
+    const int ox = g.vx + (int)blockIdx.x * (kFusedCols - 4 * R), px0 = ox - 2 * R;
+    const int y0 = g.vy + (int)blockIdx.y * g.band, y1 = min(y0 + g.band, g.vy + g.vh);
+    const int c_lo = max(0, y0 - R), c_hi = min(g.LH, y1 + R);
+    const int steps = y1 + R - c_lo + 2;
+    // Pixel columns inside the support of the viewport (its 2R halo): only
+    // these enter the moments and the non-finite check.
+    const int sx0 = max(0, g.vx - 2 * R), sx1 = min(g.LW, g.vx + g.vw + 2 * R);
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    uint32_t err = 0;
+
+    // Zero the run rows (their pads stay zero).
+    for (int i = t; i < 4 * 2 * kFusedPlane; i += kFusedThreads) (&runs[0][0][0])[i] = 0.0;
+    // Per-strip shift of the moments: the strip's first centre pixel.
+    double shift[3];
+    {
+        const float* q = g.src + ((size_t)c_lo * g.src_stride + (size_t)min(max(px0 + R, sx0), sx1 - 1)) * 3;
+        shift[0] = q[0]; shift[1] = q[1]; shift[2] = q[2];
+    }
+    const double shift_y = working_y(g, shift[0], shift[1], shift[2]);
+    __syncthreads();
+
+    if (wave < 4) {
+        // ---- coefficient waves
+        const int role = wave;                 // 0: Y, Y^2; 1 + ch: p_ch, Y p_ch
+        const int ch = role - 1;
+        const int j0 = kFusedRun * lane;       // first pixel column of the lane
+        const int x0 = px0 + j0;
+        // colok: the column enters the moments; centre: a centre of the strip
+        // inside the image (others are zero in the sums along x); need: a
+        // centre some output of the viewport reads (checked for overflow).
+        bool colok[4], centre[4], need[4];
+        double nxc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            colok[u] = x0 + u >= sx0 && x0 + u < sx1;
+            centre[u] = j0 + u >= R && j0 + u < kFusedCols - R && x0 + u >= 0 && x0 + u < g.LW;
+            need[u] = centre[u] && x0 + u >= g.vx - R && x0 + u < g.vx + g.vw + R;
+            nxc[u] = (double)clipped_count(x0 + u, R, g.LW);
+        }
+        double V0[4] = {0, 0, 0, 0}, V1[4] = {0, 0, 0, 0};
+        auto accumulate = [&](const float p[12], double sign, bool check) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float* q = p + 3 * u;
+                if (check && colok[u] && (!finite(q[0]) || !finite(q[1]) || !finite(q[2]))) err |= FLAG_INVALID;
+                const double d0 = (double)q[0] - shift[0], d1 = (double)q[1] - shift[1], d2 = (double)q[2] - shift[2];
+                const double y = working_y(g, d0, d1, d2);
+                const double dc = ch == 0 ? d0 : (ch == 1 ? d1 : d2);
+                const double m0 = role == 0 ? y : dc;
+                const double m1 = role == 0 ? y * y : y * dc;
+                V0[u] = colok[u] ? V0[u] + sign * m0 : V0[u];
+                V1[u] = colok[u] ? V1[u] + sign * m1 : V1[u];
+            }
+        };
+        // Window of centre row c_lo.
+        for (int yy = max(0, c_lo - R); yy <= min(g.LH - 1, c_lo + R); ++yy) {
+            float p[12];
+            load4(g.src, g.src_stride, yy, x0, g.LW, p);
+            accumulate(p, 1.0, role == 0);
+        }
+        double* b0 = runs[wave][0];
+        double* b1 = runs[wave][1];
+        // Rows of step s arrive two steps ahead (buffer s & 1), so a load has
+        // two steps to return before its moments are formed.
+        // The loads are unconditional (rows clamped into the image; unused
+        // rows are never accumulated), so the compiler's count of loads in
+        // flight stays exact and a wait covers only the rows it needs.
+        float enter[2][12], leave[2][12];
+        auto fetch = [&](int c, int buf) {
+            load4(g.src, g.src_stride, min(c + R, g.LH - 1), x0, g.LW, enter[buf]);
+            load4(g.src, g.src_stride, max(c - R - 1, 0), x0, g.LW, leave[buf]);
+        };
+        const double sh = ch == 0 ? shift[0] : (ch == 1 ? shift[1] : shift[2]);
+        double P0[4], P1[4], Pn[4];            // channel waves: last step's sums and 1/n
+        auto step = [&](int s, int buf) {
+            const int c = c_lo + s;
+            if (role > 0 && s >= 1 && c - 1 < c_hi) {
+                // a, b of centre row c - 1 and their sums along x -> ring
+                const double* md = &my_d[(s - 1) & 1][0][0][0];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + u;
+                    const double my = md[u * 64 + lane], inv_d = md[256 + u * 64 + lane];
+                    const double mp = P0[u] * Pn[u];
+                    const double cov = P1[u] * Pn[u] - my * mp;
+                    const double a = cov * inv_d;
+                    const double b = (mp - a * my) + (sh - a * shift_y);
+                    if (need[u] && (!__builtin_isfinite(a) || !__builtin_isfinite(b))) err |= FLAG_OVERFLOW;
+                    b0[run_index(j + R)] = centre[u] ? a : 0.0;
+                    b1[run_index(j + R)] = centre[u] ? b : 0.0;
+                }
+                double Ha[4], Hb[4];
+                run_sums<R>(b0, j0, Ha);
+                run_sums<R>(b1, j0, Hb);
+                float* slot = ring + ((c - 1) % D) * 6 * WR;
+                if (j0 >= 2 * R && j0 + 3 < kFusedCols - 2 * R) {
+                    *reinterpret_cast<float4*>(slot + ch * WR + j0 - RB) =
+                        make_float4((float)Ha[0], (float)Ha[1], (float)Ha[2], (float)Ha[3]);
+                    *reinterpret_cast<float4*>(slot + (3 + ch) * WR + j0 - RB) =
+                        make_float4((float)Hb[0], (float)Hb[1], (float)Hb[2], (float)Hb[3]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int jo = j0 + u;
+                        if (jo >= 2 * R && jo < kFusedCols - 2 * R) {
+                            slot[ch * WR + jo - RB] = (float)Ha[u];
+                            slot[(3 + ch) * WR + jo - RB] = (float)Hb[u];
+                        }
+                    }
+                }
+            }
+            if (c < c_hi) {
+                // moments of centre row c
+                if (s > 0 && c + R < g.LH) accumulate(enter[buf], 1.0, role == 0);
+                if (s > 0 && c - R - 1 >= 0) accumulate(leave[buf], -1.0, false);
+            }
+            fetch(min(c + 2, c_hi - 1), buf);
+            if (c < c_hi) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    b0[run_index(j0 + u + R)] = V0[u];
+                    b1[run_index(j0 + u + R)] = V1[u];
+                }
+                double S0[4], S1[4];
+                run_sums<R>(b0, j0, S0);
+                run_sums<R>(b1, j0, S1);
+                // 1 / n of each centre: one division per row, another only
+                // where the window is clipped at the left or right edge.
+                const int ny = clipped_count(c, R, g.LH);
+                const double inv_row = 1.0 / (double)((2 * R + 1) * ny);
+                double inv_n[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    inv_n[u] = inv_row;
+                    if (nxc[u] != (double)(2 * R + 1)) inv_n[u] = 1.0 / (nxc[u] * (double)ny);
+                }
+                if (role == 0) {
+                    double* md = &my_d[s & 1][0][0][0];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double my = S0[u] * inv_n[u];
+                        double var = S1[u] * inv_n[u] - my * my;
+                        if (var < 0) var = 0;
+                        md[u * 64 + lane] = my;
+                        md[256 + u * 64 + lane] = 1.0 / (var + g.eps);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) { P0[u] = S0[u]; P1[u] = S1[u]; Pn[u] = inv_n[u]; }
+                }
+            }
+            __syncthreads();
+        };
+        fetch(c_lo + 1, 1);
+        for (int s = 0; s < steps; s += 2) {
+            step(s, 0);
+            if (s + 1 < steps) step(s + 1, 1);
+        }
+    } else {
+        // ---- output waves
+        const int jo = t - 256;
+        const int x = px0 + jo;
+        const bool col = jo >= 2 * R && jo < kFusedCols - 2 * R && x < g.vx + g.vw;
+        const int ri = min(max(jo - RB, 0), WR - 1);
+        const int nx = clipped_count(x, R, g.LW);
+        double U[6] = {0, 0, 0, 0, 0, 0};
+        float pix[3] = {0, 0, 0};
+        // Unconditional load at a clamped in-bounds pixel (see above).
+        const int xc = min(max(x, 0), g.LW - 1);
+        auto fetch = [&](int k) {
+            const float* q = g.src + ((size_t)min(max(k, y0), y1 - 1) * g.src_stride + (size_t)xc) * 3;
+            pix[0] = q[0]; pix[1] = q[1]; pix[2] = q[2];
+        };
+        fetch(c_lo - 2 - R);
+        for (int s = 0; s < steps; ++s) {
+            const int cb = c_lo + s - 2;       // centre row written into the ring last step
+            const int k = cb - R;              // output row
+            if (s >= 2) {
+                if (cb < c_hi) {
+                    const float* a = ring + (cb % D) * 6 * WR + ri;
+#pragma unroll
+                    for (int p = 0; p < 6; ++p) U[p] += (double)a[p * WR];
+                }
+                if (cb - 2 * R - 1 >= c_lo) {
+                    const float* a = ring + ((cb - 2 * R - 1) % D) * 6 * WR + ri;
+#pragma unroll
+                    for (int p = 0; p < 6; ++p) U[p] -= (double)a[p * WR];
+                }
+            }
+            const float p0 = pix[0], p1 = pix[1], p2 = pix[2];
+            fetch(k + 1);
+            if (s >= 2 && k >= y0 && k < y1) {
+                // 1 / n: one division per row, another only at the left or right edge.
+                const int ny = clipped_count(k, R, g.LH);
+                double inv_n = 1.0 / (double)((2 * R + 1) * ny);
+                if (nx != 2 * R + 1) inv_n = 1.0 / (double)(nx * ny);
+                if (col) {
+                    const float p[3] = {p0, p1, p2};
+                    const double y = working_y(g, (double)p0, (double)p1, (double)p2);
+                    float o[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double q = (U[c] * inv_n) * y + U[3 + c] * inv_n;
+                        const double delta = q - (double)p[c];
+                        if (!__builtin_isfinite(delta)) err |= FLAG_OVERFLOW;
+                        if (delta == 0) { o[c] = p[c]; continue; }   // exact copy, as the reference
+                        const double value = (double)p[c] + delta;
+                        if (value < -3.4028234663852886e38 || value > 3.4028234663852886e38) err |= FLAG_OVERFLOW;
+                        o[c] = (float)value;
+                    }
+                    apply_ops(*g.post, 0, g.fused, o[0], o[1], o[2], err);
+                    float* d = g.out + ((size_t)(k - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
+                    d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (err && g.flags) atomicOr(g.flags, err);
+}
+
+// Static LDS of gf_fused at its largest radius.
+static_assert(sizeof(float) * FusedRing<kFusedMaxR>::floats + sizeof(double) * (4 * 2 * kFusedPlane + 2 * 2 * 4 * 64) <=
+                  160 * 1024,
+              "gf_fused LDS");
 
 // Radius 0: the filter is the identity (SpatialOps.cpp:180); only the
 // post-filter program runs.
@@ -591,6 +908,34 @@ hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t 
         const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
         hipLaunchKernelGGL(gf_identity, dim3(blocks, 1, nz), dim3(256), 0, stream, g);
         return hipGetLastError();
+    }
+    if (g.mip == 0 && g.r <= kFusedMaxR) {
+        // One pass, coefficients on chip.  Bands: about one workgroup per CU
+        // over the image (at least kFusedMinBand rows, which amortise the
+        // 2r + 1 steps before a band's first output); the band height
+        // depends on the image only, so a stack filters each image as a
+        // single call does, bit for bit.
+        const int Wo = kFusedCols - 4 * g.r;
+        const int strips = (g.vw + Wo - 1) / Wo;
+        const int per = std::max(1, 256 / strips);
+        g.band = std::max(kFusedMinBand, (g.vh + per - 1) / per);
+        g.fused = L.fused_ops;
+        dim3 grid((unsigned)strips, (unsigned)((g.vh + g.band - 1) / g.band), nz);
+        switch (g.r) {
+#define RAWOPS_GF_FUSED(R) case R: hipLaunchKernelGGL(gf_fused<R>, grid, dim3(kFusedThreads), 0, stream, g); break;
+            RAWOPS_GF_FUSED(1) RAWOPS_GF_FUSED(2) RAWOPS_GF_FUSED(3) RAWOPS_GF_FUSED(4) RAWOPS_GF_FUSED(5)
+            RAWOPS_GF_FUSED(6) RAWOPS_GF_FUSED(7) RAWOPS_GF_FUSED(8) RAWOPS_GF_FUSED(9) RAWOPS_GF_FUSED(10)
+            RAWOPS_GF_FUSED(11)
+#undef RAWOPS_GF_FUSED
+        }
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && L.fused_ops < L.post_ops) {
+            const size_t n = (size_t)g.vw * g.vh;
+            const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
+            hipLaunchKernelGGL(gf_ops_tail, dim3(blocks, 1, nz), dim3(256), 0, stream, g);
+            e = hipGetLastError();
+        }
+        return e;
     }
     g.coef = static_cast<float*>(L.workspace);
     g.zcoef = (size_t)6 * g.cw * g.ch;

@@ -16,9 +16,10 @@ Differencing only through the filter keeps the steps off the curve's knots
 (the curve is C1: its second derivative jumps there, so differences taken
 through it converge only as O(h)).  The chain stores the filter output in
 float32, as the reference's tiles do, so the device takes the curve's
-derivative at values within 2^-24 of the FP64 chain's.  Each derivative is
-therefore compared with the chain rule at the device's stored float32
-values (the derivative of the computation the device performs) and, where
+derivative at values a few float32 ulps from the FP64 chain's.  Each
+derivative is therefore compared with the chain rule at the curve input the
+device stored (read back: the derivative of the computation the device
+performs) and, where
 the sum's conditioning (sum of |terms| / |sum|) is below 300, also with the
 chain rule at the FP64 chain's values: above that, the float32 storage alone
 moves a sum of terms of both signs by more than 1e-4 (measured: 1.5e-4 for
@@ -67,7 +68,8 @@ def test_c3_chain_gradients_vs_fd(radius, eps):
     e = torch.tensor(eps, device="cuda", requires_grad=True)
     kyt = torch.tensor([k[1] for k in knots], dtype=torch.float32, device="cuda", requires_grad=True)
     q = A.guided_filter(x, e, radius, SPACE)
-    y = A.tone_curve(A.exposure(q, torch.tensor(STOPS, device="cuda")), kyt, kx, True)
+    xe = A.exposure(q, torch.tensor(STOPS, device="cuda"))
+    y = A.tone_curve(xe, kyt, kx, True)
     (y * torch.from_numpy(wgt.astype(np.float32)).cuda()).sum().backward()
     gx = x.grad.double().cpu().numpy()
     ge = float(e.grad)
@@ -86,7 +88,7 @@ def test_c3_chain_gradients_vs_fd(radius, eps):
 
     q64 = gf(p64)
     x64 = q64 * m
-    x32 = (q64.astype(np.float32) * np.float32(m)).astype(np.float64)    # the stored forward values
+    x32 = xe.detach().double().cpu().numpy()      # the curve input the forward stored
     up64 = wgt * dcurve(x64) * m          # dL/dq of the FP64 chain
     up32 = wgt * dcurve(x32) * m          # dL/dq at the stored values
 
