@@ -328,7 +328,8 @@ def leg_c4_batch(replays, seed=4):
     tw, th, tuned = batch.shape()
     ms = total / replays
     return {"workload": "64 requests, 8 sources 2000x1500..6048x4024 RGGB/GRBG, viewports 256^2..full, mip 0/1/2, "
-                        "bilinear+Menon, FP32 linear + uint8 sRGB preview, one hipGraph launch",
+                        "bilinear+Menon, FP32 linear + uint8 sRGB preview, one hipGraph launch (one workgroup per "
+                        "(request, tile) item, tile shape autotuned over 64x32 / 64x64 / 128x32)",
             "batch_ms": round(ms, 3), "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
             "aggregate_output_mpix_s": round(level_px / ms / 1e3, 1),

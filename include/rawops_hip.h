@@ -180,7 +180,7 @@ rawops_status rawops_render_tiles_host(const rawops_plan* plan, const rawops_sou
                                        void* user, const volatile int* cancel);
 
 /* ---- batched heterogeneous render (the TileScheduler caller's job,
- * TileScheduler.cpp:19-246, as one persistent device launch) ---- */
+ * TileScheduler.cpp:19-246, as one device launch) ---- */
 typedef struct {
     const rawops_plan* plan;
     const rawops_source* source;       /* RAW source */
@@ -195,12 +195,12 @@ typedef struct {
 typedef struct rawops_batch rawops_batch;
 
 /* Validates every request (as rawops_render does), builds the (request, tile)
- * work list, picks the launch configuration (shape_index <
- * rawops_batch_shape_count(): tile shape x resident workgroups per CU, or
- * UINT32_MAX to autotune: every configuration is timed on this batch and the
- * fastest kept) and captures reset + persistent kernel into a hipGraph
- * (captured on a private stream, replayed on the caller's).  Requests are
- * served smallest first. */
+ * work list, picks the tile shape (shape_index < rawops_batch_shape_count():
+ * 64x32, 64x64, 128x32, one workgroup per (request, tile) item; or
+ * UINT32_MAX to autotune: every shape is timed on this batch and the fastest
+ * kept) and captures reset + tile launch into a hipGraph (captured on a
+ * private stream, replayed on the caller's).  Requests are served smallest
+ * first. */
 rawops_status rawops_batch_create(const rawops_batch_request* requests, size_t count, uint32_t shape_index,
                                   void* stream, rawops_batch** out);
 /* Replays the captured graph (asynchronous on `stream`). */

@@ -1,5 +1,6 @@
-"""Batched heterogeneous render: many RAW render requests in one persistent,
-hipGraph-captured launch (see include/rawops_hip.h, rawops_batch_*).
+"""Batched heterogeneous render: many RAW render requests in one
+hipGraph-captured launch, one workgroup per (request, tile) item, the tile
+shape autotuned on the batch (see include/rawops_hip.h, rawops_batch_*).
 
 Each request is a node chain (compiled like Renderer's), a viewport at a
 level, and device output tensors: an FP32 RGB tensor taken after the first

@@ -19,7 +19,7 @@
 // atomics touch the raw gradient.
 #include "device_common.hpp"
 #include "menon_fwd.hpp"
-#include "menon_tile.hpp"
+#include "raw_tile.hpp"
 #include "render_launch.hpp"
 
 #include <algorithm>

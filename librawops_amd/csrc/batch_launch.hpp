@@ -21,16 +21,16 @@ struct BatchItem {
     uint32_t* flags;
 };
 
-// per_cu: resident workgroups per CU the persistent grid is sized for
-// (0 = as many as fit); -1: no persistent loop, one 512-thread workgroup per
+// A tile shape of the batched launch: one 512-thread workgroup per
 // (request, tile) item running the one-request render kernel's tile.
-struct BatchShape { int tw, th, per_cu; };
+struct BatchShape { int tw, th; };
 constexpr int kBatchShapeCount = 3;
 extern const BatchShape kBatchShapes[kBatchShapeCount];
 
 size_t batch_state_bytes();
-int batch_grid(int shape);
+// Launch start time (s_memrealtime ticks) held by a host copy of the state.
+unsigned long long batch_start_time(const void* state_host);
 hipError_t launch_batch(int shape, const BatchItem* items, int n, int total, void* state, uint32_t* remaining,
-                        unsigned long long* done, int grid, hipStream_t stream);
+                        unsigned long long* done, hipStream_t stream);
 
 }  // namespace rawops

@@ -54,7 +54,7 @@ struct rawops_batch {
     uint32_t* flags = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int shape = 0, total = 0, grid = 0;
+    int shape = 0, total = 0;
     double tuned_ms = 0.0;
     ~rawops_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -561,7 +561,6 @@ static void batch_layout(rawops_batch& b, int shape) {
     }
     b.total = first;
     b.shape = shape;
-    b.grid = batch_grid(shape);
     hip_check(hipMemcpy(b.dev_items, b.items.data(), b.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice),
               "hipMemcpy(batch)");
 }
@@ -622,11 +621,11 @@ rawops_status rawops_batch_create(const rawops_batch_request* reqs, size_t count
             double best = 1e30;
             for (int c = 0; c < kBatchShapeCount; ++c) {
                 batch_layout(*b, c);
-                hip_check(launch_batch(c, b->dev_items, int(n), b->total, b->state, b->remaining, b->done, b->grid, st),
+                hip_check(launch_batch(c, b->dev_items, int(n), b->total, b->state, b->remaining, b->done, st),
                           "batch launch");
                 hip_check(hipEventRecord(e0, st), "hipEventRecord");
                 for (int rep = 0; rep < 3; ++rep)
-                    hip_check(launch_batch(c, b->dev_items, int(n), b->total, b->state, b->remaining, b->done, b->grid, st),
+                    hip_check(launch_batch(c, b->dev_items, int(n), b->total, b->state, b->remaining, b->done, st),
                               "batch launch");
                 hip_check(hipEventRecord(e1, st), "hipEventRecord");
                 hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -647,7 +646,7 @@ rawops_status rawops_batch_create(const rawops_batch_request* reqs, size_t count
         struct Destroy { hipStream_t s; ~Destroy() { (void)hipStreamDestroy(s); } } destroy{cap};
         hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         const hipError_t le = launch_batch(shape, b->dev_items, int(n), b->total, b->state, b->remaining, b->done,
-                                           b->grid, cap);
+                                           cap);
         const hipError_t ce = hipStreamEndCapture(cap, &b->graph);
         hip_check(le, "batch launch (capture)");
         hip_check(ce, "hipStreamEndCapture");
@@ -668,13 +667,14 @@ rawops_status rawops_batch_latencies(rawops_batch* b, double* ms) {
         if (!b || !ms) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
         const size_t n = b->items.size();
         std::vector<unsigned long long> done(n);
-        struct { uint32_t next, pad; unsigned long long t0; } state{};
+        std::vector<unsigned char> state(batch_state_bytes());
         hip_check(hipMemcpy(done.data(), b->done, n * sizeof(unsigned long long), hipMemcpyDeviceToHost), "hipMemcpy");
-        hip_check(hipMemcpy(&state, b->state, sizeof(state), hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(state.data(), b->state, state.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+        const unsigned long long t0 = batch_start_time(state.data());
         int dev = 0, khz = 100000;
         (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
-        for (size_t k = 0; k < n; ++k) ms[b->order[k]] = double(done[k] - state.t0) / double(khz);
+        for (size_t k = 0; k < n; ++k) ms[b->order[k]] = double(done[k] - t0) / double(khz);
     });
 }
 

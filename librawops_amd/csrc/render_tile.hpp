@@ -1,15 +1,14 @@
 // The fused render tile (demosaic + op program + mip reduction + stores),
-// shared by the one-request render kernel and the persistent batch kernel.
+// shared by the one-request render kernel and the batched launch.
 #pragma once
 
 #include "device_common.hpp"
 #include "menon_fwd.hpp"
-#include "menon_tile.hpp"
+#include "raw_tile.hpp"
 
 namespace rawops {
 
-constexpr int kThreads = 256;        // batch kernel workgroup
-constexpr int kRenderThreads = 512;  // render kernel workgroup
+constexpr int kRenderThreads = 512;  // render and batch workgroups
 
 struct RenderArgs {
     SensorArgs s;
@@ -101,10 +100,10 @@ __device__ __forceinline__ void render_tile(const RenderArgs& a, const SensorArg
         __syncthreads();
         t.classify();
         __syncthreads();
-        if (a.decisions && a.mip == 0) {
-            // Record the tile's decisions for the adjoint: a tile row's 32
-            // non-green sites are one word (half a wave's ballot).
-            static_assert(TW == 64 && TH == 32 && ((TW / 2) * TH) % NT == 0, "one decision word per tile row");
+        // Record the tile's decisions for the adjoint (64x32 tiles only: a
+        // tile row's 32 non-green sites are one word, half a wave's ballot).
+        if constexpr (TW == 64 && TH == 32) if (a.decisions && a.mip == 0) {
+            static_assert(((TW / 2) * TH) % NT == 0, "one decision word per tile row");
 #pragma unroll
             for (int j = 0; j < (TW / 2) * TH / NT; ++j) {
                 const int i = (int)threadIdx.x + j * NT;

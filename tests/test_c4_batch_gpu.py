@@ -33,21 +33,44 @@ def build(seed=4):
                            output_mode=L.OutputMode.SrgbPreview)
     graphs = [{d: L.ImageGraph(img, recipe, demosaic=L.MENON if d == "menon" else L.BILINEAR)
                for d in ("bilinear", "menon")} for img in imgs]
-    batch_reqs = []
+    return sources, reqs, cam, graphs
+
+
+def batch_requests(reqs, graphs):
+    out = []
     for r in reqs:
         x, y, w, h = r["viewport"]
         level = L.RenderLevel(r["mip"], L.RenderQuality.Final if r["mip"] == 0 else L.RenderQuality.Preview)
-        batch_reqs.append(BatchRequest(graphs[r["source"]][r["demosaic"]].output(), L.Rect(x, y, w, h), level, 0,
-                                       torch.full((h, w, 3), float("nan"), device="cuda"),
-                                       torch.zeros((h, w, 3), dtype=torch.uint8, device="cuda")))
-    return sources, reqs, cam, batch_reqs
+        out.append(BatchRequest(graphs[r["source"]][r["demosaic"]].output(), L.Rect(x, y, w, h), level, 0,
+                                torch.full((h, w, 3), float("nan"), device="cuda"),
+                                torch.zeros((h, w, 3), dtype=torch.uint8, device="cuda")))
+    return out
 
 
-def test_c4_batch_matches_oracle():
-    sources, reqs, cam, batch_reqs = build()
-    batch = Batch(batch_reqs)
+@pytest.fixture(scope="module")
+def workload():
+    """The workload and the oracle's render of every request (once)."""
+    sources, reqs, cam, graphs = build()
+    threads = min(16, os.cpu_count() or 1)
+    want = []
+    for r in reqs:
+        meta, samples = sources[r["source"]]
+        om = oracle_meta(meta)
+        kw = dict(demosaic=r["demosaic"], wb=synth.C4_GAINS, exposure_stops=synth.C4_STOPS, camera_to_xyz=cam)
+        lin = O.render_raw(om, samples, O.Chain(output="linear", **kw), r["viewport"], r["mip"], threads=threads)
+        enc = O.render_raw(om, samples, O.Chain(output="srgb", **kw), r["viewport"], r["mip"], threads=threads)
+        want.append((lin, enc))
+    return reqs, graphs, want
+
+
+@pytest.mark.parametrize("shape", [None, 0, 1, 2])
+def test_c4_batch_matches_oracle(workload, shape):
+    """shape None: the autotuned tile; 0, 1, 2: 64x32, 64x64, 128x32."""
+    reqs, graphs, want = workload
+    batch_reqs = batch_requests(reqs, graphs)
+    batch = Batch(batch_reqs, shape=shape)
     tw, th, _ = batch.shape()
-    print(f"c4: autotuned tile {tw}x{th}", flush=True)
+    print(f"c4: tile {tw}x{th} ({'autotuned' if shape is None else 'fixed'})", flush=True)
     for replay in range(2):
         for r in batch_reqs:
             r.out_rgb.fill_(float("nan"))
@@ -55,21 +78,15 @@ def test_c4_batch_matches_oracle():
         batch.run()
         torch.cuda.synchronize()
         batch.check()
-    threads = min(16, os.cpu_count() or 1)
-    for k, (r, br) in enumerate(zip(reqs, batch_reqs)):
-        meta, samples = sources[r["source"]]
-        om = oracle_meta(meta)
-        kw = dict(demosaic=r["demosaic"], wb=synth.C4_GAINS, exposure_stops=synth.C4_STOPS, camera_to_xyz=cam)
-        lin = O.render_raw(om, samples, O.Chain(output="linear", **kw), r["viewport"], r["mip"], threads=threads)
+    for k, (r, br, (lin, enc)) in enumerate(zip(reqs, batch_reqs, want)):
         got = br.out_rgb.cpu().numpy()
-        assert got.tobytes() == lin.tobytes(), (k, r)
-        enc = O.render_raw(om, samples, O.Chain(output="srgb", **kw), r["viewport"], r["mip"], threads=threads)
+        assert got.tobytes() == lin.tobytes(), (k, r, tw, th)
         want8 = O.srgb_u8(enc)
         got8 = br.out_u8.cpu().numpy()
         diff = np.abs(got8.astype(np.int32) - want8.astype(np.int32))
-        assert diff.max() <= 1, (k, r)
+        assert diff.max() <= 1, (k, r, tw, th)
         frac = np.clip(enc, 0, 1).astype(np.float64) * 255.0 + 0.5
         near = np.abs(frac - np.rint(frac)) < 255 * 1e-5
-        assert not np.any(diff[~near]), (k, r)
-        print(f"c4: request {k} {r['demosaic']} mip {r['mip']} {r['viewport']} ok", flush=True)
-        sys.stdout.flush()
+        assert not np.any(diff[~near]), (k, r, tw, th)
+    print(f"c4: {len(reqs)} requests ok at {tw}x{th}", flush=True)
+    sys.stdout.flush()
