@@ -45,5 +45,8 @@ RgbHistogram histogram_rgb(const Node& output, Rect source_bounds, const RenderR
                            const HistogramOptions& options = {}, const CancellationToken* cancellation = nullptr);
 RgbHistogram histogram_rgb(const ImageGraph& graph, const RenderRequest& request,
                            const HistogramOptions& options = {}, const CancellationToken* cancellation = nullptr);
+// ImageAnalysis.hpp:38-40: a manifest graph's output over its output bounds.
+RgbHistogram histogram_rgb(const ExecutableEditGraph& graph, const RenderRequest& request,
+                           const HistogramOptions& options = {}, const CancellationToken* cancellation = nullptr);
 
 }  // namespace rawengine

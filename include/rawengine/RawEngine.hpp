@@ -13,6 +13,11 @@
 //   * ImageGraph(RawImage, GraphRecipe, RawDemosaicIdentity, bool linear_output)
 //     selects Menon and/or ends the chain after camera calibration;
 //   * Renderer::render_device(...) renders into caller-owned device memory.
+//
+// The ICC hook (IccDisplayTransform / IccDisplayNode, RawEngine.hpp:595-619)
+// keeps the reference's host callback contract: the transform runs on the
+// host over the device-rendered tile, because a colour-management engine
+// (LittleCMS in the reference) is host code.
 #pragma once
 
 #include <array>
@@ -68,6 +73,14 @@ namespace detail {
 struct DeviceSource;
 }
 
+class RawImage;
+class RasterImage;
+// Version-1 canonical source digests (SourceFingerprint.cpp:61-104): SHA-256
+// over the rendering-relevant metadata and the visible samples in row-major
+// order, little-endian; row padding does not enter.
+std::array<std::uint8_t, 32> fingerprint_raw_source(const RawImage& image);
+std::array<std::uint8_t, 32> fingerprint_raster_source(const RasterImage& image);
+
 // Decoded uint16 Bayer samples.  Copies share immutable storage; the samples
 // are uploaded to HBM once, on first render, and stay resident.
 class RawImage {
@@ -80,10 +93,13 @@ public:
     const std::vector<std::uint16_t>& samples() const noexcept { return *bayer_; }
     std::uint32_t width() const noexcept { return metadata_.width; }
     std::uint32_t height() const noexcept { return metadata_.height; }
+    std::array<std::uint8_t, 32> fingerprint() const;  // computed once, shared by copies
     const detail::DeviceSource& device() const;  // uploads on first use
 private:
     RawMetadata metadata_;
     std::shared_ptr<const std::vector<std::uint16_t>> bayer_;
+    std::shared_ptr<std::once_flag> fingerprint_once_ = std::make_shared<std::once_flag>();
+    std::shared_ptr<std::array<std::uint8_t, 32>> fingerprint_ = std::make_shared<std::array<std::uint8_t, 32>>();
     std::shared_ptr<std::once_flag> upload_once_ = std::make_shared<std::once_flag>();
     std::shared_ptr<std::shared_ptr<detail::DeviceSource>> device_ =
         std::make_shared<std::shared_ptr<detail::DeviceSource>>();
@@ -102,10 +118,13 @@ public:
     const std::vector<float>& pixels() const noexcept { return *pixels_; }
     std::uint32_t width() const noexcept { return metadata_.width; }
     std::uint32_t height() const noexcept { return metadata_.height; }
+    std::array<std::uint8_t, 32> fingerprint() const;
     const detail::DeviceSource& device() const;
 private:
     RasterMetadata metadata_;
     std::shared_ptr<const std::vector<float>> pixels_;
+    std::shared_ptr<std::once_flag> fingerprint_once_ = std::make_shared<std::once_flag>();
+    std::shared_ptr<std::array<std::uint8_t, 32>> fingerprint_ = std::make_shared<std::array<std::uint8_t, 32>>();
     std::shared_ptr<std::once_flag> upload_once_ = std::make_shared<std::once_flag>();
     std::shared_ptr<std::shared_ptr<detail::DeviceSource>> device_ =
         std::make_shared<std::shared_ptr<detail::DeviceSource>>();
@@ -173,6 +192,11 @@ struct ImageDescriptor {
         return {PixelFormat::RGBFloat32, PixelDomain::DisplayLinearRGB, ColorPrimaries::SRGB, WhitePoint::D65,
                 TransferFunction::Linear, ReferenceState::DisplayReferred, AlphaMode::None};
     }
+    static constexpr ImageDescriptor icc_display(std::array<std::uint8_t, 32> digest) noexcept {
+        return {PixelFormat::RGBFloat32, PixelDomain::DisplayEncodedRGB, ColorPrimaries::ICCProfile,
+                WhitePoint::Unspecified, TransferFunction::ICCProfile, ReferenceState::DisplayReferred,
+                AlphaMode::None, digest};
+    }
     bool operator==(const ImageDescriptor&) const = default;
 };
 
@@ -180,6 +204,16 @@ struct Tile {
     Rect bounds;
     std::vector<float> rgb;  // interleaved float32 RGB, row-major
     ImageDescriptor descriptor = ImageDescriptor::camera_linear();
+};
+
+// RawEngine.hpp:212-219: exact output/input profile and conversion policy.
+struct IccProfileIdentity {
+    std::array<std::uint8_t, 32> profile_sha256{};
+    std::string intent = "relative_colorimetric";
+    bool black_point_compensation = false;
+    std::string engine = "lcms2-core";
+    std::string engine_version = "2.19.1";
+    bool operator==(const IccProfileIdentity&) const = default;
 };
 
 struct RawDemosaicIdentity {
@@ -211,6 +245,8 @@ public:
         if (level.mip == 0 && level.quality == RenderQuality::Final) return input_region(output, source_bounds);
         throw std::invalid_argument("node has no mapping for this render level");
     }
+    virtual std::optional<IccProfileIdentity> input_icc_identity() const { return std::nullopt; }
+    virtual std::optional<std::array<std::uint8_t, 32>> source_fingerprint() const { return std::nullopt; }
     virtual std::optional<Rect> source_bounds() const { return std::nullopt; }
     virtual std::optional<RawDemosaicIdentity> raw_demosaic_identity() const { return std::nullopt; }
 };
@@ -230,6 +266,9 @@ public:
     Rect input_region(Rect output, Rect source_bounds) const override;
     ImageDescriptor output_descriptor() const noexcept override { return ImageDescriptor::camera_linear(); }
     std::optional<Rect> source_bounds() const override { return image_.metadata().active_area; }
+    std::optional<std::array<std::uint8_t, 32>> source_fingerprint() const override {
+        return fingerprint_raw_source(image_);
+    }
     std::optional<RawDemosaicIdentity> raw_demosaic_identity() const override { return demosaic_; }
     const RawImage& image() const noexcept { return image_; }
 private:
@@ -252,6 +291,9 @@ public:
         return ImageDescriptor::scene_linear(image_.metadata().working_space);
     }
     std::optional<Rect> source_bounds() const override { return Rect{0, 0, image_.width(), image_.height()}; }
+    std::optional<std::array<std::uint8_t, 32>> source_fingerprint() const override {
+        return fingerprint_raster_source(image_);
+    }
     const RasterImage& image() const noexcept { return image_; }
 private:
     RasterImage image_;
@@ -381,24 +423,56 @@ private:
     std::shared_ptr<const Node> input_;
 };
 
+// RawEngine.hpp:595-606: output ICC transform from display-linear sRGB to
+// bounded, profile-encoded float32 RGB (thread-safe for concurrent calls).
+class IccDisplayTransform {
+public:
+    virtual ~IccDisplayTransform() = default;
+    virtual std::array<std::uint8_t, 32> profile_sha256() const noexcept = 0;
+    virtual std::optional<IccProfileIdentity> output_icc_identity() const { return std::nullopt; }
+    virtual void apply(float* interleaved_rgb, std::size_t pixels) const = 0;
+};
+
+// RawEngine.hpp:608-619, RawEngine.cpp:1279-1304.  Renders its input on the
+// device, then runs the transform over the host tile; non-finite input raises
+// domain_error, as does output outside [0, 1].
+class IccDisplayNode final : public Node {
+public:
+    IccDisplayNode(std::shared_ptr<const Node> input, std::shared_ptr<const IccDisplayTransform> transform);
+    Tile render(Rect bounds) const override;
+    const Node* input_node() const noexcept override { return input_.get(); }
+    ImageDescriptor output_descriptor() const noexcept override { return descriptor_; }
+private:
+    std::shared_ptr<const Node> input_;
+    std::shared_ptr<const IccDisplayTransform> transform_;
+    ImageDescriptor descriptor_;
+};
+
 class ImageGraph final {
 public:
-    ImageGraph(RawImage image, GraphRecipe recipe = {});
+    ImageGraph(RawImage image, GraphRecipe recipe = {},
+               std::shared_ptr<const IccDisplayTransform> display_transform = nullptr);
     // Extension: demosaic selection and linear (post-calibration) output.
-    ImageGraph(RawImage image, GraphRecipe recipe, RawDemosaicIdentity demosaic, bool linear_output = false);
-    ImageGraph(RasterImage image, GraphRecipe recipe = {});
-    ImageGraph(std::shared_ptr<const Node> scene_linear_source, Rect source_bounds, GraphRecipe recipe = {});
+    ImageGraph(RawImage image, GraphRecipe recipe, RawDemosaicIdentity demosaic, bool linear_output = false,
+               std::shared_ptr<const IccDisplayTransform> display_transform = nullptr);
+    ImageGraph(RasterImage image, GraphRecipe recipe = {},
+               std::shared_ptr<const IccDisplayTransform> display_transform = nullptr);
+    ImageGraph(std::shared_ptr<const Node> scene_linear_source, Rect source_bounds, GraphRecipe recipe = {},
+               std::shared_ptr<const IccDisplayTransform> display_transform = nullptr);
     const RawImage& image() const;
     Rect source_bounds() const noexcept { return source_bounds_; }
     const GraphRecipe& recipe() const noexcept { return recipe_; }
     const Node& output() const noexcept { return *output_; }
 private:
-    void finish(std::shared_ptr<const Node> linear, bool linear_output);
+    void finish(std::shared_ptr<const Node> linear, bool linear_output,
+                std::shared_ptr<const IccDisplayTransform> display_transform);
     std::optional<RawImage> raw_image_;
     Rect source_bounds_;
     GraphRecipe recipe_;
     std::shared_ptr<const Node> output_;
 };
+
+class ExecutableEditGraph;
 
 class RenderCancelled final : public std::runtime_error {
 public:
@@ -418,21 +492,31 @@ public:
     using TileCallback = std::function<void(const Tile&)>;
     void render_tiles(const ImageGraph& graph, RenderRequest request, const TileCallback& callback,
                       const CancellationToken* cancellation = nullptr) const;
+    void render_tiles(const ExecutableEditGraph& graph, RenderRequest request, const TileCallback& callback,
+                      const CancellationToken* cancellation = nullptr) const;
     void render_tiles(const Node& output, Rect source_bounds, RenderRequest request, const TileCallback& callback,
                       const CancellationToken* cancellation = nullptr) const;
     void render_tiles(const ImageGraph& graph, Rect viewport, const TileCallback& callback,
+                      std::uint32_t tile_size = 256, const CancellationToken* cancellation = nullptr) const;
+    void render_tiles(const ExecutableEditGraph& graph, Rect viewport, const TileCallback& callback,
                       std::uint32_t tile_size = 256, const CancellationToken* cancellation = nullptr) const;
     void render_tiles(const Node& output, Rect source_bounds, Rect viewport, const TileCallback& callback,
                       std::uint32_t tile_size = 256, const CancellationToken* cancellation = nullptr) const;
     Tile render_image(const ImageGraph& graph, RenderRequest request,
                       const CancellationToken* cancellation = nullptr) const;
+    Tile render_image(const ExecutableEditGraph& graph, RenderRequest request,
+                      const CancellationToken* cancellation = nullptr) const;
     Tile render_image(const Node& output, Rect source_bounds, RenderRequest request,
                       const CancellationToken* cancellation = nullptr) const;
     Tile render_image(const ImageGraph& graph, Rect viewport, std::uint32_t tile_size = 256,
                       const CancellationToken* cancellation = nullptr) const;
+    Tile render_image(const ExecutableEditGraph& graph, Rect viewport, std::uint32_t tile_size = 256,
+                      const CancellationToken* cancellation = nullptr) const;
     Tile render_image(const Node& output, Rect source_bounds, Rect viewport, std::uint32_t tile_size = 256,
                       const CancellationToken* cancellation = nullptr) const;
     std::vector<float> render_roi(const ImageGraph& graph, Rect viewport, std::uint32_t tile_size = 256,
+                                  const CancellationToken* cancellation = nullptr) const;
+    std::vector<float> render_roi(const ExecutableEditGraph& graph, Rect viewport, std::uint32_t tile_size = 256,
                                   const CancellationToken* cancellation = nullptr) const;
     // Extension: render into caller-owned device memory (float32 RGB, row
     // stride = viewport width) on `stream` (a hipStream_t), asynchronously.

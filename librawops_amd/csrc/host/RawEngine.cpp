@@ -12,13 +12,17 @@
 #include "../../../include/rawengine/SpatialOps.hpp"
 #include "../../../include/rawengine/ToneOps.hpp"
 #include "../../../include/rawops_hip.h"
+#include "internal.hpp"
+#include "sha256.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <bit>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 namespace rawengine {
 namespace detail {
@@ -42,12 +46,49 @@ struct DeviceSource {
     }
 }
 
-void check(rawops_status s) {
-    if (s != RAWOPS_OK) raise(s);
-}
-
 void hip(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+// Streams outlive the threads that used them: a finished thread's stream is
+// handed to the next thread that asks, so the process creates at most as many
+// streams as it ever ran rendering threads at once.
+std::mutex stream_pool_mutex;
+std::vector<hipStream_t> stream_pool;
+
+struct ThreadStream {
+    hipStream_t s = nullptr;
+    ~ThreadStream() {
+        if (!s) return;
+        std::lock_guard lock(stream_pool_mutex);
+        stream_pool.push_back(s);
+    }
+};
+thread_local ThreadStream this_thread_stream;
+}  // namespace
+
+hipStream_t thread_stream() {
+    if (!this_thread_stream.s) {
+        {
+            std::lock_guard lock(stream_pool_mutex);
+            if (!stream_pool.empty()) {
+                this_thread_stream.s = stream_pool.back();
+                stream_pool.pop_back();
+            }
+        }
+        if (!this_thread_stream.s) hip(hipStreamCreateWithFlags(&this_thread_stream.s, hipStreamNonBlocking),
+                                       "hipStreamCreateWithFlags");
+    }
+    return this_thread_stream.s;
+}
+
+DeviceTile::DeviceTile(std::uint32_t w, std::uint32_t h, ImageDescriptor d) : width(w), height(h), descriptor(d) {
+    if (floats()) hip(hipMalloc(&ptr, floats() * sizeof(float)), "hipMalloc(tile)");
+}
+
+DeviceTile::~DeviceTile() {
+    if (ptr) (void)hipFree(ptr);
 }
 
 struct Plan {
@@ -60,17 +101,35 @@ struct Op {
     std::vector<double> xs, ys;  // curve knots backing desc.knots_x/y
 };
 
-// The compiled form of a node chain.
+// One node of a compiled chain and, for graph nodes, its tile-cache identity.
+struct Link {
+    const Node* node = nullptr;  // the node itself (a cached graph node is unwrapped)
+    TileCache* cache = nullptr;
+    Signature signature{};
+    bool has_signature = false;
+};
+
+// One device launch of a chain: segment 0 reads the source (demosaic + op
+// program, or raster reduction + op program, or a foreign node's uploaded
+// tile); every further segment is a guided filter + op program over the
+// previous segment's device output.
+struct Segment {
+    const WorkingYGuidedFilterNode* guided = nullptr;
+    std::vector<Op> native, reduced;
+    bool anchor = false;
+    std::unique_ptr<Plan> plan;
+    Link last;                 // node whose output this launch produces
+    bool cache_boundary = false;
+};
+
 struct Chain {
     enum Kind { Raw, Raster, Foreign } kind = Raw;
     const RawImage* raw = nullptr;
     const RasterImage* raster = nullptr;
     const Node* foreign = nullptr;
     uint32_t demosaic = RAWOPS_DEMOSAIC_BILINEAR;
-    bool anchor = false;
-    const WorkingYGuidedFilterNode* guided = nullptr;
-    std::vector<Op> native, reduced;
-    std::unique_ptr<Plan> plan;
+    bool fuse_source = false;  // segment 0 is the bare source: segment 1 filters it directly
+    std::vector<Segment> segs;
 };
 
 rawops_rect c(Rect r) { return {r.x, r.y, r.width, r.height}; }
@@ -154,83 +213,206 @@ std::unique_ptr<Plan> make_plan(uint32_t demosaic, bool anchor, std::vector<Op>&
     return p;
 }
 
+Link link_of(const Node* n) {
+    Link l;
+    if (auto* cn = dynamic_cast<const CachedNode*>(n)) {
+        l.node = &cn->inner();
+        l.cache = cn->cache();
+        l.signature = cn->signature();
+        l.has_signature = true;
+    } else {
+        l.node = n;
+    }
+    return l;
+}
+
+bool is_source(const Node* n) {
+    return dynamic_cast<const RawUnpackNode*>(n) || dynamic_cast<const RasterSourceNode*>(n);
+}
+
 std::unique_ptr<Chain> compile(const Node& output) {
-    std::vector<const Node*> nodes;
-    for (const Node* n = &output; n; n = n->input_node()) nodes.push_back(n);
-    std::reverse(nodes.begin(), nodes.end());
+    std::vector<Link> links;
+    for (const Node* n = &output; n; n = n->input_node()) links.push_back(link_of(n));
+    std::reverse(links.begin(), links.end());
     auto ch = std::make_unique<Chain>();
     size_t first = 1;
-    if (auto* ru = dynamic_cast<const RawUnpackNode*>(nodes[0])) {
+    if (auto* ru = dynamic_cast<const RawUnpackNode*>(links[0].node)) {
         ch->kind = Chain::Raw;
         ch->raw = &ru->image();
         ch->demosaic = ru->raw_demosaic_identity()->algorithm == "rawengine.menon_base" ? RAWOPS_DEMOSAIC_MENON
                                                                                          : RAWOPS_DEMOSAIC_BILINEAR;
-    } else if (auto* rs = dynamic_cast<const RasterSourceNode*>(nodes[0])) {
+    } else if (auto* rs = dynamic_cast<const RasterSourceNode*>(links[0].node)) {
         ch->kind = Chain::Raster;
         ch->raster = &rs->image();
     } else {
         // Render the last foreign node with its own code; run the rest on the GPU.
         size_t last = 0;
-        for (size_t i = 0; i < nodes.size(); ++i) {
+        for (size_t i = 0; i < links.size(); ++i) {
             std::vector<Op> probe;
-            const bool known = point_ops(nodes[i], probe) || dynamic_cast<const DeviceNode*>(nodes[i]);
+            const bool known = point_ops(links[i].node, probe) || dynamic_cast<const DeviceNode*>(links[i].node);
             if (!known) last = i;
         }
         ch->kind = Chain::Foreign;
-        ch->foreign = nodes[last];
+        ch->foreign = links[last].node;
         first = last + 1;
     }
-    for (size_t i = first; i < nodes.size(); ++i) {
-        const Node* n = nodes[i];
-        if (auto* gf = dynamic_cast<const WorkingYGuidedFilterNode*>(n)) {
-            if (ch->kind != Chain::Raster || ch->guided || !ch->reduced.empty())
-                throw std::invalid_argument("guided filter is fused only directly over a raster source");
-            ch->guided = gf;
+    ch->segs.emplace_back();
+    ch->segs.back().last = links[first - 1];
+    for (size_t i = first; i < links.size(); ++i) {
+        const Link& l = links[i];
+        if (auto* gf = dynamic_cast<const WorkingYGuidedFilterNode*>(l.node)) {
+            ch->segs.emplace_back();
+            ch->segs.back().guided = gf;
+            ch->segs.back().last = l;
             continue;
         }
-        auto& list = ch->kind == Chain::Raw && !ch->anchor ? ch->native : ch->reduced;
-        if (!point_ops(n, list)) throw std::invalid_argument("node type cannot be compiled into a device plan");
-        if (auto* cw = dynamic_cast<const CameraToWorkingNode*>(n))
-            if (ch->kind == Chain::Raw && cw->native_input_bounds().width) ch->anchor = true;
+        Segment& s = ch->segs.back();
+        auto& list = ch->kind == Chain::Raw && ch->segs.size() == 1 && !s.anchor ? s.native : s.reduced;
+        if (!point_ops(l.node, list)) throw std::invalid_argument("node type cannot be compiled into a device plan");
+        if (auto* cw = dynamic_cast<const CameraToWorkingNode*>(l.node))
+            if (ch->kind == Chain::Raw && ch->segs.size() == 1 && cw->native_input_bounds().width) s.anchor = true;
+        s.last = l;
     }
-    ch->plan = make_plan(ch->demosaic, ch->anchor, ch->native, ch->reduced);
+    // A bare raster (or foreign tile) under a guided filter is filtered where
+    // it lies: no copy launch.
+    const Segment& s0 = ch->segs[0];
+    ch->fuse_source = ch->kind != Chain::Raw && ch->segs.size() > 1 && s0.native.empty() && s0.reduced.empty() &&
+                      !(ch->kind == Chain::Foreign && s0.last.has_signature && s0.last.cache);
+    for (size_t k = 0; k < ch->segs.size(); ++k) {
+        Segment& s = ch->segs[k];
+        // Source tiles are resident already: a cache entry would only copy them.
+        s.cache_boundary = s.last.has_signature && s.last.cache && !is_source(s.last.node) &&
+                           !(k == 0 && ch->fuse_source);
+        s.plan = make_plan(ch->demosaic, s.anchor, s.native, s.reduced);
+    }
     return ch;
-}
-
-// Device scratch for one render call.
-struct DeviceBuffer {
-    void* p = nullptr;
-    explicit DeviceBuffer(size_t bytes) { if (bytes) hip(hipMalloc(&p, bytes), "hipMalloc"); }
-    ~DeviceBuffer() { if (p) (void)hipFree(p); }
-};
-
-void run(const Chain& ch, Rect viewport, RenderLevel level, float* dst, uint32_t* flags, hipStream_t stream) {
-    const uint32_t quality = static_cast<uint32_t>(level.quality);
-    if (ch.kind == Chain::Raw) {
-        check(rawops_render(ch.plan->handle, ch.raw->device().handle, c(viewport), level.mip, quality, dst, nullptr,
-                            viewport.width, flags, stream));
-    } else if (ch.kind == Chain::Raster) {
-        const uint32_t r = ch.guided ? ch.guided->settings().radius : 0;
-        const double eps = ch.guided ? ch.guided->settings().epsilon : 1.0;
-        check(rawops_guided_filter(ch.raster->device().handle, c(viewport), level.mip, r, eps, ch.plan->handle, dst,
-                                   viewport.width, nullptr, 0, flags, stream));
-    } else {
-        // Foreign input: its own render, uploaded, then the device op program.
-        const Tile t = ch.foreign->render_level(viewport, ch.foreign->input_level(level));
-        if (t.rgb.size() != size_t(viewport.width) * viewport.height * 3)
-            throw std::domain_error("upstream node returned an invalid tile");
-        DeviceSource src;
-        check(rawops_source_create_raster(viewport.width, viewport.height, 0, 0, t.rgb.data(), &src.handle));
-        check(rawops_guided_filter(src.handle, {0, 0, viewport.width, viewport.height}, 0, 0, 1.0, ch.plan->handle,
-                                   dst, viewport.width, nullptr, 0, flags, stream));
-        hip(hipStreamSynchronize(stream), "stream synchronize");
-    }
 }
 
 Rect request_bounds(Rect source, RenderLevel level) {
     rawops_rect out{};
     check(rawops_request_bounds(c(source), level.mip, static_cast<uint32_t>(level.quality), &out));
     return {out.x, out.y, out.width, out.height};
+}
+
+WorkingSpace space_of(const ImageDescriptor& d) {
+    return d.primaries == ColorPrimaries::Rec2020 ? WorkingSpace::LinearRec2020D65 : WorkingSpace::LinearProPhotoD50;
+}
+
+// A device buffer (previous launch's output) viewed as a raster source.
+struct WrappedRaster {
+    rawops_source* handle = nullptr;
+    WrappedRaster(const float* p, uint32_t w, uint32_t h, WorkingSpace ws) {
+        check(rawops_source_wrap_raster_device(w, h, w, uint32_t(ws), p, &handle));
+    }
+    ~WrappedRaster() { rawops_source_destroy(handle); }
+};
+
+// Runs the chain over `viewport` (level coordinates) into dst (row stride =
+// viewport width).  Segment outputs that are cache boundaries are looked up
+// top-down; rendering resumes above the highest hit.  New entries are stored
+// only after the launches have finished without an output-boundary error.
+void run(const Chain& ch, Rect viewport, RenderLevel level, float* dst, uint32_t* flags, hipStream_t stream) {
+    const size_t S = ch.segs.size();
+    std::vector<Rect> region(S);
+    region[S - 1] = viewport;
+    for (size_t k = S - 1; k >= 1; --k) {
+        const auto* gf = ch.segs[k].guided;
+        region[k - 1] = gf->input_region_level(region[k], request_bounds(gf->native_bounds(), level), level);
+    }
+    std::vector<std::shared_ptr<const DeviceTile>> out(S);
+    std::vector<uint64_t> generation(S, 0);
+    size_t start = 0;
+    for (size_t k = S; k-- > 0;) {
+        const Segment& s = ch.segs[k];
+        if (!s.cache_boundary) continue;
+        out[k] = CacheAccess::lookup(*s.last.cache, s.last.signature, region[k], level, &generation[k]);
+        if (out[k]) { start = k + 1; break; }
+    }
+    const size_t row_bytes = size_t(viewport.width) * 3 * sizeof(float);
+    if (start == S) {
+        hip(hipMemcpyAsync(dst, out[S - 1]->ptr, row_bytes * viewport.height, hipMemcpyDeviceToDevice, stream),
+            "hipMemcpyAsync(cached tile)");
+        return;
+    }
+    const uint32_t quality = static_cast<uint32_t>(level.quality);
+    std::unique_ptr<DeviceSource> foreign_upload;
+    Rect foreign_region{};
+    for (size_t k = start; k < S; ++k) {
+        const Segment& s = ch.segs[k];
+        if (k == 0 && ch.fuse_source) continue;
+        float* target = dst;
+        std::shared_ptr<DeviceTile> fresh;
+        if (k + 1 < S) {
+            fresh = std::make_shared<DeviceTile>(region[k].width, region[k].height, s.last.node->output_descriptor());
+            target = static_cast<float*>(fresh->ptr);
+            out[k] = fresh;
+        }
+        const uint32_t ostride = region[k].width;
+        if (k == 0 && ch.kind == Chain::Raw) {
+            check(rawops_render(s.plan->handle, ch.raw->device().handle, c(region[0]), level.mip, quality, target,
+                                nullptr, ostride, flags, stream));
+        } else if (k == 0 && ch.kind == Chain::Raster) {
+            check(rawops_guided_filter(ch.raster->device().handle, c(region[0]), level.mip, 0, 1.0, s.plan->handle,
+                                       target, ostride, nullptr, 0, flags, stream));
+        } else if (k == 0 || (k == 1 && ch.fuse_source && ch.kind == Chain::Foreign)) {
+            // Foreign input: its own render at this level, uploaded.
+            foreign_region = region[0];
+            const Tile t = ch.foreign->render_level(foreign_region, level);
+            if (t.bounds.x != foreign_region.x || t.bounds.y != foreign_region.y ||
+                t.bounds.width != foreign_region.width || t.bounds.height != foreign_region.height ||
+                t.rgb.size() != size_t(foreign_region.width) * foreign_region.height * 3)
+                throw std::domain_error("upstream node returned an invalid tile");
+            foreign_upload = std::make_unique<DeviceSource>();
+            check(rawops_source_create_raster(foreign_region.width, foreign_region.height, 0,
+                                              uint32_t(space_of(t.descriptor)), t.rgb.data(), &foreign_upload->handle));
+            if (k == 0) {
+                check(rawops_guided_filter(foreign_upload->handle, {0, 0, foreign_region.width, foreign_region.height},
+                                           0, 0, 1.0, s.plan->handle, target, ostride, nullptr, 0, flags, stream));
+            }
+        }
+        if (k >= 1) {
+            const auto* gf = s.guided;
+            const Rect in = region[k - 1];
+            const rawops_rect v{region[k].x - in.x, region[k].y - in.y, region[k].width, region[k].height};
+            if (k == 1 && ch.fuse_source && ch.kind == Chain::Raster) {
+                check(rawops_guided_filter(ch.raster->device().handle, c(region[k]), level.mip,
+                                           gf->settings().radius, gf->settings().epsilon, s.plan->handle, target,
+                                           ostride, nullptr, 0, flags, stream));
+            } else if (k == 1 && ch.fuse_source) {
+                check(rawops_guided_filter(foreign_upload->handle, v, 0, gf->settings().radius,
+                                           gf->settings().epsilon, s.plan->handle, target, ostride, nullptr, 0, flags,
+                                           stream));
+            } else {
+                WrappedRaster src(static_cast<const float*>(out[k - 1]->ptr), in.width, in.height,
+                                  space_of(out[k - 1]->descriptor));
+                check(rawops_guided_filter(src.handle, v, 0, gf->settings().radius, gf->settings().epsilon,
+                                           s.plan->handle, target, ostride, nullptr, 0, flags, stream));
+            }
+        }
+    }
+    // Entries enter the caches only once the launches are known to be good.
+    bool pending = false;
+    for (size_t k = start; k < S; ++k) pending |= ch.segs[k].cache_boundary;
+    if (!pending) {
+        if (foreign_upload) hip(hipStreamSynchronize(stream), "stream synchronize");
+        return;
+    }
+    if (ch.segs[S - 1].cache_boundary &&
+        CacheAccess::admits(*ch.segs[S - 1].last.cache, size_t(viewport.width) * viewport.height * 3)) {
+        auto copy = std::make_shared<DeviceTile>(viewport.width, viewport.height,
+                                                 ch.segs[S - 1].last.node->output_descriptor());
+        hip(hipMemcpyAsync(copy->ptr, dst, row_bytes * viewport.height, hipMemcpyDeviceToDevice, stream),
+            "hipMemcpyAsync(cache entry)");
+        out[S - 1] = copy;
+    }
+    hip(hipStreamSynchronize(stream), "stream synchronize");
+    if (!flags) return;  // errors of this render are not observable: nothing is cached
+    check(rawops_check_flags(flags, stream));
+    for (size_t k = start; k < S; ++k) {
+        const Segment& s = ch.segs[k];
+        if (s.cache_boundary && out[k])
+            CacheAccess::store(*s.last.cache, s.last.signature, region[k], level, out[k], generation[k]);
+    }
 }
 
 void validate_rect(Rect area, Rect r) {
@@ -246,6 +428,18 @@ const Node* source_of(const Node& n) {
 }
 
 bool compilable(const Node& n) { return dynamic_cast<const DeviceNode*>(&n) != nullptr; }
+
+void render_to_device(const Node& output, Rect source_bounds, const RenderRequest& request, float* dst,
+                      hipStream_t stream) {
+    validate_rect(request_bounds(source_bounds, request.level), request.viewport);
+    if (!output.supports_level(request.level)) throw std::invalid_argument("node does not support this render level");
+    if (!request.viewport.width || !request.viewport.height) return;
+    const auto chain = compile(output);
+    DeviceBuffer flags(sizeof(uint32_t));
+    hip(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), stream), "hipMemsetAsync(flags)");
+    run(*chain, request.viewport, request.level, dst, static_cast<uint32_t*>(flags.p), stream);
+    check(rawops_check_flags(static_cast<uint32_t*>(flags.p), stream));
+}
 
 }  // namespace detail
 
@@ -611,22 +805,25 @@ Rect WorkingYGuidedFilterNode::input_region_level(Rect output, Rect source_bound
 
 // ------------------------------------------------------------------- graph --
 
-ImageGraph::ImageGraph(RawImage image, GraphRecipe recipe)
-    : ImageGraph(std::move(image), recipe, RawDemosaicIdentity{}, false) {}
+ImageGraph::ImageGraph(RawImage image, GraphRecipe recipe, std::shared_ptr<const IccDisplayTransform> transform)
+    : ImageGraph(std::move(image), recipe, RawDemosaicIdentity{}, false, std::move(transform)) {}
 
-ImageGraph::ImageGraph(RawImage image, GraphRecipe recipe, RawDemosaicIdentity demosaic, bool linear_output)
+ImageGraph::ImageGraph(RawImage image, GraphRecipe recipe, RawDemosaicIdentity demosaic, bool linear_output,
+                       std::shared_ptr<const IccDisplayTransform> transform)
     : raw_image_(std::move(image)), source_bounds_(raw_image_->metadata().active_area), recipe_(recipe) {
     std::shared_ptr<const Node> n = std::make_shared<RawUnpackNode>(*raw_image_, std::move(demosaic));
     n = std::make_shared<WhiteBalanceNode>(n, recipe.red_gain, recipe.green_gain, recipe.blue_gain);
     n = std::make_shared<ExposureNode>(n, recipe.exposure_stops);
     if (recipe.camera_color) n = std::make_shared<CameraToWorkingNode>(n, *recipe.camera_color, source_bounds_);
-    finish(n, linear_output);
+    finish(n, linear_output, std::move(transform));
 }
 
-ImageGraph::ImageGraph(RasterImage image, GraphRecipe recipe)
-    : ImageGraph(std::make_shared<RasterSourceNode>(image), Rect{0, 0, image.width(), image.height()}, recipe) {}
+ImageGraph::ImageGraph(RasterImage image, GraphRecipe recipe, std::shared_ptr<const IccDisplayTransform> transform)
+    : ImageGraph(std::make_shared<RasterSourceNode>(image), Rect{0, 0, image.width(), image.height()}, recipe,
+                 std::move(transform)) {}
 
-ImageGraph::ImageGraph(std::shared_ptr<const Node> source, Rect bounds, GraphRecipe recipe)
+ImageGraph::ImageGraph(std::shared_ptr<const Node> source, Rect bounds, GraphRecipe recipe,
+                       std::shared_ptr<const IccDisplayTransform> transform)
     : source_bounds_(bounds), recipe_(recipe) {
     if (!source || !bounds.width || !bounds.height)
         throw std::invalid_argument("scene-linear source and valid bounds are required");
@@ -635,24 +832,134 @@ ImageGraph::ImageGraph(std::shared_ptr<const Node> source, Rect bounds, GraphRec
         throw std::invalid_argument("graph source must output a supported scene-linear space");
     if (recipe.red_gain != 1.0f || recipe.green_gain != 1.0f || recipe.blue_gain != 1.0f || recipe.camera_color)
         throw std::invalid_argument("raster input does not accept RAW calibration controls");
-    finish(std::make_shared<ExposureNode>(source, recipe.exposure_stops), false);
+    finish(std::make_shared<ExposureNode>(source, recipe.exposure_stops), false, std::move(transform));
 }
 
-void ImageGraph::finish(std::shared_ptr<const Node> linear, bool linear_output) {
-    if (recipe_.output_mode == OutputMode::IccDisplay)
-        throw std::invalid_argument("ICC display output is not supported by this library");
-    if (recipe_.output_mode != OutputMode::LegacyBounded && recipe_.output_mode != OutputMode::SrgbPreview)
+void ImageGraph::finish(std::shared_ptr<const Node> linear, bool linear_output,
+                        std::shared_ptr<const IccDisplayTransform> transform) {
+    if (recipe_.output_mode != OutputMode::LegacyBounded && recipe_.output_mode != OutputMode::SrgbPreview &&
+        recipe_.output_mode != OutputMode::IccDisplay)
         throw std::invalid_argument("unknown output mode");
     if (linear_output) {
         output_ = linear;
         return;
     }
-    if (recipe_.output_mode == OutputMode::SrgbPreview) linear = std::make_shared<WorkingToSrgbNode>(linear);
+    if (recipe_.output_mode == OutputMode::SrgbPreview || recipe_.output_mode == OutputMode::IccDisplay)
+        linear = std::make_shared<WorkingToSrgbNode>(linear);
     auto tone = std::make_shared<ToneCurveNode>(linear, recipe_.tone_shoulder, recipe_.tone_gamma);
-    output_ = recipe_.output_mode == OutputMode::SrgbPreview
-                  ? std::static_pointer_cast<const Node>(std::make_shared<SrgbEncodeNode>(tone))
-                  : std::static_pointer_cast<const Node>(std::make_shared<OutputClipNode>(tone));
+    if (recipe_.output_mode == OutputMode::IccDisplay)
+        output_ = std::make_shared<IccDisplayNode>(tone, std::move(transform));
+    else
+        output_ = recipe_.output_mode == OutputMode::SrgbPreview
+                      ? std::static_pointer_cast<const Node>(std::make_shared<SrgbEncodeNode>(tone))
+                      : std::static_pointer_cast<const Node>(std::make_shared<OutputClipNode>(tone));
 }
+
+// ------------------------------------------------------------- ICC display --
+
+IccDisplayNode::IccDisplayNode(std::shared_ptr<const Node> input, std::shared_ptr<const IccDisplayTransform> transform)
+    : input_(std::move(input)), transform_(std::move(transform)) {
+    if (!input_ || !transform_ || input_->output_descriptor() != ImageDescriptor::display_linear_srgb())
+        throw std::invalid_argument("ICC display requires display-linear sRGB and a transform");
+    const auto digest = transform_->profile_sha256();
+    if (std::all_of(digest.begin(), digest.end(), [](std::uint8_t v) { return v == 0; }))
+        throw std::invalid_argument("ICC output profile digest must be nonzero");
+    descriptor_ = ImageDescriptor::icc_display(digest);
+}
+
+Tile IccDisplayNode::render(Rect r) const {
+    // The display-linear input renders on the device (one launch for the chain).
+    Tile tile = input_->render(r);
+    if (tile.bounds.x != r.x || tile.bounds.y != r.y || tile.bounds.width != r.width ||
+        tile.bounds.height != r.height || tile.rgb.size() != size_t(r.width) * r.height * 3 ||
+        tile.descriptor != input_->output_descriptor())
+        throw std::domain_error("upstream node returned an invalid tile");
+    if (!std::all_of(tile.rgb.begin(), tile.rgb.end(), [](float v) { return std::isfinite(v); }))
+        throw std::domain_error("non-finite value at ICC input boundary");
+    if (!tile.rgb.empty()) transform_->apply(tile.rgb.data(), tile.rgb.size() / 3);
+    if (!std::all_of(tile.rgb.begin(), tile.rgb.end(),
+                     [](float v) { return std::isfinite(v) && v >= 0.0f && v <= 1.0f; }))
+        throw std::domain_error("ICC display transform returned unbounded or non-finite RGB");
+    tile.descriptor = descriptor_;
+    return tile;
+}
+
+// ------------------------------------------------------------ fingerprints --
+
+namespace {
+
+void put_u8(detail::Sha256& h, uint8_t v) { h.update(&v, 1); }
+void put_u16(detail::Sha256& h, uint16_t v) {
+    const uint8_t b[2] = {uint8_t(v), uint8_t(v >> 8)};
+    h.update(b, 2);
+}
+void put_u32(detail::Sha256& h, uint32_t v) {
+    const uint8_t b[4] = {uint8_t(v), uint8_t(v >> 8), uint8_t(v >> 16), uint8_t(v >> 24)};
+    h.update(b, 4);
+}
+void put_tag(detail::Sha256& h, const char* tag) { h.update(tag, std::strlen(tag) + 1); }
+
+// Little-endian bytes of a row of samples, staged through a stack block.
+template <class T> void put_row(detail::Sha256& h, const T* p, size_t n) {
+    static_assert(sizeof(T) == 2 || sizeof(T) == 4);
+    uint8_t block[4096];
+    constexpr size_t per = sizeof(block) / sizeof(T);
+    while (n) {
+        const size_t m = std::min(n, per);
+        for (size_t i = 0; i < m; ++i) {
+            uint32_t bits;
+            if constexpr (sizeof(T) == 2) bits = p[i];
+            else bits = std::bit_cast<uint32_t>(p[i]);
+            for (size_t b = 0; b < sizeof(T); ++b) block[i * sizeof(T) + b] = uint8_t(bits >> (8 * b));
+        }
+        h.update(block, m * sizeof(T));
+        p += m;
+        n -= m;
+    }
+}
+
+}  // namespace
+
+std::array<std::uint8_t, 32> RawImage::fingerprint() const {
+    std::call_once(*fingerprint_once_, [&] {
+        const auto& m = metadata_;
+        detail::Sha256 h;
+        put_tag(h, "librawops.source.raw.u16.v1");
+        put_u32(h, m.width);
+        put_u32(h, m.height);
+        put_u8(h, uint8_t(m.pattern));
+        put_u8(h, m.cfa_phase_x);
+        put_u8(h, m.cfa_phase_y);
+        put_u32(h, m.active_area.x);
+        put_u32(h, m.active_area.y);
+        put_u32(h, m.active_area.width);
+        put_u32(h, m.active_area.height);
+        for (auto v : m.black_levels) put_u16(h, v);
+        for (auto v : m.white_levels) put_u16(h, v);
+        for (uint32_t y = m.active_area.y; y < m.active_area.y + m.active_area.height; ++y)
+            put_row(h, bayer_->data() + size_t(y) * m.row_stride_samples + m.active_area.x, m.active_area.width);
+        *fingerprint_ = h.finish();
+    });
+    return *fingerprint_;
+}
+
+std::array<std::uint8_t, 32> RasterImage::fingerprint() const {
+    std::call_once(*fingerprint_once_, [&] {
+        const auto& m = metadata_;
+        detail::Sha256 h;
+        put_tag(h, "librawops.source.raster.f32.v1");
+        put_u32(h, m.width);
+        put_u32(h, m.height);
+        put_u8(h, uint8_t(m.working_space));
+        for (uint32_t y = 0; y < m.height; ++y)
+            put_row(h, pixels_->data() + size_t(y) * m.row_stride_pixels * 3, size_t(m.width) * 3);
+        *fingerprint_ = h.finish();
+    });
+    return *fingerprint_;
+}
+
+std::array<std::uint8_t, 32> fingerprint_raw_source(const RawImage& image) { return image.fingerprint(); }
+std::array<std::uint8_t, 32> fingerprint_raster_source(const RasterImage& image) { return image.fingerprint(); }
 
 const RawImage& ImageGraph::image() const {
     if (!raw_image_) throw std::logic_error("graph has a raster source, not a RAW source");
@@ -689,6 +996,7 @@ void Renderer::render_tiles(const Node& output, Rect source_bounds, RenderReques
     // One device render per row of tiles: the band is produced by one launch,
     // copied to pinned host memory, then handed out tile by tile.
     const auto chain = detail::compile(output);
+    const hipStream_t stream = detail::thread_stream();
     const uint32_t band_h = std::min<uint32_t>(request.tile_size, v.height);
     detail::DeviceBuffer dev(size_t(v.width) * band_h * 3 * sizeof(float));
     detail::DeviceBuffer flags(sizeof(uint32_t));
@@ -696,15 +1004,15 @@ void Renderer::render_tiles(const Node& output, Rect source_bounds, RenderReques
     detail::hip(hipHostMalloc(reinterpret_cast<void**>(&host), size_t(v.width) * band_h * 3 * sizeof(float),
                               hipHostMallocDefault), "hipHostMalloc");
     struct Free { float* p; ~Free() { (void)hipHostFree(p); } } free_host{host};
-    detail::hip(hipMemset(flags.p, 0, sizeof(uint32_t)), "hipMemset");
+    detail::hip(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), stream), "hipMemsetAsync");
     const auto desc = output.output_descriptor();
     for (uint64_t y = v.y; y < bottom; y += request.tile_size) {
         if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
         const Rect band{v.x, uint32_t(y), v.width, uint32_t(std::min<uint64_t>(request.tile_size, bottom - y))};
-        detail::run(*chain, band, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), nullptr);
-        detail::hip(hipMemcpy(host, dev.p, size_t(band.width) * band.height * 3 * sizeof(float),
-                              hipMemcpyDeviceToHost), "hipMemcpy");
-        check(rawops_check_flags(static_cast<uint32_t*>(flags.p), nullptr));
+        detail::run(*chain, band, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), stream);
+        detail::hip(hipMemcpyAsync(host, dev.p, size_t(band.width) * band.height * 3 * sizeof(float),
+                                   hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+        check(rawops_check_flags(static_cast<uint32_t*>(flags.p), stream));
         for (uint64_t x = v.x; x < right; x += request.tile_size) {
             if (cancellation && cancellation->is_cancelled()) throw RenderCancelled();
             Tile t{Rect{uint32_t(x), band.y, uint32_t(std::min<uint64_t>(request.tile_size, right - x)), band.height},
@@ -751,12 +1059,14 @@ Tile Renderer::render_image(const Node& output, Rect source_bounds, RenderReques
         return image;
     }
     const auto chain = detail::compile(output);
+    const hipStream_t stream = detail::thread_stream();
     detail::DeviceBuffer dev(image.rgb.size() * sizeof(float));
     detail::DeviceBuffer flags(sizeof(uint32_t));
-    detail::hip(hipMemset(flags.p, 0, sizeof(uint32_t)), "hipMemset");
-    detail::run(*chain, v, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), nullptr);
-    detail::hip(hipMemcpy(image.rgb.data(), dev.p, image.rgb.size() * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
-    check(rawops_check_flags(static_cast<uint32_t*>(flags.p), nullptr));
+    detail::hip(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), stream), "hipMemsetAsync");
+    detail::run(*chain, v, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), stream);
+    detail::hip(hipMemcpyAsync(image.rgb.data(), dev.p, image.rgb.size() * sizeof(float), hipMemcpyDeviceToHost,
+                               stream), "hipMemcpyAsync");
+    check(rawops_check_flags(static_cast<uint32_t*>(flags.p), stream));
     return image;
 }
 

@@ -3,6 +3,7 @@
 // the reference node chains for the render path and exposes them through a
 // small C interface so the tests can compare the oracle restatement (and the
 // HIP path) with the real reference.  Never part of the product.
+#include "EditGraph.hpp"
 #include "RawEngine.hpp"
 #include "SpatialOps.hpp"
 #include "ToneOps.hpp"
@@ -218,6 +219,62 @@ int refh_time_menon_bands(const ro_bayer* b, const uint16_t* samples, size_t cou
         const auto t0 = std::chrono::steady_clock::now();
         for (uint32_t i = 0; i < reps; ++i) round();
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+    });
+}
+
+// ---- edit manifests (EditGraph.cpp) ----
+
+// serialize_edit_manifest(parse_edit_manifest(json)) into out (NUL-terminated;
+// *len receives the full length, also when it does not fit).
+int refh_manifest_canonical(const char* json, char* out, size_t cap, size_t* len) {
+    return guarded([&] {
+        const std::string text = serialize_edit_manifest(parse_edit_manifest(json));
+        *len = text.size();
+        if (text.size() + 1 <= cap) std::memcpy(out, text.c_str(), text.size() + 1);
+    });
+}
+
+int refh_fingerprint_raw(const ro_bayer* b, const uint16_t* samples, size_t count, uint8_t out[32]) {
+    return guarded([&] {
+        const auto d = fingerprint_raw_source(make_image(b, samples, count));
+        std::memcpy(out, d.data(), 32);
+    });
+}
+
+int refh_fingerprint_raster(uint32_t w, uint32_t h, uint32_t stride, int space, const float* pixels,
+                            uint8_t out[32]) {
+    return guarded([&] {
+        const size_t n = size_t(stride ? stride : w) * h * 3;
+        RasterImage image({w, h, stride, static_cast<WorkingSpace>(space)}, std::vector<float>(pixels, pixels + n));
+        const auto d = fingerprint_raster_source(image);
+        std::memcpy(out, d.data(), 32);
+    });
+}
+
+// snapshot_legacy_recipe for a Bayer (raw != 0) or ProPhoto raster source.
+int refh_legacy_snapshot(int raw, const uint8_t digest[32], const char* source_id, const char* op_id, int era,
+                         const float gains[3], float stops, float shoulder, float gamma, int mode, int has_camera,
+                         const double cam[9], int target, char* out, size_t cap, size_t* len) {
+    return guarded([&] {
+        EditSource src;
+        src.id = source_id;
+        src.kind = raw ? EditSourceKind::DecodedBayerU16 : EditSourceKind::SceneLinearRasterF32;
+        if (!raw) src.working_space = static_cast<WorkingSpace>(target);
+        std::memcpy(src.content_sha256.data(), digest, 32);
+        GraphRecipe r;
+        r.red_gain = gains[0]; r.green_gain = gains[1]; r.blue_gain = gains[2];
+        r.exposure_stops = stops; r.tone_shoulder = shoulder; r.tone_gamma = gamma;
+        r.output_mode = static_cast<OutputMode>(mode);
+        if (has_camera) {
+            CameraColorTransform t;
+            for (int i = 0; i < 9; ++i) t.camera_to_xyz_d50[i] = cam[i];
+            t.target = static_cast<WorkingSpace>(target);
+            r.camera_color = t;
+        }
+        const std::string text = serialize_edit_manifest(
+            snapshot_legacy_recipe(src, r, static_cast<LegacyRecipeEra>(era), op_id));
+        *len = text.size();
+        if (text.size() + 1 <= cap) std::memcpy(out, text.c_str(), text.size() + 1);
     });
 }
 

@@ -5,6 +5,7 @@
 #include <SpatialOps.hpp>
 #include <ToneOps.hpp>
 #include <ImageAnalysis.hpp>
+#include <EditGraph.hpp>
 
 #include <cmath>
 #include <cstdio>
@@ -160,7 +161,313 @@ static void histogram_checks() {
     CHECK(covered, "histogram_rgb: ImageGraph overload covers every pixel on the device path");
 }
 
+
+// ------------------------------------------------------------ edit graphs --
+
+static std::string uid(unsigned i) {
+    char b[40];
+    std::snprintf(b, sizeof b, "00000000-0000-0000-0000-%012x", i);
+    return b;
+}
+
+static EditOperation make_op(unsigned id, std::string type, EditDomain in, EditDomain out, std::string upstream,
+                             EditValue::Object params = {}) {
+    EditOperation op;
+    op.id = uid(id);
+    op.type_id = std::move(type);
+    op.processing_version = kCurrentEditProcessingVersion;
+    op.input_domain = in;
+    op.output_domain = out;
+    op.inputs.emplace("image", std::move(upstream));
+    op.parameters = std::move(params);
+    return op;
+}
+
+static bool same_tile(const Tile& a, const Tile& b) {
+    return a.descriptor == b.descriptor && a.bounds.x == b.bounds.x && a.bounds.y == b.bounds.y &&
+           a.bounds.width == b.bounds.width && a.bounds.height == b.bounds.height && same_bits(a.rgb, b.rgb);
+}
+
+// Maps display-linear RGB into [0, 1] by a fixed affine map (a stand-in for an
+// ICC engine: the hook only needs a deterministic host transform).
+class HalfGreyDisplay final : public IccDisplayTransform {
+public:
+    bool unbounded = false;
+    std::array<std::uint8_t, 32> profile_sha256() const noexcept override {
+        std::array<std::uint8_t, 32> d{};
+        d.fill(0x3a);
+        return d;
+    }
+    std::optional<IccProfileIdentity> output_icc_identity() const override {
+        IccProfileIdentity id;
+        id.profile_sha256 = profile_sha256();
+        return id;
+    }
+    void apply(float* rgb, std::size_t pixels) const override {
+        for (std::size_t i = 0; i < pixels * 3; ++i)
+            rgb[i] = unbounded ? 2.0f : 0.25f + 0.5f * std::min(std::max(rgb[i], 0.0f), 1.0f);
+    }
+};
+
+static void edit_graph_checks() {
+    const std::array<double, 9> identity{1, 0, 0, 0, 1, 0, 0, 0, 1};
+    Renderer renderer;
+    // (a) A RAW manifest equals the fixed ImageGraph it restates
+    //     (edit_graph_tests.cpp:111-223 of the reference).
+    {
+        constexpr uint32_t W = 13, H = 9;
+        std::vector<uint16_t> s(W * H);
+        for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) s[y * W + x] = uint16_t(1000 + x * 1900 + y * 2300);
+        RawImage raw(W, H, s);
+        GraphRecipe recipe;
+        recipe.red_gain = 1.15f; recipe.green_gain = 0.95f; recipe.blue_gain = 1.3f;
+        recipe.exposure_stops = 0.75f; recipe.tone_shoulder = 0.35f; recipe.tone_gamma = 1.1f;
+        recipe.output_mode = OutputMode::SrgbPreview;
+        recipe.camera_color = CameraColorTransform{identity, WorkingSpace::LinearProPhotoD50};
+        ImageGraph fixed(raw, recipe);
+        EditSource source;
+        source.id = uid(1);
+        source.kind = EditSourceKind::DecodedBayerU16;
+        source.content_sha256 = fingerprint_raw_source(raw);
+        EditManifest m;
+        m.sources.push_back(source);
+        using D = EditDomain;
+        m.operations.push_back(make_op(2, "rawengine.white_balance", D::CameraLinear, D::CameraLinear, source.id,
+                                       {{"red_gain", EditValue{double(recipe.red_gain)}},
+                                        {"green_gain", EditValue{double(recipe.green_gain)}},
+                                        {"blue_gain", EditValue{double(recipe.blue_gain)}}}));
+        m.operations.push_back(make_op(3, "rawengine.exposure", D::CameraLinear, D::CameraLinear, uid(2),
+                                       {{"stops", EditValue{double(recipe.exposure_stops)}}}));
+        EditValue::Array mat;
+        for (double v : identity) mat.push_back(EditValue{v});
+        m.operations.push_back(make_op(4, "rawengine.camera_to_working", D::CameraLinear, D::SceneLinearProPhotoD50,
+                                       uid(3), {{"matrix", EditValue{mat}}}));
+        m.operations.push_back(make_op(5, "rawengine.working_to_srgb", D::SceneLinearProPhotoD50, D::SceneLinearSrgb,
+                                       uid(4)));
+        m.operations.push_back(make_op(6, "rawengine.tone_curve", D::SceneLinearSrgb, D::DisplayLinearSrgb, uid(5),
+                                       {{"shoulder", EditValue{double(recipe.tone_shoulder)}},
+                                        {"gamma", EditValue{double(recipe.tone_gamma)}}}));
+        m.operations.push_back(make_op(7, "rawengine.srgb_encode", D::DisplayLinearSrgb, D::DisplayEncodedSrgb, uid(6)));
+        m.output_id = uid(7);
+        std::reverse(m.operations.begin(), m.operations.end());
+        auto node = std::make_shared<RawUnpackNode>(raw);
+        ExecutableEditGraph graph(parse_edit_manifest(serialize_edit_manifest(m)), {{source, node, {0, 0, W, H}}});
+        bool same = true;
+        for (Rect roi : {Rect{0, 0, W, H}, Rect{2, 1, 7, 6}, Rect{12, 8, 1, 1}})
+            same = same && same_tile(renderer.render_image(graph, roi, 3), renderer.render_image(fixed, roi, 3));
+        CHECK(same, "EditGraph: RAW manifest render == fixed ImageGraph (bit-exact, 3 ROIs)");
+        std::vector<Tile> tiles;
+        renderer.render_tiles(graph, RenderRequest{Rect{0, 0, W, H}, 4}, [&](const Tile& t) { tiles.push_back(t); });
+        CHECK(tiles.size() == 12, "EditGraph: render_tiles keeps the tile partition");
+        bool rejected = true;
+        auto wrong = source;
+        wrong.content_sha256[0] ^= 1;
+        rejected &= throws_as([&] { ExecutableEditGraph(m, {{wrong, node, {0, 0, W, H}}}); }, 2);
+        auto altered = raw.samples();
+        altered[0] ^= 1;
+        rejected &= throws_as([&] {
+            ExecutableEditGraph(m, {{source, std::make_shared<RawUnpackNode>(RawImage(raw.metadata(), altered)),
+                                     {0, 0, W, H}}});
+        }, 2);
+        rejected &= throws_as([&] { ExecutableEditGraph(m, {{source, node, {0, 0, W - 1, H}}}); }, 2);
+        auto wrong_domain = m;
+        wrong_domain.operations.front().input_domain = D::CameraLinear;
+        rejected &= throws_as([&] { ExecutableEditGraph(wrong_domain, {{source, node, {0, 0, W, H}}}); }, 2);
+        auto unknown = m;
+        unknown.operations.front().type_id = "vendor.unknown.operation";
+        rejected &= serialize_edit_manifest(parse_edit_manifest(serialize_edit_manifest(unknown))) ==
+                    serialize_edit_manifest(unknown);
+        rejected &= throws_as([&] { ExecutableEditGraph(unknown, {{source, node, {0, 0, W, H}}}); }, 2);
+        auto version = m;
+        version.operations.front().processing_version = kLegacyRec2020ProcessingVersion;
+        rejected &= throws_as([&] { ExecutableEditGraph(version, {{source, node, {0, 0, W, H}}}); }, 2);
+        CHECK(rejected, "EditGraph: wrong fingerprint/samples/bounds/domain, unknown op, mixed versions rejected");
+        auto old_recipe = recipe;
+        old_recipe.camera_color->target = WorkingSpace::LinearRec2020D65;
+        ImageGraph old_fixed(raw, old_recipe);
+        auto old_manifest = snapshot_legacy_recipe(source, old_recipe, LegacyRecipeEra::ImplicitRec2020, uid(8));
+        ExecutableEditGraph old_graph(parse_edit_manifest(serialize_edit_manifest(old_manifest)),
+                                      {{source, node, {0, 0, W, H}}});
+        CHECK(same_tile(renderer.render_image(old_graph, Rect{1, 2, 8, 5}, 3),
+                        renderer.render_image(old_fixed, Rect{1, 2, 8, 5}, 3)),
+              "EditGraph: legacy Rec.2020 snapshot replay == fixed graph");
+        const Rect region = old_graph.required_source_region({1, 2, 8, 5});
+        CHECK(region.x == 0 && region.y == 1 && region.width == 10 && region.height == 7,
+              "EditGraph: required_source_region plans the demosaic halo (edit_graph_tests.cpp:219-222)");
+    }
+    // (b) RAW (Menon) -> WB -> camera -> guided filter -> exposure -> curves:
+    //     two device launches, checked against the oracle.
+    {
+        RawMetadata md;
+        md.width = 300; md.height = 200; md.row_stride_samples = 304; md.pattern = BayerPattern::BGGR;
+        md.cfa_phase_x = 1; md.active_area = {5, 3, 288, 190};
+        md.black_levels = {510, 512, 511, 513}; md.white_levels.fill(16383);
+        auto s = random_samples(304 * 200, 7, 16383);
+        RawImage raw(md, s);
+        EditSource source;
+        source.id = uid(0x40);
+        source.kind = EditSourceKind::DecodedBayerU16;
+        source.content_sha256 = fingerprint_raw_source(raw);
+        source.demosaic = RawDemosaicIdentity{"rawengine.menon_base", 1};
+        const std::array<double, 9> cam{0.7, 0.2, 0.1, 0.25, 0.7, 0.05, 0.02, 0.1, 0.88};
+        EditValue::Array mat;
+        for (double v : cam) mat.push_back(EditValue{v});
+        using D = EditDomain;
+        auto knots = [](std::initializer_list<std::pair<double, double>> k) {
+            EditValue::Array a;
+            for (auto [x, y] : k) a.push_back(EditValue{EditValue::Array{EditValue{x}, EditValue{y}}});
+            return EditValue{a};
+        };
+        EditManifest m;
+        m.format_version = 3;
+        m.sources.push_back(source);
+        m.operations.push_back(make_op(0x41, "rawengine.white_balance", D::CameraLinear, D::CameraLinear, source.id,
+                                       {{"red_gain", EditValue{2.0}}, {"green_gain", EditValue{1.0}},
+                                        {"blue_gain", EditValue{1.5}}}));
+        m.operations.push_back(make_op(0x42, "rawengine.camera_to_working", D::CameraLinear,
+                                       D::SceneLinearProPhotoD50, uid(0x41), {{"matrix", EditValue{mat}}}));
+        m.operations.push_back(make_op(0x43, "rawengine.guided_filter_working_y", D::SceneLinearProPhotoD50,
+                                       D::SceneLinearProPhotoD50, uid(0x42),
+                                       {{"radius", EditValue{std::int64_t(4)}}, {"epsilon", EditValue{1e-3}}}));
+        m.operations.push_back(make_op(0x44, "rawengine.exposure", D::SceneLinearProPhotoD50, D::SceneLinearProPhotoD50,
+                                       uid(0x43), {{"stops", EditValue{0.7}}}));
+        m.operations.push_back(make_op(0x45, "rawengine.curves_extended", D::SceneLinearProPhotoD50,
+                                       D::SceneLinearProPhotoD50, uid(0x44),
+                                       {{"master", knots({{0, 0}, {0.25, 0.3}, {1, 1}})},
+                                        {"red", knots({{0, 0}, {1, 1}})}, {"green", knots({{0, 0}, {1, 1}})},
+                                        {"blue", knots({{0, 0.1}, {0.5, 0.45}, {1, 1}})},
+                                        {"interpolation", EditValue{std::string("shape_preserving_cubic")}}}));
+        m.output_id = uid(0x45);
+        auto node = std::make_shared<RawUnpackNode>(raw, *source.demosaic);
+        ExecutableEditGraph graph(m, {{source, node, md.active_area}});
+        const Rect vp{40, 30, 200, 120};
+        const Tile got = renderer.render_image(graph, RenderRequest{vp, 64});
+        // Oracle: the guided filter's input region rendered by the restatement.
+        const Rect area = md.active_area;
+        const Rect needed = working_y_guided_filter_region(vp, area, 4);
+        ro_bayer b = oracle_meta(md);
+        std::vector<float> lin(size_t(needed.width) * needed.height * 3);
+        ro_demosaic_menon(&b, s.data(), {needed.x, needed.y, needed.width, needed.height}, lin.data());
+        const float gains[3] = {2.0f, 1.0f, 1.5f};
+        ro_scale3(lin.data(), lin.size() / 3, gains);
+        CameraToWorkingNode cw(std::make_shared<ConstantCamera>(), CameraColorTransform{cam, WorkingSpace::LinearProPhotoD50});
+        ro_mat3(lin.data(), lin.size() / 3, cw.matrix().data());
+        std::vector<float> exp(size_t(vp.width) * vp.height * 3);
+        ro_guided_filter(lin.data(), {needed.x, needed.y, needed.width, needed.height},
+                         {vp.x, vp.y, vp.width, vp.height}, {area.x, area.y, area.width, area.height}, 4, 1e-3,
+                         0.28807112822929337, 0.00008565396060525903, exp.data());
+        ro_scale(exp.data(), exp.size(), ro_exp2f(0.7f));
+        const double xs[] = {0, 0.25, 1, 0, 1, 0, 1, 0, 0.5, 1}, ys[] = {0, 0.3, 1, 0, 1, 0, 1, 0.1, 0.45, 1};
+        const uint32_t offs[5] = {0, 3, 5, 7, 10};
+        ro_curves(exp.data(), exp.size() / 3, xs, ys, offs, 1);
+        float worst = 0;
+        for (size_t i = 0; i < exp.size(); ++i)
+            worst = std::max(worst, std::abs(got.rgb[i] - exp[i]) / std::max(1.0f, std::abs(exp[i])));
+        std::printf("     manifest Menon+GF+curves max rel err %.3g\n", worst);
+        CHECK(got.rgb.size() == exp.size() && worst <= 1e-5f,
+              "EditGraph: RAW Menon -> WB -> camera -> guided filter -> exposure -> curves == oracle within 1e-5");
+        // Tile cache: warm renders hit, edits never reuse stale tiles.
+        auto cache = std::make_shared<TileCache>(64u << 20);
+        ExecutableEditGraph cached(m, {{source, node, md.active_area}}, nullptr, cache);
+        const Tile cold = renderer.render_image(cached, RenderRequest{vp, 64});
+        const auto c0 = cache->stats();
+        const Tile warm = renderer.render_image(cached, RenderRequest{vp, 64});
+        const auto c1 = cache->stats();
+        CHECK(same_tile(cold, got) && same_tile(warm, got) && c1.hits > c0.hits && c1.misses == c0.misses &&
+                  c0.entries >= 2 && c1.used_bytes <= (64u << 20),
+              "TileCache: cold and warm cached renders == uncached; warm render hits");
+        auto late = m;
+        late.operations.back().parameters["interpolation"] = EditValue{std::string("linear")};
+        ExecutableEditGraph late_graph(late, {{source, node, md.active_area}}, nullptr, cache);
+        const Tile late_out = renderer.render_image(late_graph, RenderRequest{vp, 64});
+        const auto c2 = cache->stats();
+        CHECK(same_tile(late_out, renderer.render_image(ExecutableEditGraph(late, {{source, node, md.active_area}}),
+                                                        RenderRequest{vp, 64})) &&
+                  !same_bits(late_out.rgb, got.rgb) && c2.hits > c1.hits,
+              "TileCache: a late edit reuses the unchanged guided-filter input tile and no stale output");
+        auto early = m;
+        early.operations.front().parameters["red_gain"] = EditValue{2.5};
+        ExecutableEditGraph early_graph(early, {{source, node, md.active_area}}, nullptr, cache);
+        CHECK(same_tile(renderer.render_image(early_graph, RenderRequest{vp, 64}),
+                        renderer.render_image(ExecutableEditGraph(early, {{source, node, md.active_area}}),
+                                              RenderRequest{vp, 64})),
+              "TileCache: an early edit reuses no stale downstream tile");
+        auto tiny = std::make_shared<TileCache>(700);
+        ExecutableEditGraph tiny_graph(m, {{source, node, md.active_area}}, nullptr, tiny);
+        CHECK(same_tile(renderer.render_image(tiny_graph, RenderRequest{vp, 64}), got) && tiny->stats().used_bytes <= 700,
+              "TileCache: a tile larger than the budget renders without entering the cache");
+        CancellationToken pre;
+        pre.cancel();
+        CHECK(throws_as([&] { renderer.render_image(cached, RenderRequest{vp, 64}, &pre); }, 4),
+              "TileCache: a cancelled cached render throws RenderCancelled");
+        cache->clear();
+        CHECK(cache->stats().entries == 0 && cache->stats().used_bytes == 0 && cache->stats().hits == 0,
+              "TileCache: clear() drops entries and counters");
+        // Host-tile API on a source node (EditGraph tests: level keys).
+        TileCache level_cache(4096);
+        std::array<std::uint8_t, 32> sig{};
+        sig[0] = 17;
+        const Rect one{0, 0, 1, 1};
+        auto raster = std::make_shared<RasterSourceNode>(RasterImage({8, 6, 0, WorkingSpace::LinearProPhotoD50},
+                                                                     std::vector<float>(8 * 6 * 3, 0.25f)));
+        const Tile first = level_cache.render(*raster, sig, one);
+        const Tile again = level_cache.render(*raster, sig, one);
+        bool levels = level_cache.stats().entries == 1 && level_cache.stats().hits == 1 && same_tile(first, again) &&
+                      first.rgb == std::vector<float>{0.25f, 0.25f, 0.25f};
+        level_cache.render(*raster, sig, one, {0, RenderQuality::Preview});
+        level_cache.render(*raster, sig, one, {1, RenderQuality::Preview});
+        levels = levels && level_cache.stats().entries == 3 && level_cache.stats().misses == 3;
+        levels = levels && throws_as([&] { level_cache.render(*raster, sig, one, {32, RenderQuality::Final}); }, 2);
+        CHECK(levels, "TileCache::render: one entry per (signature, bounds, mip, quality); invalid level rejected");
+        // histogram_rgb over the manifest graph's output bounds.
+        const auto h = histogram_rgb(graph, RenderRequest{vp, 64}, HistogramOptions{16, 0.0, 1.0});
+        uint64_t total = 0;
+        for (auto k : h.channels[1].counts) total += k;
+        total += h.channels[1].underflow + h.channels[1].overflow + h.channels[1].nonfinite;
+        CHECK(total == uint64_t(vp.width) * vp.height && h.descriptor == graph.output().output_descriptor(),
+              "histogram_rgb(ExecutableEditGraph) covers the viewport");
+    }
+    // (c) ICC display hook: device render up to display-linear sRGB, host transform.
+    {
+        RawMetadata md;
+        md.width = 40; md.height = 30; md.black_levels.fill(64); md.white_levels.fill(4095);
+        RawImage raw(md, random_samples(40 * 30, 9, 4095));
+        GraphRecipe r;
+        r.camera_color = CameraColorTransform{identity, WorkingSpace::LinearProPhotoD50};
+        r.output_mode = OutputMode::IccDisplay;
+        auto display = std::make_shared<HalfGreyDisplay>();
+        ImageGraph icc_graph(raw, r, display);
+        GraphRecipe lin = r;
+        lin.output_mode = OutputMode::SrgbPreview;
+        ImageGraph preview(raw, lin);
+        const auto& encode = dynamic_cast<const SrgbEncodeNode&>(preview.output());
+        Tile tone = encode.input_node()->render({0, 0, 40, 30});
+        display->apply(tone.rgb.data(), tone.rgb.size() / 3);
+        const Tile got = renderer.render_image(icc_graph, Rect{0, 0, 40, 30});
+        CHECK(same_bits(got.rgb, tone.rgb) && got.descriptor == ImageDescriptor::icc_display(display->profile_sha256()),
+              "ICC hook: IccDisplayNode == display-linear device render + host transform");
+        display->unbounded = true;
+        CHECK(throws_as([&] { renderer.render_image(icc_graph, Rect{0, 0, 4, 4}); }, 3),
+              "ICC hook: unbounded transform output -> domain_error");
+        display->unbounded = false;
+        EditSource source;
+        source.id = uid(0x60);
+        source.kind = EditSourceKind::DecodedBayerU16;
+        source.content_sha256 = fingerprint_raw_source(raw);
+        auto snapshot = snapshot_legacy_recipe(source, r, LegacyRecipeEra::ImplicitProPhoto, uid(0x61),
+                                               *display->output_icc_identity());
+        ExecutableEditGraph g(snapshot, {{source, std::make_shared<RawUnpackNode>(raw), {0, 0, 40, 30}}}, display);
+        CHECK(same_tile(renderer.render_image(g, Rect{0, 0, 40, 30}), got),
+              "ICC hook: legacy icc_display manifest == ImageGraph IccDisplay");
+        CHECK(throws_as([&] { ExecutableEditGraph(snapshot, {{source, std::make_shared<RawUnpackNode>(raw),
+                                                              {0, 0, 40, 30}}}); }, 2),
+              "ICC hook: manifest with an output profile needs the runtime transform");
+    }
+}
+
 int main() {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0);  // a line per check, also when a later check aborts
     const std::array<double, 9> identity{1, 0, 0, 0, 1, 0, 0, 0, 1};
     // 1. Bilinear linear ProPhoto through ImageGraph, 256 tiles.
     {
@@ -268,6 +575,7 @@ int main() {
         CHECK(throws_as([&] { WorkingYGuidedFilterNode(raster, Rect{0, 0, W, H}, {9, 1e-3}); }, 2), "radius 9 rejected");
     }
     histogram_checks();
+    edit_graph_checks();
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }
