@@ -104,10 +104,10 @@ void CacheAccess::store(TileCache& c, const Signature& sig, Rect b, RenderLevel 
 Tile TileCache::render(const Node& node, std::array<std::uint8_t, 32> signature, Rect bounds, RenderLevel level) {
     validate_cache_level(level);
     if (!node.supports_level(level)) throw std::invalid_argument("node does not support this cache render level");
-    const hipStream_t stream = detail::thread_stream();
     const auto to_host = [&](const detail::DeviceTile& t) {
         Tile out{bounds, std::vector<float>(t.floats()), t.descriptor};
         if (!out.rgb.empty()) {
+            const hipStream_t stream = detail::thread_stream();
             detail::hip(hipMemcpyAsync(out.rgb.data(), t.ptr, out.rgb.size() * sizeof(float), hipMemcpyDeviceToHost,
                                        stream), "hipMemcpyAsync(cache hit)");
             detail::hip(hipStreamSynchronize(stream), "stream synchronize");
@@ -125,7 +125,7 @@ Tile TileCache::render(const Node& node, std::array<std::uint8_t, 32> signature,
         auto tile = std::make_shared<detail::DeviceTile>(bounds.width, bounds.height, node.output_descriptor());
         detail::render_to_device(node, sb ? *sb : bounds,
                                  RenderRequest{bounds, std::max(bounds.width, bounds.height), level},
-                                 static_cast<float*>(tile->ptr), stream);
+                                 static_cast<float*>(tile->ptr), detail::thread_stream());
         detail::CacheAccess::store(*this, signature, bounds, level, tile, generation);
         return to_host(*tile);
     }
@@ -136,6 +136,7 @@ Tile TileCache::render(const Node& node, std::array<std::uint8_t, 32> signature,
         throw std::domain_error("node returned an invalid cache tile");
     if (!detail::CacheAccess::admits(*this, t.rgb.size())) return t;
     auto tile = std::make_shared<detail::DeviceTile>(bounds.width, bounds.height, t.descriptor);
+    const hipStream_t stream = detail::thread_stream();
     detail::hip(hipMemcpyAsync(tile->ptr, t.rgb.data(), t.rgb.size() * sizeof(float), hipMemcpyHostToDevice, stream),
                 "hipMemcpyAsync(cache entry)");
     detail::hip(hipStreamSynchronize(stream), "stream synchronize");

@@ -429,6 +429,34 @@ const Node* source_of(const Node& n) {
 
 bool compilable(const Node& n) { return dynamic_cast<const DeviceNode*>(&n) != nullptr; }
 
+bool single_launch_raw_chain(const Node& output) {
+    for (const Node* n = &output; n; n = n->input_node()) {
+        if (auto* cn = dynamic_cast<const CachedNode*>(n)) {
+            if (cn->cache()) return false;
+            n = &cn->inner();
+        }
+        std::vector<Op> probe;
+        if (dynamic_cast<const RawUnpackNode*>(n)) return true;
+        if (!point_ops(n, probe)) return false;
+    }
+    return false;
+}
+
+bool batch_plan(const Node& output, BatchPlan* out) {
+    if (!compilable(output)) return false;
+    std::shared_ptr<Chain> ch;
+    try {
+        ch = compile(output);
+    } catch (const std::exception&) {
+        return false;
+    }
+    if (ch->kind != Chain::Raw || ch->segs.size() != 1 || ch->segs[0].cache_boundary) return false;
+    out->plan = ch->segs[0].plan->handle;
+    out->source = ch->raw->device().handle;
+    out->keep = std::move(ch);
+    return true;
+}
+
 void render_to_device(const Node& output, Rect source_bounds, const RenderRequest& request, float* dst,
                       hipStream_t stream) {
     validate_rect(request_bounds(source_bounds, request.level), request.viewport);

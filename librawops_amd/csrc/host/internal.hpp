@@ -102,6 +102,18 @@ private:
 Signature signature_of_operation(const EditOperation& op, const Signature& upstream,
                                  const std::optional<IccProfileIdentity>& output_profile);
 
+// A render that one batched demosaic launch can serve: a RAW chain compiled
+// into a single launch with no tile-cache boundary.  `keep` owns the plan.
+struct BatchPlan {
+    std::shared_ptr<const void> keep;
+    const rawops_plan* plan = nullptr;
+    const rawops_source* source = nullptr;
+};
+bool batch_plan(const Node& output, BatchPlan* out);
+// Cheap structural test (no compilation, no device work) for the same
+// property: a RAW source under built-in point nodes only, no cache.
+bool single_launch_raw_chain(const Node& output);
+
 Rect request_bounds(Rect source, RenderLevel level);
 bool compilable(const Node& n);
 
