@@ -1,11 +1,42 @@
 // rawengine SpatialOps subset for the render path (reference
-// SpatialOps.hpp:42-69): the working-Y guided filter node, run on the GPU
-// as FP64 box moments with a fused post-filter op program.
+// SpatialOps.hpp:7-69): local statistics and the working-Y guided filter
+// node, run on the GPU (FP64 moments; the filter with a fused post-filter op
+// program).
 #pragma once
 
 #include "RawEngine.hpp"
 
 namespace rawengine {
+
+class ExecutableEditGraph;
+
+// SpatialOps.hpp:7-13: local moments of a rendered tile, computed on the GPU
+// in FP64 in the reference's operation order (equal to its values bit for bit).
+struct LocalStatisticsTile {
+    Rect bounds;
+    ImageDescriptor descriptor;
+    std::uint32_t radius = 3;                  // 0..8, requested-level pixels
+    std::vector<double> mean, variance;        // interleaved RGB; population variance
+    std::vector<std::uint32_t> finite_count;   // non-finite samples excluded per channel
+};
+
+// SpatialOps.hpp:15-34.  Windows clip at the true image edge; the input tile
+// must cover local_statistics_region(output, image, radius).  Empty finite
+// windows have NaN moments.
+Rect local_statistics_region(Rect output, Rect image_bounds, std::uint32_t radius);
+LocalStatisticsTile local_statistics_rgb(const Tile& input, Rect output, Rect image_bounds, std::uint32_t radius = 3);
+using LocalStatisticsCallback = std::function<void(const LocalStatisticsTile&)>;
+// Streams one statistics tile per render tile (each rendered with its halo on
+// the device); the callback's storage is temporary.  Cancellation and errors
+// stop delivery; earlier callbacks remain.
+void analyze_local_rgb(const Node& output, Rect native_bounds, const RenderRequest& request,
+                       const LocalStatisticsCallback& callback, std::uint32_t radius = 3,
+                       const CancellationToken* cancellation = nullptr);
+void analyze_local_rgb(const ImageGraph& graph, const RenderRequest& request, const LocalStatisticsCallback& callback,
+                       std::uint32_t radius = 3, const CancellationToken* cancellation = nullptr);
+void analyze_local_rgb(const ExecutableEditGraph& graph, const RenderRequest& request,
+                       const LocalStatisticsCallback& callback, std::uint32_t radius = 3,
+                       const CancellationToken* cancellation = nullptr);
 
 struct WorkingYGuidedFilterSettings {
     std::uint32_t radius = 3;    // 0..8 (reference range); requested-level pixels

@@ -342,6 +342,21 @@ rawops_status rawops_histogram_rgb(const float* rgb, uint32_t width, uint32_t he
                                    uint32_t bins, double lower, double upper, uint64_t* counts,
                                    uint64_t* tallies, uint32_t* extrema, void* stream);
 
+/* Local mean, population variance and finite-sample count per channel
+ * (local_statistics_rgb, SpatialOps.cpp:72-121) of a device FP32 RGB tile
+ * covering `needed` = `output` grown by `radius` (0..8) and clipped to
+ * `image` (level coordinates; row stride in pixels, 0 = needed.width).
+ * Windows clip at the image edge; non-finite samples are skipped; windows
+ * without finite samples give NaN moments.  FP64 Welford row moments merged
+ * in increasing row order, the reference's order: results equal its FP64
+ * values bit for bit.  mean / variance (double) and finite_count (uint32) are
+ * output.width x output.height x 3, row-major, on the device.  Replaces
+ * the per-tile computation of analyze_local_rgb (SpatialOps.cpp:123-146). */
+rawops_status rawops_local_statistics(const float* rgb, rawops_rect needed, uint32_t stride_pixels,
+                                      rawops_rect output, rawops_rect image, uint32_t radius, double* mean,
+                                      double* variance, uint32_t* finite_count, void* stream);
+
+
 /* A stack of `images` equal-size images (no reference counterpart; for
  * callers that filter many crops of one size, such as the fit over crops):
  * the raster's height splits into `images` images of height H each,

@@ -158,6 +158,7 @@ def lib():
             "rawops_histogram_rgb": ([P, u32, u32, u32, u32, ctypes.c_double, ctypes.c_double, P, P, P, P], S),
             "rawops_guided_filter_stack": ([P, u32, Rect, u32, ctypes.c_double, P, P, u32, P, P], S),
             "rawops_guided_backward_stack": ([P, u32, u32, ctypes.c_double, Rect, P, u32, P, P, P], S),
+            "rawops_local_statistics": ([P, Rect, u32, Rect, Rect, u32, P, P, P, P], S),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -180,3 +181,20 @@ def exported_symbols():
     header = os.path.join(os.path.dirname(_HERE), "include", "rawops_hip.h")
     text = open(header).read()
     return sorted(set(re.findall(r"\b(rawops_[a-z0-9_]+)\s*\(", text)))
+
+
+def local_statistics(tile, needed, output, image, radius, stream=None):
+    """rawops_local_statistics over a device float32 tensor (h, w, 3) covering
+    ``needed``: (mean, variance, finite_count) device tensors (h, w, 3)."""
+    import torch
+    assert tile.is_cuda and tile.dtype == torch.float32 and tile.is_contiguous()
+    assert tuple(tile.shape) == (needed[3], needed[2], 3)
+    shape = (output[3], output[2], 3)
+    mean = torch.empty(shape, dtype=torch.float64, device=tile.device)
+    var = torch.empty(shape, dtype=torch.float64, device=tile.device)
+    cnt = torch.empty(shape, dtype=torch.int32, device=tile.device)
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().rawops_local_statistics(tile.data_ptr(), Rect(*needed), needed[2], Rect(*output), Rect(*image),
+                                        radius, mean.data_ptr(), var.data_ptr(), cnt.data_ptr(), s))
+    torch.cuda.synchronize()
+    return mean, var, cnt

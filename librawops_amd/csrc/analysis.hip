@@ -132,6 +132,102 @@ __global__ void __launch_bounds__(kHistThreads) histogram_rgb(const HistogramLau
     }
 }
 
+
+// ---------------------------------------------------------- local statistics --
+//
+// The reference's operation order, so FP64 results are equal bit for bit:
+// per (row, column) a Welford pass over the row's window in increasing x
+// (count, mean += delta / count, m2 += delta * (v - mean), non-finite
+// samples skipped), then per output pixel Chan's pairwise merge of those row
+// moments in increasing y.  A workgroup owns a 32 x 16 output tile: its
+// threads first form the row moments of the tile's rows grown by r into LDS
+// (each read by up to 2r + 1 outputs), then merge them.
+
+constexpr int kLsThreads = 256;
+constexpr int kLsTw = 32, kLsTh = 16, kLsMaxR = 8;
+constexpr int kLsRows = kLsTh + 2 * kLsMaxR;
+
+struct Moment {
+    uint32_t n;
+    double mean, m2;
+};
+
+__device__ __forceinline__ void moment_add(Moment& m, float v) {
+    if (!__builtin_isfinite(v)) return;
+    ++m.n;
+    const double d = (double)v - m.mean;
+    m.mean += d / (double)m.n;
+    m.m2 += d * ((double)v - m.mean);
+}
+
+__device__ __forceinline__ void moment_merge(Moment& a, const Moment& b) {
+    if (!b.n) return;
+    if (!a.n) { a = b; return; }
+    const uint32_t total = a.n + b.n;
+    const double d = b.mean - a.mean;
+    a.m2 += b.m2 + d * d * ((double)a.n * (double)b.n / (double)total);
+    a.mean += d * ((double)b.n / (double)total);
+    a.n = total;
+}
+
+__global__ void __launch_bounds__(kLsThreads) local_stats_kernel(const LocalStatsLaunch L) {
+    __shared__ uint32_t sn[kLsRows * kLsTw * 3];
+    __shared__ double smean[kLsRows * kLsTw * 3];
+    __shared__ double sm2[kLsRows * kLsTw * 3];
+    const int r = (int)L.radius;
+    const int ox0 = (int)L.output.x + (int)blockIdx.x * kLsTw;   // tile origin, level coordinates
+    const int oy0 = (int)L.output.y + (int)blockIdx.y * kLsTh;
+    const int ox_end = min(ox0 + kLsTw, (int)(L.output.x + L.output.width));
+    const int oy_end = min(oy0 + kLsTh, (int)(L.output.y + L.output.height));
+    const int ix0 = (int)L.image.x, ix1 = (int)(L.image.x + L.image.width) - 1;
+    const int iy0 = (int)L.image.y, iy1 = (int)(L.image.y + L.image.height) - 1;
+    const int row0 = max(iy0, oy0 - r), row1 = min(iy1, oy_end - 1 + r);   // rows with moments
+    const int rows = row1 - row0 + 1;
+    // Row moments: (row, column) items, channels together.
+    for (int i = threadIdx.x; i < rows * kLsTw; i += kLsThreads) {
+        const int ry = i / kLsTw, cx = i % kLsTw;
+        const int x = ox0 + cx, y = row0 + ry;
+        if (x >= ox_end) continue;
+        const int left = max(ix0, x - r), right = min(ix1, x + r);
+        const float* src = L.in + ((size_t)(y - (int)L.needed.y) * L.in_stride + (size_t)(left - (int)L.needed.x)) * 3;
+        Moment m[3] = {{0, 0.0, 0.0}, {0, 0.0, 0.0}, {0, 0.0, 0.0}};
+        for (int xx = left; xx <= right; ++xx, src += 3)
+            for (int c = 0; c < 3; ++c) moment_add(m[c], src[c]);
+        for (int c = 0; c < 3; ++c) {
+            const int k = (ry * kLsTw + cx) * 3 + c;
+            sn[k] = m[c].n;
+            smean[k] = m[c].mean;
+            sm2[k] = m[c].m2;
+        }
+    }
+    __syncthreads();
+    const double nan = __builtin_nan("");
+    for (int i = threadIdx.x; i < kLsTh * kLsTw; i += kLsThreads) {
+        const int ty = i / kLsTw, cx = i % kLsTw;
+        const int x = ox0 + cx, y = oy0 + ty;
+        if (x >= ox_end || y >= oy_end) continue;
+        const int top = max(iy0, y - r), bottom = min(iy1, y + r);
+        const size_t o = ((size_t)(y - (int)L.output.y) * L.output.width + (size_t)(x - (int)L.output.x)) * 3;
+        for (int c = 0; c < 3; ++c) {
+            Moment t = {0, 0.0, 0.0};
+            for (int yy = top; yy <= bottom; ++yy) {
+                const int k = ((yy - row0) * kLsTw + cx) * 3 + c;
+                const Moment b = {sn[k], smean[k], sm2[k]};
+                moment_merge(t, b);
+            }
+            L.count[o + c] = t.n;
+            if (t.n) {
+                const double v = t.m2 / (double)t.n;
+                L.mean[o + c] = t.mean;
+                L.variance[o + c] = 0.0 < v ? v : 0.0;   // std::max(0.0, v)
+            } else {
+                L.mean[o + c] = nan;
+                L.variance[o + c] = nan;
+            }
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_histogram(const HistogramLaunch& h, hipStream_t stream) {
@@ -146,6 +242,13 @@ hipError_t launch_histogram(const HistogramLaunch& h, hipStream_t stream) {
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>({want, 512, (size_t)h.height}));
     if (h.bins <= kLdsBins) hipLaunchKernelGGL(histogram_rgb<true>, dim3(blocks), dim3(kHistThreads), 0, stream, h);
     else hipLaunchKernelGGL(histogram_rgb<false>, dim3(blocks), dim3(kHistThreads), 0, stream, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_local_stats(const LocalStatsLaunch& L, hipStream_t stream) {
+    if (!L.output.width || !L.output.height) return hipSuccess;
+    const dim3 grid((L.output.width + kLsTw - 1) / kLsTw, (L.output.height + kLsTh - 1) / kLsTh);
+    hipLaunchKernelGGL(local_stats_kernel, grid, dim3(kLsThreads), 0, stream, L);
     return hipGetLastError();
 }
 

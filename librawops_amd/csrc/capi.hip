@@ -876,6 +876,44 @@ rawops_status rawops_histogram_rgb(const float* rgb, uint32_t width, uint32_t he
     });
 }
 
+rawops_status rawops_local_statistics(const float* rgb, rawops_rect needed, uint32_t stride_pixels,
+                                      rawops_rect output, rawops_rect image, uint32_t radius, double* mean,
+                                      double* variance, uint32_t* finite_count, void* stream) {
+    return guarded([&] {
+        if (radius > 8) fail(RAWOPS_INVALID_ARGUMENT, "local statistics radius must be in 0..8");
+        auto addressable = [](rawops_rect r) {
+            return r.width && r.height && uint64_t(r.x) + r.width <= (uint64_t{1} << 32) &&
+                   uint64_t(r.y) + r.height <= (uint64_t{1} << 32);
+        };
+        if (!addressable(output) || !addressable(image))
+            fail(RAWOPS_INVALID_ARGUMENT, "spatial operation requires nonempty addressable bounds");
+        if (output.x < image.x || output.y < image.y ||
+            uint64_t(output.x) + output.width > uint64_t(image.x) + image.width ||
+            uint64_t(output.y) + output.height > uint64_t(image.y) + image.height)
+            fail(RAWOPS_OUT_OF_RANGE, "spatial viewport is outside image bounds");
+        const uint64_t left = std::max<int64_t>(image.x, int64_t(output.x) - radius);
+        const uint64_t top = std::max<int64_t>(image.y, int64_t(output.y) - radius);
+        const uint64_t right = std::min<uint64_t>(uint64_t(image.x) + image.width, uint64_t(output.x) + output.width + radius);
+        const uint64_t bottom = std::min<uint64_t>(uint64_t(image.y) + image.height, uint64_t(output.y) + output.height + radius);
+        if (needed.x != left || needed.y != top || needed.width != right - left || needed.height != bottom - top)
+            fail(RAWOPS_INVALID_ARGUMENT, "local statistics input must cover exactly the output grown by the radius");
+        if (!stride_pixels) stride_pixels = needed.width;
+        if (stride_pixels < needed.width) fail(RAWOPS_INVALID_ARGUMENT, "image stride is smaller than its width");
+        if (!rgb || !mean || !variance || !finite_count) fail(RAWOPS_INVALID_ARGUMENT, "null argument");
+        LocalStatsLaunch L{};
+        L.in = rgb;
+        L.in_stride = stride_pixels;
+        L.needed = needed;
+        L.output = output;
+        L.image = image;
+        L.radius = radius;
+        L.mean = mean;
+        L.variance = variance;
+        L.count = finite_count;
+        hip_check(launch_local_stats(L, static_cast<hipStream_t>(stream)), "local statistics launch");
+    });
+}
+
 rawops_status rawops_guided_filter_stack(const rawops_source* raster, uint32_t images, rawops_rect viewport,
                                          uint32_t radius, double epsilon, const rawops_plan* post, float* dst,
                                          uint32_t dst_stride, uint32_t* flags, void* stream) {

@@ -17,6 +17,7 @@
 #include "../../../include/rawengine/EditGraph.hpp"
 #include "../../../include/rawengine/ImageAnalysis.hpp"
 #include "../../../include/rawengine/RawEngine.hpp"
+#include "../../../include/rawengine/SpatialOps.hpp"
 #include "../../../include/rawengine/TileScheduler.hpp"
 
 #include <hip/hip_runtime.h>
@@ -728,6 +729,50 @@ public:
         });
     }
 
+    // RawEnginePython.cpp:1792-1854: one callback per statistics tile with
+    // the moments as bytes (float64 mean / variance, uint32 finite counts).
+    py::dict analyze_local(const py::object& manifest, const py::object& callback, const py::object& options,
+                           const py::object& radius_obj) const {
+        return guarded([&] {
+            if (!PyCallable_Check(callback.ptr())) throw py::type_error("callback must be callable");
+            uint32_t radius = 3;
+            if (!radius_obj.is_none()) {
+                if (!py::isinstance<py::int_>(radius_obj) || py::isinstance<py::bool_>(radius_obj))
+                    throw py::type_error("radius must be an integer");
+                const auto n = py::cast<long long>(radius_obj);
+                if (n < 0 || n > 8) throw py::value_error("radius must be in 0..8");
+                radius = uint32_t(n);
+            }
+            auto [g, r] = prepare(options, manifest);
+            uint64_t tiles = 0, pixels = 0;
+            {
+                py::gil_scoped_release unlocked;
+                analyze_local_rgb(*g, r, [&](const LocalStatisticsTile& t) {
+                    py::gil_scoped_acquire locked;
+                    py::dict payload;
+                    payload["version"] = 1;
+                    payload["bounds"] = py::make_tuple(t.bounds.x, t.bounds.y, t.bounds.width, t.bounds.height);
+                    payload["radius"] = radius;
+                    payload["mip"] = r.level.mip;
+                    payload["quality"] = r.level.quality == RenderQuality::Preview ? "preview" : "final";
+                    payload["mean_f64"] = py::bytes(reinterpret_cast<const char*>(t.mean.data()),
+                                                    t.mean.size() * sizeof(double));
+                    payload["variance_f64"] = py::bytes(reinterpret_cast<const char*>(t.variance.data()),
+                                                        t.variance.size() * sizeof(double));
+                    payload["finite_count_u32"] = py::bytes(reinterpret_cast<const char*>(t.finite_count.data()),
+                                                            t.finite_count.size() * sizeof(uint32_t));
+                    callback(payload);
+                    ++tiles;
+                    pixels += uint64_t(t.bounds.width) * t.bounds.height;
+                }, radius);
+            }
+            py::dict d;
+            d["completed_tiles"] = tiles;
+            d["pixel_count"] = pixels;
+            return d;
+        });
+    }
+
     RenderJob submit(const py::object& options, const py::object& manifest, const std::string& priority,
                      const py::object& group_obj) {
         return guarded([&] {
@@ -992,6 +1037,12 @@ template <class T, class C> void bind_session(C cls) {
              },
              py::arg("manifest"), py::arg("options") = py::none(), py::kw_only(), py::arg("bins") = 256u,
              py::arg("lower") = 0.0, py::arg("upper") = 1.0)
+        .def("analyze_local_manifest",
+             [](const T& s, const py::object& mf, const py::object& cb, const py::object& o, const py::object& radius) {
+                 return s.analyze_local(mf, cb, o, radius);
+             },
+             py::arg("manifest"), py::arg("callback"), py::arg("options") = py::none(), py::kw_only(),
+             py::arg("radius") = py::none())
         .def("source_info", [](const T& s) { return s.source_info(); })
         .def("cache_stats", [](const T& s) { return s.cache_stats(); })
         .def("clear_cache", [](T& s) { s.clear_cache(); })

@@ -114,6 +114,22 @@ struct HistogramLaunch {
 };
 hipError_t launch_histogram(const HistogramLaunch& launch, hipStream_t stream);
 
+// Local mean / population variance / finite count per channel over
+// (2r+1)^2 windows clipped to the image (local_statistics_rgb,
+// SpatialOps.cpp:79-121).  `in` covers `needed` (output grown by r, clipped to
+// the image; row stride in pixels), outputs are output.width x output.height
+// x 3, row-major.
+struct LocalStatsLaunch {
+    const float* in;
+    uint32_t in_stride;
+    rawops_rect needed, output, image;
+    uint32_t radius;
+    double* mean;
+    double* variance;
+    uint32_t* count;
+};
+hipError_t launch_local_stats(const LocalStatsLaunch& launch, hipStream_t stream);
+
 hipError_t launch_srgb(const float* x, float* y, size_t n, hipStream_t s);
 hipError_t launch_srgb_backward(const float* x, const float* gy, float* gx, size_t n, hipStream_t s);
 hipError_t launch_exposure_backward(const float* x, const float* gy, float m, float* gx, double* g_stops, size_t n,

@@ -65,6 +65,7 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, P]
         L.ro_guided_filter_bands.argtypes = [P, Rect, Rect, Rect, ctypes.c_uint32, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, P, ctypes.c_int]
+        L.ro_local_statistics.argtypes = [P, Rect, Rect, Rect, ctypes.c_uint32, P, P, P]
         L.ro_division_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
         L.ro_division_mismatches.restype = ctypes.c_longlong
         _LIB = L
@@ -221,6 +222,19 @@ def guided_filter(inp, needed, output, image, radius, eps, space, threads=1):
     if st:
         raise OverflowError("guided filter invalid intermediate")
     return out
+
+
+def local_statistics(inp, needed, output, image, radius):
+    """local_statistics_rgb SpatialOps.cpp:79-121 on a tile covering ``needed``
+    (output grown by radius, clipped to image): (mean, variance, count), each
+    (h, w, 3)."""
+    inp = np.ascontiguousarray(inp, np.float32)
+    assert inp.shape == (needed[3], needed[2], 3)
+    shape = (output[3], output[2], 3)
+    mean, var, cnt = np.zeros(shape), np.zeros(shape), np.zeros(shape, np.uint32)
+    lib().ro_local_statistics(_p(inp), Rect(*needed), Rect(*output), Rect(*image), radius, _p(mean), _p(var),
+                              _p(cnt))
+    return mean, var, cnt
 
 
 def grow_rect(r, image, radius):

@@ -669,3 +669,66 @@ long long ro_division_mismatches(uint32_t d_lo, uint32_t d_hi, int planted) {
     if (__builtin_cpu_supports("fma")) return div_mismatches_fma(d_lo, d_hi, planted);
     return div_mismatches_libm(d_lo, d_hi, planted);
 }
+
+
+/* ---- local statistics (SpatialOps.cpp:48-121) ---- */
+
+typedef struct { uint32_t n; double mean, m2; } ro_moment;
+
+static void moment_add(ro_moment* m, float v) {   /* SpatialOps.cpp:48-55 */
+    if (!isfinite(v)) return;
+    ++m->n;
+    const double d = (double)v - m->mean;
+    m->mean += d / m->n;
+    m->m2 += d * ((double)v - m->mean);
+}
+
+static void moment_merge(ro_moment* a, const ro_moment* b) {   /* SpatialOps.cpp:56-64 */
+    if (!b->n) return;
+    if (!a->n) { *a = *b; return; }
+    const uint32_t total = a->n + b->n;
+    const double d = b->mean - a->mean;
+    a->m2 += b->m2 + d * d * ((double)a->n * b->n / total);
+    a->mean += d * ((double)b->n / total);
+    a->n = total;
+}
+
+void ro_local_statistics(const float* in, ro_rect needed, ro_rect output, ro_rect image, uint32_t radius,
+                         double* mean, double* variance, uint32_t* count) {
+    const size_t hw = output.width;
+    ro_moment* h = (ro_moment*)calloc((size_t)needed.height * hw * 3, sizeof(ro_moment));
+    for (uint32_t row = 0; row < needed.height; ++row)
+        for (uint32_t col = 0; col < output.width; ++col) {
+            const int64_t x = (int64_t)output.x + col;
+            const int64_t left = x - radius > (int64_t)image.x ? x - radius : (int64_t)image.x;
+            const int64_t lastx = (int64_t)image.x + image.width - 1;
+            const int64_t right = x + radius < lastx ? x + radius : lastx;
+            for (int64_t xx = left; xx <= right; ++xx) {
+                const float* p = in + ((size_t)row * needed.width + (size_t)(xx - needed.x)) * 3;
+                for (int c = 0; c < 3; ++c) moment_add(&h[((size_t)row * hw + col) * 3 + c], p[c]);
+            }
+        }
+    for (uint32_t row = 0; row < output.height; ++row) {
+        const int64_t y = (int64_t)output.y + row;
+        const int64_t top = y - radius > (int64_t)image.y ? y - radius : (int64_t)image.y;
+        const int64_t lasty = (int64_t)image.y + image.height - 1;
+        const int64_t bottom = y + radius < lasty ? y + radius : lasty;
+        for (uint32_t col = 0; col < output.width; ++col)
+            for (int c = 0; c < 3; ++c) {
+                ro_moment t = {0, 0.0, 0.0};
+                for (int64_t yy = top; yy <= bottom; ++yy)
+                    moment_merge(&t, &h[((size_t)(yy - needed.y) * hw + col) * 3 + c]);
+                const size_t o = ((size_t)row * output.width + col) * 3 + c;
+                count[o] = t.n;
+                if (t.n) {
+                    const double v = t.m2 / t.n;
+                    mean[o] = t.mean;
+                    variance[o] = 0.0 < v ? v : 0.0;
+                } else {
+                    mean[o] = NAN;
+                    variance[o] = NAN;
+                }
+            }
+    }
+    free(h);
+}

@@ -115,6 +115,14 @@ int ro_guided_filter_bands(const float* input, ro_rect needed, ro_rect output, r
                            uint32_t radius, double epsilon, double wr, double wb, float* out,
                            int threads);
 
+/* local_statistics_rgb SpatialOps.cpp:79-121: `in` covers `needed` (output
+ * grown by radius, clipped to image); per output pixel and channel the finite
+ * count, mean and population variance (NaN moments for an empty window).
+ * Row moments by Welford's update in increasing x, merged by Chan's pairwise
+ * rule in increasing y. */
+void ro_local_statistics(const float* in, ro_rect needed, ro_rect output, ro_rect image, uint32_t radius,
+                         double* mean, double* variance, uint32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,3 +128,16 @@ def exposure_multiplier(stops):
 
 __all__ = ["available", "render_raw", "guided_chain", "matrices", "exposure_multiplier", "time_menon_bands",
            "Bayer", "Chain", "request_bounds"]
+
+
+def local_statistics(inp, needed, output, image, radius):
+    """The compiled reference's local_statistics_rgb on the same tile."""
+    L = lib()
+    f = L.refh_local_statistics
+    P = ctypes.c_void_p
+    f.argtypes = [P, Rect, Rect, Rect, ctypes.c_uint32, P, P, P]
+    inp = np.ascontiguousarray(inp, np.float32)
+    shape = (output[3], output[2], 3)
+    mean, var, cnt = np.zeros(shape), np.zeros(shape), np.zeros(shape, np.uint32)
+    _check(f(_p(inp), Rect(*needed), Rect(*output), Rect(*image), radius, _p(mean), _p(var), _p(cnt)))
+    return mean, var, cnt

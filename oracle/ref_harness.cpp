@@ -278,4 +278,18 @@ int refh_legacy_snapshot(int raw, const uint8_t digest[32], const char* source_i
     });
 }
 
+// local_statistics_rgb on a tile covering `needed` (camera-linear descriptor).
+int refh_local_statistics(const float* in, ro_rect needed, ro_rect output, ro_rect image, uint32_t radius,
+                          double* mean, double* variance, uint32_t* count) {
+    return guarded([&] {
+        Tile t{Rect{needed.x, needed.y, needed.width, needed.height},
+               std::vector<float>(in, in + size_t(needed.width) * needed.height * 3), ImageDescriptor::camera_linear()};
+        const auto r = local_statistics_rgb(t, Rect{output.x, output.y, output.width, output.height},
+                                            Rect{image.x, image.y, image.width, image.height}, radius);
+        std::memcpy(mean, r.mean.data(), r.mean.size() * sizeof(double));
+        std::memcpy(variance, r.variance.data(), r.variance.size() * sizeof(double));
+        std::memcpy(count, r.finite_count.data(), r.finite_count.size() * sizeof(uint32_t));
+    });
+}
+
 } // extern "C"

@@ -466,6 +466,124 @@ static void edit_graph_checks() {
     }
 }
 
+// ------------------------------------------------------- local statistics --
+
+// R = x - 12, G = y - 10, B = 2 (or NaN): the reference test's probe source.
+class AffineSource final : public Node {
+public:
+    bool malformed = false, nonfinite = false;
+    mutable Rect last{};
+    ImageDescriptor output_descriptor() const noexcept override {
+        return ImageDescriptor::scene_linear(WorkingSpace::LinearProPhotoD50);
+    }
+    Tile render(Rect r) const override {
+        last = r;
+        Tile t{r, {}, output_descriptor()};
+        for (uint32_t y = 0; y < r.height; ++y)
+            for (uint32_t x = 0; x < r.width; ++x) {
+                t.rgb.push_back(float(int64_t(r.x) + x - 12));
+                t.rgb.push_back(float(int64_t(r.y) + y - 10));
+                t.rgb.push_back(nonfinite ? std::nanf("") : 2.0f);
+            }
+        if (malformed) t.rgb.pop_back();
+        return t;
+    }
+};
+
+static LocalStatisticsTile collect(const Node& node, Rect image, uint32_t size, unsigned radius) {
+    const size_t n = size_t(image.width) * image.height * 3;
+    LocalStatisticsTile out{image, node.output_descriptor(), radius, std::vector<double>(n), std::vector<double>(n),
+                            std::vector<uint32_t>(n)};
+    analyze_local_rgb(node, image, RenderRequest{image, size}, [&](const LocalStatisticsTile& t) {
+        for (uint32_t y = 0; y < t.bounds.height; ++y)
+            for (uint32_t x = 0; x < t.bounds.width; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    const size_t src = (size_t(y) * t.bounds.width + x) * 3 + c;
+                    const size_t dst = ((size_t(t.bounds.y - image.y) + y) * image.width + t.bounds.x - image.x + x) * 3 + c;
+                    out.mean[dst] = t.mean[src];
+                    out.variance[dst] = t.variance[src];
+                    out.finite_count[dst] = t.finite_count[src];
+                }
+    }, radius);
+    return out;
+}
+
+static bool same_doubles(const std::vector<double>& a, const std::vector<double>& b) {
+    return a.size() == b.size() && !std::memcmp(a.data(), b.data(), a.size() * sizeof(double));
+}
+
+static void local_statistics_checks() {
+    AffineSource source;
+    const Rect image{10, 7, 7, 5};
+    bool truth = true, tiles = true;
+    for (unsigned radius : {0u, 1u, 3u, 8u}) {
+        const auto full = collect(source, image, 256, radius);
+        for (unsigned y = 0; y < 5; ++y)
+            for (unsigned x = 0; x < 7; ++x) {
+                const unsigned left = x > radius ? x - radius : 0, right = std::min(6u, x + radius);
+                const unsigned top = y > radius ? y - radius : 0, bottom = std::min(4u, y + radius);
+                const size_t i = (size_t(y) * 7 + x) * 3;
+                const double nx = right - left + 1, ny = bottom - top + 1;
+                truth = truth && std::abs(full.mean[i] - (double(left + right) / 2 - 2)) < 1e-12 &&
+                        std::abs(full.mean[i + 1] - (double(top + bottom) / 2 - 3)) < 1e-12 &&
+                        std::abs(full.variance[i] - (nx * nx - 1) / 12) < 1e-12 &&
+                        std::abs(full.variance[i + 1] - (ny * ny - 1) / 12) < 1e-12 && full.mean[i + 2] == 2 &&
+                        full.variance[i + 2] == 0 && full.finite_count[i] == nx * ny;
+            }
+        for (unsigned size : {1u, 2u, 4u}) {
+            const auto tiled = collect(source, image, size, radius);
+            tiles = tiles && same_doubles(tiled.mean, full.mean) && same_doubles(tiled.variance, full.variance) &&
+                    tiled.finite_count == full.finite_count;
+        }
+    }
+    CHECK(truth, "local statistics: affine means/variances and window counts (spatial_ops_tests.cpp:49-62)");
+    CHECK(tiles, "local statistics: tile sizes 1/2/4 equal one 256 tile bit for bit");
+    bool halo = false;
+    analyze_local_rgb(source, image, RenderRequest{{12, 9, 1, 1}, 1}, [&](const LocalStatisticsTile& t) {
+        halo = t.finite_count[0] == 9 && t.mean[0] == 0 && std::abs(t.variance[0] - 2.0 / 3) < 1e-12;
+    }, 1);
+    CHECK(halo && source.last.width == 3 && source.last.height == 3, "local statistics: interior ROI reads its halo");
+    source.nonfinite = true;
+    const auto missing = collect(source, image, 3, 1);
+    bool empty = true;
+    for (size_t i = 2; i < missing.mean.size(); i += 3)
+        empty = empty && missing.finite_count[i] == 0 && std::isnan(missing.mean[i]) && std::isnan(missing.variance[i]);
+    source.nonfinite = false;
+    CHECK(empty, "local statistics: windows without finite samples have NaN moments");
+    bool rejected = throws_as([&] { local_statistics_region(image, image, 9); }, 2) &&
+                    throws_as([&] { analyze_local_rgb(source, image, RenderRequest{image, 0}, [](const auto&) {}); }, 2) &&
+                    throws_as([&] { analyze_local_rgb(source, image, RenderRequest{image}, {}); }, 2) &&
+                    throws_as([&] { local_statistics_region({9, 7, 1, 1}, image, 1); }, 1);
+    source.malformed = true;
+    rejected = rejected && throws_as([&] { collect(source, image, 3, 1); }, 2);
+    source.malformed = false;
+    CancellationToken pre;
+    pre.cancel();
+    rejected = rejected && throws_as([&] { analyze_local_rgb(source, image, RenderRequest{image}, [](const auto&) {}, 1, &pre); }, 4);
+    CancellationToken last;
+    unsigned callbacks = 0;
+    rejected = rejected && throws_as([&] {
+        analyze_local_rgb(source, image, RenderRequest{image}, [&](const auto&) { ++callbacks; last.cancel(); }, 1, &last);
+    }, 4) && callbacks == 1;
+    CHECK(rejected, "local statistics: validation and cancellation (spatial_ops_tests.cpp:124-138)");
+    // A device chain: Menon render with its halo, moments on the device, equal to the oracle.
+    RawMetadata m;
+    m.width = 160; m.height = 120; m.pattern = BayerPattern::GBRG; m.black_levels.fill(256); m.white_levels.fill(4095);
+    auto s = random_samples(160 * 120, 13, 4095);
+    ImageGraph g(RawImage(m, s), GraphRecipe{}, RawDemosaicIdentity{"rawengine.menon_base", 1}, true);
+    const Rect img{0, 0, 160, 120};
+    const auto got = collect(g.output(), img, 48, 3);
+    ro_bayer b = oracle_meta(m);
+    std::vector<float> lin(160 * 120 * 3);
+    ro_demosaic_menon(&b, s.data(), {0, 0, 160, 120}, lin.data());
+    std::vector<double> mean(lin.size()), var(lin.size());
+    std::vector<uint32_t> cnt(lin.size());
+    ro_local_statistics(lin.data(), {0, 0, 160, 120}, {0, 0, 160, 120}, {0, 0, 160, 120}, 3, mean.data(), var.data(),
+                        cnt.data());
+    CHECK(same_doubles(got.mean, mean) && same_doubles(got.variance, var) && got.finite_count == cnt,
+          "analyze_local_rgb(ImageGraph Menon, 48-pixel tiles, r=3) == oracle bit for bit");
+}
+
 int main() {
     std::setvbuf(stdout, nullptr, _IOLBF, 0);  // a line per check, also when a later check aborts
     const std::array<double, 9> identity{1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -576,6 +694,7 @@ int main() {
     }
     histogram_checks();
     edit_graph_checks();
+    local_statistics_checks();
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }

@@ -117,6 +117,15 @@ def test_raw_session_recipe_manifest_and_cache():
     assert hist["pixel_count"] == 200 * 120
     for c in hist["channels"]:
         assert sum(c["counts"]) + c["underflow"] + c["overflow"] + c["nonfinite"] == 200 * 120
+    stats = []
+    done = session.analyze_local_manifest(manifest, stats.append, dict(roi, tile_size=64), radius=2)
+    assert done == {"completed_tiles": 8, "pixel_count": 200 * 120}
+    first = stats[0]
+    assert first["bounds"] == (20, 15, 64, 64) and first["radius"] == 2
+    mean = np.frombuffer(first["mean_f64"], np.float64).reshape(64, 64, 3)
+    lin = as_image(session.render_manifest(manifest, dict(x=18, y=13, roi_width=68, roi_height=68)))
+    exp_mean, _, _ = O.local_statistics(lin, (18, 13, 68, 68), (20, 15, 64, 64), (5, 3, 288, 190), 2)
+    assert mean.tobytes() == exp_mean.tobytes()
     with pytest.raises(ValueError):
         session.render_manifest(manifest, dict(exposure_stops=1.0))   # edits belong in the manifest
     with pytest.raises(ValueError):
