@@ -386,6 +386,81 @@ def leg_c5_chain(steps, seed=5):
             "gradient_check": "<=1e-4 relative vs FP64 finite differences (tests/test_autograd_gpu.py)"}
 
 
+def leg_dropin(steps):
+    """The drop-in entry points on the config-2 frame: a saved manifest
+    (Menon -> camera->ProPhoto) rendered through rawengine_native.RawSession
+    (C++ ExecutableEditGraph -> device launch) into host bytes (includes the
+    288 MB device-to-host copy over PCIe), into a DLPack device result, and
+    again from a warm device tile cache; plus the stream scheduler serving 64
+    queued 512x512 viewports."""
+    import numpy as np
+    import torch
+    from librawops_amd import rawengine_native as rn
+    from librawops_amd import synth
+
+    meta, samples, cam = synth.config2_menon(seed=2)
+    ax, ay, aw, ah = meta["active_area"]
+    md = dict(row_stride_samples=meta["row_stride_samples"], pattern=meta["pattern"], cfa_phase_x=meta["cfa_phase_x"],
+              cfa_phase_y=meta["cfa_phase_y"], active_x=ax, active_y=ay, active_width=aw, active_height=ah,
+              black_levels=list(meta["black_levels"]), white_levels=list(meta["white_levels"]))
+    menon = {"algorithm": "rawengine.menon_base", "processing_version": 1}
+    session = rn.RawSession(samples, meta["width"], meta["height"], md, cache_bytes=1 << 30, workers=4,
+                            max_pending=64, demosaic=menon)
+    info = session.source_info()
+    src = "00000000-0000-0000-0000-000000000001"
+    op = "00000000-0000-0000-0000-000000000002"
+    manifest = json.dumps({
+        "format_version": 3, "processing_version": 2, "working_space": "linear_prophoto_d50", "output": op,
+        "sources": [{"id": src, "kind": "decoded_bayer_u16", "content_sha256": info["content_sha256"],
+                     "demosaic": menon}],
+        "operations": [{"id": op, "type": "rawengine.camera_to_working", "schema_version": 1,
+                        "processing_version": 2, "enabled": True, "input_domain": "camera_linear",
+                        "output_domain": "scene_linear_prophoto_d50",
+                        "parameters": {"matrix": list(synth.camera_to_xyz_for(cam))}, "inputs": {"image": src},
+                        "masks": {}, "blend_mode": "normal", "opacity": 1.0}]})
+    px = aw * ah
+
+    def rate(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / n
+        return round(px / dt / 1e6, 1), round(dt * 1e3, 3)
+
+    host_rate, host_ms = rate(lambda: session.render_manifest(manifest), max(2, steps // 4))
+    session.clear_cache()
+    cold = session.cache_stats()
+
+    def device():
+        session.clear_cache()
+        return torch.from_dlpack(session.render_manifest_dlpack(manifest)[2])
+    dev_rate, dev_ms = rate(device, steps)
+    warm_rate, warm_ms = rate(lambda: torch.from_dlpack(session.render_manifest_dlpack(manifest)[2]), steps)
+    stats = session.cache_stats()
+    # 64 queued 512^2 viewports through the session's scheduler (4 workers, 4 streams).
+    rng = np.random.default_rng(9)
+    views = [dict(x=int(ax + rng.integers(0, aw - 512)), y=int(ay + rng.integers(0, ah - 512)), roi_width=512,
+                  roi_height=512) for _ in range(64)]
+    session.clear_cache()
+    t = time.perf_counter()
+    jobs = [session.submit_manifest(manifest, v) for v in views]
+    for j in jobs:
+        j.result(timeout=120)
+    sched_s = time.perf_counter() - t
+    session.close()
+    return {"workload": "saved manifest (Menon -> camera->ProPhoto) on the 6016x3992 active area through "
+                        "rawengine_native.RawSession (C++ ExecutableEditGraph, device TileCache)",
+            "host_bytes_mpix_s": host_rate, "host_bytes_ms": host_ms,
+            "device_dlpack_mpix_s": dev_rate, "device_dlpack_ms": dev_ms,
+            "warm_cache_dlpack_mpix_s": warm_rate, "warm_cache_dlpack_ms": warm_ms,
+            "cache_entries_after_warm": stats["entries"], "cache_hits": stats["hits"] - cold["hits"],
+            "scheduler_64x512sq_ms": round(sched_s * 1e3, 2),
+            "scheduler_64x512sq_mpix_s": round(64 * 512 * 512 / sched_s / 1e6, 1)}
+
+
 def compiled_matrix(node):
     """float32 camera->working coefficients the CameraToWorkingNode uses."""
     return node.matrix
@@ -405,7 +480,7 @@ def main():
     ap.add_argument("--menon-path", type=int, default=0,
                     help="Menon render path (rawops_set_menon_path): 0 automatic, 1 streaming, 2 tiles")
     ap.add_argument("--stream-band", type=int, default=0, help="rows per streaming wavefront (0: 16)")
-    ap.add_argument("--leg", choices=("c1", "c3", "c4", "c5"), default=None,
+    ap.add_argument("--leg", choices=("c1", "c3", "c4", "c5", "dropin"), default=None,
                     help="run only this configuration's leg and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (reference for rocprofv3 FETCH/WRITE_SIZE)")
@@ -437,7 +512,8 @@ def main():
     C.check(C.lib().rawops_set_menon_path(args.menon_path, args.stream_band))
     if args.leg:
         fn = {"c1": lambda: leg_c1_bilinear(args.steps), "c3": lambda: leg_c3_guided(args.steps, cpu_leg=not args.no_cpu),
-              "c4": lambda: leg_c4_batch(args.steps), "c5": lambda: leg_c5_chain(args.c5_steps)}[args.leg]
+              "c4": lambda: leg_c4_batch(args.steps), "c5": lambda: leg_c5_chain(args.c5_steps),
+              "dropin": lambda: leg_dropin(args.steps)}[args.leg]
         print(json.dumps({"leg": args.leg, **fn()}))
         return
     meta, samples, cam = synth.config2_menon(seed=2)
@@ -550,7 +626,8 @@ def main():
     # ---- the other configurations
     if not args.main_only:
         legs = (("c1_bilinear", lambda: leg_c1_bilinear(20)), ("c3_guided", lambda: leg_c3_guided(6)),
-                ("c4_batch", lambda: leg_c4_batch(5)), ("c5_chain", lambda: leg_c5_chain(args.c5_steps)))
+                ("c4_batch", lambda: leg_c4_batch(5)), ("c5_chain", lambda: leg_c5_chain(args.c5_steps)),
+                ("dropin", lambda: leg_dropin(8)))
         for name, fn in legs:
             print(f"bench: {name}", file=sys.stderr, flush=True)
             t = time.perf_counter()

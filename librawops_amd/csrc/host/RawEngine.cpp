@@ -83,6 +83,21 @@ hipStream_t thread_stream() {
     return this_thread_stream.s;
 }
 
+DeviceBuffer::DeviceBuffer(size_t bytes, hipStream_t s) : stream(s), pooled(true) {
+    // Keep released blocks in the pool instead of trimming it at every
+    // synchronisation (once per device).
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    });
+    if (bytes) hip(hipMallocAsync(&p, bytes, s), "hipMallocAsync");
+}
+
 DeviceTile::DeviceTile(std::uint32_t w, std::uint32_t h, ImageDescriptor d) : width(w), height(h), descriptor(d) {
     if (floats()) hip(hipMalloc(&ptr, floats() * sizeof(float)), "hipMalloc(tile)");
 }
@@ -463,7 +478,7 @@ void render_to_device(const Node& output, Rect source_bounds, const RenderReques
     if (!output.supports_level(request.level)) throw std::invalid_argument("node does not support this render level");
     if (!request.viewport.width || !request.viewport.height) return;
     const auto chain = compile(output);
-    DeviceBuffer flags(sizeof(uint32_t));
+    DeviceBuffer flags(sizeof(uint32_t), stream);
     hip(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), stream), "hipMemsetAsync(flags)");
     run(*chain, request.viewport, request.level, dst, static_cast<uint32_t*>(flags.p), stream);
     check(rawops_check_flags(static_cast<uint32_t*>(flags.p), stream));
@@ -1026,8 +1041,8 @@ void Renderer::render_tiles(const Node& output, Rect source_bounds, RenderReques
     const auto chain = detail::compile(output);
     const hipStream_t stream = detail::thread_stream();
     const uint32_t band_h = std::min<uint32_t>(request.tile_size, v.height);
-    detail::DeviceBuffer dev(size_t(v.width) * band_h * 3 * sizeof(float));
-    detail::DeviceBuffer flags(sizeof(uint32_t));
+    detail::DeviceBuffer dev(size_t(v.width) * band_h * 3 * sizeof(float), stream);
+    detail::DeviceBuffer flags(sizeof(uint32_t), stream);
     float* host = nullptr;
     detail::hip(hipHostMalloc(reinterpret_cast<void**>(&host), size_t(v.width) * band_h * 3 * sizeof(float),
                               hipHostMallocDefault), "hipHostMalloc");
@@ -1088,8 +1103,8 @@ Tile Renderer::render_image(const Node& output, Rect source_bounds, RenderReques
     }
     const auto chain = detail::compile(output);
     const hipStream_t stream = detail::thread_stream();
-    detail::DeviceBuffer dev(image.rgb.size() * sizeof(float));
-    detail::DeviceBuffer flags(sizeof(uint32_t));
+    detail::DeviceBuffer dev(image.rgb.size() * sizeof(float), stream);
+    detail::DeviceBuffer flags(sizeof(uint32_t), stream);
     detail::hip(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), stream), "hipMemsetAsync");
     detail::run(*chain, v, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), stream);
     detail::hip(hipMemcpyAsync(image.rgb.data(), dev.p, image.rgb.size() * sizeof(float), hipMemcpyDeviceToHost,

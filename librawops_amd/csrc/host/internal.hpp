@@ -30,10 +30,20 @@ void hip(hipError_t e, const char* what);
 // workers, concurrent callers) run on different streams.
 hipStream_t thread_stream();
 
+// Device scratch.  With a stream, the memory comes from the device's
+// stream-ordered pool (allocation and release ordered on that stream, kept in
+// the pool between renders: no device-wide synchronisation per render).
 struct DeviceBuffer {
     void* p = nullptr;
+    hipStream_t stream = nullptr;
+    bool pooled = false;
     explicit DeviceBuffer(size_t bytes) { if (bytes) hip(hipMalloc(&p, bytes), "hipMalloc"); }
-    ~DeviceBuffer() { if (p) (void)hipFree(p); }
+    DeviceBuffer(size_t bytes, hipStream_t s);
+    ~DeviceBuffer() {
+        if (!p) return;
+        if (pooled) (void)hipFreeAsync(p, stream);
+        else (void)hipFree(p);
+    }
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
 };

@@ -19,6 +19,7 @@
 #include "../../../include/rawengine/RawEngine.hpp"
 #include "../../../include/rawengine/SpatialOps.hpp"
 #include "../../../include/rawengine/TileScheduler.hpp"
+#include "internal.hpp"
 
 #include <hip/hip_runtime.h>
 #include <pybind11/pybind11.h>
@@ -217,12 +218,13 @@ py::tuple render_capsule(const Node& output, Rect source_bounds, const RenderReq
     {
         py::gil_scoped_release unlocked;
         if (hipGetDevice(&device) != hipSuccess) throw std::runtime_error("no HIP device");
-        if (bytes && hipMalloc(&r->memory, bytes) != hipSuccess) throw std::bad_alloc();
+        const hipStream_t stream = detail::thread_stream();
+        // From the stream-ordered pool; the capsule's deleter returns it.
+        if (bytes && hipMallocAsync(&r->memory, bytes, stream) != hipSuccess) throw std::bad_alloc();
         try {
-            Renderer{}.render_device(output, source_bounds, request, static_cast<float*>(r->memory), nullptr);
-            if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("device synchronisation failed");
+            detail::render_to_device(output, source_bounds, request, static_cast<float*>(r->memory), stream);
         } catch (...) {
-            if (r->memory) (void)hipFree(r->memory);
+            if (r->memory) (void)hipFreeAsync(r->memory, stream);
             r->memory = nullptr;
             throw;
         }
