@@ -420,6 +420,140 @@ __device__ __forceinline__ void run_sums(const double* __restrict__ b, int i0, d
     }
 }
 
+// Box sums for a compile-time radius R (1..11) without per-row prefix scans.
+// The strip is kRunCols = 256 pixel columns x = strip_x - R + t; thread t
+// keeps the vertical window sums (rows k-R..k+R, clipped) of the N values of
+// its column, slid down one row per step, with the row data of the next step
+// loaded a step ahead (fetch(x, y) returns it, eval(data, x, y, v) turns it
+// into the N values).  Per row:
+//   1. thread t writes its N sums into slot t of N LDS rows zero-padded by R;
+//   2. thread t = 16 m + j (m < N) forms the sums along x of value m for the
+//      16 centres 16 j .. 16 j + 15 as sliding sums (2R + 16 reads) and, once
+//      every lane of its wavefront has read (a wavefront fence orders the
+//      reads before the writes), writes the sum of centre c back into slot c
+//      of row m; a wavefront touches only the rows of its own values;
+//   3. thread t, the centre x = strip_x - R + t for t in [R, 256 - R), reads
+//      its N box sums from slot t and calls sink(data, x, k, S, cols, rows),
+//      data being what centre(x, k) loaded a step earlier, cols x rows the
+//      clipped window.
+// Slot t is written in step 1 and read in step 3 only by thread t, so two
+// barriers per row suffice.  A row holds logical index i = pixel + R at
+// run_slot(i) (one pad double per 16, so the 16 lanes of a value read
+// distinct banks); rows are plane doubles apart, plane = 16 mod 32, so the 4
+// values of a wavefront spread over the banks as well.  Windows are clipped
+// to [0, LW) x [0, LH); only columns in [c_lo, c_hi) enter the sums; fetch
+// and centre are called at clamped in-bounds coordinates, so their loads are
+// unconditional.
+constexpr int kRunCols = 256;
+constexpr int kRunThreads = 256;
+constexpr int kRunLen = 16;   // centres per lane in the sums along x
+
+__device__ __forceinline__ int run_slot(int i) { return i + (i >> 4); }
+
+template <int R> struct RunGeom {
+    static constexpr int logical = kRunCols + 2 * R;                      // zero-padded row
+    static constexpr int used = logical + (logical >> 4) + 1;
+    static constexpr int plane = ((used + 15) / 32) * 32 + 16;            // >= used, = 16 mod 32
+    static_assert(plane >= used, "run plane");
+};
+
+// s[t] = sum_{i=0}^{2R} b[run_slot(i0 + t + i)], t < kRunLen, as sliding sums
+// (two partial sums for the first window, so the adds overlap).
+template <int R>
+__device__ __forceinline__ void run_sums16(const double* __restrict__ b, int i0, double s[kRunLen]) {
+    double e = 0.0, o = 0.0;
+#pragma unroll
+    for (int i = 0; i <= 2 * R; i += 2) e += b[run_slot(i0 + i)];
+#pragma unroll
+    for (int i = 1; i <= 2 * R; i += 2) o += b[run_slot(i0 + i)];
+    double acc = e + o;
+    s[0] = acc;
+#pragma unroll
+    for (int t = 1; t < kRunLen; ++t) {
+        acc = acc + b[run_slot(i0 + 2 * R + t)] - b[run_slot(i0 + t - 1)];
+        s[t] = acc;
+    }
+}
+
+template <int N, int R, class Fetch, class Eval, class Centre, class Sink>
+__device__ __forceinline__ void strip_runs(int LW, int LH, int c_lo, int c_hi, int strip_x, int xend, int y0, int y1,
+                                           double* rows /* [N][plane] */, Fetch fetch, Eval eval, Centre centre,
+                                           Sink sink) {
+    static_assert(N * (kRunCols / kRunLen) <= kRunThreads, "one sum run per thread");
+    constexpr int P = RunGeom<R>::plane;
+    const int t = threadIdx.x;
+    const int x = strip_x - R + t;
+    const bool own = x >= c_lo && x < c_hi;
+    const int xc = min(max(x, 0), LW - 1);
+    const int slot = run_slot(t + R);
+    const int vm = t / (kRunCols / kRunLen), vj = t % (kRunCols / kRunLen);   // value, run of the sums along x
+    auto row_of = [&](int y) { return min(max(y, 0), LH - 1); };
+    for (int i = t; i < N * P; i += kRunThreads) rows[i] = 0.0;
+    double V[N];
+#pragma unroll
+    for (int m = 0; m < N; ++m) V[m] = 0.0;
+    if (own) {
+        const int ya = max(0, y0 - R), yb = min(LH, y0 + R + 1);
+        for (int y = ya; y < yb; ++y) {
+            const auto d = fetch(xc, y);
+            double v[N];
+            eval(d, x, y, v);
+#pragma unroll
+            for (int m = 0; m < N; ++m) V[m] += v[m];
+        }
+    }
+    auto en = fetch(xc, row_of(y0 + 1 + R));
+    auto lv = fetch(xc, row_of(y0 - R));
+    const bool is_centre = t >= R && t < kRunCols - R && x < xend;
+    auto cd = centre(xc, row_of(y0));
+    __syncthreads();   // pads zeroed before the first row is written
+    for (int k = y0; k < y1; ++k) {
+        if (k > y0) {
+            if (own) {
+                double v[N];
+                if (k + R < LH) {
+                    eval(en, x, k + R, v);
+#pragma unroll
+                    for (int m = 0; m < N; ++m) V[m] += v[m];
+                }
+                if (k - R - 1 >= 0) {
+                    eval(lv, x, k - R - 1, v);
+#pragma unroll
+                    for (int m = 0; m < N; ++m) V[m] -= v[m];
+                }
+            }
+            en = fetch(xc, row_of(k + 1 + R));
+            lv = fetch(xc, row_of(k - R));
+        }
+#pragma unroll
+        for (int m = 0; m < N; ++m) rows[m * P + slot] = V[m];
+        __syncthreads();
+        if (vm < N) {
+            double* b = rows + vm * P;
+            double Hs[kRunLen];
+            run_sums16<R>(b, kRunLen * vj, Hs);
+            // Every lane's reads of the wavefront's rows are done before any
+            // lane overwrites them.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int u = 0; u < kRunLen; ++u) b[run_slot(kRunLen * vj + u + R)] = Hs[u];
+        }
+        __syncthreads();
+        const auto cur = cd;
+        if (k + 1 < y1) cd = centre(xc, row_of(k + 1));
+        if (is_centre) {
+            double S[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m) S[m] = rows[m * P + slot];
+            const int cols = min(LW, x + R + 1) - max(0, x - R);
+            const int rws = min(LH, k + R + 1) - max(0, k - R);
+            sink(cur, x, k, S, cols, rws);
+        }
+    }
+}
+
 template <int R>
 __global__ void __launch_bounds__(kFusedThreads) gf_fused(const GuidedArgs g0) {
     const GuidedArgs g = slice_of(g0);
@@ -827,6 +961,179 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBack
     strip_box<11>(g.W, g.H, 0, g.W, g.r, strip_x, g.W, y0, y1, prefix, totals, src, sink);
 }
 
+// The same two passes at a compile-time radius R (1..kRunMaxR) over
+// strip_runs: the same per-pixel values and per-centre formulas, box sums
+// formed as sliding sums instead of prefix differences; 1 / n is a constant
+// for unclipped windows and a division only where the window is clipped.
+constexpr int kRunMaxR = 11;
+
+template <int R> struct RunWindow {
+    static constexpr int side = 2 * R + 1;
+    static constexpr double inv_full = 1.0 / (double)(side * side);
+};
+
+template <int R>
+__device__ __forceinline__ double inv_count(int cols, int rws) {
+    if (cols == RunWindow<R>::side && rws == RunWindow<R>::side) return RunWindow<R>::inv_full;
+    return 1.0 / (double)(cols * rws);
+}
+
+struct BackPixel {
+    float p[3];
+    float d[3];   // dL/dq at the pixel, clamped into the viewport (used only inside it)
+};
+
+__device__ __forceinline__ BackPixel back_pixel(const GuidedBackArgs& g, int x, int y, bool have_view) {
+    BackPixel o;
+    const float* q = g.p + ((size_t)y * g.stride + (size_t)x) * 3;
+    o.p[0] = q[0]; o.p[1] = q[1]; o.p[2] = q[2];
+    if (have_view) {
+        const int lx = min(max(x - g.vx, 0), g.vw - 1), ly = min(max(y - g.vy, 0), g.vh - 1);
+        const float* d = g.gq + ((size_t)ly * g.gstride + (size_t)lx) * 3;
+        o.d[0] = d[0]; o.d[1] = d[1]; o.d[2] = d[2];
+    } else {
+        o.d[0] = o.d[1] = o.d[2] = 0.0f;
+    }
+    return o;
+}
+
+template <int R>
+__global__ void __launch_bounds__(kRunThreads, 3) gf_back_centres_runs(const GuidedBackArgs g0) {
+    const GuidedBackArgs g = slice_of(g0);
+    __shared__ double rows[14 * RunGeom<R>::plane];
+    __shared__ double eps_part[kRunThreads / 64];
+    const int span = kRunCols - 2 * R;
+    const int strip_x = (int)blockIdx.x * span;
+    const int y0 = (int)blockIdx.y * g.band, y1 = min(y0 + g.band, g.H);
+    const bool have_view = g.vw > 0 && g.vh > 0;
+    double geps = 0.0;
+    auto fetch = [&](int x, int y) { return back_pixel(g, x, y, have_view); };
+    auto eval = [&](const BackPixel& o, int x, int y, double v[14]) {
+        const double pr = o.p[0], pg = o.p[1], pb = o.p[2];
+        const double Y = (pg + g.wr * (pr - pg)) + g.wb * (pb - pg);
+        v[0] = Y; v[1] = pr; v[2] = pg; v[3] = pb;
+        v[4] = Y * Y; v[5] = Y * pr; v[6] = Y * pg; v[7] = Y * pb;
+        const int lx = x - g.vx, ly = y - g.vy;
+        if (lx >= 0 && ly >= 0 && lx < g.vw && ly < g.vh) {
+            const double inv_n = inv_count<R>(min(g.W, x + R + 1) - max(0, x - R), min(g.H, y + R + 1) - max(0, y - R));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { v[8 + c] = (double)o.d[c] * Y * inv_n; v[11 + c] = (double)o.d[c] * inv_n; }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v[8 + c] = 0.0;
+        }
+    };
+    auto centre = [&](int, int) { return 0; };
+    const size_t plane = (size_t)g.W * g.H;
+    auto sink = [&](int, int x, int y, const double S[14], int cols, int rws) {
+        const double inv_n = inv_count<R>(cols, rws);
+        const double my = S[0] * inv_n, eyy = S[4] * inv_n;
+        const double var = eyy - my * my, D = (var < 0 ? 0 : var) + g.eps;
+        const double inv_D = 1.0 / D;
+        double a[3], mp[3], ga[3], gmp[3], gcov[3];
+        double gmy = 0, gD = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            mp[c] = S[1 + c] * inv_n;
+            const double cov = S[5 + c] * inv_n - my * mp[c];
+            a[c] = cov * inv_D;
+            const double GA = S[8 + c], GB = S[11 + c];
+            gmp[c] = GB;
+            ga[c] = GA - my * GB;
+            gmy -= a[c] * GB;
+            gcov[c] = ga[c] * inv_D;
+            gD -= ga[c] * a[c] * inv_D;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gmy -= gcov[c] * mp[c];
+            gmp[c] -= gcov[c] * my;
+        }
+        gmy -= 2.0 * my * gD;
+        geps += gD;
+        float* w = g.ws + (size_t)y * g.W + (size_t)x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            w[c * plane] = (float)(gmp[c] * inv_n);
+            w[(3 + c) * plane] = (float)(gcov[c] * inv_n);
+            w[(8 + c) * plane] = (float)a[c];
+        }
+        w[6 * plane] = (float)(gmy * inv_n);
+        w[7 * plane] = (float)(gD * inv_n);
+    };
+    strip_runs<14, R>(g.W, g.H, 0, g.W, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
+    for (int off = 32; off > 0; off >>= 1) geps += __shfl_xor(geps, off, 64);
+    if ((threadIdx.x & 63) == 0) eps_part[threadIdx.x >> 6] = geps;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kRunThreads / 64; ++w) t += eps_part[w];
+        atomicAdd(g.geps, t);
+    }
+}
+
+struct BackTerms {
+    float w[11];
+};
+
+template <int R>
+__global__ void __launch_bounds__(kRunThreads, 4) gf_back_pixels_runs(const GuidedBackArgs g0) {
+    const GuidedBackArgs g = slice_of(g0);
+    __shared__ double rows[11 * RunGeom<R>::plane];
+    const int span = kRunCols - 2 * R;
+    const int strip_x = (int)blockIdx.x * span;
+    const int y0 = (int)blockIdx.y * g.band, y1 = min(y0 + g.band, g.H);
+    const bool have_view = g.vw > 0 && g.vh > 0;
+    const size_t plane = (size_t)g.W * g.H;
+    auto fetch = [&](int x, int y) {
+        BackTerms o;
+        const float* w = g.ws + (size_t)y * g.W + (size_t)x;
+#pragma unroll
+        for (int m = 0; m < 11; ++m) o.w[m] = w[m * plane];
+        return o;
+    };
+    auto eval = [&](const BackTerms& o, int, int, double v[11]) {
+#pragma unroll
+        for (int m = 0; m < 11; ++m) v[m] = (double)o.w[m];
+    };
+    auto centre = [&](int x, int y) { return back_pixel(g, x, y, have_view); };
+    auto sink = [&](const BackPixel& o, int x, int y, const double S[11], int cols, int rws) {
+        const double p[3] = {o.p[0], o.p[1], o.p[2]};
+        const double Y = (p[1] + g.wr * (p[0] - p[1])) + g.wb * (p[2] - p[1]);
+        double gY = S[6] + 2.0 * S[7] * Y;
+        double gp[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gp[c] = S[c] + S[3 + c] * Y;
+            gY += S[3 + c] * p[c];
+        }
+        const int lx = x - g.vx, ly = y - g.vy;
+        if (lx >= 0 && ly >= 0 && lx < g.vw && ly < g.vh) {
+            const double inv_n = inv_count<R>(cols, rws);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gY += (double)o.d[c] * (S[8 + c] * inv_n);
+        }
+        gp[0] += gY * g.wr;
+        gp[1] += gY * (1.0 - g.wr - g.wb);
+        gp[2] += gY * g.wb;
+        float* out = g.gp + ((size_t)y * g.W + (size_t)x) * 3;
+        out[0] = (float)gp[0]; out[1] = (float)gp[1]; out[2] = (float)gp[2];
+    };
+    strip_runs<11, R>(g.W, g.H, 0, g.W, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
+}
+
+// Static LDS at the largest radius: four workgroups of either pass fit one CU.
+static_assert(sizeof(double) * (14 * RunGeom<kRunMaxR>::plane + 4) <= 160 * 1024 / 4, "gf_back_centres_runs LDS");
+
+// Rows per band for the strip_runs passes: about three workgroups per CU
+// over one image, at least 64 rows (each band repeats the 2R + 1 row
+// warm-up).  Depends on the image only, so a stack equals single calls bit
+// for bit.
+inline int run_band_rows(int strips, int rows) {
+    const int per = std::max(1, 768 / std::max(1, strips));
+    return std::max(64, (rows + per - 1) / per);
+}
+
 }  // namespace
 
 size_t guided_workspace_bytes(const GuidedLaunch& L) {
@@ -865,6 +1172,25 @@ hipError_t launch_guided_backward(const GuidedBackwardLaunch& L, hipStream_t str
     g.zgq = L.g_out_zstride;
     g.zws = (size_t)11 * g.W * g.H;
     g.zgp = (size_t)3 * g.W * g.H;
+    if (g.r >= 1 && g.r <= kRunMaxR) {
+        const int span = kRunCols - 2 * g.r;
+        const int strips = (g.W + span - 1) / span;
+        g.band = run_band_rows(strips, g.H);
+        dim3 grid((unsigned)strips, (unsigned)((g.H + g.band - 1) / g.band), nz);
+        switch (g.r) {
+#define RAWOPS_GF_BACK(R) \
+    case R: \
+        hipLaunchKernelGGL(gf_back_centres_runs<R>, grid, dim3(kRunThreads), 0, stream, g); \
+        if (hipGetLastError() != hipSuccess) break; \
+        hipLaunchKernelGGL(gf_back_pixels_runs<R>, grid, dim3(kRunThreads), 0, stream, g); \
+        break;
+            RAWOPS_GF_BACK(1) RAWOPS_GF_BACK(2) RAWOPS_GF_BACK(3) RAWOPS_GF_BACK(4) RAWOPS_GF_BACK(5)
+            RAWOPS_GF_BACK(6) RAWOPS_GF_BACK(7) RAWOPS_GF_BACK(8) RAWOPS_GF_BACK(9) RAWOPS_GF_BACK(10)
+            RAWOPS_GF_BACK(11)
+#undef RAWOPS_GF_BACK
+        }
+        return hipGetLastError();
+    }
     const int span = kStripCols - 2 * g.r;
     const int strips = (g.W + span - 1) / span;
     g.band = band_rows(strips, g.H);   // as for one image: the same sums, bit for bit
