@@ -6,16 +6,19 @@
 // with mp_k = mean_{j in W_k}(p_j - p_k), mg_k = mean(Y_j - Y_k) and
 // a_k = cov_k / (var_k + eps) equals  q_i = mean_{k in W_i}(a_k) Y_i +
 // mean_{k in W_i}(b_k),  b_k = mean_{W_k}(p) - a_k mean_{W_k}(Y).
-// Both box means are computed with sliding vertical sums and per-row
-// wavefront prefix scans (O(1) work per pixel for any radius), in FP64.  The
-// moments are accumulated on values shifted by a per-strip constant so that
+// Both box means are sums over strips of pixel columns: vertical window sums
+// slid down a band of rows per column, then sums along x over each centre's
+// window (O(1) work per pixel for any radius), in FP64.  The moments are
+// accumulated on values shifted by a per-strip constant so that
 // E[Y^2] - E[Y]^2 does not cancel catastrophically.
 //
 // Two strip kernels: `coefficients` writes a_k, b_k for every centre of the
 // viewport grown by r (computed in FP64, stored as FP32 planes: the stored
 // values carry a relative error of 2^-24, far inside the filter's 1e-5
-// tolerance, and halve the workspace traffic); `apply` box-averages them in
-// FP64, forms q and runs the fused post-filter op program (exposure, curves).
+// tolerance); `apply` box-averages them in FP64, forms q and runs the
+// post-filter op program (exposure, curves).  For r <= 16 the sums along x
+// are sliding sums over runs of 16 centres in LDS (strip_runs); larger radii
+// use per-row wavefront prefix scans (strip_box).
 #include "device_common.hpp"
 #include "render_launch.hpp"
 
@@ -41,7 +44,7 @@ inline int band_rows(int strips, int rows) {
 
 struct GuidedArgs {
     int band;                // rows per band (blockIdx.y)
-    uint32_t fused;          // post ops [0, fused) run in gf_apply, [fused, n) in gf_ops_tail
+    uint32_t fused;          // post ops [0, fused) run in the apply pass, the rest (from a curve op) after
     const float* src;
     uint32_t src_stride;
     int W, H;                // native raster size
@@ -337,95 +340,13 @@ __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
     if (err && g.flags) atomicOr(g.flags, err);
 }
 
-// ---------------------------------------------------------------- fused --
-//
-// The whole filter and its post-filter program in one pass at level 0, with
-// the per-centre coefficients kept on chip instead of a workspace round trip.
-// A workgroup (8 wavefronts) owns a strip of kFusedCols pixel columns
-// j = 0..255 at x = px0 + j, px0 = ox - 2r, and a band of output rows; the
-// centres of the strip are j in [r, 256 - r), its outputs j in [2r, 256 - 2r)
-// (Wo = 256 - 4r columns from x = ox).  Step s, with c = c_lo + s:
-//
-//   wave 0: the vertical window sums of Y and Y^2 of centre row c, per pixel
-//     column (a lane holds 4 adjacent columns), slid down one row per step;
-//     their sums along x as sliding sums over the lane's run of 4 centres,
-//     read back from a wave-private LDS row zero-padded by r at both ends;
-//     each centre's mean Y and 1 / (var + eps), written to a double-buffered
-//     row.
-//   waves 1..3 (channel ch): the same for p_ch and Y p_ch of row c (kept to
-//     the next step), and, from last step's sums and wave 0's row, a_ch and
-//     b_ch of centre row c - 1, summed along x over each output column's
-//     r-window the same way and written (FP32) into a ring of 2r + 3 centre
-//     rows in LDS.
-//   waves 4..7: a thread per output column adds the ring row c - 2 to the
-//     sums of its window and subtracts row c - 2r - 3, forms output row
-//     c - 2 - r and runs the post-filter program (curves included), and
-//     stores the pixel.
-//
-// One barrier per step; the ring slot written in a step is the one read as
-// the leaving row a step later.  One workgroup per CU (ring of 2r + 3 rows x
-// 6 planes; 98 KB at r = 8).
-constexpr int kFusedCols = 256;
-constexpr int kFusedThreads = 512;
-constexpr int kFusedMaxR = 11;
-constexpr int kFusedRun = 4;                                        // columns per coefficient lane
-constexpr int kFusedLogical = kFusedCols + 2 * kFusedMaxR;          // zero-padded run row
-constexpr int kFusedPlane = kFusedLogical + (kFusedLogical >> 2);   // + one pad double per 4 (banks)
-constexpr int kFusedMinBand = 64;                                   // rows per band at least
-
-// Ring geometry at radius R: a row of each plane holds the outputs from
-// column RB = 2R rounded down to a multiple of 4 (so a coefficient lane's 4
-// outputs are one aligned 16-byte group), WR floats (a multiple of 4).
-template <int R> struct FusedRing {
-    static constexpr int D = 2 * R + 3;
-    static constexpr int RB = (2 * R) & ~3;
-    static constexpr int WR = ((kFusedCols - 2 * R - RB) + 3) & ~3;
-    static constexpr int floats = D * 6 * WR;
-};
-
-__device__ __forceinline__ int run_index(int i) { return i + (i >> 2); }
-
-__device__ __forceinline__ int clipped_count(int v, int r, int n) {
-    return min(n, v + r + 1) - max(0, v - r);
-}
-
-// Four adjacent pixels of row yy at columns x0..x0+3; columns outside
-// [0, W) load a clamped in-bounds pixel (its moments are discarded).
-__device__ __forceinline__ void load4(const float* __restrict__ src, uint32_t stride, int yy, int x0, int W,
-                                      float p[12]) {
-    const float* row = src + (size_t)yy * stride * 3;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int x = min(max(x0 + t, 0), W - 1);
-        const float* q = row + (size_t)x * 3;
-        p[3 * t] = q[0]; p[3 * t + 1] = q[1]; p[3 * t + 2] = q[2];
-    }
-}
-
-// Sliding sums over a zero-padded run row: s[t] = sum_{i=0}^{2R} b[i0 + t + i]
-// (two partial sums for the first window, so the adds overlap).
-template <int R>
-__device__ __forceinline__ void run_sums(const double* __restrict__ b, int i0, double s[4]) {
-    double e = 0.0, o = 0.0;
-#pragma unroll
-    for (int i = 0; i <= 2 * R; i += 2) e += b[run_index(i0 + i)];
-#pragma unroll
-    for (int i = 1; i <= 2 * R; i += 2) o += b[run_index(i0 + i)];
-    double acc = e + o;
-    s[0] = acc;
-#pragma unroll
-    for (int t = 1; t < 4; ++t) {
-        acc = acc + b[run_index(i0 + 2 * R + t)] - b[run_index(i0 + t - 1)];
-        s[t] = acc;
-    }
-}
-
-// Box sums for a compile-time radius R (1..11) without per-row prefix scans.
+// Box sums for a compile-time radius R (1..16) without per-row prefix scans.
 // The strip is kRunCols = 256 pixel columns x = strip_x - R + t; thread t
 // keeps the vertical window sums (rows k-R..k+R, clipped) of the N values of
-// its column, slid down one row per step, with the row data of the next step
-// loaded a step ahead (fetch(x, y) returns it, eval(data, x, y, v) turns it
-// into the N values).  Per row:
+// its column, slid down one row per step, with the row data of a step loaded
+// two steps ahead (fetch(x, y) returns it, eval(data, x, y, v) turns it into
+// the N values; fetch must be branch-free, so that the compiler's count of
+// loads in flight stays exact and the loads overlap the steps).  Per row:
 //   1. thread t writes its N sums into slot t of N LDS rows zero-padded by R;
 //   2. thread t = 16 m + j (m < N) forms the sums along x of value m for the
 //      16 centres 16 j .. 16 j + 15 as sliding sums (2R + 16 reads) and, once
@@ -441,9 +362,10 @@ __device__ __forceinline__ void run_sums(const double* __restrict__ b, int i0, d
 // run_slot(i) (one pad double per 16, so the 16 lanes of a value read
 // distinct banks); rows are plane doubles apart, plane = 16 mod 32, so the 4
 // values of a wavefront spread over the banks as well.  Windows are clipped
-// to [0, LW) x [0, LH); only columns in [c_lo, c_hi) enter the sums; fetch
-// and centre are called at clamped in-bounds coordinates, so their loads are
-// unconditional.
+// to [0, LW) x [0, LH); only pixels of the data region [c_lo, c_hi) x
+// [r_lo, r_hi) enter the sums (it must hold every clipped window of the
+// band's centres), and fetch and centre are called only inside it
+// (coordinates clamped into it), so their loads are unconditional.
 constexpr int kRunCols = 256;
 constexpr int kRunThreads = 256;
 constexpr int kRunLen = 16;   // centres per lane in the sums along x
@@ -476,55 +398,65 @@ __device__ __forceinline__ void run_sums16(const double* __restrict__ b, int i0,
 }
 
 template <int N, int R, class Fetch, class Eval, class Centre, class Sink>
-__device__ __forceinline__ void strip_runs(int LW, int LH, int c_lo, int c_hi, int strip_x, int xend, int y0, int y1,
-                                           double* rows /* [N][plane] */, Fetch fetch, Eval eval, Centre centre,
-                                           Sink sink) {
+__device__ __forceinline__ void strip_runs(int LW, int LH, int c_lo, int c_hi, int r_lo, int r_hi, int strip_x,
+                                           int xend, int y0, int y1, double* rows /* [N][plane] */, Fetch fetch,
+                                           Eval eval, Centre centre, Sink sink) {
     static_assert(N * (kRunCols / kRunLen) <= kRunThreads, "one sum run per thread");
     constexpr int P = RunGeom<R>::plane;
     const int t = threadIdx.x;
     const int x = strip_x - R + t;
     const bool own = x >= c_lo && x < c_hi;
-    const int xc = min(max(x, 0), LW - 1);
+    const int xc = min(max(x, c_lo), c_hi - 1);
     const int slot = run_slot(t + R);
     const int vm = t / (kRunCols / kRunLen), vj = t % (kRunCols / kRunLen);   // value, run of the sums along x
-    auto row_of = [&](int y) { return min(max(y, 0), LH - 1); };
+    auto row_of = [&](int y) { return min(max(y, r_lo), r_hi - 1); };
+    // Rows that enter the sums: the image rows of the data region (every
+    // clipped window of the band's centres lies inside it).
+    const int row_lo = max(0, r_lo), row_hi = min(LH, r_hi);
     for (int i = t; i < N * P; i += kRunThreads) rows[i] = 0.0;
+    // Warm-up: the window of row y0 - 1, four rows of loads in flight.
     double V[N];
 #pragma unroll
     for (int m = 0; m < N; ++m) V[m] = 0.0;
-    if (own) {
-        const int ya = max(0, y0 - R), yb = min(LH, y0 + R + 1);
-        for (int y = ya; y < yb; ++y) {
-            const auto d = fetch(xc, y);
-            double v[N];
-            eval(d, x, y, v);
+    for (int y = y0 - 1 - R; y <= y0 - 1 + R; y += 4) {
+        decltype(fetch(0, 0)) d[4];
 #pragma unroll
-            for (int m = 0; m < N; ++m) V[m] += v[m];
+        for (int u = 0; u < 4; ++u) d[u] = fetch(xc, row_of(y + u));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int yy = y + u;
+            if (own && yy <= y0 - 1 + R && yy >= row_lo && yy < row_hi) {
+                double v[N];
+                eval(d[u], x, yy, v);
+#pragma unroll
+                for (int m = 0; m < N; ++m) V[m] += v[m];
+            }
         }
     }
-    auto en = fetch(xc, row_of(y0 + 1 + R));
-    auto lv = fetch(xc, row_of(y0 - R));
+    // Every step adds row k + R and drops row k - R - 1; the rows of steps
+    // k and k + 1 are in flight (loads are unconditional, at clamped rows, so
+    // the compiler's count of loads in flight stays exact).
     const bool is_centre = t >= R && t < kRunCols - R && x < xend;
-    auto cd = centre(xc, row_of(y0));
-    __syncthreads();   // pads zeroed before the first row is written
-    for (int k = y0; k < y1; ++k) {
-        if (k > y0) {
-            if (own) {
-                double v[N];
-                if (k + R < LH) {
-                    eval(en, x, k + R, v);
+    // One step: row k with its entering / leaving rows in en / lv and the
+    // centre data in cd; then those buffers are refilled for step k + 2.
+    // Steps alternate between two sets of buffers (no register copies, so
+    // the loads of the next two steps stay in flight).
+    auto step = [&](int k, auto& en, auto& lv, auto& cd) {
+        if (own) {
+            double v[N];
+            if (k + R < row_hi) {
+                eval(en, x, k + R, v);
 #pragma unroll
-                    for (int m = 0; m < N; ++m) V[m] += v[m];
-                }
-                if (k - R - 1 >= 0) {
-                    eval(lv, x, k - R - 1, v);
-#pragma unroll
-                    for (int m = 0; m < N; ++m) V[m] -= v[m];
-                }
+                for (int m = 0; m < N; ++m) V[m] += v[m];
             }
-            en = fetch(xc, row_of(k + 1 + R));
-            lv = fetch(xc, row_of(k - R));
+            if (k - R - 1 >= row_lo) {
+                eval(lv, x, k - R - 1, v);
+#pragma unroll
+                for (int m = 0; m < N; ++m) V[m] -= v[m];
+            }
         }
+        en = fetch(xc, row_of(k + 2 + R));
+        lv = fetch(xc, row_of(k + 1 - R));
 #pragma unroll
         for (int m = 0; m < N; ++m) rows[m * P + slot] = V[m];
         __syncthreads();
@@ -541,252 +473,30 @@ __device__ __forceinline__ void strip_runs(int LW, int LH, int c_lo, int c_hi, i
             for (int u = 0; u < kRunLen; ++u) b[run_slot(kRunLen * vj + u + R)] = Hs[u];
         }
         __syncthreads();
-        const auto cur = cd;
-        if (k + 1 < y1) cd = centre(xc, row_of(k + 1));
         if (is_centre) {
             double S[N];
 #pragma unroll
             for (int m = 0; m < N; ++m) S[m] = rows[m * P + slot];
             const int cols = min(LW, x + R + 1) - max(0, x - R);
             const int rws = min(LH, k + R + 1) - max(0, k - R);
-            sink(cur, x, k, S, cols, rws);
+            sink(cd, x, k, S, cols, rws);
         }
+        cd = centre(xc, row_of(k + 2));
+    };
+    auto en0 = fetch(xc, row_of(y0 + R));
+    auto lv0 = fetch(xc, row_of(y0 - R - 1));
+    auto cd0 = centre(xc, row_of(y0));
+    auto en1 = fetch(xc, row_of(y0 + 1 + R));
+    auto lv1 = fetch(xc, row_of(y0 - R));
+    auto cd1 = centre(xc, row_of(y0 + 1));
+    __syncthreads();   // pads zeroed before the first row is written
+    int k = y0;
+    for (; k + 1 < y1; k += 2) {
+        step(k, en0, lv0, cd0);
+        step(k + 1, en1, lv1, cd1);
     }
+    if (k < y1) step(k, en0, lv0, cd0);
 }
-
-template <int R>
-__global__ void __launch_bounds__(kFusedThreads) gf_fused(const GuidedArgs g0) {
-    const GuidedArgs g = slice_of(g0);
-    using RG = FusedRing<R>;
-    constexpr int D = RG::D, RB = RG::RB, WR = RG::WR;
-    __shared__ __attribute__((aligned(16))) float ring[RG::floats];
-    __shared__ double runs[4][2][kFusedPlane];
-    __shared__ double my_d[2][2][4][64];        // [buffer][mean Y, 1/(var+eps)][column of the lane][lane]
-    const int ox = g.vx + (int)blockIdx.x * (kFusedCols - 4 * R), px0 = ox - 2 * R;
-    const int y0 = g.vy + (int)blockIdx.y * g.band, y1 = min(y0 + g.band, g.vy + g.vh);
-    const int c_lo = max(0, y0 - R), c_hi = min(g.LH, y1 + R);
-    const int steps = y1 + R - c_lo + 2;
-    // Pixel columns inside the support of the viewport (its 2R halo): only
-    // these enter the moments and the non-finite check.
-    const int sx0 = max(0, g.vx - 2 * R), sx1 = min(g.LW, g.vx + g.vw + 2 * R);
-    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    uint32_t err = 0;
-
-    // Zero the run rows (their pads stay zero).
-    for (int i = t; i < 4 * 2 * kFusedPlane; i += kFusedThreads) (&runs[0][0][0])[i] = 0.0;
-    // Per-strip shift of the moments: the strip's first centre pixel.
-    double shift[3];
-    {
-        const float* q = g.src + ((size_t)c_lo * g.src_stride + (size_t)min(max(px0 + R, sx0), sx1 - 1)) * 3;
-        shift[0] = q[0]; shift[1] = q[1]; shift[2] = q[2];
-    }
-    const double shift_y = working_y(g, shift[0], shift[1], shift[2]);
-    __syncthreads();
-
-    if (wave < 4) {
-        // ---- coefficient waves
-        const int role = wave;                 // 0: Y, Y^2; 1 + ch: p_ch, Y p_ch
-        const int ch = role - 1;
-        const int j0 = kFusedRun * lane;       // first pixel column of the lane
-        const int x0 = px0 + j0;
-        // colok: the column enters the moments; centre: a centre of the strip
-        // inside the image (others are zero in the sums along x); need: a
-        // centre some output of the viewport reads (checked for overflow).
-        bool colok[4], centre[4], need[4];
-        double nxc[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            colok[u] = x0 + u >= sx0 && x0 + u < sx1;
-            centre[u] = j0 + u >= R && j0 + u < kFusedCols - R && x0 + u >= 0 && x0 + u < g.LW;
-            need[u] = centre[u] && x0 + u >= g.vx - R && x0 + u < g.vx + g.vw + R;
-            nxc[u] = (double)clipped_count(x0 + u, R, g.LW);
-        }
-        double V0[4] = {0, 0, 0, 0}, V1[4] = {0, 0, 0, 0};
-        auto accumulate = [&](const float p[12], double sign, bool check) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float* q = p + 3 * u;
-                if (check && colok[u] && (!finite(q[0]) || !finite(q[1]) || !finite(q[2]))) err |= FLAG_INVALID;
-                const double d0 = (double)q[0] - shift[0], d1 = (double)q[1] - shift[1], d2 = (double)q[2] - shift[2];
-                const double y = working_y(g, d0, d1, d2);
-                const double dc = ch == 0 ? d0 : (ch == 1 ? d1 : d2);
-                const double m0 = role == 0 ? y : dc;
-                const double m1 = role == 0 ? y * y : y * dc;
-                V0[u] = colok[u] ? V0[u] + sign * m0 : V0[u];
-                V1[u] = colok[u] ? V1[u] + sign * m1 : V1[u];
-            }
-        };
-        // Window of centre row c_lo.
-        for (int yy = max(0, c_lo - R); yy <= min(g.LH - 1, c_lo + R); ++yy) {
-            float p[12];
-            load4(g.src, g.src_stride, yy, x0, g.LW, p);
-            accumulate(p, 1.0, role == 0);
-        }
-        double* b0 = runs[wave][0];
-        double* b1 = runs[wave][1];
-        // Rows of step s arrive two steps ahead (buffer s & 1), so a load has
-        // two steps to return before its moments are formed.
-        // The loads are unconditional (rows clamped into the image; unused
-        // rows are never accumulated), so the compiler's count of loads in
-        // flight stays exact and a wait covers only the rows it needs.
-        float enter[2][12], leave[2][12];
-        auto fetch = [&](int c, int buf) {
-            load4(g.src, g.src_stride, min(c + R, g.LH - 1), x0, g.LW, enter[buf]);
-            load4(g.src, g.src_stride, max(c - R - 1, 0), x0, g.LW, leave[buf]);
-        };
-        const double sh = ch == 0 ? shift[0] : (ch == 1 ? shift[1] : shift[2]);
-        double P0[4], P1[4], Pn[4];            // channel waves: last step's sums and 1/n
-        auto step = [&](int s, int buf) {
-            const int c = c_lo + s;
-            if (role > 0 && s >= 1 && c - 1 < c_hi) {
-                // a, b of centre row c - 1 and their sums along x -> ring
-                const double* md = &my_d[(s - 1) & 1][0][0][0];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int j = j0 + u;
-                    const double my = md[u * 64 + lane], inv_d = md[256 + u * 64 + lane];
-                    const double mp = P0[u] * Pn[u];
-                    const double cov = P1[u] * Pn[u] - my * mp;
-                    const double a = cov * inv_d;
-                    const double b = (mp - a * my) + (sh - a * shift_y);
-                    if (need[u] && (!__builtin_isfinite(a) || !__builtin_isfinite(b))) err |= FLAG_OVERFLOW;
-                    b0[run_index(j + R)] = centre[u] ? a : 0.0;
-                    b1[run_index(j + R)] = centre[u] ? b : 0.0;
-                }
-                double Ha[4], Hb[4];
-                run_sums<R>(b0, j0, Ha);
-                run_sums<R>(b1, j0, Hb);
-                float* slot = ring + ((c - 1) % D) * 6 * WR;
-                if (j0 >= 2 * R && j0 + 3 < kFusedCols - 2 * R) {
-                    *reinterpret_cast<float4*>(slot + ch * WR + j0 - RB) =
-                        make_float4((float)Ha[0], (float)Ha[1], (float)Ha[2], (float)Ha[3]);
-                    *reinterpret_cast<float4*>(slot + (3 + ch) * WR + j0 - RB) =
-                        make_float4((float)Hb[0], (float)Hb[1], (float)Hb[2], (float)Hb[3]);
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int jo = j0 + u;
-                        if (jo >= 2 * R && jo < kFusedCols - 2 * R) {
-                            slot[ch * WR + jo - RB] = (float)Ha[u];
-                            slot[(3 + ch) * WR + jo - RB] = (float)Hb[u];
-                        }
-                    }
-                }
-            }
-            if (c < c_hi) {
-                // moments of centre row c
-                if (s > 0 && c + R < g.LH) accumulate(enter[buf], 1.0, role == 0);
-                if (s > 0 && c - R - 1 >= 0) accumulate(leave[buf], -1.0, false);
-            }
-            fetch(min(c + 2, c_hi - 1), buf);
-            if (c < c_hi) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    b0[run_index(j0 + u + R)] = V0[u];
-                    b1[run_index(j0 + u + R)] = V1[u];
-                }
-                double S0[4], S1[4];
-                run_sums<R>(b0, j0, S0);
-                run_sums<R>(b1, j0, S1);
-                // 1 / n of each centre: one division per row, another only
-                // where the window is clipped at the left or right edge.
-                const int ny = clipped_count(c, R, g.LH);
-                const double inv_row = 1.0 / (double)((2 * R + 1) * ny);
-                double inv_n[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    inv_n[u] = inv_row;
-                    if (nxc[u] != (double)(2 * R + 1)) inv_n[u] = 1.0 / (nxc[u] * (double)ny);
-                }
-                if (role == 0) {
-                    double* md = &my_d[s & 1][0][0][0];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const double my = S0[u] * inv_n[u];
-                        double var = S1[u] * inv_n[u] - my * my;
-                        if (var < 0) var = 0;
-                        md[u * 64 + lane] = my;
-                        md[256 + u * 64 + lane] = 1.0 / (var + g.eps);
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) { P0[u] = S0[u]; P1[u] = S1[u]; Pn[u] = inv_n[u]; }
-                }
-            }
-            __syncthreads();
-        };
-        fetch(c_lo + 1, 1);
-        for (int s = 0; s < steps; s += 2) {
-            step(s, 0);
-            if (s + 1 < steps) step(s + 1, 1);
-        }
-    } else {
-        // ---- output waves
-        const int jo = t - 256;
-        const int x = px0 + jo;
-        const bool col = jo >= 2 * R && jo < kFusedCols - 2 * R && x < g.vx + g.vw;
-        const int ri = min(max(jo - RB, 0), WR - 1);
-        const int nx = clipped_count(x, R, g.LW);
-        double U[6] = {0, 0, 0, 0, 0, 0};
-        float pix[3] = {0, 0, 0};
-        // Unconditional load at a clamped in-bounds pixel (see above).
-        const int xc = min(max(x, 0), g.LW - 1);
-        auto fetch = [&](int k) {
-            const float* q = g.src + ((size_t)min(max(k, y0), y1 - 1) * g.src_stride + (size_t)xc) * 3;
-            pix[0] = q[0]; pix[1] = q[1]; pix[2] = q[2];
-        };
-        fetch(c_lo - 2 - R);
-        for (int s = 0; s < steps; ++s) {
-            const int cb = c_lo + s - 2;       // centre row written into the ring last step
-            const int k = cb - R;              // output row
-            if (s >= 2) {
-                if (cb < c_hi) {
-                    const float* a = ring + (cb % D) * 6 * WR + ri;
-#pragma unroll
-                    for (int p = 0; p < 6; ++p) U[p] += (double)a[p * WR];
-                }
-                if (cb - 2 * R - 1 >= c_lo) {
-                    const float* a = ring + ((cb - 2 * R - 1) % D) * 6 * WR + ri;
-#pragma unroll
-                    for (int p = 0; p < 6; ++p) U[p] -= (double)a[p * WR];
-                }
-            }
-            const float p0 = pix[0], p1 = pix[1], p2 = pix[2];
-            fetch(k + 1);
-            if (s >= 2 && k >= y0 && k < y1) {
-                // 1 / n: one division per row, another only at the left or right edge.
-                const int ny = clipped_count(k, R, g.LH);
-                double inv_n = 1.0 / (double)((2 * R + 1) * ny);
-                if (nx != 2 * R + 1) inv_n = 1.0 / (double)(nx * ny);
-                if (col) {
-                    const float p[3] = {p0, p1, p2};
-                    const double y = working_y(g, (double)p0, (double)p1, (double)p2);
-                    float o[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const double q = (U[c] * inv_n) * y + U[3 + c] * inv_n;
-                        const double delta = q - (double)p[c];
-                        if (!__builtin_isfinite(delta)) err |= FLAG_OVERFLOW;
-                        if (delta == 0) { o[c] = p[c]; continue; }   // exact copy, as the reference
-                        const double value = (double)p[c] + delta;
-                        if (value < -3.4028234663852886e38 || value > 3.4028234663852886e38) err |= FLAG_OVERFLOW;
-                        o[c] = (float)value;
-                    }
-                    apply_ops(*g.post, 0, g.fused, o[0], o[1], o[2], err);
-                    float* d = g.out + ((size_t)(k - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
-                    d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (err && g.flags) atomicOr(g.flags, err);
-}
-
-// Static LDS of gf_fused at its largest radius.
-static_assert(sizeof(float) * FusedRing<kFusedMaxR>::floats + sizeof(double) * (4 * 2 * kFusedPlane + 2 * 2 * 4 * 64) <=
-                  160 * 1024,
-              "gf_fused LDS");
 
 // Radius 0: the filter is the identity (SpatialOps.cpp:180); only the
 // post-filter program runs.
@@ -965,7 +675,7 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBack
 // strip_runs: the same per-pixel values and per-centre formulas, box sums
 // formed as sliding sums instead of prefix differences; 1 / n is a constant
 // for unclipped windows and a division only where the window is clipped.
-constexpr int kRunMaxR = 11;
+constexpr int kRunMaxR = 16;
 
 template <int R> struct RunWindow {
     static constexpr int side = 2 * R + 1;
@@ -983,17 +693,15 @@ struct BackPixel {
     float d[3];   // dL/dq at the pixel, clamped into the viewport (used only inside it)
 };
 
+// Branch-free: with an empty viewport the dL/dq load reads the pixel itself
+// (its values are never used).
 __device__ __forceinline__ BackPixel back_pixel(const GuidedBackArgs& g, int x, int y, bool have_view) {
     BackPixel o;
     const float* q = g.p + ((size_t)y * g.stride + (size_t)x) * 3;
+    const int lx = max(min(x - g.vx, g.vw - 1), 0), ly = max(min(y - g.vy, g.vh - 1), 0);
+    const float* d = have_view ? g.gq + ((size_t)ly * g.gstride + (size_t)lx) * 3 : q;
     o.p[0] = q[0]; o.p[1] = q[1]; o.p[2] = q[2];
-    if (have_view) {
-        const int lx = min(max(x - g.vx, 0), g.vw - 1), ly = min(max(y - g.vy, 0), g.vh - 1);
-        const float* d = g.gq + ((size_t)ly * g.gstride + (size_t)lx) * 3;
-        o.d[0] = d[0]; o.d[1] = d[1]; o.d[2] = d[2];
-    } else {
-        o.d[0] = o.d[1] = o.d[2] = 0.0f;
-    }
+    o.d[0] = d[0]; o.d[1] = d[1]; o.d[2] = d[2];
     return o;
 }
 
@@ -1061,7 +769,7 @@ __global__ void __launch_bounds__(kRunThreads, 3) gf_back_centres_runs(const Gui
         w[6 * plane] = (float)(gmy * inv_n);
         w[7 * plane] = (float)(gD * inv_n);
     };
-    strip_runs<14, R>(g.W, g.H, 0, g.W, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
+    strip_runs<14, R>(g.W, g.H, 0, g.W, 0, g.H, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
     for (int off = 32; off > 0; off >>= 1) geps += __shfl_xor(geps, off, 64);
     if ((threadIdx.x & 63) == 0) eps_part[threadIdx.x >> 6] = geps;
     __syncthreads();
@@ -1077,7 +785,7 @@ struct BackTerms {
 };
 
 template <int R>
-__global__ void __launch_bounds__(kRunThreads, 4) gf_back_pixels_runs(const GuidedBackArgs g0) {
+__global__ void __launch_bounds__(kRunThreads, 3) gf_back_pixels_runs(const GuidedBackArgs g0) {
     const GuidedBackArgs g = slice_of(g0);
     __shared__ double rows[11 * RunGeom<R>::plane];
     const int span = kRunCols - 2 * R;
@@ -1119,7 +827,169 @@ __global__ void __launch_bounds__(kRunThreads, 4) gf_back_pixels_runs(const Guid
         float* out = g.gp + ((size_t)y * g.W + (size_t)x) * 3;
         out[0] = (float)gp[0]; out[1] = (float)gp[1]; out[2] = (float)gp[2];
     };
-    strip_runs<11, R>(g.W, g.H, 0, g.W, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
+    strip_runs<11, R>(g.W, g.H, 0, g.W, 0, g.H, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
+}
+
+// The two-pass forward at a compile-time radius R (1..kRunMaxR) over
+// strip_runs: gf_coef_runs writes a, b of each centre of the viewport grown
+// by R as 6 FP32 planes, gf_apply_runs box-averages them and runs the
+// post-filter program up to the first curve.  The same per-pixel moments and
+// per-centre formulas as gf_coefficients / gf_apply.
+// A pixel of the filtered level: at level 0 a plain load (branch-free, see
+// strip_runs), at mip 1/2 the clipped native footprint's average.
+template <bool Level0>
+__device__ __forceinline__ void run_pixel(const GuidedArgs& g, int x, int y, float p[3]) {
+    if (Level0) {
+        const float* q = g.src + ((size_t)y * g.src_stride + (size_t)x) * 3;
+        p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+    } else {
+        level_pixel(g, x, y, p);
+    }
+}
+
+template <int R, bool Level0>
+__device__ __forceinline__ void coef_runs(const GuidedArgs& g, double* rows) {
+    const int span = kRunCols - 2 * R;
+    const int strip_x = g.cx0 + (int)blockIdx.x * span;
+    const int k0 = g.cy0 + (int)blockIdx.y * g.band;
+    const int k1 = min(k0 + g.band, g.cy0 + g.ch);
+    uint32_t err = 0;
+    // Per-strip shift: the first centre pixel of the strip.
+    double shift[3];
+    {
+        float p[3];
+        level_pixel(g, min(strip_x, g.LW - 1), k0, p);
+        shift[0] = p[0]; shift[1] = p[1]; shift[2] = p[2];
+    }
+    const double shift_y = working_y(g, shift[0], shift[1], shift[2]);
+    struct Pix { float p[3]; };
+    auto fetch = [&](int x, int y) {
+        Pix o;
+        run_pixel<Level0>(g, x, y, o.p);
+        return o;
+    };
+    auto eval = [&](const Pix& o, int, int, double m[8]) { moments(g, o.p, shift, m, err); };
+    auto centre = [&](int, int) { return 0; };
+    const size_t plane = (size_t)g.cw * g.ch;
+    auto sink = [&](int, int kx, int k, const double S[8], int cols, int rws) {
+        const double inv_n = inv_count<R>(cols, rws);
+        const double my = S[0] * inv_n;
+        double var = S[4] * inv_n - my * my;
+        if (var < 0) var = 0;
+        const double inv_d = 1.0 / (var + g.eps);
+        float* c = g.coef + (size_t)(k - g.cy0) * g.cw + (size_t)(kx - g.cx0);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const double mp = S[1 + ch] * inv_n;
+            const double cov = S[5 + ch] * inv_n - my * mp;
+            const double a = cov * inv_d;
+            // b in unshifted coordinates: mean(p) - a mean(Y).
+            const double b = (mp - a * my) + (shift[ch] - a * shift_y);
+            if (!__builtin_isfinite(a) || !__builtin_isfinite(b)) err |= FLAG_OVERFLOW;
+            c[ch * plane] = (float)a;
+            c[(3 + ch) * plane] = (float)b;
+        }
+    };
+    // Only the pixels of the centres' windows enter the sums.
+    const int c_lo = max(0, g.cx0 - R), c_hi = min(g.LW, g.cx0 + g.cw + R);
+    strip_runs<8, R>(g.LW, g.LH, c_lo, c_hi, 0, g.LH, strip_x, g.cx0 + g.cw, k0, k1, rows, fetch, eval, centre,
+                     sink);
+    if (err && g.flags) atomicOr(g.flags, err);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kRunThreads, 3) gf_coef_runs(const GuidedArgs g0) {
+    const GuidedArgs g = slice_of(g0);
+    __shared__ double rows[8 * RunGeom<R>::plane];
+    if (g.mip == 0)
+        coef_runs<R, true>(g, rows);
+    else
+        coef_runs<R, false>(g, rows);
+}
+
+template <int R, bool Level0>
+__device__ __forceinline__ void apply_runs(const GuidedArgs& g, double* rows, double* tables) {
+    const int span = kRunCols - 2 * R;
+    const int strip_x = g.vx + (int)blockIdx.x * span;
+    const int i0 = g.vy + (int)blockIdx.y * g.band;
+    const int i1 = min(i0 + g.band, g.vy + g.vh);
+    uint32_t err = 0;
+    // Coefficients exist for centres in [cx0, cx0 + cw) x [cy0, cy0 + ch):
+    // exactly the clipped windows of the viewport's pixels.
+    const size_t plane = (size_t)g.cw * g.ch;
+    struct Coef { float c[6]; };
+    auto fetch = [&](int kx, int ky) {
+        Coef o;
+        const float* q = g.coef + (size_t)(ky - g.cy0) * g.cw + (size_t)(kx - g.cx0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) o.c[m] = q[m * plane];
+        return o;
+    };
+    auto eval = [&](const Coef& o, int, int, double c[6]) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) c[m] = (double)o.c[m];
+    };
+    struct Pix { float p[3]; };
+    auto centre = [&](int x, int i) {
+        Pix o;
+        run_pixel<Level0>(g, x, i, o.p);
+        return o;
+    };
+    // The post-filter ops from g.fused on (g.fused is a curve op) run here
+    // too, the curve op's knot tables staged in LDS when every curve has at
+    // most kTailKnots knots.
+    Const<OpList>& L = as_const(*g.post);
+    const uint32_t end = L.n;
+    bool staged = false;
+    LdsPoly curves[4];
+    uint32_t cubic = 0;
+    if (g.fused < end) {
+        const DevCurves* cv = L.op[g.fused].curves;
+        staged = true;
+        for (int c = 0; c < 4; ++c) staged = staged && (as_const(*cv).c[c].identity || as_const(*cv).c[c].n <= kTailKnots);
+        cubic = as_const(*cv).cubic;
+        if (staged)
+            for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * (kPolyRec + 1) * kTailKnots);
+    }
+    auto sink = [&](const Pix& px, int x, int i, const double S[6], int cols, int rws) {
+        const double inv_n = inv_count<R>(cols, rws);
+        const float* p = px.p;
+        const double y = working_y(g, (double)p[0], (double)p[1], (double)p[2]);
+        float o[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const double q = (S[ch] * inv_n) * y + S[3 + ch] * inv_n;
+            const double delta = q - (double)p[ch];
+            if (!__builtin_isfinite(delta)) err |= FLAG_OVERFLOW;
+            if (delta == 0) { o[ch] = p[ch]; continue; }   // exact copy, as the reference
+            const double value = (double)p[ch] + delta;
+            if (value < -3.4028234663852886e38 || value > 3.4028234663852886e38) err |= FLAG_OVERFLOW;
+            o[ch] = (float)value;
+        }
+        apply_ops(*g.post, 0, g.fused, o[0], o[1], o[2], err);
+        if (staged) {
+            curves3_lds(curves, cubic, o, err);
+            apply_ops(*g.post, g.fused + 1, end, o[0], o[1], o[2], err);
+        } else {
+            apply_ops(*g.post, g.fused, end, o[0], o[1], o[2], err);
+        }
+        float* d = g.out + ((size_t)(i - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
+        d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
+    };
+    strip_runs<6, R>(g.LW, g.LH, g.cx0, g.cx0 + g.cw, g.cy0, g.cy0 + g.ch, strip_x, g.vx + g.vw, i0, i1, rows, fetch,
+                     eval, centre, sink);
+    if (err && g.flags) atomicOr(g.flags, err);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kRunThreads, 3) gf_apply_runs(const GuidedArgs g0) {
+    const GuidedArgs g = slice_of(g0);
+    __shared__ double rows[6 * RunGeom<R>::plane];
+    __shared__ __attribute__((aligned(16))) double tables[4 * (kPolyRec + 1) * kTailKnots];
+    if (g.mip == 0)
+        apply_runs<R, true>(g, rows, tables);
+    else
+        apply_runs<R, false>(g, rows, tables);
 }
 
 // Static LDS at the largest radius: four workgroups of either pass fit one CU.
@@ -1186,7 +1056,8 @@ hipError_t launch_guided_backward(const GuidedBackwardLaunch& L, hipStream_t str
         break;
             RAWOPS_GF_BACK(1) RAWOPS_GF_BACK(2) RAWOPS_GF_BACK(3) RAWOPS_GF_BACK(4) RAWOPS_GF_BACK(5)
             RAWOPS_GF_BACK(6) RAWOPS_GF_BACK(7) RAWOPS_GF_BACK(8) RAWOPS_GF_BACK(9) RAWOPS_GF_BACK(10)
-            RAWOPS_GF_BACK(11)
+            RAWOPS_GF_BACK(11) RAWOPS_GF_BACK(12) RAWOPS_GF_BACK(13) RAWOPS_GF_BACK(14) RAWOPS_GF_BACK(15)
+            RAWOPS_GF_BACK(16)
 #undef RAWOPS_GF_BACK
         }
         return hipGetLastError();
@@ -1235,36 +1106,37 @@ hipError_t launch_guided(const GuidedLaunch& L, const OpList* post, hipStream_t 
         hipLaunchKernelGGL(gf_identity, dim3(blocks, 1, nz), dim3(256), 0, stream, g);
         return hipGetLastError();
     }
-    if (g.mip == 0 && g.r <= kFusedMaxR) {
-        // One pass, coefficients on chip.  Bands: about one workgroup per CU
-        // over the image (at least kFusedMinBand rows, which amortise the
-        // 2r + 1 steps before a band's first output); the band height
-        // depends on the image only, so a stack filters each image as a
-        // single call does, bit for bit.
-        const int Wo = kFusedCols - 4 * g.r;
-        const int strips = (g.vw + Wo - 1) / Wo;
-        const int per = std::max(1, 256 / strips);
-        g.band = std::max(kFusedMinBand, (g.vh + per - 1) / per);
-        g.fused = L.fused_ops;
-        dim3 grid((unsigned)strips, (unsigned)((g.vh + g.band - 1) / g.band), nz);
+    g.coef = static_cast<float*>(L.workspace);
+    g.zcoef = (size_t)6 * g.cw * g.ch;
+    g.fused = L.fused_ops;
+    if (g.r <= kRunMaxR) {
+        // Two strip passes over sliding sums; bands depend on the image only.
+        const int span = kRunCols - 2 * g.r;
+        const int strips_c = (g.cw + span - 1) / span;
+        GuidedArgs gc = g;
+        gc.band = run_band_rows(strips_c, g.ch);
+        dim3 grid_c((unsigned)strips_c, (unsigned)((g.ch + gc.band - 1) / gc.band), nz);
+        const int strips_a = (g.vw + span - 1) / span;
+        GuidedArgs ga = g;
+        ga.band = run_band_rows(strips_a, g.vh);
+        dim3 grid_a((unsigned)strips_a, (unsigned)((g.vh + ga.band - 1) / ga.band), nz);
+        hipError_t e = hipSuccess;
         switch (g.r) {
-#define RAWOPS_GF_FUSED(R) case R: hipLaunchKernelGGL(gf_fused<R>, grid, dim3(kFusedThreads), 0, stream, g); break;
-            RAWOPS_GF_FUSED(1) RAWOPS_GF_FUSED(2) RAWOPS_GF_FUSED(3) RAWOPS_GF_FUSED(4) RAWOPS_GF_FUSED(5)
-            RAWOPS_GF_FUSED(6) RAWOPS_GF_FUSED(7) RAWOPS_GF_FUSED(8) RAWOPS_GF_FUSED(9) RAWOPS_GF_FUSED(10)
-            RAWOPS_GF_FUSED(11)
-#undef RAWOPS_GF_FUSED
-        }
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess && L.fused_ops < L.post_ops) {
-            const size_t n = (size_t)g.vw * g.vh;
-            const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
-            hipLaunchKernelGGL(gf_ops_tail, dim3(blocks, 1, nz), dim3(256), 0, stream, g);
-            e = hipGetLastError();
+#define RAWOPS_GF_RUNS(R) \
+    case R: \
+        hipLaunchKernelGGL(gf_coef_runs<R>, grid_c, dim3(kRunThreads), 0, stream, gc); \
+        if ((e = hipGetLastError()) != hipSuccess) break; \
+        hipLaunchKernelGGL(gf_apply_runs<R>, grid_a, dim3(kRunThreads), 0, stream, ga); \
+        e = hipGetLastError(); \
+        break;
+            RAWOPS_GF_RUNS(1) RAWOPS_GF_RUNS(2) RAWOPS_GF_RUNS(3) RAWOPS_GF_RUNS(4) RAWOPS_GF_RUNS(5)
+            RAWOPS_GF_RUNS(6) RAWOPS_GF_RUNS(7) RAWOPS_GF_RUNS(8) RAWOPS_GF_RUNS(9) RAWOPS_GF_RUNS(10)
+            RAWOPS_GF_RUNS(11) RAWOPS_GF_RUNS(12) RAWOPS_GF_RUNS(13) RAWOPS_GF_RUNS(14) RAWOPS_GF_RUNS(15)
+            RAWOPS_GF_RUNS(16)
+#undef RAWOPS_GF_RUNS
         }
         return e;
     }
-    g.coef = static_cast<float*>(L.workspace);
-    g.zcoef = (size_t)6 * g.cw * g.ch;
     const int span = kStripCols - 2 * g.r;
     const int strips_c = (g.cw + span - 1) / span;
     g.band = band_rows(strips_c, g.ch);   // as for one image: the same sums, bit for bit
