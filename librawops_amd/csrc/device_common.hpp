@@ -214,20 +214,47 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
 // blends of de Casteljau's construction (ToneOps.cpp:788-792).  Same knot
 // selection, exact-knot and linear-extension rules as curve_map_in; cubic
 // values agree with it to FP64 rounding (the stored results are float32).
+//
+// For the knot search of float inputs the curve also keeps, per knot, the
+// smallest float >= x_k (for a float v, v >= x_k exactly when v >= that
+// float), padded with NaN to a power of two p2 >= n, so that the number of
+// knots <= v is a binary search over floats.
 struct LdsPoly {
     uint32_t n, identity;
     const double* x;
     const double* rec;   // 8 doubles per knot, 16-byte aligned
+    const float* xf;     // p2 float thresholds
+    uint32_t p2;
 };
 
 constexpr int kPolyRec = 8;
 
-// Stage curve c (n <= cap knots) at base (9 n doubles, 16-byte aligned);
-// the whole workgroup works, the caller synchronises.
+// LDS doubles of one staged curve of at most cap knots (even, so that
+// consecutive curves stay 16-byte aligned).
+constexpr int poly_table_doubles(int cap) { return ((kPolyRec + 1) * cap + (cap + 1) / 2 + 1) & ~1; }
+
+// The smallest float >= x (x finite; +-FLT_MAX overflow to +-inf is handled
+// by the step toward +inf).
+__device__ __forceinline__ float float_at_least(double x) {
+    float f = (float)x;
+    if ((double)f < x) {
+        const uint32_t b = __float_as_uint(f);
+        f = __uint_as_float((b & 0x80000000u) ? b - 1u : b + 1u);
+    }
+    return f;
+}
+
+// Stage curve c (n <= cap knots) at base (poly_table_doubles(cap) doubles,
+// 16-byte aligned); the whole workgroup works, the caller synchronises.
 __device__ __forceinline__ LdsPoly stage_poly(const DevCurve* c, double* base) {
     Const<DevCurve>& s = as_const(*c);
     const uint32_t n = s.n;
-    LdsPoly l{n, s.identity, base + kPolyRec * n, base};
+    uint32_t p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    float* xf = reinterpret_cast<float*>(base + (kPolyRec + 1) * n);
+    LdsPoly l{n, s.identity, base + kPolyRec * n, base, xf, p2};
+    if (!l.identity)
+        for (uint32_t k = threadIdx.x; k < p2; k += blockDim.x) xf[k] = k < n ? float_at_least(s.x[k]) : __builtin_nanf("");
     if (!l.identity)
         for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
             const uint32_t i = k < n - 2u ? k : n - 2u;
@@ -263,14 +290,12 @@ __device__ __forceinline__ double poly_eval(const LdsPoly& c, uint32_t cubic, do
     return v == xl ? yl : (linear ? lin : cub);
 }
 
-// Number of knots <= v for three values at once (one pass over the knots).
-__device__ __forceinline__ void knot_counts3(const LdsPoly& c, const double v[3], uint32_t cnt[3]) {
-    cnt[0] = cnt[1] = cnt[2] = 0;
-    for (uint32_t k = 0; k < c.n; ++k) {
-        const double xk = c.x[k];
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) cnt[ch] += v[ch] >= xk ? 1u : 0u;
-    }
+// Number of knots <= v for a float v: a binary search over the float
+// thresholds (NaN padding and a NaN v compare false).
+__device__ __forceinline__ uint32_t knot_count_f(const LdsPoly& c, float v) {
+    uint32_t cnt = 0;
+    for (uint32_t s = c.p2 >> 1; s > 0; s >>= 1) cnt += v >= c.xf[cnt + s - 1] ? s : 0u;
+    return cnt + (v >= c.xf[cnt] ? 1u : 0u);
 }
 
 // ExtendedCurvesNode::map (ToneOps.cpp:820-831) for the three channels of a
@@ -278,13 +303,11 @@ __device__ __forceinline__ void knot_counts3(const LdsPoly& c, const double v[3]
 // the checks and flags of curves_channel.
 __device__ inline void curves3_lds(const LdsPoly* curves, uint32_t cubic, float o[3], uint32_t& err) {
     double v[3], m[3];
-    uint32_t cnt[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) v[ch] = (double)o[ch];
     if (!curves[0].identity) {
-        knot_counts3(curves[0], v, cnt);
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) m[ch] = poly_eval(curves[0], cubic, v[ch], cnt[ch]);
+        for (int ch = 0; ch < 3; ++ch) m[ch] = poly_eval(curves[0], cubic, v[ch], knot_count_f(curves[0], o[ch]));
     } else {
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) m[ch] = v[ch];

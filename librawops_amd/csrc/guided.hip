@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(kStripThreads) gf_apply(const GuidedArgs g0) {
 constexpr int kTailKnots = 64;
 __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
     const GuidedArgs g = slice_of(g0);
-    __shared__ __attribute__((aligned(16))) double tables[4 * (kPolyRec + 1) * kTailKnots];
+    __shared__ __attribute__((aligned(16))) double tables[4 * poly_table_doubles(kTailKnots)];
     const size_t n = (size_t)g.vw * g.vh;
     Const<OpList>& L = as_const(*g.post);
     const uint32_t end = L.n;
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
     LdsPoly curves[4];
     const uint32_t cubic = as_const(*cv).cubic;
     if (staged) {
-        for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * (kPolyRec + 1) * kTailKnots);
+        for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * poly_table_doubles(kTailKnots));
         __syncthreads();
     }
     uint32_t err = 0;
@@ -949,7 +949,7 @@ __device__ __forceinline__ void apply_runs(const GuidedArgs& g, double* rows, do
         for (int c = 0; c < 4; ++c) staged = staged && (as_const(*cv).c[c].identity || as_const(*cv).c[c].n <= kTailKnots);
         cubic = as_const(*cv).cubic;
         if (staged)
-            for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * (kPolyRec + 1) * kTailKnots);
+            for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * poly_table_doubles(kTailKnots));
     }
     auto sink = [&](const Pix& px, int x, int i, const double S[6], int cols, int rws) {
         const double inv_n = inv_count<R>(cols, rws);
@@ -985,7 +985,7 @@ template <int R>
 __global__ void __launch_bounds__(kRunThreads, 3) gf_apply_runs(const GuidedArgs g0) {
     const GuidedArgs g = slice_of(g0);
     __shared__ double rows[6 * RunGeom<R>::plane];
-    __shared__ __attribute__((aligned(16))) double tables[4 * (kPolyRec + 1) * kTailKnots];
+    __shared__ __attribute__((aligned(16))) double tables[4 * poly_table_doubles(kTailKnots)];
     if (g.mip == 0)
         apply_runs<R, true>(g, rows, tables);
     else

@@ -71,82 +71,126 @@ __global__ void boost_bwd(const float* x, const float* base, const float* gy, fl
 // Bezier B(t) = sum_k b_k(t) P_k with P = (y_i, y_i + h T_i/3, y_i+1 - h T_i+1/3,
 // y_i+1); tangents T are the minmod of the adjacent secant slopes.  dL/dy_j
 // collects the Bernstein weights through P and T; dL/dx uses B'(t)/h.
-// LANE > 0 (curves of at most LANE knots): each thread accumulates into its
-// own LDS column, so values of one segment in a wavefront (common: the knots
-// are few) do not serialise on shared atomics; the columns are summed at the
-// end.  LANE == 0: shared LDS atomics.
+// Per element every contribution lands on the knots i-1 .. i+2 (the tangent
+// of knot j follows the secant k_j in {j-1, j} that the minmod picks, or
+// none, fixed by the curve and tabulated once per block), so the weights of
+// those four knots are formed in registers and added with four
+// read-modify-writes.  The segment is found by a binary search over float
+// thresholds: for a float v, v < x_k exactly when v < the smallest float >=
+// x_k.  LANE > 0 (curves of at most LANE knots): each thread accumulates into
+// its own LDS column, so values of one segment in a wavefront (common: the
+// knots are few) do not serialise on shared atomics; the columns are summed
+// at the end.  LANE == 0: shared LDS atomics.
 template <int LANE>
 __global__ void __launch_bounds__(kPW) curve_bwd(const float* xv, const float* gy, size_t n, const DevCurve* c,
                                                  uint32_t cubic, float* gx, double* gys) {
     constexpr int KT = LANE > 0 ? LANE : kMaxKnots;          // knots the tables hold
     __shared__ double acc[LANE > 0 ? LANE * kPW : kMaxKnots];
     __shared__ double tab[6 * KT];   // the knot tables (+ 1 / segment width), staged: lookups are data-dependent
+    __shared__ float xf[KT];         // the float thresholds, +inf beyond the last knot
+    __shared__ int src[KT];          // k_j - j (-1 or 0): the secant knot j's tangent follows, 1 for none
     __shared__ double part[LANE > 0 ? (kPW / 64) * LANE : 1];
     for (int k = threadIdx.x; k < (LANE > 0 ? LANE * kPW : kMaxKnots); k += kPW) acc[k] = 0.0;
     const uint32_t nk = as_const(*c).n;
-    for (uint32_t k = threadIdx.x; k < nk; k += kPW) {
+    uint32_t p2 = 1;
+    while (p2 < nk) p2 <<= 1;
+    for (uint32_t k = threadIdx.x; k < p2; k += kPW) {
+        xf[k] = k < nk ? float_at_least(c->x[k]) : __builtin_inff();
+        if (k >= nk) continue;
         tab[k] = c->x[k];
         tab[nk + k] = c->y[k];
         tab[2 * nk + k] = c->slope[k];
         tab[3 * nk + k] = c->c0[k];
         tab[4 * nk + k] = c->c1[k];
         tab[5 * nk + k] = k + 1 < nk ? 1.0 / (c->x[k + 1] - c->x[k]) : 0.0;
+        int sk;
+        if (k == 0) {
+            sk = 0;
+        } else if (k == nk - 1) {
+            sk = -1;
+        } else {
+            const double a = c->slope[k - 1], b = c->slope[k];
+            sk = (a == 0 || b == 0 || signbit(a) != signbit(b)) ? 1 : (fabs(a) <= fabs(b) ? -1 : 0);
+        }
+        src[k] = sk;
     }
     __syncthreads();
-    auto add = [&](uint32_t k, double w) {
-        if constexpr (LANE > 0) acc[k * kPW + threadIdx.x] += w;
-        else atomicAdd(&acc[k], w);
-    };
-    const LdsCurve l{nk, 0u, tab, tab + nk, tab + 2 * nk, tab + 3 * nk, tab + 4 * nk};
-    const LdsCurve* cl = &l;
+    const double *X = tab, *Y = tab + nk, *SL = tab + 2 * nk, *C0 = tab + 3 * nk, *C1 = tab + 4 * nk;
     const double* inv_h = tab + 5 * nk;   // 1 / (x[k+1] - x[k]): gradients need no quotient rounding
-    for (size_t e = blockIdx.x * (size_t)kPW + threadIdx.x; e < n; e += (size_t)gridDim.x * kPW) {
-        const double v = xv[e], g = gy[e];
-        uint32_t lo = 0, hi = nk;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (v < cl->x[mid]) hi = mid; else lo = mid + 1;
-        }
+    auto one = [&](size_t e, float vf, double g) {
+        const double v = vf;
+        // lo = the number of knots x_k with !(v < x_k) (all of them for NaN)
+        uint32_t lo = 0;
+        for (uint32_t step = p2 >> 1; step > 0; step >>= 1) lo += !(vf < xf[lo + step - 1]) ? step : 0u;
+        lo += !(vf < xf[lo]) ? 1u : 0u;
+        lo = lo < nk ? lo : nk;
         const uint32_t left = lo == 0 ? 0u : lo - 1u;
         const uint32_t i = left < nk - 2u ? left : nk - 2u;
-        const double h = cl->x[i + 1] - cl->x[i], ih = inv_h[i];
-        if (v == cl->x[left]) {            // exact knot: value y_left
-            add(left, g);
-            double slope = cl->slope[left < nk - 1 ? left : nk - 2];   // linear: right-hand secant
-            if (cubic) {                                                // cubic: the knot tangent
-                slope = left < nk - 1 ? 3.0 * (cl->c0[left] - cl->y[left]) / (cl->x[left + 1] - cl->x[left])
-                                      : 3.0 * (cl->y[left] - cl->c1[left - 1]) / (cl->x[left] - cl->x[left - 1]);
+        const double ih = inv_h[i];
+        // dL/dy of the knots i-1, i, i+1, i+2
+        double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+        const double gl = g, gl1 = left == i ? gl : 0.0, gl2 = left == i ? 0.0 : gl;   // dL/dy_left
+        float gxe;
+        if (v == X[left]) {               // exact knot: value y_left
+            w1 = gl1;
+            w2 = gl2;
+            double slope = SL[left < nk - 1 ? left : nk - 2];   // linear: right-hand secant
+            if (cubic) {                                       // cubic: the knot tangent
+                slope = left < nk - 1 ? 3.0 * (C0[left] - Y[left]) / (X[left + 1] - X[left])
+                                      : 3.0 * (Y[left] - C1[left - 1]) / (X[left] - X[left - 1]);
             }
-            gx[e] = (float)(g * slope);
-            continue;
-        }
-        if (!cubic || v < cl->x[0] || v > cl->x[nk - 1]) {
+            gxe = (float)(g * slope);
+        } else if (!cubic || v < X[0] || v > X[nk - 1]) {
             // y = y_left + (v - x_left) * s_i,  s_i = (y_i+1 - y_i) / h
-            const double dv = v - cl->x[left];
-            add(left, g);
-            add(i + 1, g * dv * ih);
-            add(i, -g * dv * ih);
-            gx[e] = (float)(g * cl->slope[i]);
-            continue;
+            const double dv = v - X[left];
+            const double q = g * dv * ih;
+            w1 = gl1 - q;
+            w2 = gl2 + q;
+            gxe = (float)(g * SL[i]);
+        } else {
+            const double h = X[i + 1] - X[i];
+            const double t = (v - X[i]) * ih, u = 1.0 - t;
+            const double b0 = u * u * u, b1 = 3 * u * u * t, b2 = 3 * u * t * t, b3 = t * t * t;
+            const double P0 = Y[i], P1 = C0[i], P2 = C1[i], P3 = Y[i + 1];
+            gxe = (float)(g * 3.0 * (u * u * (P1 - P0) + 2 * u * t * (P2 - P1) + t * t * (P3 - P2)) * ih);
+            w1 = g * (b0 + b1);
+            w2 = g * (b2 + b3);
+            // Tangent T_j -> secant k (minmod picks one of the adjacent secants,
+            // or none): dL/dy_k+1 += dT / h_k, dL/dy_k -= dT / h_k.
+            const double Ti = g * b1 * h * (1.0 / 3.0), Ti1 = -g * b2 * h * (1.0 / 3.0);
+            const int si = src[i], si1 = src[i + 1];
+            const double a = si == -1 ? Ti * inv_h[i > 0 ? i - 1 : 0] : 0.0;   // secant i-1
+            const double b = si == 0 ? Ti * ih : 0.0;                            // secant i
+            const double cc = si1 == -1 ? Ti1 * ih : 0.0;                        // secant i
+            const double d = si1 == 0 ? Ti1 * inv_h[i + 1 < nk - 1 ? i + 1 : i] : 0.0;   // secant i+1
+            w0 = -a;
+            w1 += a - b - cc;
+            w2 += b + cc - d;
+            w3 = d;
         }
-        const double t = (v - cl->x[i]) * ih, u = 1.0 - t;
-        const double b0 = u * u * u, b1 = 3 * u * u * t, b2 = 3 * u * t * t, b3 = t * t * t;
-        const double P0 = cl->y[i], P1 = cl->c0[i], P2 = cl->c1[i], P3 = cl->y[i + 1];
-        gx[e] = (float)(g * 3.0 * (u * u * (P1 - P0) + 2 * u * t * (P2 - P1) + t * t * (P3 - P2)) * ih);
-        add(i, g * (b0 + b1));
-        add(i + 1, g * (b2 + b3));
-        // Tangent T_j -> secant slopes (minmod picks one of them, or none).
-        auto tangent = [&](uint32_t j, double w) {   // w = dL/dT_j
-            if (j == 0) { add(1, w * inv_h[0]); add(0, -w * inv_h[0]); return; }
-            if (j == nk - 1) { add(nk - 1, w * inv_h[nk - 2]); add(nk - 2, -w * inv_h[nk - 2]); return; }
-            const double a = cl->slope[j - 1], b = cl->slope[j];
-            if (a == 0 || b == 0 || signbit(a) != signbit(b)) return;
-            const uint32_t k = fabs(a) <= fabs(b) ? j - 1 : j;
-            add(k + 1, w * inv_h[k]);
-            add(k, -w * inv_h[k]);
-        };
-        tangent(i, g * b1 * h * (1.0 / 3.0));
-        tangent(i + 1, -g * b2 * h * (1.0 / 3.0));
+        gx[e] = gxe;
+        const double w[4] = {w0, w1, w2, w3};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = (int)i - 1 + q;
+            if (j < 0 || j >= (int)nk) continue;
+            if constexpr (LANE > 0) acc[j * kPW + threadIdx.x] += w[q];
+            else if (w[q] != 0.0) atomicAdd(&acc[j], w[q]);
+        }
+    };
+    // Four elements per thread per round, their loads issued together.
+    const size_t stride = (size_t)gridDim.x * kPW;
+    for (size_t e0 = blockIdx.x * (size_t)kPW + threadIdx.x; e0 < n; e0 += 4 * stride) {
+        float vv[4], gg[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t e = min(e0 + u * stride, n - 1);
+            vv[u] = xv[e];
+            gg[u] = gy[e];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * stride < n) one(e0 + u * stride, vv[u], (double)gg[u]);
     }
     if constexpr (LANE > 0) {
         // sum the columns: per knot a wavefront reduction, then the waves

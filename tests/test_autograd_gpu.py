@@ -246,6 +246,36 @@ def test_tone_curve_gradients_vs_fd(cubic):
     assert rel(y.detach().cpu().numpy(), O.curve_map_f64(list(zip(kx, kyf)), vv, cubic)) <= 1e-6
 
 
+@pytest.mark.parametrize("cubic", [True, False])
+def test_tone_curve_gradients_at_knots_and_outside(cubic):
+    """Knot-ordinate gradients where the segment search has edge cases: inputs
+    exactly on knots (float-representable abscissae), just below and above
+    them, below the first and above the last knot, and many values in one
+    segment (several samples of a wavefront share knots)."""
+    knots = synth.monotone_knots(16, seed=9)
+    kx = [float(np.float32(k[0])) for k in knots]
+    ky = np.array([k[1] for k in knots])
+    xs = np.array(kx, np.float32)
+    vals = np.concatenate([xs, np.nextafter(xs, np.float32(-np.inf)), np.nextafter(xs, np.float32(np.inf)),
+                           np.float32([-0.5, -1e-3, 1.5, 7.0]),
+                           np.linspace(kx[3], kx[4], 64, dtype=np.float32)]).astype(np.float32)
+    rng = np.random.default_rng(5)
+    v = rng.permutation(np.resize(vals, 3 * 64 * 5)).reshape(64, 5, 3).astype(np.float32)
+    wgt = rng.normal(size=v.shape)
+    x = torch.from_numpy(v).cuda().requires_grad_(True)
+    kyt = torch.tensor(ky, dtype=torch.float32, device="cuda", requires_grad=True)
+    y = A.tone_curve(x, kyt, kx, cubic)
+    (y * torch.from_numpy(wgt.astype(np.float32)).cuda()).sum().backward()
+    kyf = kyt.detach().double().cpu().numpy()
+    vv = v.astype(np.float64)
+    f = lambda yy: float(np.sum(wgt * O.curve_map_f64(list(zip(kx, yy)), vv, cubic)))
+    fk = np.array([(f(kyf + 1e-6 * (np.arange(16) == j)) - f(kyf - 1e-6 * (np.arange(16) == j))) / 2e-6
+                   for j in range(16)])
+    assert rel(kyt.grad.cpu().numpy(), fk) <= 1e-4
+    assert rel(y.detach().cpu().numpy(), O.curve_map_f64(list(zip(kx, kyf)), vv, cubic)) <= 1e-6
+    assert np.isfinite(x.grad.cpu().numpy()).all()
+
+
 def test_point_op_gradients():
     rng = np.random.default_rng(3)
     v = rng.uniform(-0.1, 1.2, (5, 6, 3)).astype(np.float32)
