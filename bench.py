@@ -162,7 +162,8 @@ def leg_c1_bilinear(steps):
 
 def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
     """Working-Y guided filter + 0.7 EV + 16-point monotone curve on a
-    6000x4000 FP32 lognormal raster (Rec.2020 guide): fused forward, and
+    6000x4000 FP32 lognormal raster (Rec.2020 guide): the forward (two strip
+    passes, the exposure and the curve in the second), and
     the backward chain (curve, exposure, guided filter) for dL/dp, dL/deps,
     dL/dknots."""
     import ctypes
@@ -234,7 +235,8 @@ def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
     # crop of the raster, or the restatement's OpenMP bands without it.
     px = W * H
     res = {"workload": f"6000x4000 FP32 lognormal raster, Rec.2020 guide, r={radius}, eps={eps}, +0.7 EV, "
-                       "16-point cubic curve; fwd fused, bwd = curve + exposure + guided filter",
+                       "16-point cubic curve; fwd = coefficients + apply (exposure, curve), bwd = curve + exposure + "
+                       "guided filter",
            "forward_ms": round(fwd_ms, 4), "forward_gbs": round(24 * px / fwd_ms / 1e6, 1),
            "backward_chain_ms": round(bwd_ms, 4), "guided_backward_ms": round(gfb_ms, 4),
            "guided_backward_gbs": round(36 * px / gfb_ms / 1e6, 1),
