@@ -600,6 +600,43 @@ def main():
                                    + ": dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
     del gout, graw
 
+    # ---- the approximate form (rawops_set_menon_path 3): the Menon stages in
+    # FP32.  Its decisions and output are compared with the FP64 recording
+    # forward's just above (the reference's arithmetic); the headline below
+    # stays the FP64 kernel.
+    if not args.no_record:
+        ref_out, ref_dec = out.clone(), decisions.clone()
+        C.check(C.lib().rawops_set_menon_path(3, args.stream_band))
+        try:
+            forward_record()
+            torch.cuda.synchronize()
+            out32, dec32 = out.clone(), decisions.clone()
+            for _ in range(max(5, args.warmup)):
+                DeviceRender.run(compiled, area, level, out=out, flags=flags, stream=stream)
+            s32 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+            e32 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+            for i in range(args.steps):
+                s32[i].record(stream)
+                DeviceRender.run(compiled, area, level, out=out, flags=flags, stream=stream)
+                e32[i].record(stream)
+            torch.cuda.synchronize()
+            f32_ms = sum(a.elapsed_time(b) for a, b in zip(s32, e32)) / args.steps
+        finally:
+            C.check(C.lib().rawops_set_menon_path(args.menon_path, args.stream_band))
+        assert int(flags.item()) == 0
+        flips = int(np.unpackbits(torch.bitwise_xor(ref_dec, dec32).cpu().numpy()).sum())
+        sites = px // 2   # one decision per non-green photosite
+        dev = (out32 - ref_out).abs()
+        line["menon_fp32_variant"] = {
+            "kernel_ms": round(f32_ms, 4),
+            "frac": round(BYTES_PER_PX * px / (f32_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "flip_rate": flips / sites, "flips": flips, "non_green_sites": sites,
+            "max_abs": float(dev.max()),
+            "pixels_over_1e-5": int((dev > 1e-5).any(dim=2).sum()),
+            "basis": "rawops_set_menon_path(3): Menon stages in FP32; decisions and output against the FP64 "
+                     "recording forward (the reference's arithmetic) on the whole 24 MP frame"}
+        del ref_out, ref_dec, out32, dec32, dev
+
     # ---- histogram_rgb of the rendered frame (256 bins of [0, 1], 12 B/px read once)
     hbins = 256
     hcounts = torch.zeros(3 * hbins, dtype=torch.int64, device="cuda")

@@ -157,9 +157,13 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* source
  * switch, process-wide).  path 0: automatic (the streaming kernel for
  * viewports large enough to fill the device, about 2.75 MP and up, the tile
  * kernel below); 1: the streaming kernel wherever the active area is at
- * least 16 x 16; 2: the tile kernel only.  band_rows: rows each streaming
- * wavefront renders (0 = 16, else a multiple of 4 in 4..4096).  Every path
- * renders the same bits. */
+ * least 16 x 16; 2: the tile kernel only; 3: as 1, with the Menon stages of
+ * linear chains (white balance, exposure, 3x3) computed in FP32 instead of
+ * the reference's FP64: an approximation whose classifier decisions and
+ * outputs differ from the reference's (bench.py reports its flip rate and
+ * largest deviation).  band_rows: rows each streaming wavefront renders
+ * (0 = 16, else a multiple of 4 in 4..4096).  Paths 0, 1 and 2 render the
+ * same bits. */
 rawops_status rawops_set_menon_path(uint32_t path, uint32_t band_rows);
 
 /* Reads a device_flags word (synchronising `stream`) and resets it to zero. */

@@ -448,7 +448,9 @@ rawops_status rawops_render(const rawops_plan* plan, const rawops_source* src, r
 
 rawops_status rawops_set_menon_path(uint32_t path, uint32_t band_rows) {
     return guarded([&] {
-        if (path > 2) fail(RAWOPS_INVALID_ARGUMENT, "Menon render path must be 0 (automatic), 1 (streaming) or 2 (tiles)");
+        if (path > 3)
+            fail(RAWOPS_INVALID_ARGUMENT,
+                 "Menon render path must be 0 (automatic), 1 (streaming), 2 (tiles) or 3 (streaming, FP32 stages)");
         if (band_rows && (band_rows < 4 || band_rows > 4096 || band_rows % 4))
             fail(RAWOPS_INVALID_ARGUMENT, "streaming band rows must be 0 or a multiple of 4 in 4..4096");
         g_menon_path.store(path, std::memory_order_relaxed);

@@ -217,21 +217,25 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
 //
 // For the knot search of float inputs the curve also keeps, per knot, the
 // smallest float >= x_k (for a float v, v >= x_k exactly when v >= that
-// float), padded with NaN to a power of two p2 >= n, so that the number of
-// knots <= v is a binary search over floats.
+// float), padded with NaN to a power of two p2 >= max(n, 4), so that the
+// number of knots <= v is a search over floats: for p2 <= 16 two levels
+// (thresholds 4, 8 and 12 held in registers pick a group of four, whose
+// thresholds are one 16-byte LDS read), a binary search otherwise.
 struct LdsPoly {
     uint32_t n, identity;
     const double* x;
     const double* rec;   // 8 doubles per knot, 16-byte aligned
-    const float* xf;     // p2 float thresholds
+    const float* xf;     // p2 float thresholds, 16-byte aligned
     uint32_t p2;
+    float t4, t8, t12;   // xf[4], xf[8], xf[12] (NaN at k >= n)
+    double x_first, x_last;
 };
 
 constexpr int kPolyRec = 8;
 
 // LDS doubles of one staged curve of at most cap knots (even, so that
 // consecutive curves stay 16-byte aligned).
-constexpr int poly_table_doubles(int cap) { return ((kPolyRec + 1) * cap + (cap + 1) / 2 + 1) & ~1; }
+constexpr int poly_table_doubles(int cap) { return ((kPolyRec + 1) * cap + (cap + 1) / 2 + 3) & ~1; }
 
 // The smallest float >= x (x finite; +-FLT_MAX overflow to +-inf is handled
 // by the step toward +inf).
@@ -249,10 +253,13 @@ __device__ __forceinline__ float float_at_least(double x) {
 __device__ __forceinline__ LdsPoly stage_poly(const DevCurve* c, double* base) {
     Const<DevCurve>& s = as_const(*c);
     const uint32_t n = s.n;
-    uint32_t p2 = 1;
+    uint32_t p2 = 4;
     while (p2 < n) p2 <<= 1;
-    float* xf = reinterpret_cast<float*>(base + (kPolyRec + 1) * n);
-    LdsPoly l{n, s.identity, base + kPolyRec * n, base, xf, p2};
+    float* xf = reinterpret_cast<float*>(base + (((kPolyRec + 1) * n + 1) & ~1u));
+    const float nan = __builtin_nanf("");
+    LdsPoly l{n, s.identity, base + kPolyRec * n, base, xf, p2,
+              n > 4 ? float_at_least(s.x[4]) : nan, n > 8 ? float_at_least(s.x[8]) : nan,
+              n > 12 ? float_at_least(s.x[12]) : nan, s.identity ? 0.0 : s.x[0], s.identity ? 0.0 : s.x[n - 1]};
     if (!l.identity)
         for (uint32_t k = threadIdx.x; k < p2; k += blockDim.x) xf[k] = k < n ? float_at_least(s.x[k]) : __builtin_nanf("");
     if (!l.identity)
@@ -286,13 +293,18 @@ __device__ __forceinline__ double poly_eval(const LdsPoly& c, uint32_t cubic, do
     const double lin = sl == 0 ? yl : yl + (v - xl) * sl;
     const double t = (v - xl) * inv;
     const double cub = __builtin_fma(__builtin_fma(__builtin_fma(q23.y, t, q23.x), t, q01.y), t, q01.x);
-    const bool linear = !cubic || v < c.x[0] || v > c.x[c.n - 1];
+    const bool linear = !cubic || v < c.x_first || v > c.x_last;
     return v == xl ? yl : (linear ? lin : cub);
 }
 
 // Number of knots <= v for a float v: a binary search over the float
 // thresholds (NaN padding and a NaN v compare false).
 __device__ __forceinline__ uint32_t knot_count_f(const LdsPoly& c, float v) {
+    if (c.p2 <= 16) {
+        const uint32_t g = (v >= c.t4 ? 1u : 0u) + (v >= c.t8 ? 1u : 0u) + (v >= c.t12 ? 1u : 0u);
+        const float4 q = reinterpret_cast<const float4*>(c.xf)[g];
+        return 4u * g + (v >= q.x ? 1u : 0u) + (v >= q.y ? 1u : 0u) + (v >= q.z ? 1u : 0u) + (v >= q.w ? 1u : 0u);
+    }
     uint32_t cnt = 0;
     for (uint32_t s = c.p2 >> 1; s > 0; s >>= 1) cnt += v >= c.xf[cnt + s - 1] ? s : 0u;
     return cnt + (v >= c.xf[cnt] ? 1u : 0u);

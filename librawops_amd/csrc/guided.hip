@@ -379,8 +379,10 @@ template <int R> struct RunGeom {
     static_assert(plane >= used, "run plane");
 };
 
-// s[t] = sum_{i=0}^{2R} b[run_slot(i0 + t + i)], t < kRunLen, as sliding sums
-// (two partial sums for the first window, so the adds overlap).
+// s[t] = sum_{i=0}^{2R} b[run_slot(i0 + t + i)], t < kRunLen, as sliding sums:
+// the first window as two partial sums (so the adds overlap), then the
+// entering-minus-leaving differences, formed independently of each other, so
+// the dependent chain is one add per centre.
 template <int R>
 __device__ __forceinline__ void run_sums16(const double* __restrict__ b, int i0, double s[kRunLen]) {
     double e = 0.0, o = 0.0;
@@ -388,32 +390,37 @@ __device__ __forceinline__ void run_sums16(const double* __restrict__ b, int i0,
     for (int i = 0; i <= 2 * R; i += 2) e += b[run_slot(i0 + i)];
 #pragma unroll
     for (int i = 1; i <= 2 * R; i += 2) o += b[run_slot(i0 + i)];
+    double d[kRunLen];
+#pragma unroll
+    for (int t = 1; t < kRunLen; ++t) d[t] = b[run_slot(i0 + 2 * R + t)] - b[run_slot(i0 + t - 1)];
     double acc = e + o;
     s[0] = acc;
 #pragma unroll
     for (int t = 1; t < kRunLen; ++t) {
-        acc = acc + b[run_slot(i0 + 2 * R + t)] - b[run_slot(i0 + t - 1)];
+        acc += d[t];
         s[t] = acc;
     }
 }
 
-template <int N, int R, class Fetch, class Eval, class Centre, class Sink>
+template <int N, int R, int RP, class Fetch, class Eval, class Centre, class Sink>
 __device__ __forceinline__ void strip_runs(int LW, int LH, int c_lo, int c_hi, int r_lo, int r_hi, int strip_x,
-                                           int xend, int y0, int y1, double* rows /* [N][plane] */, Fetch fetch,
+                                           int xend, int y0, int y1, double* rows /* [RP][N][plane] */, Fetch fetch,
                                            Eval eval, Centre centre, Sink sink) {
-    static_assert(N * (kRunCols / kRunLen) <= kRunThreads, "one sum run per thread");
+    static_assert(RP == 1 || RP == 2, "rows per step");
     constexpr int P = RunGeom<R>::plane;
+    constexpr int kRuns = kRunCols / kRunLen;          // runs of 16 centres per value and row
+    constexpr int kTasks = RP * N * kRuns;             // (row, value, run) sum tasks per step
+    constexpr int kPasses = (kTasks + kRunThreads - 1) / kRunThreads;
     const int t = threadIdx.x;
     const int x = strip_x - R + t;
     const bool own = x >= c_lo && x < c_hi;
     const int xc = min(max(x, c_lo), c_hi - 1);
     const int slot = run_slot(t + R);
-    const int vm = t / (kRunCols / kRunLen), vj = t % (kRunCols / kRunLen);   // value, run of the sums along x
     auto row_of = [&](int y) { return min(max(y, r_lo), r_hi - 1); };
     // Rows that enter the sums: the image rows of the data region (every
     // clipped window of the band's centres lies inside it).
     const int row_lo = max(0, r_lo), row_hi = min(LH, r_hi);
-    for (int i = t; i < N * P; i += kRunThreads) rows[i] = 0.0;
+    for (int i = t; i < RP * N * P; i += kRunThreads) rows[i] = 0.0;
     // Warm-up: the window of row y0 - 1, four rows of loads in flight.
     double V[N];
 #pragma unroll
@@ -433,15 +440,10 @@ __device__ __forceinline__ void strip_runs(int LW, int LH, int c_lo, int c_hi, i
             }
         }
     }
-    // Every step adds row k + R and drops row k - R - 1; the rows of steps
-    // k and k + 1 are in flight (loads are unconditional, at clamped rows, so
-    // the compiler's count of loads in flight stays exact).
     const bool is_centre = t >= R && t < kRunCols - R && x < xend;
-    // One step: row k with its entering / leaving rows in en / lv and the
-    // centre data in cd; then those buffers are refilled for step k + 2.
-    // Steps alternate between two sets of buffers (no register copies, so
-    // the loads of the next two steps stay in flight).
-    auto step = [&](int k, auto& en, auto& lv, auto& cd) {
+    // Window of row k: add row k + R, drop row k - R - 1 (data in en / lv),
+    // then publish the N sums in slot t of the LDS rows at dst.
+    auto slide = [&](int k, const auto& en, const auto& lv, double* dst) {
         if (own) {
             double v[N];
             if (k + R < row_hi) {
@@ -455,47 +457,94 @@ __device__ __forceinline__ void strip_runs(int LW, int LH, int c_lo, int c_hi, i
                 for (int m = 0; m < N; ++m) V[m] -= v[m];
             }
         }
-        en = fetch(xc, row_of(k + 2 + R));
-        lv = fetch(xc, row_of(k + 1 - R));
 #pragma unroll
-        for (int m = 0; m < N; ++m) rows[m * P + slot] = V[m];
-        __syncthreads();
-        if (vm < N) {
-            double* b = rows + vm * P;
-            double Hs[kRunLen];
-            run_sums16<R>(b, kRunLen * vj, Hs);
-            // Every lane's reads of the wavefront's rows are done before any
-            // lane overwrites them.
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int m = 0; m < N; ++m) dst[m * P + slot] = V[m];
+    };
+    // Sums along x of every published row: task u = 16 (RP-row value index)
+    // + run; the 16 tasks of one LDS row are 16 consecutive lanes of one
+    // wavefront, so a wavefront fence orders all reads of a row before its
+    // in-place writes.
+    auto runs = [&]() {
 #pragma unroll
-            for (int u = 0; u < kRunLen; ++u) b[run_slot(kRunLen * vj + u + R)] = Hs[u];
+        for (int pass = 0; pass < kPasses; ++pass) {
+            const int u = t + pass * kRunThreads;
+            if (kTasks % kRunThreads == 0 || u < kTasks) {
+                double* b = rows + (u / kRuns) * P;
+                const int vj = u % kRuns;
+                double Hs[kRunLen];
+                run_sums16<R>(b, kRunLen * vj, Hs);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int i = 0; i < kRunLen; ++i) b[run_slot(kRunLen * vj + i + R)] = Hs[i];
+            }
         }
-        __syncthreads();
+    };
+    auto finish = [&](int k, const double* src, const auto& cd) {
         if (is_centre) {
             double S[N];
 #pragma unroll
-            for (int m = 0; m < N; ++m) S[m] = rows[m * P + slot];
+            for (int m = 0; m < N; ++m) S[m] = src[m * P + slot];
             const int cols = min(LW, x + R + 1) - max(0, x - R);
             const int rws = min(LH, k + R + 1) - max(0, k - R);
             sink(cd, x, k, S, cols, rws);
         }
-        cd = centre(xc, row_of(k + 2));
     };
-    auto en0 = fetch(xc, row_of(y0 + R));
-    auto lv0 = fetch(xc, row_of(y0 - R - 1));
-    auto cd0 = centre(xc, row_of(y0));
-    auto en1 = fetch(xc, row_of(y0 + 1 + R));
-    auto lv1 = fetch(xc, row_of(y0 - R));
-    auto cd1 = centre(xc, row_of(y0 + 1));
-    __syncthreads();   // pads zeroed before the first row is written
-    int k = y0;
-    for (; k + 1 < y1; k += 2) {
-        step(k, en0, lv0, cd0);
-        step(k + 1, en1, lv1, cd1);
+    if constexpr (RP == 1) {
+        // One row per step; steps alternate between two sets of buffers (no
+        // register copies, so the loads of the next two steps stay in flight).
+        auto step = [&](int k, auto& en, auto& lv, auto& cd) {
+            slide(k, en, lv, rows);
+            en = fetch(xc, row_of(k + 2 + R));
+            lv = fetch(xc, row_of(k + 1 - R));
+            __syncthreads();
+            runs();
+            __syncthreads();
+            finish(k, rows, cd);
+            cd = centre(xc, row_of(k + 2));
+        };
+        auto en0 = fetch(xc, row_of(y0 + R));
+        auto lv0 = fetch(xc, row_of(y0 - R - 1));
+        auto cd0 = centre(xc, row_of(y0));
+        auto en1 = fetch(xc, row_of(y0 + 1 + R));
+        auto lv1 = fetch(xc, row_of(y0 - R));
+        auto cd1 = centre(xc, row_of(y0 + 1));
+        __syncthreads();   // pads zeroed before the first row is written
+        int k = y0;
+        for (; k + 1 < y1; k += 2) {
+            step(k, en0, lv0, cd0);
+            step(k + 1, en1, lv1, cd1);
+        }
+        if (k < y1) step(k, en0, lv0, cd0);
+    } else {
+        // Two rows per step (two barriers per row pair): rows k and k + 1
+        // are published to two LDS row sets, their sums along x formed by
+        // all threads at once; the next pair's loads are issued before the
+        // first barrier.
+        auto en0 = fetch(xc, row_of(y0 + R));
+        auto lv0 = fetch(xc, row_of(y0 - R - 1));
+        auto en1 = fetch(xc, row_of(y0 + 1 + R));
+        auto lv1 = fetch(xc, row_of(y0 - R));
+        auto cd0 = centre(xc, row_of(y0));
+        auto cd1 = centre(xc, row_of(y0 + 1));
+        __syncthreads();   // pads zeroed before the first row is written
+        for (int k = y0; k < y1; k += 2) {
+            slide(k, en0, lv0, rows);
+            slide(k + 1, en1, lv1, rows + N * P);
+            en0 = fetch(xc, row_of(k + 2 + R));
+            lv0 = fetch(xc, row_of(k + 1 - R));
+            en1 = fetch(xc, row_of(k + 3 + R));
+            lv1 = fetch(xc, row_of(k + 2 - R));
+            __syncthreads();
+            runs();
+            __syncthreads();
+            finish(k, rows, cd0);
+            if (k + 1 < y1) finish(k + 1, rows + N * P, cd1);
+            cd0 = centre(xc, row_of(k + 2));
+            cd1 = centre(xc, row_of(k + 3));
+        }
     }
-    if (k < y1) step(k, en0, lv0, cd0);
 }
 
 // Radius 0: the filter is the identity (SpatialOps.cpp:180); only the
@@ -677,6 +726,12 @@ __global__ void __launch_bounds__(kStripThreads) gf_back_pixels(const GuidedBack
 // for unclipped windows and a division only where the window is clipped.
 constexpr int kRunMaxR = 16;
 
+// Rows per strip_runs step, by pass (r = 8 on 6000 x 4000, µs, one / two
+// rows per step): coefficients 325 / 277 (eight values: one row keeps half
+// the threads idle in the sums along x), apply 548 / 558, backward centres
+// 488 / 549 (two rows need 68 KB of LDS), backward pixels 601 / 610.
+constexpr int kRpCoef = 2, kRpApply = 1, kRpCentres = 1, kRpPixels = 1;
+
 template <int R> struct RunWindow {
     static constexpr int side = 2 * R + 1;
     static constexpr double inv_full = 1.0 / (double)(side * side);
@@ -708,7 +763,7 @@ __device__ __forceinline__ BackPixel back_pixel(const GuidedBackArgs& g, int x, 
 template <int R>
 __global__ void __launch_bounds__(kRunThreads, 3) gf_back_centres_runs(const GuidedBackArgs g0) {
     const GuidedBackArgs g = slice_of(g0);
-    __shared__ double rows[14 * RunGeom<R>::plane];
+    __shared__ double rows[kRpCentres * 14 * RunGeom<R>::plane];
     __shared__ double eps_part[kRunThreads / 64];
     const int span = kRunCols - 2 * R;
     const int strip_x = (int)blockIdx.x * span;
@@ -769,7 +824,7 @@ __global__ void __launch_bounds__(kRunThreads, 3) gf_back_centres_runs(const Gui
         w[6 * plane] = (float)(gmy * inv_n);
         w[7 * plane] = (float)(gD * inv_n);
     };
-    strip_runs<14, R>(g.W, g.H, 0, g.W, 0, g.H, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
+    strip_runs<14, R, kRpCentres>(g.W, g.H, 0, g.W, 0, g.H, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
     for (int off = 32; off > 0; off >>= 1) geps += __shfl_xor(geps, off, 64);
     if ((threadIdx.x & 63) == 0) eps_part[threadIdx.x >> 6] = geps;
     __syncthreads();
@@ -787,7 +842,7 @@ struct BackTerms {
 template <int R>
 __global__ void __launch_bounds__(kRunThreads, 3) gf_back_pixels_runs(const GuidedBackArgs g0) {
     const GuidedBackArgs g = slice_of(g0);
-    __shared__ double rows[11 * RunGeom<R>::plane];
+    __shared__ double rows[kRpPixels * 11 * RunGeom<R>::plane];
     const int span = kRunCols - 2 * R;
     const int strip_x = (int)blockIdx.x * span;
     const int y0 = (int)blockIdx.y * g.band, y1 = min(y0 + g.band, g.H);
@@ -827,7 +882,7 @@ __global__ void __launch_bounds__(kRunThreads, 3) gf_back_pixels_runs(const Guid
         float* out = g.gp + ((size_t)y * g.W + (size_t)x) * 3;
         out[0] = (float)gp[0]; out[1] = (float)gp[1]; out[2] = (float)gp[2];
     };
-    strip_runs<11, R>(g.W, g.H, 0, g.W, 0, g.H, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
+    strip_runs<11, R, kRpPixels>(g.W, g.H, 0, g.W, 0, g.H, strip_x, g.W, y0, y1, rows, fetch, eval, centre, sink);
 }
 
 // The two-pass forward at a compile-time radius R (1..kRunMaxR) over
@@ -892,7 +947,7 @@ __device__ __forceinline__ void coef_runs(const GuidedArgs& g, double* rows) {
     };
     // Only the pixels of the centres' windows enter the sums.
     const int c_lo = max(0, g.cx0 - R), c_hi = min(g.LW, g.cx0 + g.cw + R);
-    strip_runs<8, R>(g.LW, g.LH, c_lo, c_hi, 0, g.LH, strip_x, g.cx0 + g.cw, k0, k1, rows, fetch, eval, centre,
+    strip_runs<8, R, kRpCoef>(g.LW, g.LH, c_lo, c_hi, 0, g.LH, strip_x, g.cx0 + g.cw, k0, k1, rows, fetch, eval, centre,
                      sink);
     if (err && g.flags) atomicOr(g.flags, err);
 }
@@ -900,7 +955,7 @@ __device__ __forceinline__ void coef_runs(const GuidedArgs& g, double* rows) {
 template <int R>
 __global__ void __launch_bounds__(kRunThreads, 3) gf_coef_runs(const GuidedArgs g0) {
     const GuidedArgs g = slice_of(g0);
-    __shared__ double rows[8 * RunGeom<R>::plane];
+    __shared__ double rows[kRpCoef * 8 * RunGeom<R>::plane];
     if (g.mip == 0)
         coef_runs<R, true>(g, rows);
     else
@@ -976,7 +1031,7 @@ __device__ __forceinline__ void apply_runs(const GuidedArgs& g, double* rows, do
         float* d = g.out + ((size_t)(i - g.vy) * g.ostride + (size_t)(x - g.vx)) * 3;
         d[0] = o[0]; d[1] = o[1]; d[2] = o[2];
     };
-    strip_runs<6, R>(g.LW, g.LH, g.cx0, g.cx0 + g.cw, g.cy0, g.cy0 + g.ch, strip_x, g.vx + g.vw, i0, i1, rows, fetch,
+    strip_runs<6, R, kRpApply>(g.LW, g.LH, g.cx0, g.cx0 + g.cw, g.cy0, g.cy0 + g.ch, strip_x, g.vx + g.vw, i0, i1, rows, fetch,
                      eval, centre, sink);
     if (err && g.flags) atomicOr(g.flags, err);
 }
@@ -984,7 +1039,7 @@ __device__ __forceinline__ void apply_runs(const GuidedArgs& g, double* rows, do
 template <int R>
 __global__ void __launch_bounds__(kRunThreads, 3) gf_apply_runs(const GuidedArgs g0) {
     const GuidedArgs g = slice_of(g0);
-    __shared__ double rows[6 * RunGeom<R>::plane];
+    __shared__ double rows[kRpApply * 6 * RunGeom<R>::plane];
     __shared__ __attribute__((aligned(16))) double tables[4 * poly_table_doubles(kTailKnots)];
     if (g.mip == 0)
         apply_runs<R, true>(g, rows, tables);
@@ -993,7 +1048,7 @@ __global__ void __launch_bounds__(kRunThreads, 3) gf_apply_runs(const GuidedArgs
 }
 
 // Static LDS at the largest radius: four workgroups of either pass fit one CU.
-static_assert(sizeof(double) * (14 * RunGeom<kRunMaxR>::plane + 4) <= 160 * 1024 / 4, "gf_back_centres_runs LDS");
+static_assert(sizeof(double) * (kRpCentres * 14 * RunGeom<kRunMaxR>::plane + 4) <= 160 * 1024 / 4, "gf_back_centres_runs LDS");
 
 // Rows per band for the strip_runs passes: about three workgroups per CU
 // over one image, at least 32 rows (each band repeats the 2R + 1 row

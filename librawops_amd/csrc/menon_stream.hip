@@ -50,6 +50,15 @@ __device__ __forceinline__ double next_d(double v) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// The same moves for the element type of the windows (FP64, or FP32 in the
+// approximate form).
+__device__ __forceinline__ float prev_t(float v) { return prev_f(v); }
+__device__ __forceinline__ double prev_t(double v) { return prev_d(v); }
+__device__ __forceinline__ float next_t(float v) { return next_f(v); }
+__device__ __forceinline__ double next_t(double v) { return next_d(v); }
+__device__ __forceinline__ float abs_t(float v) { return __builtin_fabsf(v); }
+__device__ __forceinline__ double abs_t(double v) { return __builtin_fabs(v); }
+
 // Whole-sample reflection into [lo, hi] (MenonDemosaic.cpp:60-66) for a
 // coordinate at most one area size outside, then clamped so that far-out
 // positions (whose values no output reads) still load in bounds.
@@ -64,6 +73,10 @@ __device__ __forceinline__ double lane_read(double v, int src) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)b);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(b >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ float lane_read(float v, int src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, v)));
 }
 
 // N0: lane column (0 = xA, 1 = xA + 1) of the non-green site in rows of even
@@ -81,7 +94,7 @@ __device__ __forceinline__ double lane_read(double v, int src) {
 // column's lane, and the windows' rows outside the area from their mirror
 // rows where the next stage reads them.  (Needs an area of at least 16 x 16,
 // so that one reflection reaches every position a result depends on.)
-template <int N0, int RED0, int OPS, bool E, bool REC>
+template <int N0, int RED0, int OPS, bool E, bool REC, typename T>
 struct Strip {
     static constexpr int ncol(int q) { return N0 ^ q; }
     static constexpr bool red_row(int q) { return (RED0 ^ q) != 0; }
@@ -98,12 +111,12 @@ struct Strip {
 
     // Windows, index k = row s + k (red / blue: row s + k - 1).
     float rn[7] = {}, rg[7] = {};     // normalised samples at the non-green / green site
-    double gh[3] = {}, gv[3] = {};    // directional greens, rows s+2..s+4 (index k - 2)
-    double dh[5] = {}, dhp[5] = {};   // horizontal gradients (own, lane L-1), rows s..s+4
-    double dv[3] = {}, dvp[3] = {}, dvn[3] = {};   // vertical gradients (own, L-1, L+1), rows s..s+2
-    double gr[3] = {};                // selected green at the non-green site, rows s..s+2
+    T gh[3] = {}, gv[3] = {};    // directional greens, rows s+2..s+4 (index k - 2)
+    T dh[5] = {}, dhp[5] = {};   // horizontal gradients (own, lane L-1), rows s..s+4
+    T dv[3] = {}, dvp[3] = {}, dvn[3] = {};   // vertical gradients (own, L-1, L+1), rows s..s+2
+    T gr[3] = {};                // selected green at the non-green site, rows s..s+2
     bool hz[3] = {};                  // classifier decision (horizontal), rows s..s+2
-    double red[3] = {}, blue[3] = {}; // red / blue at the green site, rows s-1..s+1
+    T red[3] = {}, blue[3] = {}; // red / blue at the green site, rows s-1..s+1
     // Codes at xA / xA + 1 of rows s+6 .. s+5+kPrefetch, each straight from
     // its 16-bit load (combining the two halves would make the loop wait for
     // the loads it has just issued).
@@ -191,21 +204,21 @@ struct Strip {
             rg[6] = ncol(Q) ? fa : fb;
         }
         // ---- directional greens, ch, dh at row s + 4 (MenonDemosaic.cpp:107-119) ----
-        double gh4, gv4, dh4, dhp4;
+        T gh4, gv4, dh4, dhp4;
         {
-            const double own = rn[4];
+            const T own = rn[4];
             const float np = prev_f(rn[4]), nn = next_f(rn[4]);
             float gl, gright;
             if (ncol(Q) == 0) { gl = prev_f(rg[4]); gright = rg[4]; }
             else { gl = rg[4]; gright = next_f(rg[4]); }
-            gh4 = 0.5 * ((double)gl + (double)gright) + 0.5 * own - 0.25 * ((double)np + (double)nn);
-            gv4 = 0.5 * ((double)rg[3] + (double)rg[5]) + 0.5 * own - 0.25 * ((double)rn[2] + (double)rn[6]);
-            const double ch = own - gh4;
-            dh4 = fabs(ch - next_d(ch));          // dh = |ch(x) - ch(x + 2)|
+            gh4 = T(0.5) * ((T)gl + (T)gright) + T(0.5) * own - T(0.25) * ((T)np + (T)nn);
+            gv4 = T(0.5) * ((T)rg[3] + (T)rg[5]) + T(0.5) * own - T(0.25) * ((T)rn[2] + (T)rn[6]);
+            const T ch = own - gh4;
+            dh4 = abs_t(ch - next_t(ch));          // dh = |ch(x) - ch(x + 2)|
             if constexpr (E) {
                 if (out_n[Q] || row_out(s + 4)) dh4 = 0.0;   // read as zero outside the area
             }
-            dhp4 = prev_d(dh4);
+            dhp4 = prev_t(dh4);
         }
         gh[2] = gh4;
         gv[2] = gv4;
@@ -213,34 +226,34 @@ struct Strip {
         dhp[4] = dhp4;
         // ---- cv, dv at row s + 2: dv = |cv(y) - cv(y + 2)| ------------------
         if constexpr ((ST & ST_DV) != 0) {
-            const double cv2 = (double)rn[2] - gv[0];
-            const double cv4 = (double)rn[4] - gv[2];
-            double d = fabs(cv2 - cv4);
+            const T cv2 = (T)rn[2] - gv[0];
+            const T cv4 = (T)rn[4] - gv[2];
+            T d = abs_t(cv2 - cv4);
             if constexpr (E) {
                 if (out_n[Q] || row_out(s + 2)) d = 0.0;
             }
             dv[2] = d;
-            dvp[2] = prev_d(d);
-            dvn[2] = next_d(d);
+            dvp[2] = prev_t(d);
+            dvn[2] = next_t(d);
         }
         // ---- classifier at row s + 2 (MenonDemosaic.cpp:126-147) ---------
         // Terms in the reference's order, weights {1,1,1,3,3,1,1,1}; the
         // gradients are >= +0, so the first term equals 0 + term.
         if constexpr ((ST & ST_CLS) != 0) {
             constexpr int n = ncol(Q);
-            double h = dh[4];                      // ( 0,  2)
+            T h = dh[4];                      // ( 0,  2)
             h += dhp[4];                           // (-2,  2)
             h += n == 0 ? dhp[3] : dh[3];          // (-1,  1)
-            h += 3.0 * dh[2];                      // ( 0,  0)
-            h += 3.0 * dhp[2];                     // (-2,  0)
+            h += T(3.0) * dh[2];                      // ( 0,  0)
+            h += T(3.0) * dhp[2];                     // (-2,  0)
             h += n == 0 ? dhp[1] : dh[1];          // (-1, -1)
             h += dh[0];                            // ( 0, -2)
             h += dhp[0];                           // (-2, -2)
-            double v = dvn[2];                     // ( 2,  0)
+            T v = dvn[2];                     // ( 2,  0)
             v += dvn[0];                           // ( 2, -2)
             v += n == 0 ? dv[1] : dvn[1];          // ( 1, -1)
-            v += 3.0 * dv[2];                      // ( 0,  0)
-            v += 3.0 * dv[0];                      // ( 0, -2)
+            v += T(3.0) * dv[2];                      // ( 0,  0)
+            v += T(3.0) * dv[0];                      // ( 0, -2)
             v += n == 0 ? dvp[1] : dv[1];          // (-1, -1)
             v += dvp[2];                           // (-2,  0)
             v += dvp[0];                           // (-2, -2)
@@ -258,13 +271,13 @@ struct Strip {
                 if (s + 1 == row_lo) gr[0] = gr[2];
                 if (s + 1 == row_hi) gr[2] = gr[0];
             }
-            const double own = rg[1];
+            const T own = rg[1];
             float rl, rr;
-            double gl, grt;
-            if (ncol(q1) == 0) { rl = rn[1]; gl = gr[1]; rr = next_f(rn[1]); grt = next_d(gr[1]); }
-            else { rl = prev_f(rn[1]); gl = prev_d(gr[1]); rr = rn[1]; grt = gr[1]; }
-            const double ch = own + 0.5 * ((double)rl + (double)rr) - 0.5 * (gl + grt);
-            const double cv = own + 0.5 * ((double)rn[0] + (double)rn[2]) - 0.5 * (gr[0] + gr[2]);
+            T gl, grt;
+            if (ncol(q1) == 0) { rl = rn[1]; gl = gr[1]; rr = next_f(rn[1]); grt = next_t(gr[1]); }
+            else { rl = prev_f(rn[1]); gl = prev_t(gr[1]); rr = rn[1]; grt = gr[1]; }
+            const T ch = own + T(0.5) * ((T)rl + (T)rr) - T(0.5) * (gl + grt);
+            const T cv = own + T(0.5) * ((T)rn[0] + (T)rn[2]) - T(0.5) * (gr[0] + gr[2]);
             red[2] = red_row(q1) ? ch : cv;
             blue[2] = red_row(q1) ? cv : ch;
         }
@@ -317,13 +330,13 @@ struct Strip {
         // Red / blue along the decision at the non-green site; the two
         // horizontal neighbours are green sites of row s, the two vertical
         // ones green sites of rows s -+ 1 in this column.
-        double hl_r, hl_b, hr_r, hr_b;
-        if (n == 0) { hl_r = prev_d(red[1]); hl_b = prev_d(blue[1]); hr_r = red[1]; hr_b = blue[1]; }
-        else { hl_r = red[1]; hl_b = blue[1]; hr_r = next_d(red[1]); hr_b = next_d(blue[1]); }
+        T hl_r, hl_b, hr_r, hr_b;
+        if (n == 0) { hl_r = prev_t(red[1]); hl_b = prev_t(blue[1]); hr_r = red[1]; hr_b = blue[1]; }
+        else { hl_r = red[1]; hl_b = blue[1]; hr_r = next_t(red[1]); hr_b = next_t(blue[1]); }
         const bool horizontal = hz[0];
-        const double mr = 0.5 * (horizontal ? hl_r + hr_r : red[0] + red[2]);
-        const double mb = 0.5 * (horizontal ? hl_b + hr_b : blue[0] + blue[2]);
-        const double own = rn[0];
+        const T mr = T(0.5) * (horizontal ? hl_r + hr_r : red[0] + red[2]);
+        const T mb = T(0.5) * (horizontal ? hl_b + hr_b : blue[0] + blue[2]);
+        const T own = rn[0];
         float px[2][3];
         float* P = px[n];
         float* G = px[1 - n];
@@ -408,10 +421,10 @@ struct Strip {
     }
 };
 
-template <int N0, int RED0, int OPS, bool E, bool REC>
+template <int N0, int RED0, int OPS, bool E, bool REC, typename T>
 __device__ __forceinline__ void run_strip(const StreamArgs& a, int lane, int xA, int yb, int ye) {
     const int ye4 = yb + ((ye - yb + 3) & ~3);                // whole groups of four rows
-    Strip<N0, RED0, OPS, E, REC> t(a, lane, xA, yb, ye);
+    Strip<N0, RED0, OPS, E, REC, T> t(a, lane, xA, yb, ye);
     // Rows yb-6 .. yb-3 of the window before the first step (s = yb - 8
     // reads rows s+2 .. s+6), then the rows in flight.
 #pragma unroll
@@ -430,7 +443,7 @@ __device__ __forceinline__ void run_strip(const StreamArgs& a, int lane, int xA,
     // classifier from yb - 2, red / blue from yb - 1), then the band, four
     // rows per iteration: the windows' registers rename within an iteration
     // and only the few values older than four rows move.
-    using S = Strip<N0, RED0, OPS, E, REC>;
+    using S = Strip<N0, RED0, OPS, E, REC, T>;
     t.template step<0, 0>(yb - 8, false, false);
     t.template step<1, 0>(yb - 7, false, false);
     t.template step<0, S::ST_DV>(yb - 6, false, false);
@@ -448,7 +461,7 @@ __device__ __forceinline__ void run_strip(const StreamArgs& a, int lane, int xA,
     if (t.e_all && a.flags) atomicOr(a.flags, t.e_all);
 }
 
-template <int N0, int RED0, int OPS, bool REC = false>
+template <int N0, int RED0, int OPS, bool REC = false, typename T = double>
 __global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const StreamArgs a) {
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kStreamThreads / 64) + (threadIdx.x >> 6)));
     if (wave >= a.wave_cols * a.bands) return;   // whole wavefronts leave together
@@ -462,8 +475,8 @@ __global__ void __launch_bounds__(kStreamThreads) menon_stream_kernel(const Stre
     // rows and the warm-up rows above them) come within reach of the area's
     // edge takes the edge form.
     const bool edge = x0 < a.ax + 8 || x0 + 128 > a.ax + a.aw - 8 || yb - 16 < a.ay || ye + 16 > a.ay + a.ah;
-    if (edge) run_strip<N0, RED0, OPS, true, REC>(a, lane, xA, yb, ye);
-    else run_strip<N0, RED0, OPS, false, REC>(a, lane, xA, yb, ye);
+    if (edge) run_strip<N0, RED0, OPS, true, REC, T>(a, lane, xA, yb, ye);
+    else run_strip<N0, RED0, OPS, false, REC, T>(a, lane, xA, yb, ye);
 }
 
 __global__ void zero_words(uint32_t* p, size_t n) {
@@ -488,29 +501,53 @@ hipError_t launch_menon_stream(const StreamArgs& args, int n0, int red0, int ops
     const unsigned blocks = (unsigned)((waves + kStreamThreads / 64 - 1) / (kStreamThreads / 64));
     const dim3 grid(blocks), block(kStreamThreads);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, stream, a); };
+    const bool fp32 = a.fp32 && ops != STREAM_OPS_LISTS;
     if (a.decisions) {
         // The record's words are OR-ed into: clear them first.
         const size_t words = (size_t)a.dtiles * (size_t)((a.Y1 - a.Y0 + 31) / 32) * 32;
         hipLaunchKernelGGL(zero_words, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, stream,
                            a.decisions, words);
         if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
-#define RAWOPS_STREAM_REC(N, R)                                                                \
+#define RAWOPS_STREAM_REC(N, R, T)                                                             \
     switch (ops) {                                                                             \
-    case STREAM_OPS_LINEAR: go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR, true>); break;     \
+    case STREAM_OPS_LINEAR: go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR, true, T>); break;  \
     case STREAM_OPS_LINEAR | STREAM_LIN_GAINS:                                                 \
-        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS, true>); break;      \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS, true, T>); break;   \
     case STREAM_OPS_LINEAR | STREAM_LIN_MATRIX:                                                \
-        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_MATRIX, true>); break;     \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_MATRIX, true, T>); break;  \
     default:                                                                                   \
-        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS | STREAM_LIN_MATRIX, true>); \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS | STREAM_LIN_MATRIX, true, T>); \
+        break;                                                                                 \
+    }
+#define RAWOPS_STREAM_REC_T(T)                                                                 \
+        if (n0) {                                                                              \
+            if (red0) { RAWOPS_STREAM_REC(1, 1, T) } else { RAWOPS_STREAM_REC(1, 0, T) }       \
+        } else {                                                                               \
+            if (red0) { RAWOPS_STREAM_REC(0, 1, T) } else { RAWOPS_STREAM_REC(0, 0, T) }       \
+        }
+        if (fp32) { RAWOPS_STREAM_REC_T(float) } else { RAWOPS_STREAM_REC_T(double) }
+#undef RAWOPS_STREAM_REC_T
+#undef RAWOPS_STREAM_REC
+        return hipGetLastError();
+    }
+    if (fp32) {
+#define RAWOPS_STREAM_F32(N, R)                                                                \
+    switch (ops) {                                                                             \
+    case STREAM_OPS_LINEAR: go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR, false, float>); break; \
+    case STREAM_OPS_LINEAR | STREAM_LIN_GAINS:                                                 \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS, false, float>); break; \
+    case STREAM_OPS_LINEAR | STREAM_LIN_MATRIX:                                                \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_MATRIX, false, float>); break; \
+    default:                                                                                   \
+        go(menon_stream_kernel<N, R, STREAM_OPS_LINEAR | STREAM_LIN_GAINS | STREAM_LIN_MATRIX, false, float>); \
         break;                                                                                 \
     }
         if (n0) {
-            if (red0) { RAWOPS_STREAM_REC(1, 1) } else { RAWOPS_STREAM_REC(1, 0) }
+            if (red0) { RAWOPS_STREAM_F32(1, 1) } else { RAWOPS_STREAM_F32(1, 0) }
         } else {
-            if (red0) { RAWOPS_STREAM_REC(0, 1) } else { RAWOPS_STREAM_REC(0, 0) }
+            if (red0) { RAWOPS_STREAM_F32(0, 1) } else { RAWOPS_STREAM_F32(0, 0) }
         }
-#undef RAWOPS_STREAM_REC
+#undef RAWOPS_STREAM_F32
         return hipGetLastError();
     }
 #define RAWOPS_STREAM_FORMS(N, R)                                                              \

@@ -44,7 +44,8 @@ uint32_t site_of_host(const SensorArgs& s, int x, int y) {
 } // namespace
 
 // Menon mip-0 render path (rawops_set_menon_path): 0 automatic, 1 streaming
-// wherever the area allows it, 2 tile kernel only; rows per streaming
+// wherever the area allows it, 2 tile kernel only, 3 as 1 with the Menon
+// stages of linear chains in FP32 (approximate); rows per streaming
 // wavefront (0: 16).
 std::atomic<uint32_t> g_menon_path{0}, g_stream_band{0};
 
@@ -60,7 +61,7 @@ bool use_stream(const RenderArgs& a) {
     if (S.aw < 16 || S.ah < 16 || a.ostride > (1u << 23)) return false;
     const uint32_t path = g_menon_path.load(std::memory_order_relaxed);
     if (path == 2) return false;
-    if (path == 1) return true;
+    if (path == 1 || path == 3) return true;
     const int band = stream_band_rows() ? stream_band_rows() : 16;
     return (long)((a.vw + 111) / 112) * ((a.vh + band - 1) / band) >= 1536;
 }
@@ -156,6 +157,7 @@ hipError_t launch(const RenderLaunch& L, const OpList* native, const OpList* red
         t.flags = a.flags;
         t.decisions = a.decisions;
         t.dtiles = a.dtiles;
+        t.fp32 = g_menon_path.load(std::memory_order_relaxed) == 3 ? 1 : 0;
         const int n0 = (int)(((S.ng_parity >> ((uint32_t)t.Y0 & 1u)) & 1u) ^ ((uint32_t)t.X0 & 1u));
         const uint32_t site_ng = site_of_host(S, t.X0 + n0, t.Y0);
         const int red0 = ((S.colors >> (2u * site_ng)) & 3u) == 0 ? 1 : 0;

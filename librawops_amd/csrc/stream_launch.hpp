@@ -34,6 +34,10 @@ struct StreamArgs {
     // render_launch.hpp, zeroed by the launcher), or null.
     uint32_t* decisions;
     uint32_t dtiles;             // record tiles per viewport row
+    // Approximate form: the Menon stages in FP32 instead of the reference's
+    // FP64 (linear chains only; its outputs and decisions differ from the
+    // reference's, the flip rate is measured, not bounded).
+    int fp32;
 };
 
 // Op program forms of the streaming kernel: the device op lists run by the

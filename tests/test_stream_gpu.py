@@ -46,7 +46,7 @@ def test_paths_render_the_same_bits():
         C.check(C.lib().rawops_set_menon_path(1, 0))
     for o in outs[1:]:
         assert bits_equal(o, outs[0])
-    assert C.lib().rawops_set_menon_path(3, 0) == C.INVALID_ARGUMENT
+    assert C.lib().rawops_set_menon_path(4, 0) == C.INVALID_ARGUMENT
     assert C.lib().rawops_set_menon_path(1, 6) == C.INVALID_ARGUMENT
 
 CAM = (0.7, 0.2, 0.1, 0.25, 0.7, 0.05, 0.02, 0.1, 0.88)
@@ -151,3 +151,49 @@ def test_stream_recording_matches_tile_recording(vp_rel):
     assert torch.equal(out_s.view(torch.int32), out_t.view(torch.int32))
     n = ((vp.width + 63) // 64) * ((vp.height + 31) // 32) * 32
     assert torch.equal(dec_s.view(torch.int32)[:n], dec_t.view(torch.int32)[:n])
+
+
+def test_fp32_form_is_an_approximation_of_the_oracle():
+    """rawops_set_menon_path(3): the Menon stages of linear chains in FP32.
+    Not the reference's arithmetic, so no parity claim: its classifier
+    decisions (recorded like the FP64 form's) agree with the FP64 form's on
+    all but a small fraction of the non-green sites, and its output stays
+    within FP32 rounding of the oracle except near flipped decisions.
+    Chains that are not linear (tone + clip here) keep the FP64 kernel, bit
+    for bit."""
+    import torch
+    from librawops_amd import _capi as C
+    from librawops_amd import autograd as A
+    w, h, stride, area = 600, 420, 608, (4, 3, 590, 410)
+    m = meta_dict(w, h, stride, 1, 1, 0, area, (510, 512, 511, 513), (16383,) * 4)
+    s = random_samples(h, stride, seed=31, lo=400, hi=16000)
+    img = gpu_image(m, s)
+    om = oracle_meta(m)
+    ax, ay, aw, ah = area
+    vp = L.Rect(ax + 2, ay + 1, aw - 5, ah - 3)
+    vpt = (vp.x, vp.y, vp.width, vp.height)
+    req = L.RenderRequest(vp, 64)
+    c = CHAINS["wb_exposure_matrix"]
+    cam = np.array([[1.05, -0.03, -0.02], [-0.06, 1.02, 0.04], [0.03, 0.09, 0.88]], np.float32)
+    try:
+        C.check(C.lib().rawops_set_menon_path(1, 0))
+        out64, dec64 = A.render_record(img, [1.7, 1.0, 1.3], 0.4, cam, vp)
+        C.check(C.lib().rawops_set_menon_path(3, 0))
+        out32, dec32 = A.render_record(img, [1.7, 1.0, 1.3], 0.4, cam, vp)
+        approx = L.Renderer().render_image(chain_node(img, c), img.metadata.active_area, req).rgb
+        tone = L.Renderer().render_image(chain_node(img, CHAINS["tone_clip"]), img.metadata.active_area, req).rgb
+        torch.cuda.synchronize()
+    finally:
+        C.check(C.lib().rawops_set_menon_path(1, 0))
+    assert bits_equal(tone, O.render_raw(om, s, CHAINS["tone_clip"], vpt))
+    # decisions: one bit per non-green site of the viewport
+    n = ((vp.width + 63) // 64) * ((vp.height + 31) // 32) * 32
+    d64 = dec64.view(torch.int32)[:n].cpu().numpy().view(np.uint8)
+    d32 = dec32.view(torch.int32)[:n].cpu().numpy().view(np.uint8)
+    flips = int(np.unpackbits(d64 ^ d32).sum())
+    assert 0 <= flips < 0.01 * (vp.width * vp.height // 2), flips
+    # outputs: within FP32 rounding except near flipped decisions
+    for got, ref in ((approx, O.render_raw(om, s, c, vpt)), (out32.cpu().numpy(), out64.cpu().numpy())):
+        dev = np.abs(got - ref)
+        assert float(np.median(dev)) <= 1e-6
+        assert float(np.mean(np.any(dev > 1e-5, axis=2))) < 0.05
