@@ -283,7 +283,7 @@ def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
     return res
 
 
-def leg_c4_batch(replays, seed=4):
+def leg_c4_batch(replays, seed=4, cpu_leg=True):
     """64 heterogeneous requests over 8 Bayer sources (2000x1500 .. 6048x4024,
     RGGB/GRBG; synth.c4_workload): viewports 256^2 .. full frame, (mip 0,
     Final) / (mip 1 or 2, Preview), bilinear and Menon, FP32 linear + uint8
@@ -329,18 +329,42 @@ def leg_c4_batch(replays, seed=4):
     lat = np.asarray(lat)
     tw, th, tuned = batch.shape()
     ms = total / replays
-    return {"workload": "64 requests, 8 sources 2000x1500..6048x4024 RGGB/GRBG, viewports 256^2..full, mip 0/1/2, "
-                        "bilinear+Menon, FP32 linear + uint8 sRGB preview, one hipGraph launch (one workgroup per "
-                        "(request, tile) item, tile shape autotuned over 64x32 / 64x64 / 128x32)",
-            "batch_ms": round(ms, 3), "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
-            "aggregate_output_mpix_s": round(level_px / ms / 1e3, 1),
-            "aggregate_native_mpix_s": round(native_px / ms / 1e3, 1),
-            "native_mpix": round(native_px / 1e6, 2),
-            "tile": f"{tw}x{th}", "autotune_ms": round(tuned, 3)}
+    res = {"workload": "64 requests, 8 sources 2000x1500..6048x4024 RGGB/GRBG, viewports 256^2..full, mip 0/1/2, "
+                       "bilinear+Menon, FP32 linear + uint8 sRGB preview, one hipGraph launch (one workgroup per "
+                       "(request, tile) item, tile shape autotuned over 64x32 / 64x64 / 128x32)",
+           "batch_ms": round(ms, 3), "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
+           "aggregate_output_mpix_s": round(level_px / ms / 1e3, 1),
+           "aggregate_native_mpix_s": round(native_px / ms / 1e3, 1),
+           "native_mpix": round(native_px / 1e6, 2),
+           "tile": f"{tw}x{th}", "autotune_ms": round(tuned, 3)}
+    if not cpu_leg:
+        return res
+    # CPU: the same 64 requests served by the compiled reference on a pool of
+    # worker threads (one request at a time each, one OpenMP thread per
+    # worker, as a TileScheduler worker pool runs jobs), SrgbPreview chain.
+    try:
+        from oracle import ref as R
+        have_ref = R.available()
+    except Exception:
+        have_ref = False
+    if have_ref:
+        from oracle import oracle as O
+        threads = cpu_threads()
+        src = [(O.Bayer(m["width"], m["height"], m["row_stride_samples"], m["pattern"], m["cfa_phase_x"],
+                        m["cfa_phase_y"], m["active_area"], m["black_levels"], m["white_levels"]), s)
+               for m, s in sources]
+        sec = R.time_requests(src, specs, synth.C4_GAINS, synth.C4_STOPS, cam, threads, 1)
+        res["cpu_baseline"] = {"value": round(level_px / sec / 1e6, 2), "unit": "Mpix/s (output)", "kind": "reference",
+                               "cores": threads, "seconds_per_batch": round(sec, 3),
+                               "sample": f"the whole 64-request batch once after one untimed round, {threads} "
+                                         "worker threads of the compiled reference (one request each at a time, "
+                                         "one OpenMP thread per worker), SrgbPreview chain"}
+        res["gpu_over_cpu"] = round(sec * 1e3 / ms, 1)
+    return res
 
 
-def leg_c5_chain(steps, seed=5):
+def leg_c5_chain(steps, seed=5, cpu_leg=True):
     """16 crops of 512x512 from a 4000x3000 RGGB scene through Menon -> WB ->
     3x3 -> exposure -> guided-filter detail boost (r=8, eps=1e-3) -> sRGB,
     fitted to a seeded target render with Adam (gains, matrix, exposure,
@@ -381,11 +405,38 @@ def leg_c5_chain(steps, seed=5):
     el = time.perf_counter() - t0
     losses = [float(v) for v in losses]
     px = 16 * 512 * 512 * steps
-    return {"workload": f"16 x 512^2 crops of 4000x3000 RGGB, Menon->WB->3x3->exposure->detail boost(r=8,eps=1e-3)"
-                        f"->sRGB, Adam {steps} steps",
-            "fwd_bwd_mpix_s": round(px / el / 1e6, 1), "seconds": round(el, 2),
-            "loss_first": round(losses[0], 6), "loss_last": round(losses[-1], 6),
-            "gradient_check": "<=1e-4 relative vs FP64 finite differences (tests/test_autograd_gpu.py)"}
+    res = {"workload": f"16 x 512^2 crops of 4000x3000 RGGB, Menon->WB->3x3->exposure->detail boost(r=8,eps=1e-3)"
+                       f"->sRGB, Adam {steps} steps",
+           "fwd_bwd_mpix_s": round(px / el / 1e6, 1), "seconds": round(el, 2),
+           "loss_first": round(losses[0], 6), "loss_last": round(losses[-1], 6),
+           "gradient_check": "<=1e-4 relative vs FP64 finite differences on 512^2 crops "
+                             "(tests/test_c5_gradients_gpu.py, tests/test_autograd_gpu.py)"}
+    if not cpu_leg:
+        return res
+    # CPU: the forward of the same chain over the 16 crops by the compiled
+    # reference (its nodes: Menon, WB, exposure, camera->ProPhoto, working-Y
+    # guided filter r = 8, to-sRGB, encode; the detail-boost blend, a
+    # pointwise op of this library, has no reference node), one crop per
+    # worker thread.  The reference has no backward: GPU fwd+bwd against the
+    # CPU forward.
+    try:
+        from oracle import ref as R
+        have_ref = R.available()
+    except Exception:
+        have_ref = False
+    if have_ref:
+        from oracle import oracle as O
+        threads = cpu_threads()
+        om = O.Bayer(W, H, W, 0, 0, 0, (0, 0, W, H), (512,) * 4, (16383,) * 4)
+        cam_xyz = synth.camera_to_xyz_for(synth.camera_to_prophoto_seeded(seed))
+        sec = R.time_guided_crops(om, s, (1.5, 1.0, 1.25), 0.4, cam_xyz, 8, 1e-3,
+                                  [(c.x, c.y, c.width, c.height) for c in crops], threads, 1)
+        cpu = round(16 * 512 * 512 / sec / 1e6, 2)
+        res["cpu_baseline_forward"] = {"value": cpu, "unit": "Mpix/s", "kind": "reference", "cores": threads,
+                                       "sample": f"the 16 crops once after one untimed round, {threads} worker "
+                                                 "threads (one crop each at a time, one OpenMP thread per worker)"}
+        res["gpu_fwd_bwd_over_cpu_fwd"] = round(res["fwd_bwd_mpix_s"] / cpu, 1)
+    return res
 
 
 def leg_dropin(steps):
@@ -514,7 +565,7 @@ def main():
     C.check(C.lib().rawops_set_menon_path(args.menon_path, args.stream_band))
     if args.leg:
         fn = {"c1": lambda: leg_c1_bilinear(args.steps), "c3": lambda: leg_c3_guided(args.steps, cpu_leg=not args.no_cpu),
-              "c4": lambda: leg_c4_batch(args.steps), "c5": lambda: leg_c5_chain(args.c5_steps),
+              "c4": lambda: leg_c4_batch(args.steps, cpu_leg=not args.no_cpu), "c5": lambda: leg_c5_chain(args.c5_steps, cpu_leg=not args.no_cpu),
               "dropin": lambda: leg_dropin(args.steps)}[args.leg]
         print(json.dumps({"leg": args.leg, **fn()}))
         return
@@ -664,8 +715,8 @@ def main():
 
     # ---- the other configurations
     if not args.main_only:
-        legs = (("c1_bilinear", lambda: leg_c1_bilinear(20)), ("c3_guided", lambda: leg_c3_guided(6)),
-                ("c4_batch", lambda: leg_c4_batch(5)), ("c5_chain", lambda: leg_c5_chain(args.c5_steps)),
+        legs = (("c1_bilinear", lambda: leg_c1_bilinear(20)), ("c3_guided", lambda: leg_c3_guided(6, cpu_leg=not args.no_cpu)),
+                ("c4_batch", lambda: leg_c4_batch(5, cpu_leg=not args.no_cpu)), ("c5_chain", lambda: leg_c5_chain(args.c5_steps, cpu_leg=not args.no_cpu)),
                 ("dropin", lambda: leg_dropin(8)))
         for name, fn in legs:
             print(f"bench: {name}", file=sys.stderr, flush=True)

@@ -42,6 +42,12 @@ def lib():
         L.refh_matrices.argtypes = [P, ctypes.c_int, P, P, P]
         L.refh_time_menon_bands.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_float, P, Rect, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+        L.refh_time_requests.argtypes = [P, P, P, ctypes.c_uint32, P, P, P, P, P, ctypes.c_uint32, P,
+                                         ctypes.c_float, P, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+        L.refh_time_guided_crops.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_float, P, ctypes.c_uint32,
+                                             ctypes.c_double, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.POINTER(ctypes.c_double)]
         L.refh_exposure_multiplier.argtypes = [ctypes.c_float]
         L.refh_exposure_multiplier.restype = ctypes.c_float
         _LIB = L
@@ -109,6 +115,49 @@ def time_menon_bands(meta: Bayer, samples, wb, stops, camera_to_xyz, rect, threa
     out = ctypes.c_double()
     _check(L.refh_time_menon_bands(ctypes.byref(meta.struct()), _p(s), s.size, _p(w), float(stops), _p(cam),
                                    Rect(*rect), threads, reps, ctypes.byref(out)))
+    return out.value
+
+
+def time_requests(sources, requests, wb, stops, camera_to_xyz, threads, reps, output="srgb",
+                  tone_shoulder=0.25, tone_gamma=1.0):
+    """Seconds per round of serving ``requests`` (dicts with source, demosaic,
+    mip, viewport, tile_size) over ``sources`` [(Bayer, samples)] on
+    ``threads`` worker threads of the compiled reference, one OpenMP thread
+    each (Menon / bilinear -> WB -> exposure -> camera-to-ProPhoto, then the
+    SrgbPreview stages for output "srgb")."""
+    L = lib()
+    metas = (type(sources[0][0].struct()) * len(sources))(*[m.struct() for m, _ in sources])
+    arrays = [np.ascontiguousarray(s, np.uint16) for _, s in sources]
+    ptrs = (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+    counts = (ctypes.c_size_t * len(arrays))(*[a.size for a in arrays])
+    n = len(requests)
+    u32 = lambda v: (ctypes.c_uint32 * n)(*v)
+    src = u32([r["source"] for r in requests])
+    menon = u32([1 if r["demosaic"] == "menon" else 0 for r in requests])
+    mip = u32([r["mip"] for r in requests])
+    tile = u32([r["tile_size"] for r in requests])
+    vps = (Rect * n)(*[Rect(*r["viewport"]) for r in requests])
+    w = np.asarray(wb, np.float32)
+    cam = np.asarray(camera_to_xyz, np.float64)
+    out = ctypes.c_double()
+    _check(L.refh_time_requests(metas, ptrs, counts, len(sources), src, menon, vps, mip, tile, n, _p(w),
+                                float(stops), _p(cam), {"linear": 0, "srgb": 2}[output], float(tone_shoulder),
+                                float(tone_gamma), threads, reps, ctypes.byref(out)))
+    return out.value
+
+
+def time_guided_crops(meta: Bayer, samples, wb, stops, camera_to_xyz, radius, eps, crops, threads, reps):
+    """Seconds per round of the full chain's forward (Menon -> WB -> exposure
+    -> camera-to-ProPhoto -> guided filter -> sRGB) over ``crops`` (x, y, w, h)
+    of one source, on ``threads`` worker threads of the compiled reference."""
+    L = lib()
+    s = np.ascontiguousarray(samples, np.uint16)
+    w = np.asarray(wb, np.float32)
+    cam = np.asarray(camera_to_xyz, np.float64)
+    rects = (Rect * len(crops))(*[Rect(*c) for c in crops])
+    out = ctypes.c_double()
+    _check(L.refh_time_guided_crops(ctypes.byref(meta.struct()), _p(s), s.size, _p(w), float(stops), _p(cam),
+                                    int(radius), float(eps), rects, len(crops), threads, reps, ctypes.byref(out)))
     return out.value
 
 

@@ -8,7 +8,9 @@
 #include "SpatialOps.hpp"
 #include "ToneOps.hpp"
 
+#include <atomic>
 #include <chrono>
+#include <exception>
 #include <cstring>
 #include <thread>
 #include <memory>
@@ -113,6 +115,139 @@ int refh_render_raw(const ro_bayer* b, const uint16_t* samples, size_t count, in
                           RenderLevel{mip, static_cast<RenderQuality>(quality)}};
         const Tile t = Renderer{}.render_image(*n, area, req);
         std::memcpy(out, t.rgb.data(), t.rgb.size() * sizeof(float));
+    });
+}
+
+// Wall seconds per round of `nreq` render requests served by `threads`
+// worker threads, the way a TileScheduler worker pool serves them: each
+// worker takes the next request and renders it (render_image of its
+// viewport at its level and tile size) through the chain of refh_render_raw
+// (output_mode 2: SrgbPreview) of its source, one OpenMP thread per worker;
+// `reps` rounds after one untimed round.  Request k: source req_src[k],
+// Menon if req_menon[k], viewport req_vp[k], mip req_mip[k] (Final at mip
+// 0, Preview above), tile size req_tile[k].
+int refh_time_requests(const ro_bayer* metas, const uint16_t* const* samples, const size_t* counts, uint32_t nsrc,
+                       const uint32_t* req_src, const uint32_t* req_menon, const ro_rect* req_vp,
+                       const uint32_t* req_mip, const uint32_t* req_tile, uint32_t nreq, const float wb[3],
+                       float stops, const double cam[9], int output_mode, float shoulder, float gamma,
+                       uint32_t threads, uint32_t reps, double* seconds) {
+    return guarded([&] {
+        std::vector<RawImage> images;
+        for (uint32_t i = 0; i < nsrc; ++i) images.push_back(make_image(&metas[i], samples[i], counts[i]));
+        std::vector<std::shared_ptr<const Node>> chains;   // [source][demosaic]
+        for (uint32_t i = 0; i < nsrc; ++i)
+            for (int menon = 0; menon < 2; ++menon) {
+                RawDemosaicIdentity id;
+                if (menon) id = {"rawengine.menon_base", 1};
+                const Rect area = images[i].metadata().active_area;
+                std::shared_ptr<const Node> n = std::make_shared<RawUnpackNode>(images[i], id);
+                n = std::make_shared<WhiteBalanceNode>(n, wb[0], wb[1], wb[2]);
+                n = std::make_shared<ExposureNode>(n, stops);
+                CameraColorTransform t;
+                for (int k = 0; k < 9; ++k) t.camera_to_xyz_d50[k] = cam[k];
+                t.target = WorkingSpace::LinearProPhotoD50;
+                n = std::make_shared<CameraToWorkingNode>(n, t, area);
+                if (output_mode == 2) {
+                    n = std::make_shared<WorkingToSrgbNode>(n);
+                    n = std::make_shared<ToneCurveNode>(n, shoulder, gamma);
+                    n = std::make_shared<SrgbEncodeNode>(n);
+                }
+                chains.push_back(n);
+            }
+        // A worker's exception is carried out of its thread and rethrown
+        // here (the first one), after every worker has stopped.
+        auto round = [&] {
+            std::atomic<uint32_t> next{0};
+            std::vector<std::exception_ptr> errors(threads);
+            std::vector<std::thread> pool;
+            for (uint32_t w = 0; w < threads; ++w)
+                pool.emplace_back([&, w] {
+                    try {
+#ifdef _OPENMP
+                        omp_set_num_threads(1);
+#endif
+                        for (uint32_t k = next++; k < nreq; k = next++) {
+                            const uint32_t i = req_src[k];
+                            if (i >= nsrc) throw std::out_of_range("request source out of range");
+                            const ro_rect& v = req_vp[k];
+                            const RenderRequest req{Rect{v.x, v.y, v.width, v.height}, req_tile[k],
+                                                    RenderLevel{req_mip[k], req_mip[k] == 0 ? RenderQuality::Final
+                                                                                            : RenderQuality::Preview}};
+                            const Tile out = Renderer{}.render_image(*chains[2 * i + (req_menon[k] ? 1 : 0)],
+                                                                     images[i].metadata().active_area, req);
+                            if (out.rgb.empty()) throw std::runtime_error("empty render");
+                        }
+                    } catch (...) {
+                        errors[w] = std::current_exception();
+                        next = nreq;   // the other workers stop after their current request
+                    }
+                });
+            for (auto& th : pool) th.join();
+            for (auto& e : errors)
+                if (e) std::rethrow_exception(e);
+        };
+        round();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < reps; ++i) round();
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+    });
+}
+
+// Wall seconds per round of rendering `n` crops of one RAW source through
+// Menon -> white balance -> exposure -> camera-to-ProPhoto -> working-Y
+// guided filter (radius, epsilon) -> working-to-sRGB -> tone (shoulder 0.25,
+// gamma 1) -> sRGB encode, the
+// forward of the differentiable full chain, one crop per worker at a time
+// on `threads` worker threads (one OpenMP thread each); `reps` rounds after
+// one untimed round.
+int refh_time_guided_crops(const ro_bayer* b, const uint16_t* samples, size_t count, const float wb[3], float stops,
+                           const double cam[9], uint32_t radius, double epsilon, const ro_rect* crops, uint32_t n,
+                           uint32_t threads, uint32_t reps, double* seconds) {
+    return guarded([&] {
+        const RawImage image = make_image(b, samples, count);
+        const Rect area = image.metadata().active_area;
+        std::shared_ptr<const Node> node =
+            std::make_shared<RawUnpackNode>(image, RawDemosaicIdentity{"rawengine.menon_base", 1});
+        node = std::make_shared<WhiteBalanceNode>(node, wb[0], wb[1], wb[2]);
+        node = std::make_shared<ExposureNode>(node, stops);
+        CameraColorTransform t;
+        for (int k = 0; k < 9; ++k) t.camera_to_xyz_d50[k] = cam[k];
+        t.target = WorkingSpace::LinearProPhotoD50;
+        node = std::make_shared<CameraToWorkingNode>(node, t, area);
+        node = std::make_shared<WorkingYGuidedFilterNode>(node, area, WorkingYGuidedFilterSettings{radius, epsilon});
+        node = std::make_shared<WorkingToSrgbNode>(node);
+        node = std::make_shared<ToneCurveNode>(node, 0.25f, 1.0f);   // display-linear, as SrgbEncodeNode requires
+        node = std::make_shared<SrgbEncodeNode>(node);
+        auto round = [&] {
+            std::atomic<uint32_t> next{0};
+            std::vector<std::exception_ptr> errors(threads);
+            std::vector<std::thread> pool;
+            for (uint32_t w = 0; w < threads; ++w)
+                pool.emplace_back([&, w] {
+                    try {
+#ifdef _OPENMP
+                        omp_set_num_threads(1);
+#endif
+                        for (uint32_t k = next++; k < n; k = next++) {
+                            const ro_rect& v = crops[k];
+                            const RenderRequest req{Rect{v.x, v.y, v.width, v.height}, std::max(v.width, v.height),
+                                                    RenderLevel{}};
+                            const Tile out = Renderer{}.render_image(*node, area, req);
+                            if (out.rgb.empty()) throw std::runtime_error("empty render");
+                        }
+                    } catch (...) {
+                        errors[w] = std::current_exception();
+                        next = n;
+                    }
+                });
+            for (auto& th : pool) th.join();
+            for (auto& e : errors)
+                if (e) std::rethrow_exception(e);
+        };
+        round();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < reps; ++i) round();
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
     });
 }
 

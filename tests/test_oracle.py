@@ -301,6 +301,37 @@ def test_oracle_matches_compiled_reference_chains(demosaic):
 
 
 @needs_ref
+def test_compiled_reference_request_pool_times_a_batch():
+    """The CPU baseline of the batch leg: requests of two sources served by a
+    worker pool of the compiled reference (bad requests raise)."""
+    rng = np.random.default_rng(5)
+    srcs = []
+    for w, h, pat in [(96, 64, 0), (130, 90, 2)]:
+        srcs.append((O.Bayer(w, h, w, pat, 0, 0, (0, 0, w, h), (512,) * 4, (16383,) * 4),
+                     rng.integers(0, 16383, (h, w)).astype(np.uint16)))
+    reqs = [dict(source=0, demosaic="menon", mip=0, viewport=(0, 0, 96, 64), tile_size=32),
+            dict(source=1, demosaic="bilinear", mip=1, viewport=(3, 2, 40, 30), tile_size=128),
+            dict(source=1, demosaic="menon", mip=2, viewport=(0, 0, 33, 23), tile_size=256)]
+    assert R.time_requests(srcs, reqs, (2.0, 1.0, 1.5), 0.3, CAM, 3, 2) > 0.0
+    bad = dict(source=0, demosaic="menon", mip=0, viewport=(90, 0, 20, 10), tile_size=32)
+    with pytest.raises(IndexError):
+        R.time_requests(srcs, [bad], (2.0, 1.0, 1.5), 0.3, CAM, 2, 1)
+
+
+@needs_ref
+def test_compiled_reference_crop_pool_times_the_full_chain():
+    """The CPU baseline of the differentiable-chain leg: crops of one source
+    through Menon ... guided filter ... sRGB on a worker pool."""
+    rng = np.random.default_rng(6)
+    m = O.Bayer(120, 80, 120, 0, 0, 0, (0, 0, 120, 80), (512,) * 4, (16383,) * 4)
+    s = rng.integers(0, 16383, (80, 120)).astype(np.uint16)
+    crops = [(0, 0, 40, 40), (60, 20, 40, 40), (78, 38, 42, 42)]
+    assert R.time_guided_crops(m, s, (1.5, 1.0, 1.25), 0.4, CAM, 4, 1e-3, crops, 2, 1) > 0.0
+    with pytest.raises(IndexError):
+        R.time_guided_crops(m, s, (1.5, 1.0, 1.25), 0.4, CAM, 4, 1e-3, [(100, 0, 40, 40)], 2, 1)
+
+
+@needs_ref
 def test_oracle_matches_compiled_reference_matrices_and_exposure():
     for space in (0, 1):
         a, b, c = R.matrices(CAM, space)
