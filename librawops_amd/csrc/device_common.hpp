@@ -205,15 +205,24 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
     return (float)mapped;
 }
 
-// A curve staged in LDS for the filter passes: the knot abscissae (for the
-// knot search) and, per knot k, {x_k, y_k, slope_i, 1 / (x_k+1 - x_k),
-// q0..q3} with i = min(k, n - 2) and q the power-basis coefficients of the
-// segment's cubic (control points y_k, c0_k, c1_k, y_k+1: q0 = y_k,
-// q1 = 3 (c0 - y_k), q2 = 3 (y_k - 2 c0 + c1), q3 = y_k+1 - y_k + 3 (c0 - c1)),
-// so a cubic value is three fused multiply-adds (Horner) instead of the six
-// blends of de Casteljau's construction (ToneOps.cpp:788-792).  Same knot
-// selection, exact-knot and linear-extension rules as curve_map_in; cubic
-// values agree with it to FP64 rounding (the stored results are float32).
+// A curve staged in LDS for the filter passes: per knot count k (the number
+// of knots <= v, 0..n) one record {x_k', c0, c1, c2, c3} with which the
+// curve at v is c0 + u (c1 + u (c2 + u c3)), u = v - x_k' (three fused
+// multiply-adds, no selects):
+//   k = 0       left of the first knot, the linear extension
+//               (x_0, y_0, slope_0, 0, 0);
+//   1 <= k < n  segment i = k - 1 from x_i: the cubic (cubic curves: control
+//               points y_i, c0_i, c1_i, y_i+1; power-basis coefficients in
+//               t = u / h, q0 = y_i, q1 = 3 (c0 - y_i), q2 = 3 (y_i - 2 c0 + c1),
+//               q3 = y_i+1 - y_i + 3 (c0 - c1), divided by 1, h, h^2, h^3) or
+//               the secant (y_i, slope_i, 0, 0);
+//   k = n       at or right of the last knot, the linear extension
+//               (x_n-1, y_n-1, slope_n-2, 0, 0).
+// u = 0 (v at a knot) returns c0 = y exactly, and a zero slope returns y:
+// the exact-knot and flat-segment rules of curve_map_in.  The records bake in
+// the knot selection, the linear extensions and the cubic / linear choice
+// of curve_map_in; cubic values agree with its de Casteljau blends
+// (ToneOps.cpp:788-792) to FP64 rounding (the stored results are float32).
 //
 // For the knot search of float inputs the curve also keeps, per knot, the
 // smallest float >= x_k (for a float v, v >= x_k exactly when v >= that
@@ -223,19 +232,18 @@ __device__ inline float curves_channel(const DevCurves* cv, int channel, float v
 // thresholds are one 16-byte LDS read), a binary search otherwise.
 struct LdsPoly {
     uint32_t n, identity;
-    const double* x;
-    const double* rec;   // 8 doubles per knot, 16-byte aligned
+    const double* x;     // the n knot abscissae
+    const double* rec;   // n + 1 records of kPolyRec doubles, 16-byte aligned
     const float* xf;     // p2 float thresholds, 16-byte aligned
     uint32_t p2;
     float t4, t8, t12;   // xf[4], xf[8], xf[12] (NaN at k >= n)
-    double x_first, x_last;
 };
 
-constexpr int kPolyRec = 8;
+constexpr int kPolyRec = 6;   // x, c0, c1, c2, c3, pad
 
-// LDS doubles of one staged curve of at most cap knots (even, so that
-// consecutive curves stay 16-byte aligned).
-constexpr int poly_table_doubles(int cap) { return ((kPolyRec + 1) * cap + (cap + 1) / 2 + 3) & ~1; }
+// LDS doubles of one staged curve of at most cap knots, cap a power of two
+// (even, so that consecutive curves stay 16-byte aligned).
+constexpr int poly_table_doubles(int cap) { return (kPolyRec * (cap + 1) + cap + 1 + cap / 2 + 1) & ~1; }
 
 // The smallest float >= x (x finite; +-FLT_MAX overflow to +-inf is handled
 // by the step toward +inf).
@@ -248,53 +256,57 @@ __device__ __forceinline__ float float_at_least(double x) {
     return f;
 }
 
-// Stage curve c (n <= cap knots) at base (poly_table_doubles(cap) doubles,
-// 16-byte aligned); the whole workgroup works, the caller synchronises.
-__device__ __forceinline__ LdsPoly stage_poly(const DevCurve* c, double* base) {
+// Stage curve c (2 <= n <= cap knots) at base (poly_table_doubles(cap)
+// doubles, 16-byte aligned) for the cubic or linear form; the whole
+// workgroup works, the caller synchronises.
+__device__ __forceinline__ LdsPoly stage_poly(const DevCurve* c, uint32_t cubic, double* base) {
     Const<DevCurve>& s = as_const(*c);
     const uint32_t n = s.n;
     uint32_t p2 = 4;
     while (p2 < n) p2 <<= 1;
-    float* xf = reinterpret_cast<float*>(base + (((kPolyRec + 1) * n + 1) & ~1u));
+    double* x = base + kPolyRec * (n + 1);
+    float* xf = reinterpret_cast<float*>(base + ((kPolyRec * (n + 1) + n + 1) & ~1u));
     const float nan = __builtin_nanf("");
-    LdsPoly l{n, s.identity, base + kPolyRec * n, base, xf, p2,
+    LdsPoly l{n, s.identity, x, base, xf, p2,
               n > 4 ? float_at_least(s.x[4]) : nan, n > 8 ? float_at_least(s.x[8]) : nan,
-              n > 12 ? float_at_least(s.x[12]) : nan, s.identity ? 0.0 : s.x[0], s.identity ? 0.0 : s.x[n - 1]};
-    if (!l.identity)
-        for (uint32_t k = threadIdx.x; k < p2; k += blockDim.x) xf[k] = k < n ? float_at_least(s.x[k]) : __builtin_nanf("");
-    if (!l.identity)
-        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-            const uint32_t i = k < n - 2u ? k : n - 2u;
-            double* r = base + kPolyRec * k;
-            r[0] = s.x[k];
-            r[1] = s.y[k];
-            r[2] = s.slope[i];
-            if (k + 1 < n) {
-                const double y0 = s.y[k], y1 = s.y[k + 1], c0 = s.c0[k], c1 = s.c1[k];
-                r[3] = 1.0 / (s.x[k + 1] - s.x[k]);
-                r[4] = y0;
-                r[5] = 3.0 * (c0 - y0);
-                r[6] = 3.0 * ((y0 - 2.0 * c0) + c1);
-                r[7] = (y1 - y0) + 3.0 * (c0 - c1);
+              n > 12 ? float_at_least(s.x[12]) : nan};
+    if (l.identity) return l;
+    for (uint32_t k = threadIdx.x; k < p2; k += blockDim.x) xf[k] = k < n ? float_at_least(s.x[k]) : nan;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) x[k] = s.x[k];
+    for (uint32_t k = threadIdx.x; k <= n; k += blockDim.x) {
+        double* r = base + kPolyRec * k;
+        double c2 = 0.0, c3 = 0.0;
+        if (k == 0) {
+            r[0] = s.x[0]; r[1] = s.y[0]; r[2] = s.slope[0];
+        } else if (k == n) {
+            r[0] = s.x[n - 1]; r[1] = s.y[n - 1]; r[2] = s.slope[n - 2];
+        } else {
+            const uint32_t i = k - 1;
+            r[0] = s.x[i];
+            r[1] = s.y[i];
+            if (cubic) {
+                const double y0 = s.y[i], y1 = s.y[i + 1], p = s.c0[i], q = s.c1[i];
+                const double inv = 1.0 / (s.x[i + 1] - s.x[i]);
+                r[2] = (3.0 * (p - y0)) * inv;
+                c2 = ((3.0 * ((y0 - 2.0 * p) + q)) * inv) * inv;
+                c3 = ((((y1 - y0) + 3.0 * (p - q)) * inv) * inv) * inv;
             } else {
-                r[3] = r[4] = r[5] = r[6] = r[7] = 0.0;
+                r[2] = s.slope[i];
             }
-            base[kPolyRec * n + k] = s.x[k];
         }
+        r[3] = c2;
+        r[4] = c3;
+        r[5] = 0.0;
+    }
     return l;
 }
 
 // The curve at v given cnt = the number of knots <= v (no branches).
-__device__ __forceinline__ double poly_eval(const LdsPoly& c, uint32_t cubic, double v, uint32_t cnt) {
-    const uint32_t left = cnt == 0 ? 0u : cnt - 1u;
-    const double2* r = reinterpret_cast<const double2*>(c.rec + kPolyRec * left);
-    const double2 a = r[0], b = r[1], q01 = r[2], q23 = r[3];
-    const double xl = a.x, yl = a.y, sl = b.x, inv = b.y;
-    const double lin = sl == 0 ? yl : yl + (v - xl) * sl;
-    const double t = (v - xl) * inv;
-    const double cub = __builtin_fma(__builtin_fma(__builtin_fma(q23.y, t, q23.x), t, q01.y), t, q01.x);
-    const bool linear = !cubic || v < c.x_first || v > c.x_last;
-    return v == xl ? yl : (linear ? lin : cub);
+__device__ __forceinline__ double poly_eval(const LdsPoly& c, double v, uint32_t cnt) {
+    const double2* r = reinterpret_cast<const double2*>(c.rec + kPolyRec * cnt);
+    const double2 a = r[0], b = r[1], d = r[2];
+    const double u = v - a.x;
+    return __builtin_fma(__builtin_fma(__builtin_fma(d.x, u, b.y), u, b.x), u, a.y);
 }
 
 // Number of knots <= v for a float v: a binary search over the float
@@ -313,13 +325,13 @@ __device__ __forceinline__ uint32_t knot_count_f(const LdsPoly& c, float v) {
 // ExtendedCurvesNode::map (ToneOps.cpp:820-831) for the three channels of a
 // pixel over LDS-staged curves (master curves[0], channel curves[1..3]):
 // the checks and flags of curves_channel.
-__device__ inline void curves3_lds(const LdsPoly* curves, uint32_t cubic, float o[3], uint32_t& err) {
+__device__ inline void curves3_lds(const LdsPoly* curves, float o[3], uint32_t& err) {
     double v[3], m[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) v[ch] = (double)o[ch];
     if (!curves[0].identity) {
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) m[ch] = poly_eval(curves[0], cubic, v[ch], knot_count_f(curves[0], o[ch]));
+        for (int ch = 0; ch < 3; ++ch) m[ch] = poly_eval(curves[0], v[ch], knot_count_f(curves[0], o[ch]));
     } else {
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) m[ch] = v[ch];
@@ -332,7 +344,7 @@ __device__ inline void curves3_lds(const LdsPoly* curves, uint32_t cubic, float 
         if (!cc.identity) {
             uint32_t k = 0;
             for (uint32_t j = 0; j < cc.n; ++j) k += m[ch] >= cc.x[j] ? 1u : 0u;
-            mapped = poly_eval(cc, cubic, m[ch], k);
+            mapped = poly_eval(cc, m[ch], k);
         }
         const bool bad = !finite(o[ch]) || !__builtin_isfinite(m[ch]) || !__builtin_isfinite(mapped) ||
                          fabs(mapped) > 3.4028234663852886e38;

@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
     LdsPoly curves[4];
     const uint32_t cubic = as_const(*cv).cubic;
     if (staged) {
-        for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * poly_table_doubles(kTailKnots));
+        for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], cubic, tables + c * poly_table_doubles(kTailKnots));
         __syncthreads();
     }
     uint32_t err = 0;
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(256) gf_ops_tail(const GuidedArgs g0) {
         float* d = g.out + ((size_t)y * g.ostride + (size_t)x) * 3;
         float p[3] = {d[0], d[1], d[2]};
         if (staged) {
-            curves3_lds(curves, cubic, p, err);
+            curves3_lds(curves, p, err);
             apply_ops(*g.post, g.fused + 1, end, p[0], p[1], p[2], err);
         } else {
             apply_ops(*g.post, g.fused, end, p[0], p[1], p[2], err);
@@ -1004,7 +1004,7 @@ __device__ __forceinline__ void apply_runs(const GuidedArgs& g, double* rows, do
         for (int c = 0; c < 4; ++c) staged = staged && (as_const(*cv).c[c].identity || as_const(*cv).c[c].n <= kTailKnots);
         cubic = as_const(*cv).cubic;
         if (staged)
-            for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], tables + c * poly_table_doubles(kTailKnots));
+            for (int c = 0; c < 4; ++c) curves[c] = stage_poly(&cv->c[c], cubic, tables + c * poly_table_doubles(kTailKnots));
     }
     auto sink = [&](const Pix& px, int x, int i, const double S[6], int cols, int rws) {
         const double inv_n = inv_count<R>(cols, rws);
@@ -1023,7 +1023,7 @@ __device__ __forceinline__ void apply_runs(const GuidedArgs& g, double* rows, do
         }
         apply_ops(*g.post, 0, g.fused, o[0], o[1], o[2], err);
         if (staged) {
-            curves3_lds(curves, cubic, o, err);
+            curves3_lds(curves, o, err);
             apply_ops(*g.post, g.fused + 1, end, o[0], o[1], o[2], err);
         } else {
             apply_ops(*g.post, g.fused, end, o[0], o[1], o[2], err);

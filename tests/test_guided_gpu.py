@@ -138,3 +138,37 @@ def test_stack_matches_single_images(radius):
     torch.cuda.synchronize()
     assert torch.equal(got.view(torch.int32), want.view(torch.int32))
     assert abs(ge_got.item() - ge_want.item()) <= 1e-9 * max(1.0, abs(ge_want.item()))
+
+
+@pytest.mark.parametrize("cubic", [True, False])
+@pytest.mark.parametrize("nk", [2, 5, 16, 17, 40])
+def test_staged_curves_against_oracle(cubic, nk):
+    """The filter's LDS-staged curves (one record per knot count: linear
+    extensions left of the first and right of the last knot, cubic or
+    secant segments between) against the oracle's ExtendedCurvesNode
+    (ToneOps.cpp:795-844): master and channel curves, non-monotone knots
+    (zero and sign-changing tangents), knot counts on both knot searches
+    (two levels up to 16, binary above), outputs landing exactly on knots
+    and outside the knot range."""
+    rng = np.random.default_rng(nk + 100 * int(cubic))
+    H, W = 40, 53
+    raster = rng.lognormal(-1.0, 1.2, (H, W, 3)).astype(np.float32)
+    xs = np.linspace(0.0, 2.0, nk)
+    xs[1:-1] += rng.uniform(-0.3, 0.3, nk - 2) * (2.0 / (nk - 1))
+    master = [(float(x), float(y)) for x, y in zip(xs, rng.uniform(-0.3, 1.8, xs.size))]
+    def chan(seed):
+        r = np.random.default_rng(seed)
+        cx = np.linspace(-0.5, 2.0, 4 + seed % 3)
+        return [(float(x), float(y)) for x, y in zip(cx, np.sort(r.uniform(-0.4, 2.2, cx.size)))]
+    knots = [master, chan(1), [(0.0, 0.0), (1.0, 1.0)], chan(2)]
+    for vp in [(0, 0, W, H), (5, 3, 31, 22)]:
+        got = gpu_guided(raster, 0, 3, 1e-3, vp, stops=0.0, knots=knots, cubic=cubic)
+        exp = O.guided_chain(raster, 0, 3, 1e-3, vp, stops=0.0, knots=knots, cubic=cubic)
+        assert close(got, exp) <= 1e-5, (vp, close(got, exp))
+    # A constant raster filters to itself, so the curves see exactly these
+    # values: knots of the master curve, points between, beyond both ends.
+    for v in [xs[0], xs[1], xs[-1], 0.5 * (xs[0] + xs[1]), -1.0, 7.5, float(xs[xs.size // 2])]:
+        flat = np.full((9, 70, 3), np.float32(v), np.float32)
+        got = gpu_guided(flat, 0, 2, 1e-3, (0, 0, 70, 9), knots=knots, cubic=cubic)
+        exp = O.guided_chain(flat, 0, 2, 1e-3, (0, 0, 70, 9), knots=knots, cubic=cubic)
+        assert close(got, exp) <= 1e-5, (v, close(got, exp))
