@@ -22,9 +22,11 @@
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -245,6 +247,97 @@ py::tuple render_capsule(const Node& output, Rect source_bounds, const RenderReq
     return py::make_tuple(v.width, v.height, cap);
 }
 
+// Fault in the pages of a fresh host buffer in one call
+// (MADV_POPULATE_WRITE, Linux 5.14+; elsewhere the copy faults them in),
+// instead of one page fault per 4 KiB page during the copy.
+void populate(char* p, size_t bytes) {
+#ifdef MADV_POPULATE_WRITE
+    const uintptr_t page = 4096, a = (reinterpret_cast<uintptr_t>(p) + page - 1) & ~(page - 1);
+    const uintptr_t e = (reinterpret_cast<uintptr_t>(p) + bytes) & ~(page - 1);
+    if (e > a) (void)madvise(reinterpret_cast<void*>(a), e - a, MADV_POPULATE_WRITE);
+#else
+    (void)p;
+    (void)bytes;
+#endif
+}
+
+// Device-to-host copy into pageable memory through two pinned staging
+// buffers of this thread (allocated once): chunk i's DMA runs while chunk
+// i - 1 is copied on the host into dst.
+void copy_to_pageable(char* dst, const void* src, size_t bytes, hipStream_t stream) {
+    constexpr size_t kChunk = size_t(16) << 20;
+    struct Staging {
+        void* pin[2] = {nullptr, nullptr};
+        hipEvent_t done[2] = {nullptr, nullptr};
+        ~Staging() {
+            for (int k = 0; k < 2; ++k) {
+                if (pin[k]) (void)hipHostFree(pin[k]);
+                if (done[k]) (void)hipEventDestroy(done[k]);
+            }
+        }
+    };
+    thread_local Staging st;
+    if (!st.pin[0]) {
+        for (int k = 0; k < 2; ++k) {
+            detail::hip(hipHostMalloc(&st.pin[k], kChunk, hipHostMallocDefault), "hipHostMalloc(staging)");
+            detail::hip(hipEventCreateWithFlags(&st.done[k], hipEventDisableTiming), "hipEventCreate");
+        }
+    }
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    auto len = [&](size_t i) { return std::min(kChunk, bytes - i * kChunk); };
+    for (size_t i = 0; i <= chunks; ++i) {
+        if (i < chunks) {
+            detail::hip(hipMemcpyAsync(st.pin[i & 1], static_cast<const char*>(src) + i * kChunk, len(i),
+                                       hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(staging)");
+            detail::hip(hipEventRecord(st.done[i & 1], stream), "hipEventRecord");
+        }
+        if (i > 0) {
+            const size_t j = i - 1;
+            detail::hip(hipEventSynchronize(st.done[j & 1]), "hipEventSynchronize");
+            std::memcpy(dst + j * kChunk, st.pin[j & 1], len(j));
+        }
+    }
+}
+
+// Host-bytes result of a render: compiled chains render on the device and
+// are copied from device memory into the Python bytes object through the
+// pinned staging buffers (no zero-filled intermediate Tile); other chains take
+// Renderer::render_image.  The bytes object is not visible to Python until
+// it is returned, so it is filled without the GIL.
+py::tuple render_bytes(const Node& output, Rect source_bounds, const RenderRequest& request) {
+    if (!detail::compilable(output)) {
+        Tile t;
+        {
+            py::gil_scoped_release unlocked;
+            t = Renderer{}.render_image(output, source_bounds, request);
+        }
+        return tile_result(t);
+    }
+    const Rect v = request.viewport;
+    const size_t bytes = size_t(v.width) * v.height * 3 * sizeof(float);
+    if (bytes > size_t(PY_SSIZE_T_MAX)) throw std::length_error("requested ROI exceeds Python bytes capacity");
+    PyObject* obj = PyBytes_FromStringAndSize(nullptr, Py_ssize_t(bytes));
+    if (!obj) throw py::error_already_set();
+    py::bytes result = py::reinterpret_steal<py::bytes>(obj);
+    char* host = PyBytes_AS_STRING(obj);
+    {
+        py::gil_scoped_release unlocked;
+        const hipStream_t stream = detail::thread_stream();
+        void* dev = nullptr;
+        if (bytes && hipMallocAsync(&dev, bytes, stream) != hipSuccess) throw std::bad_alloc();
+        try {
+            detail::render_to_device(output, source_bounds, request, static_cast<float*>(dev), stream);
+            populate(host, bytes);
+            copy_to_pageable(host, dev, bytes, stream);
+        } catch (...) {
+            if (dev) (void)hipFreeAsync(dev, stream);
+            throw;
+        }
+        if (dev) detail::hip(hipFreeAsync(dev, stream), "hipFreeAsync");
+    }
+    return py::make_tuple(v.width, v.height, result);
+}
+
 // ---- one-shot RAW render (RawEnginePython.cpp:387-519) ----
 
 struct OneShot {
@@ -338,12 +431,7 @@ OneShot prepare_render(const py::object& source, unsigned width, unsigned height
 py::tuple render(const py::object& bayer, unsigned width, unsigned height, const py::object& options) {
     return guarded([&] {
         OneShot s = prepare_render(bayer, width, height, options);
-        Tile t;
-        {
-            py::gil_scoped_release unlocked;
-            t = Renderer{}.render_image(*s.graph, s.request);
-        }
-        return tile_result(t);
+        return render_bytes(s.graph->output(), s.graph->source_bounds(), s.request);
     });
 }
 
@@ -399,13 +487,12 @@ py::tuple render_raster(const py::object& rgb, unsigned width, unsigned height, 
                                      "source format");
         std::vector<float> px(rows * 3);
         std::memcpy(px.data(), raw.data(), raw.size());
-        Tile t;
+        std::optional<ImageGraph> graph;
         {
             py::gil_scoped_release unlocked;
-            ImageGraph graph(RasterImage({width, height, stride, space}, std::move(px)), recipe);
-            t = Renderer{}.render_image(graph, request);
+            graph.emplace(RasterImage({width, height, stride, space}, std::move(px)), recipe);
         }
-        return tile_result(t);
+        return render_bytes(graph->output(), graph->source_bounds(), request);
     });
 }
 
@@ -656,12 +743,7 @@ public:
     py::tuple render(const py::object& options, const py::object& manifest) const {
         return guarded([&] {
             auto [g, r] = prepare(options, manifest);
-            Tile t;
-            {
-                py::gil_scoped_release unlocked;
-                t = Renderer{}.render_image(*g, r);
-            }
-            return tile_result(t);
+            return render_bytes(g->output(), g->output_bounds(), r);
         });
     }
 
