@@ -68,6 +68,44 @@ struct ThreadStream {
 thread_local ThreadStream this_thread_stream;
 }  // namespace
 
+// Two pinned staging buffers per thread (allocated on first use): chunk i's
+// DMA runs while chunk i - 1 is copied on the host into dst.
+void copy_to_host(void* dst_v, const void* src, size_t bytes, hipStream_t stream) {
+    char* dst = static_cast<char*>(dst_v);
+    constexpr size_t kChunk = size_t(16) << 20;
+    struct Staging {
+        void* pin[2] = {nullptr, nullptr};
+        hipEvent_t done[2] = {nullptr, nullptr};
+        ~Staging() {
+            for (int k = 0; k < 2; ++k) {
+                if (pin[k]) (void)hipHostFree(pin[k]);
+                if (done[k]) (void)hipEventDestroy(done[k]);
+            }
+        }
+    };
+    thread_local Staging st;
+    if (!st.pin[0]) {
+        for (int k = 0; k < 2; ++k) {
+            hip(hipHostMalloc(&st.pin[k], kChunk, hipHostMallocDefault), "hipHostMalloc(staging)");
+            hip(hipEventCreateWithFlags(&st.done[k], hipEventDisableTiming), "hipEventCreate");
+        }
+    }
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    auto len = [&](size_t i) { return std::min(kChunk, bytes - i * kChunk); };
+    for (size_t i = 0; i <= chunks; ++i) {
+        if (i < chunks) {
+            hip(hipMemcpyAsync(st.pin[i & 1], static_cast<const char*>(src) + i * kChunk, len(i),
+                               hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(staging)");
+            hip(hipEventRecord(st.done[i & 1], stream), "hipEventRecord");
+        }
+        if (i > 0) {
+            const size_t j = i - 1;
+            hip(hipEventSynchronize(st.done[j & 1]), "hipEventSynchronize");
+            std::memcpy(dst + j * kChunk, st.pin[j & 1], len(j));
+        }
+    }
+}
+
 hipStream_t thread_stream() {
     if (!this_thread_stream.s) {
         {
@@ -1107,8 +1145,7 @@ Tile Renderer::render_image(const Node& output, Rect source_bounds, RenderReques
     detail::DeviceBuffer flags(sizeof(uint32_t), stream);
     detail::hip(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), stream), "hipMemsetAsync");
     detail::run(*chain, v, request.level, static_cast<float*>(dev.p), static_cast<uint32_t*>(flags.p), stream);
-    detail::hip(hipMemcpyAsync(image.rgb.data(), dev.p, image.rgb.size() * sizeof(float), hipMemcpyDeviceToHost,
-                               stream), "hipMemcpyAsync");
+    detail::copy_to_host(image.rgb.data(), dev.p, image.rgb.size() * sizeof(float), stream);
     check(rawops_check_flags(static_cast<uint32_t*>(flags.p), stream));
     return image;
 }

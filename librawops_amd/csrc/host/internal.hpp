@@ -30,6 +30,11 @@ void hip(hipError_t e, const char* what);
 // workers, concurrent callers) run on different streams.
 hipStream_t thread_stream();
 
+// Device-to-host copy of `bytes` into pageable memory on `stream`, staged
+// through two pinned buffers of the calling thread so that each chunk's host
+// copy overlaps the next chunk's DMA; returns when dst holds the data.
+void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 // Device scratch.  With a stream, the memory comes from the device's
 // stream-ordered pool (allocation and release ordered on that stream, kept in
 // the pool between renders: no device-wide synchronisation per render).

@@ -261,44 +261,6 @@ void populate(char* p, size_t bytes) {
 #endif
 }
 
-// Device-to-host copy into pageable memory through two pinned staging
-// buffers of this thread (allocated once): chunk i's DMA runs while chunk
-// i - 1 is copied on the host into dst.
-void copy_to_pageable(char* dst, const void* src, size_t bytes, hipStream_t stream) {
-    constexpr size_t kChunk = size_t(16) << 20;
-    struct Staging {
-        void* pin[2] = {nullptr, nullptr};
-        hipEvent_t done[2] = {nullptr, nullptr};
-        ~Staging() {
-            for (int k = 0; k < 2; ++k) {
-                if (pin[k]) (void)hipHostFree(pin[k]);
-                if (done[k]) (void)hipEventDestroy(done[k]);
-            }
-        }
-    };
-    thread_local Staging st;
-    if (!st.pin[0]) {
-        for (int k = 0; k < 2; ++k) {
-            detail::hip(hipHostMalloc(&st.pin[k], kChunk, hipHostMallocDefault), "hipHostMalloc(staging)");
-            detail::hip(hipEventCreateWithFlags(&st.done[k], hipEventDisableTiming), "hipEventCreate");
-        }
-    }
-    const size_t chunks = (bytes + kChunk - 1) / kChunk;
-    auto len = [&](size_t i) { return std::min(kChunk, bytes - i * kChunk); };
-    for (size_t i = 0; i <= chunks; ++i) {
-        if (i < chunks) {
-            detail::hip(hipMemcpyAsync(st.pin[i & 1], static_cast<const char*>(src) + i * kChunk, len(i),
-                                       hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(staging)");
-            detail::hip(hipEventRecord(st.done[i & 1], stream), "hipEventRecord");
-        }
-        if (i > 0) {
-            const size_t j = i - 1;
-            detail::hip(hipEventSynchronize(st.done[j & 1]), "hipEventSynchronize");
-            std::memcpy(dst + j * kChunk, st.pin[j & 1], len(j));
-        }
-    }
-}
-
 // Host-bytes result of a render: compiled chains render on the device and
 // are copied from device memory into the Python bytes object through the
 // pinned staging buffers (no zero-filled intermediate Tile); other chains take
@@ -328,7 +290,7 @@ py::tuple render_bytes(const Node& output, Rect source_bounds, const RenderReque
         try {
             detail::render_to_device(output, source_bounds, request, static_cast<float*>(dev), stream);
             populate(host, bytes);
-            copy_to_pageable(host, dev, bytes, stream);
+            detail::copy_to_host(host, dev, bytes, stream);
         } catch (...) {
             if (dev) (void)hipFreeAsync(dev, stream);
             throw;
