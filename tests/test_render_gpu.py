@@ -210,3 +210,29 @@ def test_working_space_convert_chain(mip, source, target):
     O.mat3(exp, O.working_to_working_f32(source, target))
     assert bits_equal(got, exp), first_mismatch(got, exp)
     assert node.output_descriptor() == L.ImageDescriptor.scene_linear(L.WorkingSpace(target))
+
+
+def test_srgb_encode_power_segment_within_bar():
+    """The sRGB encode's power segment (x^(1/2.4) from the hardware log2 and
+    exp2, RawEngine.cpp:1264-1275) against the exact power in FP64 over every
+    eighth float of (0.0031308, 1], plus the segment ends, the linear
+    segment and out-of-range inputs: within 1e-6 absolute (bar 1e-5)."""
+    import ctypes
+    import torch
+    from librawops_amd import _capi as C
+    lo = np.float32(0.0031308).view(np.uint32)
+    hi = np.float32(1.0).view(np.uint32)
+    bits = np.arange(lo, hi + 1, 8, dtype=np.uint32)
+    extra = np.array([0.0031308, np.nextafter(np.float32(0.0031308), np.float32(1)), 1.0, 0.5, 0.002, 0.0, -0.0,
+                      -1.0, 2.0, 1e-30], np.float32)
+    x = np.concatenate([bits.view(np.float32), extra])
+    dx = torch.from_numpy(x).cuda()
+    dy = torch.empty_like(dx)
+    C.check(C.lib().rawops_srgb_encode(ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr()), x.size, None))
+    torch.cuda.synchronize()
+    y = dy.cpu().numpy().astype(np.float64)
+    lin = np.clip(x.astype(np.float64), 0.0, 1.0)
+    exact = np.where(lin <= np.float64(np.float32(0.0031308)), np.float64(np.float32(12.92)) * lin,
+                     np.float64(np.float32(1.055)) * lin ** (1.0 / 2.4) - np.float64(np.float32(0.055)))
+    assert np.max(np.abs(y - exact)) <= 1e-6
+    assert y[-3] == 0.0 and y[-4] == 0.0 and y[-5] == 0.0   # -1, -0 and 0 encode to zero
