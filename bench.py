@@ -465,7 +465,8 @@ def leg_dropin(steps):
     (C++ ExecutableEditGraph -> device launch) into host bytes (includes the
     288 MB device-to-host copy over PCIe), into a DLPack device result, and
     again from a warm device tile cache; plus the stream scheduler serving 64
-    queued 512x512 viewports."""
+    queued 512x512 viewports to host bytes (mean of three rounds after the
+    first, which also starts the workers)."""
     import numpy as np
     import torch
     from librawops_amd import rawengine_native as rn
@@ -517,12 +518,15 @@ def leg_dropin(steps):
     rng = np.random.default_rng(9)
     views = [dict(x=int(ax + rng.integers(0, aw - 512)), y=int(ay + rng.integers(0, ah - 512)), roi_width=512,
                   roi_height=512) for _ in range(64)]
-    session.clear_cache()
-    t = time.perf_counter()
-    jobs = [session.submit_manifest(manifest, v) for v in views]
-    for j in jobs:
-        j.result(timeout=120)
-    sched_s = time.perf_counter() - t
+    def serve():
+        session.clear_cache()
+        t = time.perf_counter()
+        jobs = [session.submit_manifest(manifest, v) for v in views]
+        for j in jobs:
+            j.result(timeout=120)
+        return time.perf_counter() - t
+    cold_s = serve()   # the first round also starts the workers, their streams and staging buffers
+    sched_s = sum(serve() for _ in range(3)) / 3
     session.close()
     return {"workload": "saved manifest (Menon -> camera->ProPhoto) on the 6016x3992 active area through "
                         "rawengine_native.RawSession (C++ ExecutableEditGraph, device TileCache)",
@@ -531,7 +535,8 @@ def leg_dropin(steps):
             "warm_cache_dlpack_mpix_s": warm_rate, "warm_cache_dlpack_ms": warm_ms,
             "cache_entries_after_warm": stats["entries"], "cache_hits": stats["hits"] - cold["hits"],
             "scheduler_64x512sq_ms": round(sched_s * 1e3, 2),
-            "scheduler_64x512sq_mpix_s": round(64 * 512 * 512 / sched_s / 1e6, 1)}
+            "scheduler_64x512sq_mpix_s": round(64 * 512 * 512 / sched_s / 1e6, 1),
+            "scheduler_first_round_ms": round(cold_s * 1e3, 2)}
 
 
 def compiled_matrix(node):
