@@ -249,6 +249,33 @@ def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
         C.check(C.lib().rawops_guided_backward(img._handle, radius, eps, vp.c(), P(g2), W, P(gp), P(ge),
                                                P(wsb), wsb.numel(), sp))
     gfb_ms = _timed(gf_only, max(1, steps // 2), st)
+
+    # The config's other radii: the same forward chain and the guided-filter
+    # backward at r = 4 and 16 (eps 1e-3).
+    others = {}
+    for r2 in (4, 16):
+        if r2 == radius:
+            continue
+        n2 = L.WorkingYGuidedFilterNode(L.RasterSourceNode(img), L.Rect(0, 0, W, H),
+                                        L.WorkingYGuidedFilterSettings(r2, eps), max_radius=32)
+        n2 = L.ExtendedCurvesNode(L.ExposureNode(n2, 0.7),
+                                  L.ExtendedCurvesSettings(knots, [ident, ident, ident],
+                                                           L.CurveInterpolation.ShapePreservingCubic))
+        c2 = compile_chain(n2)
+        n2b = ctypes.c_size_t()
+        C.check(C.lib().rawops_guided_workspace_size(W, H, vp.c(), 0, r2, ctypes.byref(n2b)))
+        ws2 = torch.empty(max(1, n2b.value), dtype=torch.uint8, device="cuda")
+        f2 = lambda: DeviceRender.run(c2, vp, L.RenderLevel(), out=q, stream=st, workspace=ws2)
+        b2 = lambda: C.check(C.lib().rawops_guided_backward(img._handle, r2, eps, vp.c(), P(g2), W, P(gp), P(ge),
+                                                            P(wsb), wsb.numel(), sp))
+        f2()
+        clock_ramp(f2)
+        f2_ms = _timed(f2, steps, st)
+        b2()
+        clock_ramp(b2)
+        b2_ms = _timed(b2, max(1, steps // 2), st)
+        others[f"r{r2}"] = {"forward_ms": round(f2_ms, 4), "guided_backward_ms": round(b2_ms, 4)}
+        del ws2
     # CPU forward: the reference's guided filter chain (serial per request,
     # SpatialOps.cpp:168-287) as concurrent row-band requests over a 1024^2
     # crop of the raster, or the restatement's OpenMP bands without it.
@@ -259,7 +286,7 @@ def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
            "forward_ms": round(fwd_ms, 4), "forward_gbs": round(24 * px / fwd_ms / 1e6, 1),
            "backward_chain_ms": round(bwd_ms, 4), "guided_backward_ms": round(gfb_ms, 4),
            "guided_backward_gbs": round(36 * px / gfb_ms / 1e6, 1),
-           "fwd_bwd_mpix_s": round(px / (fwd_ms + bwd_ms) / 1e3, 1)}
+           "fwd_bwd_mpix_s": round(px / (fwd_ms + bwd_ms) / 1e3, 1), "other_radii": others}
     if not cpu_leg:
         return res
     import threading
