@@ -431,25 +431,32 @@ def leg_c5_chain(steps, seed=5, cpu_leg=True):
     with torch.no_grad():
         targets = A.full_chain_crops(img, crops, torch.tensor([1.5, 1.0, 1.25], device="cuda"),
                                      torch.tensor(0.4, device="cuda"), M0, torch.tensor(0.6, device="cuda"))
-    gains = torch.tensor([1.0, 1.0, 1.0], device="cuda", requires_grad=True)
-    stops = torch.tensor(0.0, device="cuda", requires_grad=True)
-    mat = (M0 * 0.9).clone().requires_grad_(True)
-    strength = torch.tensor(0.2, device="cuda", requires_grad=True)
-    opt = torch.optim.Adam([gains, stops, mat, strength], lr=0.01)
-    losses = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        opt.zero_grad()
-        # sum over crops of the per-crop mean squared error (equal-size crops:
-        # crops x the overall mean), all 16 crops in one autograd node
-        y = A.full_chain_crops(img, crops, gains, stops, mat, strength)
-        loss = torch.mean((y - targets) ** 2) * len(crops)
-        loss.backward()
-        opt.step()
-        losses.append(loss.detach() / len(crops))
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    def fit(n):
+        gains = torch.tensor([1.0, 1.0, 1.0], device="cuda", requires_grad=True)
+        stops = torch.tensor(0.0, device="cuda", requires_grad=True)
+        mat = (M0 * 0.9).clone().requires_grad_(True)
+        strength = torch.tensor(0.2, device="cuda", requires_grad=True)
+        opt = torch.optim.Adam([gains, stops, mat, strength], lr=0.01)
+        losses = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            opt.zero_grad()
+            # sum over crops of the per-crop mean squared error (equal-size
+            # crops: crops x the overall mean), all 16 crops in one autograd node
+            y = A.full_chain_crops(img, crops, gains, stops, mat, strength)
+            loss = torch.mean((y - targets) ** 2) * len(crops)
+            loss.backward()
+            opt.step()
+            losses.append(loss.detach() / len(crops))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, losses
+    # An untimed 20-step fit first: the first steps after setup also fill the
+    # caching allocator and load kernels (the first 200-step fit ran at about
+    # half the rate of the next ones); the timed fit starts from the same
+    # initial values.
+    fit(20)
+    el, losses = fit(steps)
     losses = [float(v) for v in losses]
     px = 16 * 512 * 512 * steps
     res = {"workload": f"16 x 512^2 crops of 4000x3000 RGGB, Menon->WB->3x3->exposure->detail boost(r=8,eps=1e-3)"
