@@ -706,8 +706,8 @@ def main():
     line["fwd_bwd"] = {"value": round(px * bsteps / fb_wall / 1e6, 2), "unit": "Mpix/s",
                        "ms_per_step": round(fb_wall * 1e3 / bsteps, 4), "backward_kernel_ms": round(bwd_ms, 4),
                        "forward_record_ms": round(fwd_rec_ms, 4), "clock_ramp_steps": ramp_fb,
-                       "backward": ("menon_backward<64,20> (classifier recomputed)" if args.no_record else
-                                    "menon_backward_recorded<64,20> (classifier decisions recorded by the forward) + menon_backward<64,20> on border tiles")
+                       "backward": ("menon_backward<64,32> (classifier recomputed)" if args.no_record else
+                                    "menon_backward_recorded<64,32> (classifier decisions recorded by the forward) + menon_backward<64,32> on border tiles")
                                    + ": dL/d raw (24 MP plane) + dL/d gains, exposure, 3x3"}
     del gout, graw
 

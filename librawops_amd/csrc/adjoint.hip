@@ -28,6 +28,12 @@ namespace rawops {
 namespace {
 
 constexpr int kNT = 512;
+// Menon adjoint tile height (64-wide tiles; 24 MP frame, recorded /
+// recomputing adjoint, ms): 20 rows 0.279 / 0.331, 24: 0.256 / 0.386,
+// 28: 0.243 / 0.347, 32: 0.228 / 0.343.  One height for both kernels, so that
+// a site is gathered by the same (interior or border) tile code in both and
+// their raw gradients agree bit for bit.
+constexpr int kAdjTH = 32;
 
 struct BackwardArgs {
     SensorArgs s;
@@ -949,11 +955,12 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
     a.graw = L.g_raw;
     a.gparams = L.g_params;
     // Only raw samples within 4 of the viewport receive gradient (the
-    // adjoint's reach); tiles are anchored at the active-area origin.  The
-    // Menon tile is 64x20 so its recompute region (tile +-4) fits three
-    // workgroups per CU in LDS.
+    // adjoint's reach); tiles are anchored at the active-area origin, 64 x
+    // kAdjTH (Menon: the recorded kernel 48 KB of LDS, three workgroups per
+    // CU; the recomputing kernel, whose recompute region is tile +-4, 71 KB,
+    // two) or 64 x 32 (bilinear).
     const bool menon = L.demosaic == RAWOPS_DEMOSAIC_MENON;
-    const int TW = 64, TH = menon ? 20 : 32;
+    const int TW = 64, TH = menon ? kAdjTH : 32;
     const int x0 = std::max(0, a.vx - 4 - a.s.ax), y0 = std::max(0, a.vy - 4 - a.s.ay);
     const int x1 = std::min(a.s.aw, a.vx + a.vw + 4 - a.s.ax), y1 = std::min(a.s.ah, a.vy + a.vh + 4 - a.s.ay);
     a.tile0_x = x0 / TW;
@@ -1011,19 +1018,19 @@ hipError_t launch_render_backward(const BackwardLaunch& L, hipStream_t stream) {
         a.rw = (int)L.recorded.width; a.rh = (int)L.recorded.height;
         const unsigned rec = (unsigned)((a.r1 - a.r0) * (a.c1 - a.c0));
         if (rec) {
-            hipLaunchKernelGGL((menon_backward_recorded<64, 20>), dim3(rec), dim3(kNT), 0, stream, a);
+            hipLaunchKernelGGL((menon_backward_recorded<64, kAdjTH>), dim3(rec), dim3(kNT), 0, stream, a);
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
         // the recomputing kernel's border-first order lists the tiles outside
         // the block [r0, r1) x [c0, c1) first: launch only those
         if (blocks - rec)
-            hipLaunchKernelGGL((menon_backward<64, 20>), dim3(blocks - rec), dim3(kNT), 0, stream, a);
+            hipLaunchKernelGGL((menon_backward<64, kAdjTH>), dim3(blocks - rec), dim3(kNT), 0, stream, a);
         return hipGetLastError();
     }
     if (a.r0 == a.r1 || a.c0 == a.c1) a.r0 = a.r1 = a.c0 = a.c1 = 0;
     if (menon)
-        hipLaunchKernelGGL((menon_backward<64, 20>), dim3(blocks), dim3(kNT), 0, stream, a);
+        hipLaunchKernelGGL((menon_backward<64, kAdjTH>), dim3(blocks), dim3(kNT), 0, stream, a);
     else
         hipLaunchKernelGGL((bilinear_backward<64, 32>), dim3(blocks), dim3(kNT), 0, stream, a);
     return hipGetLastError();
