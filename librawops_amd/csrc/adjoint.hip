@@ -30,7 +30,8 @@ namespace {
 constexpr int kNT = 512;
 // Menon adjoint tile height (64-wide tiles; 24 MP frame, recorded /
 // recomputing adjoint, ms): 20 rows 0.279 / 0.331, 24: 0.256 / 0.386,
-// 28: 0.243 / 0.347, 32: 0.228 / 0.343.  One height for both kernels, so that
+// 28: 0.243 / 0.347, 32: 0.228 / 0.343, 40: 0.258 / 0.507, 48: 0.237 /
+// 0.462.  One height for both kernels, so that
 // a site is gathered by the same (interior or border) tile code in both and
 // their raw gradients agree bit for bit.
 constexpr int kAdjTH = 32;
