@@ -67,115 +67,135 @@ __global__ void boost_bwd(const float* x, const float* base, const float* gy, fl
     block_add(acc, part, gs);
 }
 
-// Master-curve adjoint.  For the segment the value falls in, the cubic is the
-// Bezier B(t) = sum_k b_k(t) P_k with P = (y_i, y_i + h T_i/3, y_i+1 - h T_i+1/3,
-// y_i+1); tangents T are the minmod of the adjacent secant slopes.  dL/dy_j
-// collects the Bernstein weights through P and T; dL/dx uses B'(t)/h.
-// Per element every contribution lands on the knots i-1 .. i+2 (the tangent
-// of knot j follows the secant k_j in {j-1, j} that the minmod picks, or
-// none, fixed by the curve and tabulated once per block), so the weights of
-// those four knots are formed in registers and added with four
-// read-modify-writes.  The segment is found by a binary search over float
-// thresholds: for a float v, v < x_k exactly when v < the smallest float >=
-// x_k.  LANE > 0 (curves of at most LANE knots): each thread accumulates into
-// its own LDS column, so values of one segment in a wavefront (common: the
-// knots are few) do not serialise on shared atomics; the columns are summed
-// at the end.  LANE == 0: shared LDS atomics.
-template <int LANE>
+// Master-curve adjoint.  The curve at v is one of n + 1 pieces, chosen by the
+// number k of knots <= v: the linear extension left of the first knot
+// (k = 0), segment k - 1 (1 <= k < n: the cubic with Bernstein control points
+// y_i, y_i + h T_i / 3, y_i+1 - h T_i+1 / 3, y_i+1 of a cubic curve, T the
+// minmod tangents, each the secant it follows or zero; or the secant of a
+// linear curve), the linear extension right of the last knot (k = n).  On
+// piece k, with t = (v - x_k') / h_k, the weight of each knot ordinate in the
+// value is a polynomial of degree <= 3 in t whose coefficients depend only on
+// the knots, and only the four knots k - 2 .. k + 1 have weights: they are
+// tabulated per piece once per workgroup, so an element costs one table
+// lookup, four Horner evaluations and four adds into its lane's accumulator
+// column, with no branches.  dL/dv = g f'(v) from the piece's power-basis
+// coefficients (f = c0 + c1 u + c2 u^2 + c3 u^3, u = v - x_k').  At a knot
+// (t = 0) the weights and the derivative are those of the piece starting
+// there: the value y_j exactly and the knot tangent (cubic) or right-hand
+// secant (linear), as the forward's exact-knot rule gives (ToneOps.cpp:780-792).
+// The piece is found by a search over float thresholds (for a float v,
+// v >= x_k exactly when v >= the smallest float >= x_k).  LANE: each thread
+// adds into its own LDS column (curves of at most 16 knots), so values of one
+// piece in a wavefront do not serialise on shared atomics; else shared LDS
+// atomics.
+constexpr int kRec = 6;    // per piece: x', d1 = c1, d2 = 2 c2, d3 = 3 c3, 1 / h, pad
+constexpr int kWts = 16;   // per piece: knot k - 2 + q, power p of t: [q][p]
+
+// Minmod tangent source of knot j (ToneOps.cpp:795-816): -1 the secant j - 1,
+// 0 the secant j, 1 none (zero tangent).
+__device__ __forceinline__ int tangent_source(Const<DevCurve>& c, uint32_t j) {
+    const uint32_t n = c.n;
+    if (j == 0) return 0;
+    if (j == n - 1) return -1;
+    const double a = c.slope[j - 1], b = c.slope[j];
+    return (a == 0 || b == 0 || signbit(a) != signbit(b)) ? 1 : (fabs(a) <= fabs(b) ? -1 : 0);
+}
+
+template <bool LANE>
 __global__ void __launch_bounds__(kPW) curve_bwd(const float* xv, const float* gy, size_t n, const DevCurve* c,
                                                  uint32_t cubic, float* gx, double* gys) {
-    constexpr int KT = LANE > 0 ? LANE : kMaxKnots;          // knots the tables hold
-    __shared__ double acc[LANE > 0 ? LANE * kPW : kMaxKnots];
-    __shared__ double tab[6 * KT];   // the knot tables (+ 1 / segment width), staged: lookups are data-dependent
-    __shared__ float xf[KT];         // the float thresholds, +inf beyond the last knot
-    __shared__ int src[KT];          // k_j - j (-1 or 0): the secant knot j's tangent follows, 1 for none
-    __shared__ double part[LANE > 0 ? (kPW / 64) * LANE : 1];
-    for (int k = threadIdx.x; k < (LANE > 0 ? LANE * kPW : kMaxKnots); k += kPW) acc[k] = 0.0;
-    const uint32_t nk = as_const(*c).n;
-    uint32_t p2 = 1;
+    constexpr int KMAX = LANE ? 16 : kMaxKnots;                 // knots the tables hold
+    constexpr int COLS = KMAX + 4;                              // knot columns -2 .. KMAX + 1
+    __shared__ __attribute__((aligned(16))) double rec[kRec * (KMAX + 1)];
+    __shared__ __attribute__((aligned(16))) double wts[kWts * (KMAX + 1)];
+    __shared__ __attribute__((aligned(16))) float xf[KMAX < 16 ? 16 : KMAX];
+    __shared__ double acc[LANE ? COLS * kPW : COLS];
+    __shared__ double part[LANE ? (kPW / 64) * KMAX : 1];
+    Const<DevCurve>& s = as_const(*c);
+    const uint32_t nk = s.n;
+    uint32_t p2 = 4;
     while (p2 < nk) p2 <<= 1;
-    for (uint32_t k = threadIdx.x; k < p2; k += kPW) {
-        xf[k] = k < nk ? float_at_least(c->x[k]) : __builtin_inff();
-        if (k >= nk) continue;
-        tab[k] = c->x[k];
-        tab[nk + k] = c->y[k];
-        tab[2 * nk + k] = c->slope[k];
-        tab[3 * nk + k] = c->c0[k];
-        tab[4 * nk + k] = c->c1[k];
-        tab[5 * nk + k] = k + 1 < nk ? 1.0 / (c->x[k + 1] - c->x[k]) : 0.0;
-        int sk;
-        if (k == 0) {
-            sk = 0;
-        } else if (k == nk - 1) {
-            sk = -1;
+    const float nan = __builtin_nanf("");
+    const float t4 = nk > 4 ? float_at_least(s.x[4]) : nan, t8 = nk > 8 ? float_at_least(s.x[8]) : nan,
+                t12 = nk > 12 ? float_at_least(s.x[12]) : nan;
+    for (uint32_t k = threadIdx.x; k < p2; k += kPW) xf[k] = k < nk ? float_at_least(s.x[k]) : nan;
+    for (uint32_t k = threadIdx.x; k < (LANE ? COLS * kPW : COLS); k += kPW) acc[k] = 0.0;
+    for (uint32_t k = threadIdx.x; k <= nk; k += kPW) {
+        double* r = rec + kRec * k;
+        double* w = wts + kWts * k;   // w[4 q + p]: knot k - 2 + q, t^p
+        for (int q = 0; q < kWts; ++q) w[q] = 0.0;
+        double x, d1, d2 = 0.0, d3 = 0.0, h;
+        if (k == 0) {                  // y0 + t (y1 - y0)
+            x = s.x[0]; d1 = s.slope[0]; h = s.x[1] - s.x[0];
+            w[4 * 2 + 0] = 1.0; w[4 * 2 + 1] = -1.0;
+            w[4 * 3 + 1] = 1.0;
+        } else if (k == nk) {          // y_n-1 + t (y_n-1 - y_n-2)
+            x = s.x[nk - 1]; d1 = s.slope[nk - 2]; h = s.x[nk - 1] - s.x[nk - 2];
+            w[4 * 0 + 1] = -1.0;
+            w[4 * 1 + 0] = 1.0; w[4 * 1 + 1] = 1.0;
         } else {
-            const double a = c->slope[k - 1], b = c->slope[k];
-            sk = (a == 0 || b == 0 || signbit(a) != signbit(b)) ? 1 : (fabs(a) <= fabs(b) ? -1 : 0);
+            const uint32_t i = k - 1;  // knots i - 1 .. i + 2 are q = 0 .. 3
+            x = s.x[i]; h = s.x[i + 1] - s.x[i];
+            if (cubic) {
+                const double y0 = s.y[i], y1 = s.y[i + 1], p = s.c0[i], q = s.c1[i], inv = 1.0 / h;
+                d1 = (3.0 * (p - y0)) * inv;
+                d2 = 2.0 * (((3.0 * ((y0 - 2.0 * p) + q)) * inv) * inv);
+                d3 = 3.0 * (((((y1 - y0) + 3.0 * (p - q)) * inv) * inv) * inv);
+                // b0 + b1 = 1 - 3t^2 + 2t^3 on y_i, b2 + b3 = 3t^2 - 2t^3 on y_i+1
+                w[4 * 1 + 0] = 1.0; w[4 * 1 + 2] = -3.0; w[4 * 1 + 3] = 2.0;
+                w[4 * 2 + 2] = 3.0; w[4 * 2 + 3] = -2.0;
+                // (h / 3) b1 T_i, b1 / 3 = t - 2t^2 + t^3; T_i a secant: d/dy = +-1 / its width
+                const double B1[4] = {0.0, 1.0, -2.0, 1.0};
+                const int si = tangent_source(s, i);
+                if (si == 0) {
+                    for (int e = 0; e < 4; ++e) { w[4 * 2 + e] += B1[e]; w[4 * 1 + e] -= B1[e]; }
+                } else if (si == -1) {
+                    const double r = h / (s.x[i] - s.x[i - 1]);
+                    for (int e = 0; e < 4; ++e) { w[4 * 1 + e] += r * B1[e]; w[4 * 0 + e] -= r * B1[e]; }
+                }
+                // -(h / 3) b2 T_i+1, b2 / 3 = t^2 - t^3
+                const double B2[4] = {0.0, 0.0, -1.0, 1.0};
+                const int si1 = tangent_source(s, i + 1);
+                if (si1 == -1) {
+                    for (int e = 0; e < 4; ++e) { w[4 * 2 + e] += B2[e]; w[4 * 1 + e] -= B2[e]; }
+                } else if (si1 == 0) {
+                    const double r = h / (s.x[i + 2] - s.x[i + 1]);
+                    for (int e = 0; e < 4; ++e) { w[4 * 3 + e] += r * B2[e]; w[4 * 2 + e] -= r * B2[e]; }
+                }
+            } else {                   // y_i + t (y_i+1 - y_i)
+                d1 = s.slope[i];
+                w[4 * 1 + 0] = 1.0; w[4 * 1 + 1] = -1.0;
+                w[4 * 2 + 1] = 1.0;
+            }
         }
-        src[k] = sk;
+        r[0] = x; r[1] = d1; r[2] = d2; r[3] = d3; r[4] = 1.0 / h; r[5] = 0.0;
     }
     __syncthreads();
-    const double *X = tab, *Y = tab + nk, *SL = tab + 2 * nk, *C0 = tab + 3 * nk, *C1 = tab + 4 * nk;
-    const double* inv_h = tab + 5 * nk;   // 1 / (x[k+1] - x[k]): gradients need no quotient rounding
-    auto one = [&](size_t e, float vf, double g) {
-        const double v = vf;
-        // lo = the number of knots x_k with !(v < x_k) (all of them for NaN)
-        uint32_t lo = 0;
-        for (uint32_t step = p2 >> 1; step > 0; step >>= 1) lo += !(vf < xf[lo + step - 1]) ? step : 0u;
-        lo += !(vf < xf[lo]) ? 1u : 0u;
-        lo = lo < nk ? lo : nk;
-        const uint32_t left = lo == 0 ? 0u : lo - 1u;
-        const uint32_t i = left < nk - 2u ? left : nk - 2u;
-        const double ih = inv_h[i];
-        // dL/dy of the knots i-1, i, i+1, i+2
-        double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
-        const double gl = g, gl1 = left == i ? gl : 0.0, gl2 = left == i ? 0.0 : gl;   // dL/dy_left
-        float gxe;
-        if (v == X[left]) {               // exact knot: value y_left
-            w1 = gl1;
-            w2 = gl2;
-            double slope = SL[left < nk - 1 ? left : nk - 2];   // linear: right-hand secant
-            if (cubic) {                                       // cubic: the knot tangent
-                slope = left < nk - 1 ? 3.0 * (C0[left] - Y[left]) / (X[left + 1] - X[left])
-                                      : 3.0 * (Y[left] - C1[left - 1]) / (X[left] - X[left - 1]);
-            }
-            gxe = (float)(g * slope);
-        } else if (!cubic || v < X[0] || v > X[nk - 1]) {
-            // y = y_left + (v - x_left) * s_i,  s_i = (y_i+1 - y_i) / h
-            const double dv = v - X[left];
-            const double q = g * dv * ih;
-            w1 = gl1 - q;
-            w2 = gl2 + q;
-            gxe = (float)(g * SL[i]);
-        } else {
-            const double h = X[i + 1] - X[i];
-            const double t = (v - X[i]) * ih, u = 1.0 - t;
-            const double b0 = u * u * u, b1 = 3 * u * u * t, b2 = 3 * u * t * t, b3 = t * t * t;
-            const double P0 = Y[i], P1 = C0[i], P2 = C1[i], P3 = Y[i + 1];
-            gxe = (float)(g * 3.0 * (u * u * (P1 - P0) + 2 * u * t * (P2 - P1) + t * t * (P3 - P2)) * ih);
-            w1 = g * (b0 + b1);
-            w2 = g * (b2 + b3);
-            // Tangent T_j -> secant k (minmod picks one of the adjacent secants,
-            // or none): dL/dy_k+1 += dT / h_k, dL/dy_k -= dT / h_k.
-            const double Ti = g * b1 * h * (1.0 / 3.0), Ti1 = -g * b2 * h * (1.0 / 3.0);
-            const int si = src[i], si1 = src[i + 1];
-            const double a = si == -1 ? Ti * inv_h[i > 0 ? i - 1 : 0] : 0.0;   // secant i-1
-            const double b = si == 0 ? Ti * ih : 0.0;                            // secant i
-            const double cc = si1 == -1 ? Ti1 * ih : 0.0;                        // secant i
-            const double d = si1 == 0 ? Ti1 * inv_h[i + 1 < nk - 1 ? i + 1 : i] : 0.0;   // secant i+1
-            w0 = -a;
-            w1 += a - b - cc;
-            w2 += b + cc - d;
-            w3 = d;
+    auto piece_of = [&](float v) -> uint32_t {
+        if (p2 <= 16) {
+            const uint32_t g = (v >= t4 ? 1u : 0u) + (v >= t8 ? 1u : 0u) + (v >= t12 ? 1u : 0u);
+            const float4 q = reinterpret_cast<const float4*>(xf)[g];
+            return 4u * g + (v >= q.x ? 1u : 0u) + (v >= q.y ? 1u : 0u) + (v >= q.z ? 1u : 0u) + (v >= q.w ? 1u : 0u);
         }
-        gx[e] = gxe;
-        const double w[4] = {w0, w1, w2, w3};
+        uint32_t cnt = 0;
+        for (uint32_t st = p2 >> 1; st > 0; st >>= 1) cnt += v >= xf[cnt + st - 1] ? st : 0u;
+        return cnt + (v >= xf[cnt] ? 1u : 0u);
+    };
+    auto one = [&](size_t e, float vf, double g) {
+        const uint32_t k = piece_of(vf);
+        const double2* r = reinterpret_cast<const double2*>(rec + kRec * k);
+        const double2 a = r[0], b = r[1], hh = r[2];
+        const double du = (double)vf - a.x;
+        gx[e] = (float)(g * __builtin_fma(__builtin_fma(b.y, du, b.x), du, a.y));
+        const double t = du * hh.x;
+        const double2* w = reinterpret_cast<const double2*>(wts + kWts * k);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int j = (int)i - 1 + q;
-            if (j < 0 || j >= (int)nk) continue;
-            if constexpr (LANE > 0) acc[j * kPW + threadIdx.x] += w[q];
-            else if (w[q] != 0.0) atomicAdd(&acc[j], w[q]);
+            const double2 lo = w[2 * q], hi = w[2 * q + 1];
+            const double wq = g * __builtin_fma(__builtin_fma(__builtin_fma(hi.y, t, hi.x), t, lo.y), t, lo.x);
+            const int col = (int)k + q;   // column of knot k - 2 + q
+            if constexpr (LANE) acc[col * kPW + threadIdx.x] += wq;
+            else if (wq != 0.0) atomicAdd(&acc[col], wq);
         }
     };
     // Four elements per thread per round, their loads issued together.
@@ -192,23 +212,23 @@ __global__ void __launch_bounds__(kPW) curve_bwd(const float* xv, const float* g
         for (int u = 0; u < 4; ++u)
             if (e0 + u * stride < n) one(e0 + u * stride, vv[u], (double)gg[u]);
     }
-    if constexpr (LANE > 0) {
-        // sum the columns: per knot a wavefront reduction, then the waves
+    if constexpr (LANE) {
+        // sum the columns of knots 0 .. nk - 1: per knot a wavefront reduction, then the waves
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         for (uint32_t k = 0; k < nk; ++k) {
-            double v = acc[k * kPW + threadIdx.x];
+            double v = acc[(k + 2) * kPW + threadIdx.x];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) part[wave * LANE + k] = v;
+            if (lane == 0) part[wave * KMAX + k] = v;
         }
         __syncthreads();
         if (threadIdx.x < nk) {
             double t = 0;
-            for (int w = 0; w < kPW / 64; ++w) t += part[w * LANE + threadIdx.x];
+            for (int w = 0; w < kPW / 64; ++w) t += part[w * KMAX + threadIdx.x];
             atomicAdd(gys + threadIdx.x, t);
         }
     } else {
         __syncthreads();
-        for (int k = threadIdx.x; k < (int)nk; k += kPW) atomicAdd(gys + k, acc[k]);
+        for (uint32_t k = threadIdx.x; k < nk; k += kPW) atomicAdd(gys + k, acc[k + 2]);
     }
 }
 
@@ -254,9 +274,9 @@ hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, cons
                                  float* gx, double* gys, hipStream_t s) {
     if (!n) return hipSuccess;
     if (knots <= 16)   // per-thread accumulator columns
-        hipLaunchKernelGGL(curve_bwd<16>, dim3(reduce_blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
+        hipLaunchKernelGGL(curve_bwd<true>, dim3(reduce_blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
     else
-        hipLaunchKernelGGL(curve_bwd<0>, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
+        hipLaunchKernelGGL(curve_bwd<false>, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
     return hipGetLastError();
 }
 
