@@ -111,6 +111,8 @@ struct HistogramLaunch {
     unsigned long long* counts;
     unsigned long long* tallies;
     uint32_t* extrema;
+    uint32_t* partial;                // set by the launcher: per-workgroup partial results, or null
+    uint32_t pstride;                 // words per workgroup in `partial`
 };
 hipError_t launch_histogram(const HistogramLaunch& launch, hipStream_t stream);
 

@@ -141,11 +141,31 @@ def test_device_histogram_bin_edges(bins, lower, upper):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("height,width", [(1500, 2000), (64, 4100), (4000, 50)])
+def test_device_histogram_partial_rows(height, width):
+    """Images of 64 rows or more at up to 2048 bins: each workgroup stores its
+    bins, tallies and extremum keys into a row of a partial array and a second
+    kernel sums the rows into the caller's counters.  1500x2000 at 256 bins
+    takes the full 1024 workgroups (two rows each, so the last 274 hold no
+    pixels) and 64 merge chunks; 64 rows is the fewest that take the path;
+    4000x50 at 256 bins gives each of its 75 workgroups 54 narrow rows.  Two
+    bands add into the same counters."""
+    img = random_image(height, width, seed=height + width)
+    for bins, lower, upper in ((256, 0.0, 1.0), (2048, -0.25, 1.5)):
+        want = O.histogram_rgb(img, bins, lower, upper)
+        for bands in (1, 2):
+            got = device_histogram(img, bins, lower, upper, bands)
+            for c in range(3):
+                assert np.array_equal(got[c][0], want[c][0]), (bins, c, bands)
+                assert got[c][1:] == want[c][1:], (bins, c, bands)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("width", [2049, 4100, 6001])
 @pytest.mark.parametrize("bins,lower,upper", [(256, 0.0, 1.0), (1000, -0.3, 1.7), (4096, 0.0, 1.0)])
 def test_device_histogram_wide_rows(width, bins, lower, upper):
-    """Rows wider than one pass of the workgroup (256 threads x 8 pixel
-    slots): every pixel slot p >= 1, column batches starting at 2048 and
+    """Rows wider than one pass of the workgroup (256 threads x 4 pixel
+    slots): every pixel slot p >= 1, column batches starting at 1024 and
     beyond, and a ragged last batch.  Bin-edge samples (on, and one float32
     step either side of, every edge) sit at columns 256 p + j, so each slot
     meets them."""
