@@ -220,12 +220,11 @@ def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
     C.check(C.lib().rawops_guided_filter(img._handle, vp.c(), 0, radius, eps, empty.handle,
                                          ctypes.c_void_p(base.data_ptr()), W, ctypes.c_void_p(ws.data_ptr()),
                                          ws.numel(), None, sp))
-    lin = base * m
     kx = [k[0] for k in knots]
     X = (ctypes.c_double * 22)(*(kx + [0.0, 1.0] * 3))
     Y = (ctypes.c_double * 22)(*([k[1] for k in knots] + [0.0, 1.0] * 3))
     plan = Program.from_ops(C.DEMOSAIC_BILINEAR, False, [], [(C.OP_CURVES, (), (X, Y, [0, 16, 18, 20, 22], 1))])
-    g1, g2, gp = torch.empty_like(p), torch.empty_like(p), torch.empty_like(p)
+    g2, gp = torch.empty_like(p), torch.empty_like(p)
     gk = torch.zeros(16, dtype=torch.float64, device="cuda")
     gs = torch.zeros(1, dtype=torch.float64, device="cuda")
     ge = torch.zeros(1, dtype=torch.float64, device="cuda")
@@ -236,8 +235,9 @@ def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
     N = p.numel()
 
     def bwd():
-        C.check(C.lib().rawops_curve_backward(plan.handle, P(lin), P(gq), N, P(g1), P(gk), sp))
-        C.check(C.lib().rawops_exposure_backward(P(base), P(g1), m, P(g2), P(gs), N, sp))
+        # exposure + curve adjoints in one pass (the same dL/dx as
+        # rawops_curve_backward on m x then rawops_exposure_backward)
+        C.check(C.lib().rawops_exposure_curve_backward(plan.handle, P(base), m, P(gq), N, P(g2), P(gk), P(gs), sp))
         C.check(C.lib().rawops_guided_backward(img._handle, radius, eps, vp.c(), P(g2), W, P(gp), P(ge),
                                                P(wsb), wsb.numel(), sp))
     for _ in range(2):
@@ -281,7 +281,7 @@ def leg_c3_guided(steps, radius=8, eps=1e-3, cpu_leg=True):
     # crop of the raster, or the restatement's OpenMP bands without it.
     px = W * H
     res = {"workload": f"6000x4000 FP32 lognormal raster, Rec.2020 guide, r={radius}, eps={eps}, +0.7 EV, "
-                       "16-point cubic curve; fwd = coefficients + apply (exposure, curve), bwd = curve + exposure + "
+                       "16-point cubic curve; fwd = coefficients + apply (exposure, curve), bwd = exposure + curve (one pass) + "
                        "guided filter",
            "forward_ms": round(fwd_ms, 4), "forward_gbs": round(24 * px / fwd_ms / 1e6, 1),
            "backward_chain_ms": round(bwd_ms, 4), "guided_backward_ms": round(gfb_ms, 4),

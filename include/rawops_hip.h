@@ -291,6 +291,15 @@ rawops_status rawops_detail_boost_backward(const float* x, const float* base, co
  * gradients g_knots_y (device double[knot count], accumulated). */
 rawops_status rawops_curve_backward(const rawops_plan* curve_plan, const float* x, const float* g_y, size_t n,
                                     float* g_x, double* g_knots_y, void* stream);
+/* ExposureNode (y = m x, m from rawops_exposure_multiplier) followed by the
+ * master curve: the adjoints of both in one pass over x (the exposure input)
+ * and g_y.  g_x, g_knots_y and *g_stops as rawops_curve_backward on m x
+ * followed by rawops_exposure_backward give them (g_x bit for bit; the FP64
+ * sums up to their order), without the intermediate gradient in memory.  No
+ * reference counterpart: derivatives are new. */
+rawops_status rawops_exposure_curve_backward(const rawops_plan* curve_plan, const float* x, float multiplier,
+                                             const float* g_y, size_t n, float* g_x, double* g_knots_y,
+                                             double* g_stops, void* stream);
 
 /* ---- guided filter (SpatialOps.cpp:159-317) + fused point epilogue ---- */
 /* Working-Y guided filter of a raster source at level `mip`, output rect

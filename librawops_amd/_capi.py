@@ -150,6 +150,7 @@ def lib():
             "rawops_detail_boost": ([P, P, ctypes.c_float, P, ctypes.c_size_t, P], S),
             "rawops_detail_boost_backward": ([P, P, P, ctypes.c_float, P, P, P, ctypes.c_size_t, P], S),
             "rawops_curve_backward": ([P, P, P, ctypes.c_size_t, P, P, P], S),
+            "rawops_exposure_curve_backward": ([P, P, ctypes.c_float, P, ctypes.c_size_t, P, P, P, P], S),
             "rawops_guided_workspace_size": ([u32, u32, Rect, u32, u32, P], S),
             "rawops_guided_filter": ([P, Rect, u32, u32, ctypes.c_double, P, P, u32, P,
                                       ctypes.c_size_t, P, P], S),

@@ -801,6 +801,19 @@ rawops_status rawops_curve_backward(const rawops_plan* plan, const float* x, con
     });
 }
 
+rawops_status rawops_exposure_curve_backward(const rawops_plan* plan, const float* x, float m, const float* gy,
+                                             size_t n, float* gx, double* gys, double* g_stops, void* stream) {
+    return guarded([&] {
+        if (!plan || plan->reduced.n < 1 || plan->reduced.op[0].kind != OP_CURVES)
+            fail(RAWOPS_INVALID_ARGUMENT, "curve backward needs a plan whose first reduced op is a curve");
+        if (!g_stops) fail(RAWOPS_INVALID_ARGUMENT, "g_stops is null");
+        const DevCurves* t = plan->reduced.op[0].curves;
+        hip_check(launch_curve_backward(x, gy, n, &t->c[0], plan->reduced.op[0].knots, plan->curve_cubic, gx, gys,
+                                        static_cast<hipStream_t>(stream), m, g_stops),
+                  "exposure + curve backward launch");
+    });
+}
+
 rawops_status rawops_guided_backward_workspace_size(uint32_t width, uint32_t height, size_t* bytes) {
     return guarded([&] {
         if (!bytes) fail(RAWOPS_INVALID_ARGUMENT, "output pointer is null");

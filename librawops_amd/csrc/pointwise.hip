@@ -87,7 +87,10 @@ __global__ void boost_bwd(const float* x, const float* base, const float* gy, fl
 // v >= x_k exactly when v >= the smallest float >= x_k).  LANE: each thread
 // adds into its own LDS column (curves of at most 16 knots), so values of one
 // piece in a wavefront do not serialise on shared atomics; else shared LDS
-// atomics.
+// atomics.  EXPO: the curve follows an exposure v = m x (the forward's FP32
+// product); the kernel reads x, writes dL/dx = m (g f'(v)) and adds
+// ln2 sum(g f'(v) m x) into *gst, the values exposure_bwd gives from the
+// curve adjoint's output, without that output's round trip through memory.
 constexpr int kRec = 6;    // per piece: x', d1 = c1, d2 = 2 c2, d3 = 3 c3, 1 / h, pad
 constexpr int kWts = 16;   // per piece: knot k - 2 + q, power p of t: [q][p]
 
@@ -101,9 +104,9 @@ __device__ __forceinline__ int tangent_source(Const<DevCurve>& c, uint32_t j) {
     return (a == 0 || b == 0 || signbit(a) != signbit(b)) ? 1 : (fabs(a) <= fabs(b) ? -1 : 0);
 }
 
-template <bool LANE>
+template <bool LANE, bool EXPO>
 __global__ void __launch_bounds__(kPW) curve_bwd(const float* xv, const float* gy, size_t n, const DevCurve* c,
-                                                 uint32_t cubic, float* gx, double* gys) {
+                                                 uint32_t cubic, float* gx, double* gys, float m, double* gst) {
     constexpr int KMAX = LANE ? 16 : kMaxKnots;                 // knots the tables hold
     constexpr int COLS = KMAX + 4;                              // knot columns -2 .. KMAX + 1
     __shared__ __attribute__((aligned(16))) double rec[kRec * (KMAX + 1)];
@@ -181,12 +184,20 @@ __global__ void __launch_bounds__(kPW) curve_bwd(const float* xv, const float* g
         for (uint32_t st = p2 >> 1; st > 0; st >>= 1) cnt += v >= xf[cnt + st - 1] ? st : 0u;
         return cnt + (v >= xf[cnt] ? 1u : 0u);
     };
-    auto one = [&](size_t e, float vf, double g) {
+    double sacc = 0.0;   // EXPO: sum of g f'(v) m x
+    auto one = [&](size_t e, float xe, double g) {
+        const float vf = EXPO ? xe * m : xe;
         const uint32_t k = piece_of(vf);
         const double2* r = reinterpret_cast<const double2*>(rec + kRec * k);
         const double2 a = r[0], b = r[1], hh = r[2];
         const double du = (double)vf - a.x;
-        gx[e] = (float)(g * __builtin_fma(__builtin_fma(b.y, du, b.x), du, a.y));
+        const float gc = (float)(g * __builtin_fma(__builtin_fma(b.y, du, b.x), du, a.y));
+        if constexpr (EXPO) {
+            gx[e] = gc * m;
+            sacc += (double)gc * (double)xe * (double)m;
+        } else {
+            gx[e] = gc;
+        }
         const double t = du * hh.x;
         const double2* w = reinterpret_cast<const double2*>(wts + kWts * k);
 #pragma unroll
@@ -230,6 +241,10 @@ __global__ void __launch_bounds__(kPW) curve_bwd(const float* xv, const float* g
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < nk; k += kPW) atomicAdd(gys + k, acc[k + 2]);
     }
+    if constexpr (EXPO) {
+        __shared__ double spart[kPW / 64];
+        block_add(sacc * 0.69314718055994530942, spart, gst);   // d(2^s)/ds = ln2 * 2^s
+    }
 }
 
 unsigned blocks_for(size_t n) { return (unsigned)std::min<size_t>((n + kPW - 1) / kPW, 8192); }
@@ -270,13 +285,17 @@ hipError_t launch_boost_backward(const float* x, const float* base, const float*
     return hipGetLastError();
 }
 hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t knots,
-                                 uint32_t cubic,
-                                 float* gx, double* gys, hipStream_t s) {
+                                 uint32_t cubic, float* gx, double* gys, hipStream_t s, float m, double* g_stops) {
     if (!n) return hipSuccess;
-    if (knots <= 16)   // per-thread accumulator columns
-        hipLaunchKernelGGL(curve_bwd<true>, dim3(reduce_blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
-    else
-        hipLaunchKernelGGL(curve_bwd<false>, dim3(blocks_for(n)), dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys);
+    const bool lane = knots <= 16;   // per-thread accumulator columns
+    const dim3 grid(lane ? reduce_blocks_for(n) : blocks_for(n));
+    if (g_stops) {
+        if (lane) hipLaunchKernelGGL((curve_bwd<true, true>), grid, dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys, m, g_stops);
+        else hipLaunchKernelGGL((curve_bwd<false, true>), grid, dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys, m, g_stops);
+    } else {
+        if (lane) hipLaunchKernelGGL((curve_bwd<true, false>), grid, dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys, 1.0f, nullptr);
+        else hipLaunchKernelGGL((curve_bwd<false, false>), grid, dim3(kPW), 0, s, x, gy, n, curve, cubic, gx, gys, 1.0f, nullptr);
+    }
     return hipGetLastError();
 }
 

@@ -139,8 +139,10 @@ hipError_t launch_exposure_backward(const float* x, const float* gy, float m, fl
 hipError_t launch_boost(const float* x, const float* base, float strength, float* y, size_t n, hipStream_t s);
 hipError_t launch_boost_backward(const float* x, const float* base, const float* gy, float strength, float* gx,
                                  float* gbase, double* gs, size_t n, hipStream_t s);
+// With g_stops set, x is the input of an exposure y = m x that feeds the
+// curve: the exposure's adjoint is taken in the same pass.
 hipError_t launch_curve_backward(const float* x, const float* gy, size_t n, const DevCurve* curve, uint32_t knots,
-                                 uint32_t cubic,
-                                 float* gx, double* gys, hipStream_t s);
+                                 uint32_t cubic, float* gx, double* gys, hipStream_t s, float m = 1.0f,
+                                 double* g_stops = nullptr);
 
 }  // namespace rawops

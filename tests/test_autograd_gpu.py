@@ -2,6 +2,7 @@
 restatements (frozen Menon decisions make the taken branch exactly linear).
 Tolerance: 1e-4 relative to the largest gradient magnitude of the check (the
 FP32 forward/adjoint accumulate in FP32/FP64 mixes)."""
+import ctypes
 import math
 
 import numpy as np
@@ -221,6 +222,42 @@ def test_guided_filter_gradients_vs_fd(radius):
         assert abs(float(e.grad) - fe) <= 1e-4 * max(1e-12, abs(fe)) + 1e-6, (float(e.grad), fe)
         # forward agrees with the FP64 ordered restatement
         assert rel(q.detach().cpu().numpy(), O.guided_filter_f64(p64, radius, 1e-3, space)) <= 1e-5
+
+
+@pytest.mark.parametrize("cubic", [True, False])
+@pytest.mark.parametrize("nk", [16, 20])
+def test_exposure_curve_backward_matches_the_two_adjoints(cubic, nk):
+    """rawops_exposure_curve_backward (exposure then curve, one pass) against
+    rawops_curve_backward on m x followed by rawops_exposure_backward: dL/dx bit
+    for bit, the FP64 knot and stop sums to 1e-12 (their summation order
+    differs).  16 knots take the per-lane accumulator form, 20 the shared one;
+    values span both extensions and every piece."""
+    C = A.C
+    knots = synth.monotone_knots(nk, seed=nk)
+    X = (ctypes.c_double * (nk + 6))(*([k[0] for k in knots] + [0.0, 1.0] * 3))
+    Y = (ctypes.c_double * (nk + 6))(*([k[1] for k in knots] + [0.0, 1.0] * 3))
+    plan = A.Program.from_ops(C.DEMOSAIC_BILINEAR, False, [],
+                              [(C.OP_CURVES, (), (X, Y, [0, nk, nk + 2, nk + 4, nk + 6], int(cubic)))])
+    m = A._multiplier(0.7)
+    rng = np.random.default_rng(nk + cubic)
+    for shape in ((700, 1000, 3), (5, 7, 3)):
+        x = torch.from_numpy(rng.uniform(-0.1, 1.0, shape).astype(np.float32)).cuda()
+        gy = torch.from_numpy(rng.normal(size=shape).astype(np.float32)).cuda()
+        n = x.numel()
+        lin = x * m
+        g1, g2, gf = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+        gk, gkf = (torch.zeros(nk, dtype=torch.float64, device="cuda") for _ in range(2))
+        gs, gsf = (torch.zeros(1, dtype=torch.float64, device="cuda") for _ in range(2))
+        P = lambda t: ctypes.c_void_p(t.data_ptr())
+        C.check(C.lib().rawops_curve_backward(plan.handle, P(lin), P(gy), n, P(g1), P(gk), None))
+        C.check(C.lib().rawops_exposure_backward(P(x), P(g1), m, P(g2), P(gs), n, None))
+        C.check(C.lib().rawops_exposure_curve_backward(plan.handle, P(x), m, P(gy), n, P(gf), P(gkf), P(gsf), None))
+        torch.cuda.synchronize()
+        assert torch.equal(gf, g2), shape
+        assert np.allclose(gkf.cpu().numpy(), gk.cpu().numpy(), rtol=1e-12, atol=1e-12 * float(gk.abs().max())), shape
+        assert abs(float(gsf) - float(gs)) <= 1e-12 * abs(float(gs)), shape
+    with pytest.raises(C.InvalidArgument):
+        C.check(C.lib().rawops_exposure_curve_backward(plan.handle, P(x), m, P(gy), n, P(gf), P(gkf), None, None))
 
 
 @pytest.mark.parametrize("cubic", [True, False])
